@@ -1,0 +1,300 @@
+/*
+ * nsfem_gpu.h -- C ABI of the MI355X (gfx950) CG + semi-Lagrangian advection
+ * library (libnsfem_hip.so).
+ *
+ * This is the drop-in boundary for the GPU simulation device of
+ * vasil-pashov/Navier_Stokes_FEM: every entry point below names the
+ * reference interface it replaces (file:line in that repository).  Plain
+ * pointers and sizes only; every function returns an int status (0 = ok,
+ * like EC::ErrorCode, include/error_code.h:8-36) and the message of the last
+ * failure on the calling thread is available from nsfem_last_error().
+ *
+ * Precision: the reference is FP32 throughout (gpu_cpu_shared/misc_common.cuh:8).
+ * This library computes the plain method in FP64 (vectors, matrix values,
+ * dot products).  Point location is FP32 exactly as the reference computes it
+ * (mesh coordinates are float), so KD-tree element indices are bit-exact with
+ * the reference.  nsfem_advect() is the reference's FP32 call verbatim.
+ *
+ * Ownership: the caller owns every host array and may free it when the call
+ * returns (data are copied).  The device handle owns every device allocation
+ * made through it; nsfem_device_destroy() frees them.  Calls on one device
+ * handle are serialised on the handle's single HIP stream; host-pointer calls
+ * are synchronous (they return after the result has been copied back, like
+ * the reference's cuMemcpyDtoH).  Matrix uploads are thread-safe (the
+ * reference uploads from three TBB tasks at once, include/assembly.h:1321,
+ * 1365,1405).
+ */
+#ifndef NSFEM_GPU_H
+#define NSFEM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define NSFEM_OK 0
+#define NSFEM_ERROR (-1)                 /* EC::ErrorCode(const char*) default */
+#define NSFEM_MAX_ITERATIONS_REACHED (-2)/* cpp/gpu_simulation_device.cpp:382,514 */
+#define NSFEM_INVALID_ARGUMENT (-3)
+#define NSFEM_HIP_ERROR (-4)
+#define NSFEM_OUT_OF_MEMORY (-5)
+#define NSFEM_NOT_READY (-6)             /* slot / tree / buffer not uploaded */
+#define NSFEM_BREAKDOWN (-7)             /* (Ap, p) <= 0: matrix not SPD */
+#define NSFEM_BARRIER_TIMEOUT (-8)       /* persistent kernel barrier gave up */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *nsfem_last_error(void);
+
+/* ---------------------------------------------------------------- device */
+typedef struct nsfem_device nsfem_device;
+typedef struct nsfem_buffer nsfem_buffer;
+
+/* GPUSimulationDevice::SimMatrix (include/gpu_simulation_device.h:16-23).
+ * Slots 5..NSFEM_MAX_SLOTS-1 are free for the caller. */
+#define NSFEM_PRESSURE_STIFFNESS 0
+#define NSFEM_VELOCITY_MASS 1
+#define NSFEM_DIFFUSION 2
+#define NSFEM_VELOCITY_DIVERGENCE 3
+#define NSFEM_PRESSURE_DIVERGENCE 4
+#define NSFEM_MAX_SLOTS 16
+
+int nsfem_get_device_count(int *count);
+
+/* GPUSimulationDevice::init(int) (cpp/gpu_simulation_device.cpp:11-20) and
+ * GPUSimulationDeviceManager::init() (:524-550): the kernels are part of the
+ * library's code object, there is no module to load (replaces loadModules,
+ * :22-70). */
+int nsfem_device_create(int device_index, nsfem_device **out);
+int nsfem_device_destroy(nsfem_device *dev);
+/* Name, CU count, HBM bytes, clock (kHz) of the device behind the handle. */
+int nsfem_device_info(nsfem_device *dev, char *name, int name_len, int *cu_count,
+                      int64_t *hbm_bytes, int *clock_khz);
+/* Block until all work issued on the handle's stream has finished. */
+int nsfem_device_synchronize(nsfem_device *dev);
+
+/* ------------------------------------------------------ matrices (CSR) */
+/* uploadMatrix(SimMatrix, const SMM::TripletMatrix<float>&)
+ * (cpp/gpu_simulation_device.cpp:133-140 -> GPUSparseMatrix::upload,
+ * cpp/gpu_sparse_matrix.cpp:7-49): triplets are sorted by (row, col) and
+ * duplicates summed in input order; the device layout is the reference's
+ * CSR (rowStart[rows+1], columnIndex[nnz] ascending per row, values[nnz]). */
+int nsfem_upload_matrix_triplet(nsfem_device *dev, int slot, int rows, int cols,
+                                int64_t count, const int32_t *row,
+                                const int32_t *col, const double *value);
+/* Same slot upload from a ready CSR (columns must ascend within a row, as the
+ * reference asserts at cpp/gpu_sparse_matrix.cpp:29). */
+int nsfem_upload_matrix_csr(nsfem_device *dev, int slot, int rows, int cols,
+                            int64_t nnz, const int32_t *row_start,
+                            const int32_t *col, const double *value);
+int nsfem_matrix_info(nsfem_device *dev, int slot, int *rows, int *cols, int64_t *nnz);
+int nsfem_free_matrix(nsfem_device *dev, int slot);
+
+/* ------------------------------------------------------ device buffers */
+/* GPU::GPUBuffer (include/gpu_host_common.h, cpp/gpu_host_common.cpp:250-333):
+ * init / uploadBuffer / downloadBuffer / RAII free. */
+int nsfem_buffer_create(nsfem_device *dev, int64_t byte_size, nsfem_buffer **out);
+int nsfem_buffer_destroy(nsfem_buffer *buf);
+int nsfem_buffer_upload(nsfem_buffer *buf, const void *src, int64_t byte_size);
+int nsfem_buffer_download(const nsfem_buffer *buf, void *dst, int64_t byte_size);
+int64_t nsfem_buffer_byte_size(const nsfem_buffer *buf);
+void *nsfem_buffer_device_pointer(const nsfem_buffer *buf);
+
+/* --------------------------------------------- CSR products and BLAS-1 */
+/* All vectors are FP64 device buffers.  Each call is asynchronous on the
+ * handle's stream (the reference's launchers are, too). */
+
+/* spRMult: res = A*mult (gpu_simulation_device.cpp:142-166, kernel
+ * gpu/matrix_math.cu:19-51).  mult must not alias res. */
+int nsfem_sp_r_mult(nsfem_device *dev, int slot, const nsfem_buffer *mult,
+                    nsfem_buffer *res);
+/* spRMultSub: res = lhs - A*mult; res may alias lhs (:173-199,
+ * gpu/matrix_math.cu:66-99). */
+int nsfem_sp_r_mult_sub(nsfem_device *dev, int slot, const nsfem_buffer *lhs,
+                        const nsfem_buffer *mult, nsfem_buffer *res);
+/* saxpy: y += a*x (:201-222, gpu/matrix_math.cu:106-122). */
+int nsfem_saxpy(nsfem_device *dev, int n, double a, const nsfem_buffer *x,
+                nsfem_buffer *y);
+/* saxpby: result = a*x + b*y; result may alias y (:224-249,
+ * gpu/matrix_math.cu:131-155). */
+int nsfem_saxpby(nsfem_device *dev, int n, double a, double b,
+                 const nsfem_buffer *x, const nsfem_buffer *y,
+                 nsfem_buffer *result);
+/* dotProduct: result[0] += (a, b) (:251-291, gpu/matrix_math.cu:162-194).
+ * Like the reference the sum is ADDED to result[0] (the caller zeroes it);
+ * unlike the reference's float atomics the sum is deterministic. */
+int nsfem_dot_product(nsfem_device *dev, int n, const nsfem_buffer *a,
+                      const nsfem_buffer *b, nsfem_buffer *result);
+
+/* ------------------------------------------------- conjugate gradient */
+#define NSFEM_PRECISION_FP64 0          /* plain method */
+#define NSFEM_PRECISION_FP32_REFINED 1  /* FP32 SpMV/vectors, FP64 refinement */
+
+typedef struct {
+    int32_t iterations;       /* CG iterations (inner FP32 ones when refined) */
+    int32_t refinements;      /* FP64 residual recomputations (refined mode) */
+    int32_t status;           /* same as the return value */
+    int32_t columns;          /* right-hand sides solved together */
+    double rhs_norm;          /* ||b||_2 */
+    double initial_residual;  /* ||b - A x0||_2 */
+    double final_residual;    /* true ||b - A x||_2, recomputed in FP64 at exit */
+    double tolerance;         /* the absolute tolerance the solve ran to */
+    double seconds;           /* device time of the solve (HIP events) */
+} nsfem_cg_stats;
+
+/* conjugateGradient(SimMatrix, const float* b, const float* x0, float* x,
+ * int maxIterations, float eps, bool useMegakernel)
+ * (include/gpu_simulation_device.h:62-76; megakernel path
+ * cpp/gpu_simulation_device.cpp:385-515, kernel gpu/matrix_math.cu:251-287).
+ * Host arrays; absolute stopping test ||r|| < eps exactly as the reference
+ * (eps^2 > ||r||^2, :440-448 and gpu/matrix_math.cu:271); max_iterations -1
+ * means the row count.  use_megakernel is accepted and ignored: the
+ * persistent kernel always runs.  Returns NSFEM_MAX_ITERATIONS_REACHED when
+ * the limit is hit; unlike the reference x is ALWAYS written (also on an
+ * early exit, :446-448) and holds the last iterate. */
+int nsfem_conjugate_gradient(nsfem_device *dev, int slot, const double *b,
+                             const double *x0, double *x, int max_iterations,
+                             double eps, int use_megakernel);
+
+/* Device-resident solve of A x = b for `columns` right-hand sides
+ * (1 or 2; column-major buffers of rows*columns doubles: all of column 0,
+ * then all of column 1) with the stopping rule
+ *   ||b - A x|| <= max(rtol*||b||, atol)   (||r0|| replaces ||b|| if b = 0)
+ * applied per column.  x holds x0 on entry and the solution on return.
+ * precision: NSFEM_PRECISION_FP64 or NSFEM_PRECISION_FP32_REFINED.
+ * stats may be NULL; with columns == 2 it must hold 2 entries. */
+int nsfem_cg_solve(nsfem_device *dev, int slot, const nsfem_buffer *b,
+                   nsfem_buffer *x, int columns, int max_iterations, double rtol,
+                   double atol, int precision, nsfem_cg_stats *stats);
+
+/* ------------------------------------------------------------ KD tree */
+typedef struct nsfem_kdtree nsfem_kdtree;
+
+/* KDTreeBuilder::buildCPUOwner / build (cpp/kd_tree_builder.cpp:17-25,
+ * 81-148) on the host: midpoint splits of the bbox FemGrid2D computes
+ * (cpp/grid.cpp:30 with BBox2D::expand, gpu_cpu_shared/misc_common.cuh:59-65),
+ * round-robin axes, straddling triangles on both sides, leaf at
+ * min_leaf_size (16 when < 0) or max_depth (min(29, round(8+1.3f*ln E))
+ * when < 0).  xy: 2 floats per node; elements: 6 ints per element. */
+int nsfem_kdtree_build(int node_count, const float *xy, int element_count,
+                       const int32_t *elements, int max_depth, int min_leaf_size,
+                       nsfem_kdtree **out);
+int nsfem_kdtree_destroy(nsfem_kdtree *tree);
+/* Sizes, then copies of the KDNode words (2 per node, the layout of
+ * gpu_cpu_shared/kd_tree_common.cuh:63-154), leaf triangle indices, bbox. */
+int nsfem_kdtree_info(const nsfem_kdtree *tree, int32_t *node_count,
+                      int64_t *leaf_index_count, int32_t *max_depth, float bbox[4]);
+int nsfem_kdtree_copy(const nsfem_kdtree *tree, uint32_t *node_words,
+                      int32_t *leaf_indices);
+
+/* uploadKDTree(const KDTreeCPUOwner&) (cpp/gpu_simulation_device.cpp:72-77
+ * -> KDTreeCPUOwner::upload, cpp/kd_tree_builder.cpp:36-65): tree nodes,
+ * leaf indices, mesh nodes (float xy) and elements (6 ints).  Also sizes
+ * the velocity buffers to node_count (initVelocityBuffers, :79-89). */
+int nsfem_upload_kdtree(nsfem_device *dev, const nsfem_kdtree *tree,
+                        int node_count, const float *xy, int element_count,
+                        const int32_t *elements);
+/* Same from raw arrays (a tree built elsewhere in the reference layout). */
+int nsfem_upload_kdtree_arrays(nsfem_device *dev, int32_t tree_node_count,
+                               const uint32_t *node_words,
+                               int64_t leaf_index_count,
+                               const int32_t *leaf_indices, int node_count,
+                               const float *xy, int element_count,
+                               const int32_t *elements);
+/* initVelocityBuffers (cpp/gpu_simulation_device.cpp:79-89). */
+int nsfem_init_velocity_buffers(nsfem_device *dev, int n);
+
+/* advect(int, const float* u, const float* v, float dt, float* uOut,
+ * float* vOut) (cpp/gpu_simulation_device.cpp:91-131, kernel
+ * gpu/advection.cu:14-57): host arrays, FP32 throughout, bit-exact with
+ * the reference for leaf offsets < 2^24. */
+int nsfem_advect(nsfem_device *dev, int n, const float *u, const float *v,
+                 float dt, float *u_out, float *v_out);
+/* Device-resident FP64 form: FP32 point location (start point from
+ * (float)u, (float)v, (float)dt), P2 interpolation in FP64. */
+int nsfem_advect_f64(nsfem_device *dev, int n, const nsfem_buffer *u,
+                     const nsfem_buffer *v, double dt, nsfem_buffer *u_out,
+                     nsfem_buffer *v_out);
+/* Point location only (for tests and tools): element (or -1), xi, eta and,
+ * when not found, the nearest node -- KDTree::findElement
+ * (gpu_cpu_shared/kd_tree.cuh:40-141). Host arrays of n points. */
+int nsfem_locate(nsfem_device *dev, int n, const float *points, int32_t *element,
+                 float *xi, float *eta, int32_t *nearest);
+
+/* ------------------------------------------------- FEM assembly (host) */
+/* The five device matrices of NavierStokesAssembly::assembleAllMatrices
+ * (include/assembly.h:1302-1475) in FP64 from float node coordinates, plus
+ * the constant Dirichlet right-hand-side corrections.  Host only. */
+typedef struct nsfem_assembly nsfem_assembly;
+#define NSFEM_ASM_VELOCITY_MASS 0
+#define NSFEM_ASM_DIFFUSION 1
+#define NSFEM_ASM_PRESSURE_STIFFNESS 2
+#define NSFEM_ASM_VELOCITY_DIVERGENCE 3
+#define NSFEM_ASM_PRESSURE_DIVERGENCE 4
+
+int nsfem_assemble(int node_count, int pressure_count, const float *xy,
+                   int element_count, const int32_t *elements,
+                   int vel_bc_count, const int32_t *vel_bc_nodes,
+                   const double *vel_bc_u, const double *vel_bc_v,
+                   int p_bc_count, const int32_t *p_bc_nodes,
+                   const double *p_bc_values, double viscosity, double dt,
+                   int threads, nsfem_assembly **out);
+int nsfem_assembly_destroy(nsfem_assembly *a);
+int nsfem_assembly_matrix_info(const nsfem_assembly *a, int which, int *rows,
+                               int *cols, int64_t *nnz);
+int nsfem_assembly_matrix_copy(const nsfem_assembly *a, int which,
+                               int32_t *row_start, int32_t *col, double *val);
+/* Dirichlet corrections: c_u, c_v (node_count each) and c_p
+ * (pressure_count): the sums the reference subtracts per boundary entry
+ * (include/assembly.h:503-524, 583-603). */
+int nsfem_assembly_bc_copy(const nsfem_assembly *a, double *c_u, double *c_v,
+                           double *c_p);
+
+/* --------------------------------------------- device-resident stepping */
+/* NavierStokesAssembly::semiLagrangianSolve (include/assembly.h:1477-1557)
+ * with every step on the GPU: advect, two diffusion solves, the pressure
+ * solve and two velocity-mass solves per step, right-hand sides and
+ * Dirichlet patches computed on the device. */
+typedef struct nsfem_simulation nsfem_simulation;
+
+typedef struct {
+    double viscosity;
+    double dt;
+    double rtol;              /* per-solve relative tolerance (1e-8) */
+    int32_t max_iterations;   /* per solve, -1 = rows */
+    int32_t precision;        /* NSFEM_PRECISION_* */
+    int32_t pair_velocity;    /* solve u and v together (2-column CG) */
+    int32_t threads;          /* host threads for assembly / KD build, 0 = all */
+} nsfem_sim_options;
+
+typedef struct {
+    int32_t steps;
+    int64_t cg_iterations[5]; /* diffusion u, diffusion v, pressure, mass u, mass v */
+    double max_relative_residual[5];
+    double seconds;           /* device time of the steps (HIP events) */
+    double advect_seconds;
+    double cg_seconds[5];
+} nsfem_step_stats;
+
+int nsfem_simulation_create(nsfem_device *dev, int node_count,
+                            int pressure_count, const float *xy,
+                            int element_count, const int32_t *elements,
+                            int vel_bc_count, const int32_t *vel_bc_nodes,
+                            const double *vel_bc_u, const double *vel_bc_v,
+                            int p_bc_count, const int32_t *p_bc_nodes,
+                            const double *p_bc_values,
+                            const nsfem_sim_options *options,
+                            nsfem_simulation **out);
+int nsfem_simulation_destroy(nsfem_simulation *sim);
+/* Run `steps` update steps; stats accumulate over them (may be NULL). */
+int nsfem_simulation_step(nsfem_simulation *sim, int steps, nsfem_step_stats *stats);
+/* Copy fields (u, v: node_count doubles; p: pressure_count); NULL skips. */
+int nsfem_simulation_get_fields(nsfem_simulation *sim, double *u, double *v, double *p);
+int nsfem_simulation_set_fields(nsfem_simulation *sim, const double *u,
+                                const double *v, const double *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1088 @@
+// CSR products, BLAS-1 and the persistent conjugate-gradient kernel for
+// gfx950.  Replaces gpu/matrix_math.cu of the reference (spRMult :19-51,
+// spRMultSub :66-99, saxpy :106-122, saxpby :131-155, dotProduct :162-194,
+// GridSync :209-249, conjugateGradientMegakernel :251-287) and the host
+// orchestration of cpp/gpu_simulation_device.cpp:142-515.
+//
+// Layout in HBM: the reference CSR (int32 row_start[n+1], int32 col[nnz]
+// ascending per row, values[nnz]) plus a tile table cutting the rows into
+// runs of <= kTileRows rows and <= kTileNnz nonzeros.  A tile is staged
+// through LDS: one coalesced pass over col/val computes every product
+// val*x[col] into LDS, then one thread per row adds its products in row
+// order.  That keeps the column and value streams fully coalesced for the
+// 7 (P1) to 19 (P2) nonzeros per row of these matrices, and it adds the
+// products of a row in the same order as the reference's row loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <type_traits>
+
+#include "nsfem_internal.hpp"
+
+namespace nsfem {
+
+// ----------------------------------------------------------------- helpers
+namespace {
+
+constexpr int kWaves = kBlock / 64;
+constexpr int kItems = kTileNnz / kBlock;  // nonzeros per thread per tile
+
+// Workgroups are dispatched round-robin over the 8 XCDs; this maps the
+// hardware block id to a logical id so that each XCD (and its L2) works on
+// one contiguous range of rows.
+__device__ __forceinline__ int xcd_swizzle(int b, int nb) {
+    constexpr int nx = 8;
+    const int per = nb / nx, rem = nb % nx;
+    const int xcd = b % nx, local = b / nx;
+    return xcd < rem ? xcd * (per + 1) + local
+                     : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Deterministic block sum of K doubles; every thread receives the totals.
+template <int K, int BS>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *lds) {
+    constexpr int NW = BS / 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[k * NW + w] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) s += lds[k * NW + i];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c);
+template <>
+__device__ __forceinline__ double fma_t<double>(double a, double b, double c) {
+    return __builtin_fma(a, b, c);
+}
+template <>
+__device__ __forceinline__ float fma_t<float>(float a, float b, float c) {
+    return __builtin_fmaf(a, b, c);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ tile tables
+void build_tiles(Matrix &m, const std::vector<int32_t> &rs) {
+    std::vector<int32_t> t;
+    t.reserve((size_t)m.rows / 64 + 2);
+    int r = 0;
+    t.push_back(0);
+    while (r < m.rows) {
+        int end = r;
+        int64_t nz = 0;
+        while (end < m.rows && end - r < kTileRows) {
+            const int64_t rn = rs[end + 1] - rs[end];
+            if (end > r && nz + rn > kTileNnz) break;
+            nz += rn;
+            ++end;
+            if (nz > kTileNnz) break;  // a single long row forms its own tile
+        }
+        t.push_back(end);
+        r = end;
+    }
+    m.n_tiles = (int)t.size() - 1;
+    m.host_tile_nnz_prefix.assign(t.size(), 0);
+    for (int i = 0; i < m.n_tiles; ++i)
+        m.host_tile_nnz_prefix[i + 1] =
+            m.host_tile_nnz_prefix[i] + (rs[t[i + 1]] - rs[t[i]]) + (t[i + 1] - t[i]);
+    m.host_tiles = t;
+    m.tiles.resize(t.size());
+    NSFEM_HIP(hipMemcpy(m.tiles.ptr, t.data(), t.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+    m.block_tiles_grid = 0;
+}
+
+// Persistent blocks own contiguous tile ranges balanced by (nnz + rows).
+void ensure_block_tiles(Device &d, Matrix &m, int grid) {
+    if (m.block_tiles_grid == grid && m.block_tiles.ptr) return;
+    std::vector<int32_t> bt(grid + 1);
+    const int64_t total = m.host_tile_nnz_prefix[m.n_tiles];
+    int t = 0;
+    for (int b = 0; b <= grid; ++b) {
+        const int64_t target = total * b / grid;
+        while (t < m.n_tiles && m.host_tile_nnz_prefix[t] < target) ++t;
+        bt[b] = t;
+    }
+    bt[0] = 0;
+    bt[grid] = m.n_tiles;
+    m.block_tiles.resize(grid + 1);
+    NSFEM_HIP(hipMemcpyAsync(m.block_tiles.ptr, bt.data(), bt.size() * sizeof(int32_t),
+                             hipMemcpyHostToDevice, d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    m.block_tiles_grid = grid;
+}
+
+namespace {
+__global__ void k_to_float(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+}  // namespace
+
+void ensure_val32(Device &d, Matrix &m) {
+    if (m.val32.ptr && m.val32.count == (size_t)m.nnz) return;
+    m.val32.resize((size_t)m.nnz);
+    if (m.nnz) {
+        const int grid = (int)std::min<int64_t>((m.nnz + 255) / 256, 65536);
+        k_to_float<<<grid, 256, 0, d.stream>>>(m.nnz, m.val.ptr, m.val32.ptr);
+        NSFEM_HIP(hipGetLastError());
+    }
+}
+
+// ------------------------------------------------------------- CSR product
+namespace {
+
+// MODE 0: y = A x.  MODE 1: y = lhs - A x (y may alias lhs).
+// MODE 2: y = A x - lhs.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, const int32_t *__restrict__ tiles,
+                                                 const int32_t *__restrict__ rs,
+                                                 const int32_t *__restrict__ col,
+                                                 const double *__restrict__ val,
+                                                 const double *__restrict__ x,
+                                                 const double *lhs, double *y) {
+    __shared__ double prod[kTileNnz];
+    __shared__ double red[kWaves];
+    const int t = xcd_swizzle(blockIdx.x, gridDim.x);
+    if (t >= n_tiles) return;
+    const int tid = threadIdx.x;
+    const int r0 = tiles[t], r1 = tiles[t + 1];
+    const int k0 = rs[r0];
+    const int nz = rs[r1] - k0;
+    if (nz <= kTileNnz) {
+        int cj[kItems];
+        double vj[kItems];
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+            const int k = tid + i * kBlock;
+            if (k < nz) {
+                cj[i] = col[k0 + k];
+                vj[i] = val[k0 + k];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kItems; ++i) {
+            const int k = tid + i * kBlock;
+            if (k < nz) prod[k] = vj[i] * x[cj[i]];
+        }
+        __syncthreads();
+        const int row = r0 + tid;
+        if (row < r1) {
+            const int a = rs[row] - k0, b = rs[row + 1] - k0;
+            double s = 0.0;
+            for (int k = a; k < b; ++k) s += prod[k];
+            if (MODE == 0) y[row] = s;
+            if (MODE == 1) y[row] = lhs[row] - s;
+            if (MODE == 2) y[row] = s - lhs[row];
+        }
+    } else {  // one row longer than a tile
+        double s = 0.0;
+        for (int k = tid; k < nz; k += kBlock) s += val[k0 + k] * x[col[k0 + k]];
+        s = wave_sum(s);
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) {
+            double tot = 0.0;
+            for (int i = 0; i < kWaves; ++i) tot += red[i];
+            if (MODE == 0) y[r0] = tot;
+            if (MODE == 1) y[r0] = lhs[r0] - tot;
+            if (MODE == 2) y[r0] = tot - lhs[r0];
+        }
+    }
+}
+
+__global__ void k_axpy(int n, double a, const double *__restrict__ x, double *y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        y[i] = __builtin_fma(a, x[i], y[i]);
+}
+
+__global__ void k_axpby(int n, double a, double b, const double *x, const double *y,
+                        double *out) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = __builtin_fma(a, x[i], b * y[i]);
+}
+
+// Deterministic dot: block k sums the contiguous chunk k, a single block adds
+// the chunk sums in order.
+__global__ __launch_bounds__(kBlock) void k_dot_partial(int n, int chunk,
+                                                        const double *__restrict__ a,
+                                                        const double *__restrict__ b,
+                                                        double *__restrict__ partial) {
+    __shared__ double lds[kWaves];
+    const int lo = blockIdx.x * chunk;
+    const int hi = min(n, lo + chunk);
+    double s[1] = {0.0};
+    for (int i = lo + threadIdx.x; i < hi; i += kBlock) s[0] = __builtin_fma(a[i], b[i], s[0]);
+    block_sum<1, kBlock>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_final(int nb, const double *__restrict__ partial,
+                                                      double *result) {
+    __shared__ double lds[kWaves];
+    double s[1] = {0.0};
+    for (int i = threadIdx.x; i < nb; i += kBlock) s[0] += partial[i];
+    block_sum<1, kBlock>(s, lds);
+    if (threadIdx.x == 0) result[0] += s[0];
+}
+
+__global__ void k_scatter(int count, const int32_t *__restrict__ idx,
+                          const double *__restrict__ val, double *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[idx[i]] = val[i];
+}
+
+__global__ void k_add(int n, const double *__restrict__ a, double *__restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        y[i] = a[i] + y[i];
+}
+
+__global__ void k_fill(int n, double v, double *__restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        y[i] = v;
+}
+
+__global__ void k_copy(int n, const double *__restrict__ a, double *__restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        y[i] = a[i];
+}
+
+inline int ew_grid(int n) { return std::max(1, std::min((n + kBlock - 1) / kBlock, 8192)); }
+
+}  // namespace
+
+void launch_spmv(Device &d, Matrix &m, const double *x, const double *lhs, double *y,
+                 int mode) {
+    if (m.rows == 0) return;
+    const int grid = m.n_tiles;
+    if (mode == 0)
+        k_spmv<0><<<grid, kBlock, 0, d.stream>>>(m.n_tiles, m.tiles.ptr, m.row_start.ptr,
+                                                 m.col.ptr, m.val.ptr, x, lhs, y);
+    else if (mode == 1)
+        k_spmv<1><<<grid, kBlock, 0, d.stream>>>(m.n_tiles, m.tiles.ptr, m.row_start.ptr,
+                                                 m.col.ptr, m.val.ptr, x, lhs, y);
+    else
+        k_spmv<2><<<grid, kBlock, 0, d.stream>>>(m.n_tiles, m.tiles.ptr, m.row_start.ptr,
+                                                 m.col.ptr, m.val.ptr, x, lhs, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_spmv_affine(Device &d, Matrix &m, const double *x, const double *c, double *y) {
+    launch_spmv(d, m, x, c, y, 2);
+}
+
+void launch_axpy(Device &d, int n, double a, const double *x, double *y) {
+    if (n <= 0) return;
+    k_axpy<<<ew_grid(n), kBlock, 0, d.stream>>>(n, a, x, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_axpby(Device &d, int n, double a, double b, const double *x, const double *y,
+                  double *out) {
+    if (n <= 0) return;
+    k_axpby<<<ew_grid(n), kBlock, 0, d.stream>>>(n, a, b, x, y, out);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_dot_accumulate(Device &d, int n, const double *a, const double *b,
+                           double *result) {
+    const int nb = std::max(1, std::min((n + 4095) / 4096, 1024));
+    const int chunk = std::max(1, (n + nb - 1) / nb);
+    d.dot_partial.ensure(1024);
+    k_dot_partial<<<nb, kBlock, 0, d.stream>>>(n, chunk, a, b, d.dot_partial.ptr);
+    k_dot_final<<<1, kBlock, 0, d.stream>>>(nb, d.dot_partial.ptr, result);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_scatter(Device &d, int count, const int32_t *idx, const double *val, double *dst) {
+    if (count <= 0) return;
+    k_scatter<<<(count + kBlock - 1) / kBlock, kBlock, 0, d.stream>>>(count, idx, val, dst);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_add(Device &d, int n, const double *a, double *y) {
+    if (n <= 0) return;
+    k_add<<<ew_grid(n), kBlock, 0, d.stream>>>(n, a, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_fill(Device &d, int n, double v, double *y) {
+    if (n <= 0) return;
+    k_fill<<<ew_grid(n), kBlock, 0, d.stream>>>(n, v, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_copy(Device &d, int n, const double *a, double *y) {
+    if (n <= 0) return;
+    k_copy<<<ew_grid(n), kBlock, 0, d.stream>>>(n, a, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
+// =====================================================================
+// Persistent conjugate gradient
+// =====================================================================
+//
+// One cooperative launch runs a whole solve.  Each block owns a contiguous
+// run of tiles (rows); per iteration there are two grid-wide reductions:
+//
+//   phase A  p_new = r + beta*p_old (owner rows, written),
+//            q = A p_new, gathering r and p_old (p_new of a neighbour is
+//            recomputed from them, bit-identically, so no extra barrier),
+//            x += alpha_prev * p_old (the x update of the previous step,
+//            deferred to here), delta += (p_new, q)        -> reduce delta
+//   phase B  alpha = gamma/delta; r -= alpha q; gamma' += (r, r) -> reduce
+//
+// This is textbook CG (thesis Alg. CG; gpu/matrix_math.cu:256-286) with the
+// reference's 4 grid barriers per iteration fused down to 2.
+//
+// Mixed precision (NSFEM_PRECISION_FP32_REFINED): the iteration runs on FP32
+// values and vectors with every dot accumulated in FP64; the solution
+// correction d is accumulated in FP32 and folded into the FP64 x at
+// "reliable updates" (Sleijpen & van der Vorst 1996): whenever ||r|| has
+// fallen by reliable_delta since the last one, or the recurrence says the
+// tolerance is met, x += d in FP64 and r = b - A x is recomputed in FP64 and
+// rounded to FP32, keeping the search direction.  Convergence is decided
+// only on that FP64 residual.
+//
+// Grid reduction: partial sums are stored per block; arrivals count on one
+// counter per group of kGroup blocks, the last arrival of a group adds the
+// group's partials in block order and arrives at the top counter, the last
+// top arrival adds the group sums in group order and releases every group.
+// The summation order is fixed, so results are deterministic.
+
+namespace {
+
+constexpr int kGroup = 16;
+constexpr int kMaxGroups = kMaxGridBlocks / kGroup;
+
+struct alignas(256) SyncState {
+    unsigned int arrive_top;
+    unsigned int pad0[63];
+    unsigned int abort;
+    unsigned int pad1[63];
+    unsigned int arrive_group[kMaxGroups][64];
+    unsigned int release_group[kMaxGroups][64];
+    double result[kMaxRed];
+    double pad2[24];
+    double gpartial[kMaxRed][kMaxGroups];
+    double partial[kMaxRed][kMaxGridBlocks];
+};
+
+struct CgArgs {
+    int n;
+    const int32_t *tiles;
+    const int32_t *block_tiles;
+    const int32_t *rs;
+    const int32_t *col;
+    const double *val64;
+    const float *val32;
+    const double *b;  // [NC][n]
+    double *x;        // [NC][n]
+    void *r, *p0, *p1, *q, *d;  // interleaved [n][NC]
+    double rtol, atol, rdelta;
+    int max_iter, max_refine;
+    unsigned long long timeout_ticks;
+    SyncState *sync;
+    CgColumnResult *res;
+};
+
+__global__ void k_sync_reset(SyncState *s, int ngroups) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        s->arrive_top = 0;
+        s->abort = 0;
+    }
+    if (i < ngroups) {
+        s->arrive_group[i][0] = 0;
+        s->release_group[i][0] = 0;
+    }
+}
+
+__device__ __forceinline__ unsigned ld_acquire(const unsigned *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_relaxed(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid-wide sum of K values with barrier semantics.  Returns false if the
+// barrier timed out or another block aborted (the caller must return).
+template <int K, int BS>
+__device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *lds,
+                         unsigned long long timeout) {
+    // Each flag is written by thread 0 before a __syncthreads and read by
+    // the block after it; none is rewritten until the next call, which
+    // every thread reaches only after passing several more barriers.
+    __shared__ int s_group_last, s_top_last, s_ok;
+    block_sum<K, BS>(v, lds);
+    const int nb = gridDim.x;
+    const int b = blockIdx.x;
+    const int g = b / kGroup;
+    const int ng = (nb + kGroup - 1) / kGroup;
+    const int gsize = min(kGroup, nb - g * kGroup);
+    ++epoch;
+    __threadfence();  // publish this phase's global writes (release, agent)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) S->partial[k][b] = v[k];
+        const unsigned prev = __hip_atomic_fetch_add(&S->arrive_group[g][0], 1u, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+        s_group_last = (prev == epoch * (unsigned)gsize - 1u) ? 1 : 0;
+    }
+    __syncthreads();
+    if (s_group_last) {
+        __threadfence();
+        // group sum, members in block order (lane i of wave 0 holds member i)
+        double gs[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double t = 0.0;
+            if (threadIdx.x < gsize) t = ld_relaxed(&S->partial[k][g * kGroup + threadIdx.x]);
+            gs[k] = threadIdx.x < 64 ? wave_sum(t) : 0.0;
+        }
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) S->gpartial[k][g] = gs[k];
+            const unsigned prev = __hip_atomic_fetch_add(&S->arrive_top, 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            s_top_last = (prev == epoch * (unsigned)ng - 1u) ? 1 : 0;
+        }
+        __syncthreads();
+        if (s_top_last) {
+            __threadfence();
+            double ts[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                double acc = 0.0;
+                for (int i = threadIdx.x; i < ng; i += BS) acc += ld_relaxed(&S->gpartial[k][i]);
+                ts[k] = acc;
+            }
+            block_sum<K, BS>(ts, lds);
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) S->result[k] = ts[k];
+            }
+            __threadfence();
+            __syncthreads();
+            for (int i = threadIdx.x; i < ng; i += BS)
+                __hip_atomic_store(&S->release_group[i][0], epoch, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    // wait for the release of this block's group (bounded)
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned long long t0 = wall_clock64();
+        while (ld_acquire(&S->release_group[g][0]) < epoch) {
+            if (__hip_atomic_load(&S->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > timeout) {
+                __hip_atomic_store(&S->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    const bool ok = s_ok != 0;
+    __threadfence();  // acquire for every thread of the block
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = ld_relaxed(&S->result[k]);
+    __syncthreads();
+    return ok;
+}
+
+template <int NC, typename T>
+struct VecN {
+    T v[NC];
+};
+
+template <int NC, typename T>
+__device__ __forceinline__ VecN<NC, T> ldv(const T *p, int i) {
+    VecN<NC, T> o;
+    if constexpr (NC == 1) {
+        o.v[0] = p[i];
+    } else if constexpr (std::is_same<T, double>::value) {
+        const double2 t = reinterpret_cast<const double2 *>(p)[i];
+        o.v[0] = t.x;
+        o.v[1] = t.y;
+    } else {
+        const float2 t = reinterpret_cast<const float2 *>(p)[i];
+        o.v[0] = t.x;
+        o.v[1] = t.y;
+    }
+    return o;
+}
+
+template <int NC, typename T>
+__device__ __forceinline__ void stv(T *p, int i, const VecN<NC, T> &o) {
+    if constexpr (NC == 1) {
+        p[i] = o.v[0];
+    } else if constexpr (std::is_same<T, double>::value) {
+        reinterpret_cast<double2 *>(p)[i] = make_double2(o.v[0], o.v[1]);
+    } else {
+        reinterpret_cast<float2 *>(p)[i] = make_float2(o.v[0], o.v[1]);
+    }
+}
+
+// FP64 residual phase over the block's tiles: r = b - A x (x column-major).
+// SETUP: also p_old = 0 (and d = 0 for the mixed solver) and b.b.
+// Accumulates per column: acc[c] = r.r (FP64), acc[NC+c] = b.b,
+// acc[2NC+c] = (FP32 r).(FP32 r).
+template <int NC, int BS, bool MIXED, bool SETUP>
+__device__ void residual_phase(const CgArgs &A, int T0, int T1, double *prod,
+                               double (&acc)[3 * NC]) {
+    using VT = typename std::conditional<MIXED, float, double>::type;
+    constexpr int Q = BS / kBlock;
+    const int quad = threadIdx.x / kBlock, qt = threadIdx.x % kBlock;
+    double *P = prod + quad * kTileNnz * NC;
+    VT *r = (VT *)A.r;
+    VT *pold = (VT *)A.p0;
+    VT *d = (VT *)A.d;
+    for (int base = T0; base < T1; base += Q) {
+        const int t = base + quad;
+        const bool act = t < T1;
+        int r0 = 0, r1 = 0, k0 = 0, nz = 0;
+        if (act) {
+            r0 = A.tiles[t];
+            r1 = A.tiles[t + 1];
+            k0 = A.rs[r0];
+            nz = A.rs[r1] - k0;
+        }
+        const bool small = nz <= kTileNnz;
+        if (act && small) {
+            int cj[kItems];
+            double vj[kItems];
+#pragma unroll
+            for (int i = 0; i < kItems; ++i) {
+                const int k = qt + i * kBlock;
+                if (k < nz) {
+                    cj[i] = A.col[k0 + k];
+                    vj[i] = A.val64[k0 + k];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kItems; ++i) {
+                const int k = qt + i * kBlock;
+                if (k < nz) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) P[k * NC + c] = vj[i] * A.x[(size_t)c * A.n + cj[i]];
+                }
+            }
+        }
+        __syncthreads();
+        if (act) {
+            const int row = r0 + qt;
+            const bool mine = small ? row < r1 : qt == 0;
+            double s[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) s[c] = 0.0;
+            if (small && row < r1) {
+                const int a = A.rs[row] - k0, e = A.rs[row + 1] - k0;
+                for (int k = a; k < e; ++k) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) s[c] += P[k * NC + c];
+                }
+            } else if (!small && qt == 0) {  // long row: serial in row order
+                for (int k = 0; k < nz; ++k) {
+                    const double vv = A.val64[k0 + k];
+                    const int j = A.col[k0 + k];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) s[c] += vv * A.x[(size_t)c * A.n + j];
+                }
+            }
+            if (mine) {
+                const int rw = small ? row : r0;
+                VecN<NC, VT> rv, zero;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const double bb = A.b[(size_t)c * A.n + rw];
+                    const double rr = bb - s[c];
+                    acc[c] += rr * rr;
+                    if (SETUP) acc[NC + c] += bb * bb;
+                    rv.v[c] = (VT)rr;
+                    if (MIXED) acc[2 * NC + c] += (double)rv.v[c] * (double)rv.v[c];
+                    zero.v[c] = (VT)0;
+                }
+                stv<NC, VT>(r, rw, rv);
+                if (SETUP) {
+                    stv<NC, VT>(pold, rw, zero);
+                    if (MIXED) stv<NC, VT>(d, rw, zero);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int NC, int BS, bool MIXED>
+__global__ __launch_bounds__(BS) void k_cg(const CgArgs A) {
+    using VT = typename std::conditional<MIXED, float, double>::type;
+    using MT = VT;
+    constexpr int Q = BS / kBlock;
+    __shared__ double prod_raw[Q * kTileNnz * NC];
+    __shared__ double lds[kMaxRed * (BS / 64)];
+    const int nb = gridDim.x;
+    const int L = xcd_swizzle(blockIdx.x, nb);
+    const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
+    const int R0 = A.tiles[T0], R1 = A.tiles[T1];
+    const int quad = threadIdx.x / kBlock, qt = threadIdx.x % kBlock;
+    VT *prod = reinterpret_cast<VT *>(prod_raw) + quad * kTileNnz * NC;
+    const MT *val = MIXED ? (const MT *)A.val32 : (const MT *)A.val64;
+    VT *r = (VT *)A.r;
+    VT *q = (VT *)A.q;
+    VT *dv = (VT *)A.d;
+    VT *pbuf[2] = {(VT *)A.p0, (VT *)A.p1};
+    unsigned epoch = 0;
+    const bool writer = (L == 0 && threadIdx.x == 0);
+
+    // ------------------------------------------------------------ setup
+    double acc[3 * NC];
+#pragma unroll
+    for (int k = 0; k < 3 * NC; ++k) acc[k] = 0.0;
+    residual_phase<NC, BS, MIXED, true>(A, T0, T1, prod_raw, acc);
+    if (!grid_sum<3 * NC, BS>(A.sync, acc, epoch, lds, A.timeout_ticks)) goto timeout;
+    {
+        double tol2[NC], gamma[NC], rmax2[NC];
+        bool active[NC];
+        int iters[NC], status[NC], refines = 0;
+        VT beta[NC], pend[NC];
+        double last_rr[NC];
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const double bn = sqrt(acc[NC + c]), r0n = sqrt(acc[c]);
+            const double ref = bn > 0.0 ? bn : r0n;
+            const double tol = fmax(A.rtol * ref, A.atol);
+            tol2[c] = tol * tol;
+            active[c] = acc[c] > tol2[c];
+            any = any || active[c];
+            gamma[c] = MIXED ? acc[2 * NC + c] : acc[c];
+            rmax2[c] = acc[c];
+            last_rr[c] = acc[c];
+            iters[c] = 0;
+            status[c] = NSFEM_OK;
+            beta[c] = (VT)0;
+            pend[c] = (VT)0;
+            if (writer) {
+                A.res[c].rhs_norm = bn;
+                A.res[c].initial_residual = r0n;
+                A.res[c].tolerance = tol;
+            }
+        }
+        int k = 0;
+        int cur = 0;  // pbuf[cur] = p_old
+        bool has_pend = false;
+        while (any) {
+            VT *pold = pbuf[cur];
+            VT *pnew = pbuf[cur ^ 1];
+            // ------------------------------------------------- phase A
+            double delta[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) delta[c] = 0.0;
+            for (int base = T0; base < T1; base += Q) {
+                const int t = base + quad;
+                const bool act = t < T1;
+                int r0 = 0, r1 = 0, k0 = 0, nz = 0;
+                if (act) {
+                    r0 = A.tiles[t];
+                    r1 = A.tiles[t + 1];
+                    k0 = A.rs[r0];
+                    nz = A.rs[r1] - k0;
+                }
+                const bool small = nz <= kTileNnz;
+                if (act && small) {
+                    int cj[kItems];
+                    MT vj[kItems];
+#pragma unroll
+                    for (int i = 0; i < kItems; ++i) {
+                        const int kk = qt + i * kBlock;
+                        if (kk < nz) {
+                            cj[i] = A.col[k0 + kk];
+                            vj[i] = val[k0 + kk];
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < kItems; ++i) {
+                        const int kk = qt + i * kBlock;
+                        if (kk < nz) {
+                            const VecN<NC, VT> rj = ldv<NC, VT>(r, cj[i]);
+                            const VecN<NC, VT> pj = ldv<NC, VT>(pold, cj[i]);
+#pragma unroll
+                            for (int c = 0; c < NC; ++c)
+                                prod[kk * NC + c] = vj[i] * fma_t<VT>(beta[c], pj.v[c], rj.v[c]);
+                        }
+                    }
+                }
+                __syncthreads();
+                if (act) {
+                    const int row = r0 + qt;
+                    const bool mine = small ? row < r1 : qt == 0;
+                    VT s[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) s[c] = (VT)0;
+                    if (small && row < r1) {
+                        const int a = A.rs[row] - k0, e = A.rs[row + 1] - k0;
+                        for (int kk = a; kk < e; ++kk) {
+#pragma unroll
+                            for (int c = 0; c < NC; ++c) s[c] += prod[kk * NC + c];
+                        }
+                    } else if (!small && qt == 0) {
+                        for (int kk = 0; kk < nz; ++kk) {
+                            const int j = A.col[k0 + kk];
+                            const VecN<NC, VT> rj = ldv<NC, VT>(r, j);
+                            const VecN<NC, VT> pj = ldv<NC, VT>(pold, j);
+#pragma unroll
+                            for (int c = 0; c < NC; ++c)
+                                s[c] += val[k0 + kk] * fma_t<VT>(beta[c], pj.v[c], rj.v[c]);
+                        }
+                    }
+                    if (mine) {
+                        const int rw = small ? row : r0;
+                        const VecN<NC, VT> rv = ldv<NC, VT>(r, rw);
+                        const VecN<NC, VT> pv = ldv<NC, VT>(pold, rw);
+                        VecN<NC, VT> pn, qq;
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) {
+                            pn.v[c] = fma_t<VT>(beta[c], pv.v[c], rv.v[c]);
+                            qq.v[c] = s[c];
+                            delta[c] += (double)pn.v[c] * (double)s[c];
+                        }
+                        stv<NC, VT>(pnew, rw, pn);
+                        stv<NC, VT>(q, rw, qq);
+                        if (has_pend) {
+                            if (MIXED) {
+                                VecN<NC, VT> dd = ldv<NC, VT>(dv, rw);
+#pragma unroll
+                                for (int c = 0; c < NC; ++c) dd.v[c] = fma_t<VT>(pend[c], pv.v[c], dd.v[c]);
+                                stv<NC, VT>(dv, rw, dd);
+                            } else {
+#pragma unroll
+                                for (int c = 0; c < NC; ++c) {
+                                    double *xp = A.x + (size_t)c * A.n + rw;
+                                    *xp = __builtin_fma((double)pend[c], (double)pv.v[c], *xp);
+                                }
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            if (!grid_sum<NC, BS>(A.sync, delta, epoch, lds, A.timeout_ticks)) goto timeout;
+            VT alpha[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                alpha[c] = (VT)0;
+                if (active[c]) {
+                    if (!(delta[c] > 0.0)) {
+                        status[c] = NSFEM_BREAKDOWN;
+                        active[c] = false;
+                    } else {
+                        alpha[c] = (VT)(gamma[c] / delta[c]);
+                    }
+                }
+            }
+            // ------------------------------------------------- phase B
+            double g[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) g[c] = 0.0;
+            for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) {
+                VecN<NC, VT> rv = ldv<NC, VT>(r, row);
+                const VecN<NC, VT> qv = ldv<NC, VT>(q, row);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    rv.v[c] = fma_t<VT>(-alpha[c], qv.v[c], rv.v[c]);
+                    g[c] += (double)rv.v[c] * (double)rv.v[c];
+                }
+                stv<NC, VT>(r, row, rv);
+            }
+            if (!grid_sum<NC, BS>(A.sync, g, epoch, lds, A.timeout_ticks)) goto timeout;
+            ++k;
+            bool want_ru = false;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                if (active[c]) {
+                    iters[c] = k;
+                    if (MIXED) {
+                        if (g[c] <= tol2[c] || g[c] < A.rdelta * A.rdelta * rmax2[c]) want_ru = true;
+                    } else if (g[c] <= tol2[c]) {
+                        active[c] = false;  // its last update is still pending
+                        last_rr[c] = g[c];
+                    }
+                }
+                pend[c] = alpha[c];
+            }
+            has_pend = true;
+            const bool out_of_iters = k >= A.max_iter;
+            if (MIXED && (want_ru || out_of_iters)) {
+                // fold d + alpha*p_new into the FP64 solution, recompute r
+                for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) {
+                    VecN<NC, VT> dd = ldv<NC, VT>(dv, row);
+                    const VecN<NC, VT> pv = ldv<NC, VT>(pnew, row);
+                    VecN<NC, VT> zero;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        dd.v[c] = fma_t<VT>(pend[c], pv.v[c], dd.v[c]);
+                        A.x[(size_t)c * A.n + row] += (double)dd.v[c];
+                        zero.v[c] = (VT)0;
+                    }
+                    stv<NC, VT>(dv, row, zero);
+                }
+                has_pend = false;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) pend[c] = (VT)0;
+                double dummy[1] = {0.0};
+                if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks)) goto timeout;
+                double ra[3 * NC];
+#pragma unroll
+                for (int kk = 0; kk < 3 * NC; ++kk) ra[kk] = 0.0;
+                residual_phase<NC, BS, MIXED, false>(A, T0, T1, prod_raw, ra);
+                if (!grid_sum<3 * NC, BS>(A.sync, ra, epoch, lds, A.timeout_ticks)) goto timeout;
+                ++refines;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    last_rr[c] = ra[c];
+                    if (active[c]) {
+                        if (ra[c] <= tol2[c]) {
+                            active[c] = false;
+                        } else {
+                            g[c] = ra[2 * NC + c];
+                            rmax2[c] = ra[c];
+                        }
+                    }
+                }
+                if (refines >= A.max_refine) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+                        if (active[c]) {
+                            status[c] = NSFEM_MAX_ITERATIONS_REACHED;
+                            active[c] = false;
+                        }
+                }
+            }
+            any = false;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                if (active[c] && out_of_iters) {
+                    status[c] = NSFEM_MAX_ITERATIONS_REACHED;
+                    active[c] = false;
+                    last_rr[c] = g[c];
+                }
+                any = any || active[c];
+                beta[c] = active[c] ? (VT)(g[c] / gamma[c]) : (VT)0;
+                if (active[c]) gamma[c] = g[c];
+            }
+            if (!any) {
+                // flush the pending x update of the last iteration (FP64 path)
+                if (!MIXED && has_pend) {
+                    for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) {
+                        const VecN<NC, VT> pv = ldv<NC, VT>(pnew, row);
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) {
+                            double *xp = A.x + (size_t)c * A.n + row;
+                            *xp = __builtin_fma((double)pend[c], (double)pv.v[c], *xp);
+                        }
+                    }
+                }
+                break;
+            }
+            cur ^= 1;
+        }
+        if (writer) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                A.res[c].iterations = iters[c];
+                A.res[c].refinements = refines;
+                A.res[c].status = status[c];
+                A.res[c].last_residual = sqrt(last_rr[c]);
+            }
+        }
+        return;
+    }
+timeout:
+    if (writer) {
+        for (int c = 0; c < NC; ++c) A.res[c].status = NSFEM_BARRIER_TIMEOUT;
+    }
+}
+
+template <int NC, int BS, bool MIXED>
+void cg_launch(Device &d, const CgArgs &args, int grid) {
+    void *kargs[] = {(void *)&args};
+    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg<NC, BS, MIXED>, dim3(grid),
+                                         dim3(BS), kargs, 0, d.stream));
+}
+
+template <int NC, int BS, bool MIXED>
+int cg_max_grid(Device &d) {
+    int per_cu = 0;
+    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg<NC, BS, MIXED>, BS, 0));
+    return std::max(1, std::min(per_cu * d.cu_count, kMaxGridBlocks));
+}
+
+}  // namespace
+
+static constexpr int kCgBlock = 256;
+
+namespace {
+// true residual ||b_c - A x_c|| for each column (FP64), into host res[c]
+void true_residuals(Device &d, Matrix &m, const double *b, const double *x, int nc,
+                    double *tmp, double *out) {
+    d.scalars.ensure(4);
+    launch_fill(d, nc, 0.0, d.scalars.ptr);
+    for (int c = 0; c < nc; ++c) {
+        launch_spmv(d, m, x + (size_t)c * m.rows, b + (size_t)c * m.rows, tmp, 1);
+        launch_dot_accumulate(d, m.rows, tmp, tmp, d.scalars.ptr + c);
+    }
+    double h[2] = {0, 0};
+    NSFEM_HIP(hipMemcpyAsync(h, d.scalars.ptr, sizeof(double) * nc, hipMemcpyDeviceToHost,
+                             d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    for (int c = 0; c < nc; ++c) out[c] = std::sqrt(h[c]);
+}
+}  // namespace
+
+int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
+    Matrix &m = *req.m;
+    NSFEM_REQUIRE(m.ready, NSFEM_NOT_READY, "matrix slot is empty");
+    NSFEM_REQUIRE(m.rows == m.cols, NSFEM_INVALID_ARGUMENT, "CG needs a square matrix");
+    NSFEM_REQUIRE(req.ncol == 1 || req.ncol == 2, NSFEM_INVALID_ARGUMENT,
+                  "columns must be 1 or 2");
+    const bool mixed = req.precision == NSFEM_PRECISION_FP32_REFINED;
+    NSFEM_REQUIRE(mixed || req.precision == NSFEM_PRECISION_FP64, NSFEM_INVALID_ARGUMENT,
+                  "unknown precision");
+    const int n = m.rows, nc = req.ncol;
+    if (mixed) ensure_val32(d, m);
+
+    int maxg;
+    if (nc == 1)
+        maxg = mixed ? cg_max_grid<1, kCgBlock, true>(d) : cg_max_grid<1, kCgBlock, false>(d);
+    else
+        maxg = mixed ? cg_max_grid<2, kCgBlock, true>(d) : cg_max_grid<2, kCgBlock, false>(d);
+    int grid = req.grid > 0 ? std::min(req.grid, maxg)
+                            : std::min(maxg, std::max(1, m.n_tiles / 2));
+    grid = std::max(1, std::min(grid, std::max(1, m.n_tiles)));
+    ensure_block_tiles(d, m, grid);
+
+    const size_t vt = mixed ? sizeof(float) : sizeof(double);
+    const size_t vec_bytes = ((size_t)n * nc * vt + 255) / 256 * 256;
+    d.work.ensure(vec_bytes * 5);
+    d.sync_mem.ensure(sizeof(SyncState) + 256);
+    d.cg_res.ensure(2);
+
+    CgArgs a{};
+    a.n = n;
+    a.tiles = m.tiles.ptr;
+    a.block_tiles = m.block_tiles.ptr;
+    a.rs = m.row_start.ptr;
+    a.col = m.col.ptr;
+    a.val64 = m.val.ptr;
+    a.val32 = m.val32.ptr;
+    a.b = req.b;
+    a.x = req.x;
+    unsigned char *w = d.work.ptr;
+    a.r = w;
+    a.p0 = w + vec_bytes;
+    a.p1 = w + 2 * vec_bytes;
+    a.q = w + 3 * vec_bytes;
+    a.d = w + 4 * vec_bytes;
+    a.rtol = req.rtol;
+    a.atol = req.atol;
+    a.rdelta = req.reliable_delta;
+    a.max_iter = req.max_iter < 0 ? n : req.max_iter;
+    a.max_refine = 400;
+    a.timeout_ticks = (unsigned long long)d.wall_khz * 1000ull * 20ull;  // 20 s per barrier
+    unsigned char *sm = d.sync_mem.ptr;
+    a.sync = reinterpret_cast<SyncState *>(sm + (256 - ((uintptr_t)sm % 256)) % 256);
+    a.res = d.cg_res.ptr;
+    const int ngroups = (grid + kGroup - 1) / kGroup;
+
+    CgColumnResult res[2];
+    int total_iters[2] = {0, 0}, total_ref[2] = {0, 0};
+    double final_res[2] = {0, 0};
+    float total_ms = 0.f;
+    // The FP64 path ends on the recursive residual; the true residual is
+    // recomputed and, should rounding drift have left it above the
+    // tolerance, the solve restarts from the current x (at most twice).
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        NSFEM_HIP(hipEventRecord(d.ev_start, d.stream));
+        k_sync_reset<<<(ngroups + 255) / 256, 256, 0, d.stream>>>(a.sync, ngroups);
+        NSFEM_HIP(hipGetLastError());
+        if (nc == 1) {
+            if (mixed) cg_launch<1, kCgBlock, true>(d, a, grid);
+            else cg_launch<1, kCgBlock, false>(d, a, grid);
+        } else {
+            if (mixed) cg_launch<2, kCgBlock, true>(d, a, grid);
+            else cg_launch<2, kCgBlock, false>(d, a, grid);
+        }
+        NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
+        NSFEM_HIP(hipMemcpyAsync(res, d.cg_res.ptr, sizeof(CgColumnResult) * nc,
+                                 hipMemcpyDeviceToHost, d.stream));
+        NSFEM_HIP(hipStreamSynchronize(d.stream));
+        float ms = 0.f;
+        NSFEM_HIP(hipEventElapsedTime(&ms, d.ev_start, d.ev_stop));
+        total_ms += ms;
+        bool drift = false;
+        if (!mixed) {
+            true_residuals(d, m, req.b, req.x, nc, (double *)a.q, final_res);
+        }
+        for (int c = 0; c < nc; ++c) {
+            total_iters[c] += res[c].iterations;
+            total_ref[c] += res[c].refinements;
+            if (mixed) final_res[c] = res[c].last_residual;
+            if (res[c].status == NSFEM_OK && final_res[c] > res[c].tolerance * (1.0 + 1e-12))
+                drift = true;
+            if (res[c].status == NSFEM_BARRIER_TIMEOUT)
+                fail(NSFEM_BARRIER_TIMEOUT, "CG kernel barrier timed out");
+        }
+        if (!drift || mixed) break;
+    }
+
+    int worst = NSFEM_OK;
+    for (int c = 0; c < nc; ++c) {
+        if (res[c].status != NSFEM_OK && worst == NSFEM_OK) worst = res[c].status;
+        if (stats) {
+            nsfem_cg_stats &s = stats[c];
+            s.iterations = total_iters[c];
+            s.refinements = total_ref[c];
+            s.status = res[c].status;
+            s.columns = nc;
+            s.rhs_norm = res[c].rhs_norm;
+            s.initial_residual = res[c].initial_residual;
+            s.final_residual = final_res[c];
+            s.tolerance = res[c].tolerance;
+            s.seconds = total_ms * 1e-3;
+        }
+    }
+    return worst;
+}
+
+}  // namespace nsfem

@@ -1,0 +1,268 @@
+// Internal types of libnsfem_hip.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nsfem_gpu.h"
+
+namespace nsfem {
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+[[noreturn]] void fail(int code, const std::string &msg);
+
+#define NSFEM_HIP(call)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            ::nsfem::fail(e_ == hipErrorOutOfMemory ? NSFEM_OUT_OF_MEMORY      \
+                                                    : NSFEM_HIP_ERROR,         \
+                          std::string(#call) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+#define NSFEM_REQUIRE(cond, code, msg)                                         \
+    do {                                                                       \
+        if (!(cond)) ::nsfem::fail((code), (msg));                             \
+    } while (0)
+
+// Run `body` and translate exceptions into a status code + last error.
+template <typename F>
+int guard(F &&body) {
+    try {
+        body();
+        return NSFEM_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host out of memory");
+        return NSFEM_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return NSFEM_ERROR;
+    }
+}
+
+// ------------------------------------------------------------ device memory
+template <typename T>
+struct DeviceArray {
+    T *ptr = nullptr;
+    size_t count = 0;
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray &) = delete;
+    DeviceArray &operator=(const DeviceArray &) = delete;
+    DeviceArray(DeviceArray &&o) noexcept : ptr(o.ptr), count(o.count) {
+        o.ptr = nullptr;
+        o.count = 0;
+    }
+    DeviceArray &operator=(DeviceArray &&o) noexcept {
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            count = o.count;
+            o.ptr = nullptr;
+            o.count = 0;
+        }
+        return *this;
+    }
+    ~DeviceArray() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    void resize(size_t n) {  // contents are not preserved
+        if (n == count && ptr) return;
+        release();
+        if (n) NSFEM_HIP(hipMalloc(&ptr, n * sizeof(T)));
+        count = n;
+    }
+    void ensure(size_t n) {
+        if (n > count) resize(n);
+    }
+    size_t bytes() const { return count * sizeof(T); }
+};
+
+// --------------------------------------------------------------- CSR matrix
+// Rows are cut into tiles of at most kTileRows rows and kTileNnz nonzeros
+// (a single longer row is a tile of its own); tiles are the unit of the
+// LDS-staged CSR product and of the persistent CG's row ownership.
+constexpr int kBlock = 256;
+constexpr int kTileRows = 256;
+constexpr int kTileNnz = 2048;
+constexpr int kMaxGridBlocks = 4096;
+
+struct Matrix {
+    int rows = 0, cols = 0;
+    int64_t nnz = 0;
+    DeviceArray<int32_t> row_start, col;
+    DeviceArray<double> val;
+    DeviceArray<float> val32;  // FP32 copy for the refined solver (lazy)
+    DeviceArray<int32_t> tiles;  // [n_tiles + 1] first row of each tile
+    int n_tiles = 0;
+    std::vector<int32_t> host_tiles;
+    std::vector<int64_t> host_tile_nnz_prefix;  // [n_tiles + 1]
+    // ownership of tiles by persistent blocks, cached per grid size
+    DeviceArray<int32_t> block_tiles;
+    int block_tiles_grid = 0;
+    bool ready = false;
+    std::mutex mtx;
+};
+
+struct KdTreeGpu {
+    DeviceArray<uint32_t> nodes;     // 2 words per node
+    DeviceArray<int32_t> leaf;       // leaf triangle indices
+    DeviceArray<float> xy;           // 2 per mesh node
+    DeviceArray<int32_t> elements;   // 6 per element
+    DeviceArray<float> tri_xy;       // 6 per element: vertex coordinates
+    int32_t node_count = 0;          // tree nodes
+    int64_t leaf_count = 0;
+    int mesh_nodes = 0;
+    int elements_count = 0;
+    bool ready = false;
+};
+
+constexpr int kMaxRed = 8;
+
+struct CgColumnResult {
+    int32_t iterations;
+    int32_t refinements;
+    int32_t status;
+    int32_t pad;
+    double rhs_norm;
+    double initial_residual;
+    double tolerance;
+    double last_residual;  // sqrt of the last FP64 (true) residual^2 computed in-kernel
+};
+
+struct Device {
+    int index = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t props{};
+    int cu_count = 0;
+    Matrix slots[NSFEM_MAX_SLOTS];
+    KdTreeGpu tree;
+    DeviceArray<float> vel32[4];  // initVelocityBuffers: u in, v in, u out, v out
+    int vel_n = 0;
+    // scratch for CG / dot
+    DeviceArray<unsigned char> work;      // CG vectors
+    DeviceArray<unsigned char> sync_mem;  // persistent-kernel barrier state
+    DeviceArray<CgColumnResult> cg_res;
+    DeviceArray<double> dot_partial;
+    DeviceArray<double> scalars;
+    int wall_khz = 100000;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    std::mutex mtx;  // serialises solves on this handle
+    void set_current() const { NSFEM_HIP(hipSetDevice(index)); }
+};
+
+struct Buffer {
+    Device *dev = nullptr;
+    DeviceArray<unsigned char> mem;
+};
+
+// ------------------------------------------------------------ entry points
+// (implemented in the .hip translation units, called by the C ABI layer)
+void build_tiles(Matrix &m, const std::vector<int32_t> &row_start);
+void ensure_block_tiles(Device &d, Matrix &m, int grid);
+void ensure_val32(Device &d, Matrix &m);
+
+void launch_spmv(Device &d, Matrix &m, const double *x, const double *lhs, double *y,
+                 int mode);
+void launch_axpy(Device &d, int n, double a, const double *x, double *y);
+void launch_axpby(Device &d, int n, double a, double b, const double *x,
+                  const double *y, double *out);
+void launch_dot_accumulate(Device &d, int n, const double *a, const double *b,
+                           double *result);
+
+struct CgRequest {
+    Matrix *m;
+    const double *b;  // [ncol][rows]
+    double *x;        // [ncol][rows]
+    int ncol;
+    int max_iter;
+    double rtol, atol;
+    int precision;
+    double reliable_delta = 0.1;
+    int grid = 0;     // 0 = automatic
+};
+// Runs the persistent CG; fills stats[ncol].  Returns the worst status.
+int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats);
+
+void launch_advect_f32(Device &d, int n, const float *u, const float *v, float dt,
+                       float *uo, float *vo);
+void launch_advect_f64(Device &d, int n, const double *u, const double *v, double dt,
+                       double *uo, double *vo);
+void launch_locate(Device &d, int n, const float *pts, int32_t *elem, float *xi,
+                   float *eta, int32_t *nearest);
+void build_tri_xy(Device &d);
+
+// elementwise helpers for the step driver
+void launch_scatter(Device &d, int count, const int32_t *idx, const double *val,
+                    double *dst);
+void launch_add(Device &d, int n, const double *a, double *y);  // y += a
+void launch_copy(Device &d, int n, const double *src, double *dst);
+void launch_spmv_affine(Device &d, Matrix &m, const double *x, const double *c,
+                        double *y);  // y = A x - c
+
+// Host-side helpers
+struct HostKdTree {
+    std::vector<uint32_t> nodes;
+    std::vector<int32_t> leaf;
+    float bbox[4];
+    int max_depth;
+};
+void kd_build_host(int node_count, const float *xy, int elem_count,
+                   const int32_t *elems, int max_depth, int min_leaf,
+                   HostKdTree &out);
+
+struct HostCsr {
+    int rows = 0, cols = 0;
+    std::vector<int32_t> rs, col;
+    std::vector<double> val;
+};
+
+struct HostAssembly {
+    HostCsr m[5];  // NSFEM_ASM_* order
+    std::vector<double> c_u, c_v, c_p;
+};
+
+void assemble_host(int V, int P, const float *xy, int E, const int32_t *el, int nvb,
+                   const int32_t *vb, const double *gu, const double *gv, int npb,
+                   const int32_t *pb, const double *gp, double nu, double dt, int threads,
+                   HostAssembly &out);
+
+// Validate and upload a host CSR into a device slot (builds the tile table).
+void upload_csr(Device &d, Matrix &m, int rows, int cols, int64_t nnz, const int32_t *rs,
+                const int32_t *col, const double *val);
+// Validate and upload a KD tree + mesh.
+void upload_tree(Device &d, int32_t tree_nodes, const uint32_t *words, int64_t leaf_count,
+                 const int32_t *leaf, int mesh_nodes, const float *xy, int elem_count,
+                 const int32_t *elems);
+void launch_fill(Device &d, int n, double v, double *dst);
+
+}  // namespace nsfem
+
+struct nsfem_device {
+    nsfem::Device impl;
+};
+struct nsfem_buffer {
+    nsfem::Buffer impl;
+};
+struct nsfem_kdtree {
+    nsfem::HostKdTree impl;
+};
+struct nsfem_assembly {
+    nsfem::HostAssembly impl;
+};

@@ -1,0 +1,331 @@
+// Device-resident time stepping: NavierStokesAssembly::semiLagrangianSolve
+// (include/assembly.h:1477-1557) with diffusionSolve (:489-558) and
+// applyPressure (:560-712), every operation on the GPU.
+//
+// Per step (the reference's order):
+//   1. (ut, vt) = advect(u, v)                                   (:1521-1526)
+//   2. rhs_u = M ut - c_u, rhs_u[b] = g_u;  same for v           (:501-524)
+//      u = CG(D, rhs_u, x0 = ut);  v = CG(D, rhs_v, x0 = vt)     (:530-537)
+//   3. rhs_p = Bdiv [u; v] - c_p, rhs_p[b] = g_p                 (:580-603)
+//      p = CG(Kp, rhs_p, x0 = p)                                 (:608-615)
+//   4. rhs = Pdiv p;  du = CG(M, rhs_u, x0 = u), dv likewise     (:637-657)
+//   5. u += du, v += dv; u[b] = g_u, v[b] = g_v                  (:687-708)
+// The reference runs steps 2 and 4 as two separate solves; with
+// pair_velocity they run as one 2-column CG (same iterates per column,
+// the matrix is read once for both).
+#include <algorithm>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+#include "nsfem_internal.hpp"
+
+namespace nsfem {
+
+struct Simulation {
+    Device *dev = nullptr;
+    int V = 0, P = 0;
+    nsfem_sim_options opt{};
+    DeviceArray<double> uv, utv, rhs, prhs, p, du, c_uv, c_p;
+    DeviceArray<int32_t> vb_idx, pb_idx;
+    DeviceArray<double> vb_u, vb_v, pb_val;
+    int nvb = 0, npb = 0;
+};
+
+namespace {
+
+// Last listing of a node wins, as in the reference's assignment loop.
+void last_wins(int n, const int32_t *idx, const double *a, const double *b,
+               std::vector<int32_t> &oi, std::vector<double> &oa, std::vector<double> &ob) {
+    std::unordered_map<int32_t, int> pos;
+    for (int k = 0; k < n; ++k) {
+        auto it = pos.find(idx[k]);
+        if (it == pos.end()) {
+            pos[idx[k]] = (int)oi.size();
+            oi.push_back(idx[k]);
+            oa.push_back(a[k]);
+            if (b) ob.push_back(b[k]);
+        } else {
+            oa[it->second] = a[k];
+            if (b) ob[it->second] = b[k];
+        }
+    }
+}
+
+template <typename T>
+void put(Device &d, DeviceArray<T> &dst, const T *src, size_t n) {
+    dst.resize(n);
+    if (n)
+        NSFEM_HIP(hipMemcpyAsync(dst.ptr, src, n * sizeof(T), hipMemcpyHostToDevice, d.stream));
+}
+
+}  // namespace
+
+Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
+                              const int32_t *el, int nvb, const int32_t *vb, const double *gu,
+                              const double *gv, int npb, const int32_t *pb, const double *gp,
+                              const nsfem_sim_options &opt) {
+    NSFEM_REQUIRE(opt.dt > 0 && opt.viscosity >= 0, NSFEM_INVALID_ARGUMENT,
+                  "simulation: dt must be > 0 and viscosity >= 0");
+    NSFEM_REQUIRE(opt.precision == NSFEM_PRECISION_FP64 ||
+                      opt.precision == NSFEM_PRECISION_FP32_REFINED,
+                  NSFEM_INVALID_ARGUMENT, "simulation: unknown precision");
+    HostAssembly A;
+    assemble_host(V, P, xy, E, el, nvb, vb, gu, gv, npb, pb, gp, opt.viscosity, opt.dt,
+                  opt.threads, A);
+    HostKdTree tree;
+    kd_build_host(V, xy, E, el, -1, -1, tree);
+    d.set_current();
+    const int slot_of[5] = {NSFEM_VELOCITY_MASS, NSFEM_DIFFUSION, NSFEM_PRESSURE_STIFFNESS,
+                            NSFEM_VELOCITY_DIVERGENCE, NSFEM_PRESSURE_DIVERGENCE};
+    for (int k = 0; k < 5; ++k) {
+        HostCsr &c = A.m[k];
+        upload_csr(d, d.slots[slot_of[k]], c.rows, c.cols, (int64_t)c.col.size(), c.rs.data(),
+                   c.col.data(), c.val.data());
+        HostCsr().rs.swap(c.rs);
+        std::vector<int32_t>().swap(c.col);
+        std::vector<double>().swap(c.val);
+    }
+    upload_tree(d, (int32_t)(tree.nodes.size() / 2), tree.nodes.data(), (int64_t)tree.leaf.size(),
+                tree.leaf.data(), V, xy, E, el);
+
+    auto *s = new Simulation();
+    s->dev = &d;
+    s->V = V;
+    s->P = P;
+    s->opt = opt;
+    s->uv.resize(2 * (size_t)V);
+    s->utv.resize(2 * (size_t)V);
+    s->rhs.resize(2 * (size_t)V);
+    s->du.resize(2 * (size_t)V);
+    s->prhs.resize(P);
+    s->p.resize(P);
+    std::vector<double> cuv(2 * (size_t)V);
+    std::copy(A.c_u.begin(), A.c_u.end(), cuv.begin());
+    std::copy(A.c_v.begin(), A.c_v.end(), cuv.begin() + V);
+    put(d, s->c_uv, cuv.data(), cuv.size());
+    put(d, s->c_p, A.c_p.data(), A.c_p.size());
+    std::vector<int32_t> vi, pi;
+    std::vector<double> vu, vv, pv, dummy;
+    last_wins(nvb, vb, gu, gv, vi, vu, vv);
+    last_wins(npb, pb, gp, nullptr, pi, pv, dummy);
+    s->nvb = (int)vi.size();
+    s->npb = (int)pi.size();
+    put(d, s->vb_idx, vi.data(), vi.size());
+    put(d, s->vb_u, vu.data(), vu.size());
+    put(d, s->vb_v, vv.data(), vv.size());
+    put(d, s->pb_idx, pi.data(), pi.size());
+    put(d, s->pb_val, pv.data(), pv.size());
+    // zero initial field with the velocity Dirichlet values (:1503-1507)
+    launch_fill(d, 2 * V, 0.0, s->uv.ptr);
+    launch_fill(d, P, 0.0, s->p.ptr);
+    launch_scatter(d, s->nvb, s->vb_idx.ptr, s->vb_u.ptr, s->uv.ptr);
+    launch_scatter(d, s->nvb, s->vb_idx.ptr, s->vb_v.ptr, s->uv.ptr + V);
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    return s;
+}
+
+namespace {
+
+struct SolveOut {
+    int64_t iters[2];
+    double rel[2];
+    double seconds;
+};
+
+void solve(Simulation &s, int slot, const double *b, double *x, int ncol, SolveOut &o) {
+    Device &d = *s.dev;
+    CgRequest rq{};
+    rq.m = &d.slots[slot];
+    rq.b = b;
+    rq.x = x;
+    rq.ncol = ncol;
+    rq.max_iter = s.opt.max_iterations;
+    rq.rtol = s.opt.rtol;
+    rq.atol = 0.0;
+    rq.precision = s.opt.precision;
+    nsfem_cg_stats st[2]{};
+    const int status = run_cg(d, rq, st);
+    if (status != NSFEM_OK) {
+        static const char *names[] = {"pressure stiffness", "velocity mass", "diffusion", "", ""};
+        fail(status, std::string("CG on the ") + names[slot] + " matrix did not converge (status " +
+                         std::to_string(status) + ")");
+    }
+    for (int c = 0; c < ncol; ++c) {
+        o.iters[c] = st[c].iterations;
+        o.rel[c] = st[c].rhs_norm > 0 ? st[c].final_residual / st[c].rhs_norm : st[c].final_residual;
+    }
+    o.seconds = st[0].seconds;
+}
+
+}  // namespace
+
+void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
+    Device &d = *s.dev;
+    d.set_current();
+    const int V = s.V, P = s.P;
+    double *u = s.uv.ptr, *v = s.uv.ptr + V;
+    double *ut = s.utv.ptr, *vt = s.utv.ptr + V;
+    Matrix &M = d.slots[NSFEM_VELOCITY_MASS];
+    Matrix &BD = d.slots[NSFEM_VELOCITY_DIVERGENCE];
+    Matrix &PD = d.slots[NSFEM_PRESSURE_DIVERGENCE];
+    hipEvent_t e0, e1, ea;
+    NSFEM_HIP(hipEventCreate(&e0));
+    NSFEM_HIP(hipEventCreate(&e1));
+    NSFEM_HIP(hipEventCreate(&ea));
+    for (int step = 0; step < steps; ++step) {
+        NSFEM_HIP(hipEventRecord(e0, d.stream));
+        // 1. advection
+        launch_advect_f64(d, V, u, v, s.opt.dt, ut, vt);
+        NSFEM_HIP(hipEventRecord(ea, d.stream));
+        // 2. diffusion
+        launch_spmv(d, M, ut, s.c_uv.ptr, s.rhs.ptr, 2);
+        launch_spmv(d, M, vt, s.c_uv.ptr + V, s.rhs.ptr + V, 2);
+        launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, s.rhs.ptr);
+        launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_v.ptr, s.rhs.ptr + V);
+        launch_copy(d, 2 * V, s.utv.ptr, s.uv.ptr);
+        SolveOut o[5]{};
+        if (s.opt.pair_velocity) {
+            solve(s, NSFEM_DIFFUSION, s.rhs.ptr, s.uv.ptr, 2, o[0]);
+            o[1].iters[0] = o[0].iters[1];
+            o[1].rel[0] = o[0].rel[1];
+        } else {
+            solve(s, NSFEM_DIFFUSION, s.rhs.ptr, u, 1, o[0]);
+            solve(s, NSFEM_DIFFUSION, s.rhs.ptr + V, v, 1, o[1]);
+        }
+        // 3. pressure
+        launch_spmv(d, BD, s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
+        launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.prhs.ptr);
+        solve(s, NSFEM_PRESSURE_STIFFNESS, s.prhs.ptr, s.p.ptr, 1, o[2]);
+        // 4. velocity correction
+        launch_spmv(d, PD, s.p.ptr, nullptr, s.rhs.ptr, 0);
+        launch_copy(d, 2 * V, s.uv.ptr, s.du.ptr);
+        if (s.opt.pair_velocity) {
+            solve(s, NSFEM_VELOCITY_MASS, s.rhs.ptr, s.du.ptr, 2, o[3]);
+            o[4].iters[0] = o[3].iters[1];
+            o[4].rel[0] = o[3].rel[1];
+        } else {
+            solve(s, NSFEM_VELOCITY_MASS, s.rhs.ptr, s.du.ptr, 1, o[3]);
+            solve(s, NSFEM_VELOCITY_MASS, s.rhs.ptr + V, s.du.ptr + V, 1, o[4]);
+        }
+        // 5. update and Dirichlet values
+        launch_add(d, 2 * V, s.du.ptr, s.uv.ptr);
+        launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, u);
+        launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_v.ptr, v);
+        NSFEM_HIP(hipEventRecord(e1, d.stream));
+        NSFEM_HIP(hipEventSynchronize(e1));
+        if (stats) {
+            float ms = 0.f, ma = 0.f;
+            NSFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            NSFEM_HIP(hipEventElapsedTime(&ma, e0, ea));
+            stats->steps += 1;
+            stats->seconds += ms * 1e-3;
+            stats->advect_seconds += ma * 1e-3;
+            for (int k = 0; k < 5; ++k) {
+                stats->cg_iterations[k] += o[k].iters[0];
+                stats->max_relative_residual[k] =
+                    std::max(stats->max_relative_residual[k], o[k].rel[0]);
+            }
+            const double t[5] = {o[0].seconds, s.opt.pair_velocity ? 0.0 : o[1].seconds,
+                                 o[2].seconds, o[3].seconds,
+                                 s.opt.pair_velocity ? 0.0 : o[4].seconds};
+            for (int k = 0; k < 5; ++k) stats->cg_seconds[k] += t[k];
+        }
+    }
+    (void)P;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(ea);
+}
+
+void simulation_get(Simulation &s, double *u, double *v, double *p) {
+    Device &d = *s.dev;
+    d.set_current();
+    if (u) NSFEM_HIP(hipMemcpyAsync(u, s.uv.ptr, sizeof(double) * s.V, hipMemcpyDeviceToHost, d.stream));
+    if (v)
+        NSFEM_HIP(hipMemcpyAsync(v, s.uv.ptr + s.V, sizeof(double) * s.V, hipMemcpyDeviceToHost,
+                                 d.stream));
+    if (p) NSFEM_HIP(hipMemcpyAsync(p, s.p.ptr, sizeof(double) * s.P, hipMemcpyDeviceToHost, d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+}
+
+void simulation_set(Simulation &s, const double *u, const double *v, const double *p) {
+    Device &d = *s.dev;
+    d.set_current();
+    if (u) NSFEM_HIP(hipMemcpyAsync(s.uv.ptr, u, sizeof(double) * s.V, hipMemcpyHostToDevice, d.stream));
+    if (v)
+        NSFEM_HIP(hipMemcpyAsync(s.uv.ptr + s.V, v, sizeof(double) * s.V, hipMemcpyHostToDevice,
+                                 d.stream));
+    if (p) NSFEM_HIP(hipMemcpyAsync(s.p.ptr, p, sizeof(double) * s.P, hipMemcpyHostToDevice, d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+}
+
+void simulation_destroy(Simulation *s) { delete s; }
+
+}  // namespace nsfem
+
+struct nsfem_simulation {
+    nsfem::Simulation *impl;
+};
+
+using namespace nsfem;
+
+extern "C" {
+
+int nsfem_simulation_create(nsfem_device *h, int V, int P, const float *xy, int E,
+                            const int32_t *el, int nvb, const int32_t *vb, const double *gu,
+                            const double *gv, int npb, const int32_t *pb, const double *gp,
+                            const nsfem_sim_options *opt, nsfem_simulation **out) {
+    return guard([&] {
+        NSFEM_REQUIRE(h && out && opt && xy && el, NSFEM_INVALID_ARGUMENT, "null argument");
+        NSFEM_REQUIRE((nvb == 0 || (vb && gu && gv)) && (npb == 0 || (pb && gp)),
+                      NSFEM_INVALID_ARGUMENT, "null boundary arrays");
+        std::lock_guard<std::mutex> lk(h->impl.mtx);
+        auto *w = new nsfem_simulation();
+        try {
+            w->impl = simulation_create(h->impl, V, P, xy, E, el, nvb, vb, gu, gv, npb, pb, gp,
+                                        *opt);
+        } catch (...) {
+            delete w;
+            throw;
+        }
+        *out = w;
+    });
+}
+
+int nsfem_simulation_destroy(nsfem_simulation *s) {
+    return guard([&] {
+        if (!s) return;
+        if (s->impl) {
+            (void)hipSetDevice(s->impl->dev->index);
+            (void)hipStreamSynchronize(s->impl->dev->stream);
+        }
+        simulation_destroy(s->impl);
+        delete s;
+    });
+}
+
+int nsfem_simulation_step(nsfem_simulation *s, int steps, nsfem_step_stats *stats) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && steps >= 0, NSFEM_INVALID_ARGUMENT, "bad simulation call");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_step(*s->impl, steps, stats);
+    });
+}
+
+int nsfem_simulation_get_fields(nsfem_simulation *s, double *u, double *v, double *p) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl, NSFEM_INVALID_ARGUMENT, "null simulation");
+        simulation_get(*s->impl, u, v, p);
+    });
+}
+
+int nsfem_simulation_set_fields(nsfem_simulation *s, const double *u, const double *v,
+                                const double *p) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl, NSFEM_INVALID_ARGUMENT, "null simulation");
+        simulation_set(*s->impl, u, v, p);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,343 @@
+"""P2-P1 triangular meshes in the reference's layout.
+
+The reference reads a JSON mesh (cpp/grid.cpp:9-77) whose element rows are
+``[v0, v1, v2, m12, m02, m01]`` -- three CCW vertices followed by the
+midpoints opposite each vertex, the order P2ShapeEval expects
+(gpu/advection.cu:5-12, include/assembly.h:201-235).  Vertices come first
+and are the pressure nodes ``0..P-1`` (include/grid.h:118-121).  Node
+coordinates are ``float`` (gpu_cpu_shared/misc_common.cuh:8), so every mesh
+here stores ``float32`` coordinates; FP64 arithmetic on the mesh starts from
+those values.
+
+Dirichlet data are carried as explicit per-node values (the reference
+evaluates expression strings per node, include/assembly.h:503-524); lists are
+kept in boundary order, which is the order the reference patches right-hand
+sides in.  :func:`load_json` reads the reference's own file format, including
+its boundary-condition expressions (cpp/expression.cpp grammar: ``+ - * / ^``,
+unary minus, ``sin``, ``cos``, ``pi``, variables ``x`` and ``y``).
+"""
+from __future__ import annotations
+
+import json
+import math
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+__all__ = ["Mesh", "cavity", "channel", "load_json", "evaluate_expression"]
+
+
+@dataclass
+class Mesh:
+    nodes: np.ndarray              # float32 [V, 2]
+    elements: np.ndarray           # int32 [E, 6]
+    pressure_count: int            # P: nodes 0..P-1 are the P1 vertices
+    vel_bc_nodes: np.ndarray       # int32 [B]
+    vel_bc_u: np.ndarray           # float64 [B]
+    vel_bc_v: np.ndarray           # float64 [B]
+    p_bc_nodes: np.ndarray         # int32 [Bp]
+    p_bc_values: np.ndarray        # float64 [Bp]
+    name: str = "mesh"
+    info: dict = field(default_factory=dict)
+
+    @property
+    def node_count(self) -> int:
+        return int(self.nodes.shape[0])
+
+    @property
+    def element_count(self) -> int:
+        return int(self.elements.shape[0])
+
+    def xy(self) -> np.ndarray:
+        return np.ascontiguousarray(self.nodes, dtype=np.float32).reshape(-1)
+
+    def elems(self) -> np.ndarray:
+        return np.ascontiguousarray(self.elements, dtype=np.int32).reshape(-1)
+
+    def check(self) -> None:
+        """Layout invariants the kernels rely on (raise on violation)."""
+        e = self.elements
+        V, P = self.node_count, self.pressure_count
+        if e.ndim != 2 or e.shape[1] != 6:
+            raise ValueError("elements must be [E, 6]")
+        if e.min() < 0 or e.max() >= V:
+            raise ValueError("element node index out of range")
+        if e[:, :3].max() >= P or e[:, 3:].min() < P:
+            raise ValueError("vertices must be nodes 0..P-1, midpoints P..V-1")
+        xy = self.nodes.astype(np.float64)
+        a, b, c = xy[e[:, 0]], xy[e[:, 1]], xy[e[:, 2]]
+        det = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])
+        if not np.all(det > 0):
+            raise ValueError("all triangles must be counter-clockwise")
+        if len(self.vel_bc_u) != len(self.vel_bc_nodes) or len(self.vel_bc_v) != len(self.vel_bc_nodes):
+            raise ValueError("velocity Dirichlet values must match the node list")
+        if len(self.p_bc_values) != len(self.p_bc_nodes):
+            raise ValueError("pressure Dirichlet values must match the node list")
+
+
+def _p2_from_p1(xy_v: np.ndarray, tris: np.ndarray):
+    """Add one midpoint node per unique edge.  Midpoints are numbered after
+    the vertices in ascending (min vertex, max vertex) order, which keeps a
+    row-major vertex numbering local for the midpoints too."""
+    P = xy_v.shape[0]
+    t = tris.astype(np.int64)
+    # edges opposite v0, v1, v2: (v1,v2), (v0,v2), (v0,v1)
+    pairs = np.stack([t[:, [1, 2]], t[:, [0, 2]], t[:, [0, 1]]], axis=1)  # [E,3,2]
+    lo = pairs.min(axis=2)
+    hi = pairs.max(axis=2)
+    key = lo * P + hi
+    uniq, inv, counts = np.unique(key.reshape(-1), return_inverse=True, return_counts=True)
+    mid_idx = (P + inv).reshape(-1, 3)
+    ulo = uniq // P
+    uhi = uniq % P
+    mids = ((xy_v[ulo].astype(np.float64) + xy_v[uhi].astype(np.float64)) * 0.5).astype(np.float32)
+    nodes = np.concatenate([xy_v.astype(np.float32), mids], axis=0)
+    elements = np.concatenate([tris.astype(np.int64), mid_idx], axis=1).astype(np.int32)
+    edges = np.stack([ulo, uhi], axis=1)
+    return nodes, elements, edges, counts
+
+
+def cavity(n: int, lid_speed: float = 1.0) -> Mesh:
+    """Lid-driven cavity on the unit square: ``n x n`` squares, each split
+    along the (i,j)-(i+1,j+1) diagonal into two CCW triangles.  Lid
+    ``u = (lid_speed, 0)`` on ``y = 1`` excluding the two top corners, no-slip
+    on every other boundary node; the pure-Neumann pressure is pinned with
+    ``p = 0`` at node 0 (the reference's pDirichlet mechanism,
+    include/assembly.h:583-603)."""
+    if n < 1:
+        raise ValueError("n must be >= 1")
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="xy")
+    xy_v = np.stack([ii.reshape(-1) / n, jj.reshape(-1) / n], axis=1).astype(np.float32)
+    sq_i, sq_j = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+    sq_i = sq_i.reshape(-1)
+    sq_j = sq_j.reshape(-1)
+    v00 = sq_j * (n + 1) + sq_i
+    v10 = v00 + 1
+    v01 = v00 + (n + 1)
+    v11 = v01 + 1
+    t1 = np.stack([v00, v10, v11], axis=1)
+    t2 = np.stack([v00, v11, v01], axis=1)
+    tris = np.stack([t1, t2], axis=1).reshape(-1, 3)
+    nodes, elements, edges, counts = _p2_from_p1(xy_v, tris)
+    P = xy_v.shape[0]
+    vi, vj = ii.reshape(-1), jj.reshape(-1)
+    on_b = (vi == 0) | (vi == n) | (vj == 0) | (vj == n)
+    top_v = (vj == n) & (vi > 0) & (vi < n)
+    bedge = counts == 1
+    e_top = bedge & (vj[edges[:, 0]] == n) & (vj[edges[:, 1]] == n)
+    lid = np.concatenate([np.nonzero(top_v)[0], P + np.nonzero(e_top)[0]])
+    walls = np.concatenate([np.nonzero(on_b & ~top_v)[0], P + np.nonzero(bedge & ~e_top)[0]])
+    lid = np.sort(lid)
+    walls = np.sort(walls)
+    vb = np.concatenate([lid, walls]).astype(np.int32)
+    gu = np.concatenate([np.full(len(lid), float(lid_speed)), np.zeros(len(walls))])
+    gv = np.zeros(len(vb))
+    m = Mesh(nodes=nodes, elements=elements, pressure_count=P,
+             vel_bc_nodes=vb, vel_bc_u=gu, vel_bc_v=gv,
+             p_bc_nodes=np.array([0], dtype=np.int32), p_bc_values=np.zeros(1),
+             name=f"cavity{n}", info={"n": n, "lid_speed": float(lid_speed)})
+    return m
+
+
+def _inflow(y: np.ndarray, umax: float, height: float) -> np.ndarray:
+    # 4*Um*y*(H-y)/H^2, the inflow profile of assets/mesh_small.json
+    return 4.0 * umax * y * (height - y) / (height * height)
+
+
+def channel(nx: int = 512, ny: int = 128, seed: int = 2, jitter: float = 0.2,
+            length: float = 2.2, height: float = 0.41,
+            disk: tuple = (0.2, 0.2, 0.05), umax: float = 1.5) -> Mesh:
+    """Channel past a cylinder: ``[0,length] x [0,height]`` on an ``nx x ny``
+    grid, interior vertices jittered uniformly by ``+-jitter*h`` per axis
+    (seeded), triangles whose centroid lies inside the disk removed, orphan
+    vertices dropped.  Parabolic inflow on ``x = 0``, no-slip on the walls and
+    the hole, do-nothing outlet with ``p = 0`` on ``x = length``."""
+    hx, hy = length / nx, height / ny
+    rng = np.random.default_rng(seed)
+    ii, jj = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="xy")
+    ii = ii.reshape(-1)
+    jj = jj.reshape(-1)
+    x = ii * hx
+    y = jj * hy
+    x[ii == nx] = length
+    y[jj == ny] = height
+    dx = rng.uniform(-jitter * hx, jitter * hx, size=x.shape)
+    dy = rng.uniform(-jitter * hy, jitter * hy, size=y.shape)
+    interior = (ii > 0) & (ii < nx) & (jj > 0) & (jj < ny)
+    x = np.where(interior, x + dx, x)
+    y = np.where(interior, y + dy, y)
+    xy_all = np.stack([x, y], axis=1).astype(np.float32)
+    sq_i, sq_j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    sq_i = sq_i.reshape(-1)
+    sq_j = sq_j.reshape(-1)
+    v00 = sq_j * (nx + 1) + sq_i
+    v10 = v00 + 1
+    v01 = v00 + (nx + 1)
+    v11 = v01 + 1
+    tris = np.stack([np.stack([v00, v10, v11], 1), np.stack([v00, v11, v01], 1)], 1).reshape(-1, 3)
+    cxy = xy_all[tris].astype(np.float64).mean(axis=1)
+    cx, cy, r = disk
+    keep = (cxy[:, 0] - cx) ** 2 + (cxy[:, 1] - cy) ** 2 >= r * r
+    tris = tris[keep]
+    used = np.zeros(xy_all.shape[0], dtype=bool)
+    used[tris.reshape(-1)] = True
+    remap = -np.ones(xy_all.shape[0], dtype=np.int64)
+    remap[used] = np.arange(int(used.sum()))
+    tris = remap[tris]
+    xy_v = xy_all[used]
+    gi, gj = ii[used], jj[used]
+    nodes, elements, edges, counts = _p2_from_p1(xy_v, tris)
+    P = xy_v.shape[0]
+    left_v, right_v = gi == 0, gi == nx
+    bot_v, top_v = gj == 0, gj == ny
+    bedge = counts == 1
+    e0, e1 = edges[:, 0], edges[:, 1]
+    e_left = bedge & left_v[e0] & left_v[e1]
+    e_right = bedge & right_v[e0] & right_v[e1]
+    e_bot = bedge & bot_v[e0] & bot_v[e1]
+    e_top = bedge & top_v[e0] & top_v[e1]
+    e_hole = bedge & ~(e_left | e_right | e_bot | e_top)
+    hole_v = np.zeros(P, dtype=bool)
+    hole_v[e0[e_hole]] = True
+    hole_v[e1[e_hole]] = True
+    inflow = np.concatenate([np.nonzero(left_v)[0], P + np.nonzero(e_left)[0]])
+    walls = np.concatenate([np.nonzero((bot_v | top_v) & ~left_v)[0],
+                            P + np.nonzero(e_bot | e_top)[0]])
+    hole = np.concatenate([np.nonzero(hole_v)[0], P + np.nonzero(e_hole)[0]])
+    inflow, walls, hole = np.sort(inflow), np.sort(walls), np.sort(hole)
+    vb = np.concatenate([inflow, walls, hole]).astype(np.int32)
+    yin = nodes[inflow, 1].astype(np.float64)
+    gu = np.concatenate([_inflow(yin, umax, height), np.zeros(len(walls) + len(hole))])
+    gv = np.zeros(len(vb))
+    pb = np.nonzero(right_v)[0].astype(np.int32)
+    m = Mesh(nodes=nodes, elements=elements, pressure_count=P,
+             vel_bc_nodes=vb, vel_bc_u=gu, vel_bc_v=gv,
+             p_bc_nodes=pb, p_bc_values=np.zeros(len(pb)),
+             name=f"channel{nx}x{ny}s{seed}",
+             info={"nx": nx, "ny": ny, "seed": seed, "umax": umax,
+                   "removed_triangles": int((~keep).sum())})
+    return m
+
+
+# ---------------------------------------------------------------------------
+# Reference JSON mesh format (cpp/grid.cpp:9-77) and its BC expressions.
+# ---------------------------------------------------------------------------
+_TOKEN = re.compile(r"\s*(?:(\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)|(sin|cos|pi)|([xy])|(.))")
+_BIN_PREC = {"+": 0, "-": 0, "*": 1, "/": 1, "^": 3}
+
+
+def _tokenize(s: str):
+    out = []
+    pos = 0
+    s = s.strip()
+    while pos < len(s):
+        m = _TOKEN.match(s, pos)
+        if not m or m.end() == pos:
+            raise ValueError(f"cannot parse expression {s!r} at {pos}")
+        num, fn, var, op = m.groups()
+        if num is not None:
+            out.append(("num", float(num)))
+        elif fn is not None:
+            out.append(("num", math.pi) if fn == "pi" else ("fn", fn))
+        elif var is not None:
+            out.append(("var", var))
+        elif op is not None and op.strip():
+            if op not in "+-*/^()":
+                raise ValueError(f"unexpected symbol {op!r} in {s!r}")
+            out.append(("op", op))
+        pos = m.end()
+    return out
+
+
+def evaluate_expression(expr: str, x, y):
+    """Evaluate a reference BC expression (operator precedence of
+    cpp/expression.cpp:195-213: + - < * / < unary minus < ^ < sin cos; all
+    binary operators left-associative) in float64, vectorised over x, y."""
+    toks = _tokenize(expr)
+    pos = 0
+
+    def peek():
+        return toks[pos] if pos < len(toks) else (None, None)
+
+    def parse(min_prec):
+        nonlocal pos
+        kind, val = peek()
+        if kind == "op" and val == "-":
+            pos += 1
+            lhs = -parse(2)
+        elif kind == "op" and val == "(":
+            pos += 1
+            lhs = parse(0)
+            if peek() != ("op", ")"):
+                raise ValueError(f"missing ')' in {expr!r}")
+            pos += 1
+        elif kind == "fn":
+            pos += 1
+            arg = parse(10)
+            lhs = np.sin(arg) if val == "sin" else np.cos(arg)
+        elif kind == "num":
+            pos += 1
+            lhs = np.asarray(val, dtype=np.float64)
+        elif kind == "var":
+            pos += 1
+            lhs = np.asarray(x if val == "x" else y, dtype=np.float64)
+        else:
+            raise ValueError(f"unexpected token in {expr!r}")
+        while True:
+            kind, val = peek()
+            if kind != "op" or val not in _BIN_PREC or _BIN_PREC[val] < min_prec:
+                break
+            prec = _BIN_PREC[val]
+            pos += 1
+            rhs = parse(prec + 1)
+            if val == "+":
+                lhs = lhs + rhs
+            elif val == "-":
+                lhs = lhs - rhs
+            elif val == "*":
+                lhs = lhs * rhs
+            elif val == "/":
+                lhs = lhs / rhs
+            else:
+                lhs = np.power(lhs, rhs)
+        return lhs
+
+    res = parse(0)
+    if pos != len(toks):
+        raise ValueError(f"trailing tokens in {expr!r}")
+    return np.broadcast_to(res, np.broadcast(np.asarray(x), np.asarray(y)).shape).astype(np.float64)
+
+
+def load_json(path_or_dict) -> Mesh:
+    """Read the reference's mesh JSON (keys ``elements, nodes, elementsCount,
+    velocityNodesCount, pressureNodesCount, elementSize, uDirichlet,
+    pDirichlet``; cpp/grid.cpp:19-72)."""
+    d = path_or_dict
+    if not isinstance(d, dict):
+        with open(path_or_dict) as f:
+            d = json.load(f)
+    if int(d["elementSize"]) != 6:
+        raise ValueError("only P2-P1 elements (elementSize 6) are supported")
+    nodes = np.asarray(d["nodes"], dtype=np.float64).astype(np.float32)
+    elements = np.asarray(d["elements"], dtype=np.int32).reshape(-1, 6)
+    if nodes.shape[0] != int(d["velocityNodesCount"]) or elements.shape[0] != int(d["elementsCount"]):
+        raise ValueError("node/element count mismatch")
+    xs = nodes[:, 0].astype(np.float64)
+    ys = nodes[:, 1].astype(np.float64)
+    vb, gu, gv = [], [], []
+    for bc in d["uDirichlet"]:
+        idx = np.asarray(bc["nodes"], dtype=np.int32)
+        vb.append(idx)
+        gu.append(evaluate_expression(str(bc["u"]), xs[idx], ys[idx]))
+        gv.append(evaluate_expression(str(bc["v"]), xs[idx], ys[idx]))
+    pb, gp = [], []
+    for bc in d["pDirichlet"]:
+        idx = np.asarray(bc["nodes"], dtype=np.int32)
+        pb.append(idx)
+        gp.append(evaluate_expression(str(bc["p"]), xs[idx], ys[idx]))
+    cat = lambda a, dt: np.concatenate(a).astype(dt) if a else np.zeros(0, dt)
+    return Mesh(nodes=nodes, elements=elements, pressure_count=int(d["pressureNodesCount"]),
+                vel_bc_nodes=cat(vb, np.int32), vel_bc_u=cat(gu, np.float64),
+                vel_bc_v=cat(gv, np.float64), p_bc_nodes=cat(pb, np.int32),
+                p_bc_values=cat(gp, np.float64), name="json")

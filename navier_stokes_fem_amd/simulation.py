@@ -1,0 +1,71 @@
+"""Device-resident semi-Lagrangian Navier-Stokes stepping
+(NavierStokesAssembly::semiLagrangianSolve, include/assembly.h:1477-1557).
+
+``Simulation(mesh, viscosity, dt)`` assembles the matrices on the host,
+builds the KD tree, uploads everything and keeps ``u, v, p`` in HBM; each
+:meth:`Simulation.step` runs advect + 5 CG solves per step on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .device import GPUSimulationDevice
+
+__all__ = ["Simulation"]
+
+
+class Simulation:
+    def __init__(self, mesh, viscosity: float, dt: float, device: GPUSimulationDevice = None,
+                 rtol: float = 1e-8, precision: str = "fp64", pair_velocity: bool = True,
+                 max_iterations: int = -1, threads: int = 0):
+        self.mesh = mesh
+        self.device = device if device is not None else GPUSimulationDevice(0)
+        self.options = N.SimOptions(viscosity=float(viscosity), dt=float(dt), rtol=float(rtol),
+                                    max_iterations=int(max_iterations),
+                                    precision={"fp64": N.PRECISION_FP64,
+                                               "fp32_refined": N.PRECISION_FP32_REFINED}[precision],
+                                    pair_velocity=int(bool(pair_velocity)),
+                                    threads=int(threads))
+        xy, el = mesh.xy(), mesh.elems()
+        vb = np.ascontiguousarray(mesh.vel_bc_nodes, np.int32)
+        gu = np.ascontiguousarray(mesh.vel_bc_u, np.float64)
+        gv = np.ascontiguousarray(mesh.vel_bc_v, np.float64)
+        pb = np.ascontiguousarray(mesh.p_bc_nodes, np.int32)
+        gp = np.ascontiguousarray(mesh.p_bc_values, np.float64)
+        h = C.c_void_p()
+        N.check(N.lib().nsfem_simulation_create(
+            self.device.handle, mesh.node_count, mesh.pressure_count, N.ptr(xy),
+            mesh.element_count, N.ptr(el), len(vb), N.ptr(vb) if len(vb) else None,
+            N.ptr(gu) if len(vb) else None, N.ptr(gv) if len(vb) else None, len(pb),
+            N.ptr(pb) if len(pb) else None, N.ptr(gp) if len(pb) else None,
+            C.byref(self.options), C.byref(h)))
+        self.handle = h
+
+    def step(self, steps: int = 1) -> dict:
+        st = N.StepStats()
+        N.check(N.lib().nsfem_simulation_step(self.handle, int(steps), C.byref(st)))
+        return st.as_dict()
+
+    def fields(self):
+        V, P = self.mesh.node_count, self.mesh.pressure_count
+        u, v, p = np.empty(V), np.empty(V), np.empty(P)
+        N.check(N.lib().nsfem_simulation_get_fields(self.handle, N.ptr(u), N.ptr(v), N.ptr(p)))
+        return u, v, p
+
+    def set_fields(self, u=None, v=None, p=None):
+        a = [None if x is None else np.ascontiguousarray(x, np.float64) for x in (u, v, p)]
+        N.check(N.lib().nsfem_simulation_set_fields(self.handle, *[N.ptr(x) for x in a]))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            N.lib().nsfem_simulation_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

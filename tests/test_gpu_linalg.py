@@ -1,0 +1,210 @@
+"""GPU parity of the CSR products, BLAS-1 and CG against the CPU oracle.
+
+Bars: the CSR products add each row's products in row order exactly like the
+reference's row loop (and the oracle), so they are compared BIT-EXACTLY.
+Dot products are deterministic but tree-ordered: relative 1e-13.  CG: every
+solve meets its stated tolerance with the true residual recomputed in FP64,
+and the solution matches the oracle's FP64 CG within cond(A)*tol.
+"""
+import numpy as np
+import pytest
+
+from conftest import dense, random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def _buf(gpu, a):
+    return gpu.buffer(np.ascontiguousarray(a, dtype=np.float64))
+
+
+@pytest.mark.parametrize("n,m,density", [(1, 1, 1.0), (7, 5, 0.5), (300, 300, 0.02),
+                                         (5000, 4000, 0.003), (2000, 300, 0.3)])
+def test_spmv_bit_exact(gpu, oracle, n, m, density):
+    rng = np.random.default_rng(n * 31 + m)
+    rs, col, val = random_csr(n, m, density, rng)
+    gpu.upload_csr(7, rs, col, val, shape=(n, m))
+    x = rng.standard_normal(m)
+    lhs = rng.standard_normal(n)
+    bx, by, bl = _buf(gpu, x), _buf(gpu, np.zeros(n)), _buf(gpu, lhs)
+    gpu.sp_r_mult(7, bx, by)
+    y = by.download()
+    ref = oracle.spmv((rs, col, val), x)
+    assert np.array_equal(y, ref)
+    assert np.allclose(y, dense(rs, col, val, (n, m)) @ x, rtol=1e-12, atol=1e-12)
+    gpu.sp_r_mult_sub(7, bl, bx, bl)  # res aliases lhs (allowed by the reference)
+    assert np.array_equal(bl.download(), oracle.spmv_sub((rs, col, val), lhs, x))
+
+
+def test_spmv_long_row(gpu, oracle):
+    """A row longer than one LDS tile takes the block-reduction path."""
+    rng = np.random.default_rng(3)
+    n, m = 4, 6000
+    rs = np.array([0, 3, 3 + 5000, 5003 + 2, 5005 + 100], np.int32)
+    col = np.concatenate([np.sort(rng.choice(m, 3, replace=False)),
+                          np.sort(rng.choice(m, 5000, replace=False)),
+                          np.sort(rng.choice(m, 2, replace=False)),
+                          np.sort(rng.choice(m, 100, replace=False))]).astype(np.int32)
+    val = rng.standard_normal(len(col))
+    gpu.upload_csr(8, rs, col, val, shape=(n, m))
+    x = rng.standard_normal(m)
+    by = _buf(gpu, np.zeros(n))
+    gpu.sp_r_mult(8, _buf(gpu, x), by)
+    y = by.download()
+    ref = oracle.spmv((rs, col, val), x)
+    assert np.array_equal(y[[0, 2, 3]], ref[[0, 2, 3]])
+    assert abs(y[1] - ref[1]) <= 1e-12 * np.abs(val[3:5003] * x[col[3:5003]]).sum()
+
+
+def test_triplet_upload_sums_duplicates(gpu, oracle):
+    rng = np.random.default_rng(5)
+    n = 50
+    ti = rng.integers(0, n, 600).astype(np.int32)
+    tj = rng.integers(0, n, 600).astype(np.int32)
+    tv = rng.standard_normal(600)
+    gpu.upload_matrix(9, n, n, ti, tj, tv)
+    rs, col, val = oracle.triplet_to_csr(n, ti, tj, tv)
+    x = rng.standard_normal(n)
+    by = _buf(gpu, np.zeros(n))
+    gpu.sp_r_mult(9, _buf(gpu, x), by)
+    assert np.array_equal(by.download(), oracle.spmv((rs, col, val), x))
+    assert gpu.matrix_info(9) == (n, n, len(col))
+
+
+def test_blas1(gpu, oracle):
+    rng = np.random.default_rng(11)
+    n = 100_003
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    bx, by, bo = _buf(gpu, x), _buf(gpu, y), _buf(gpu, np.zeros(n))
+    gpu.saxpy(n, 0.75, bx, by)
+    assert np.allclose(by.download(), y + 0.75 * x, rtol=1e-15, atol=1e-15)
+    gpu.saxpby(n, 2.0, -0.5, bx, by, bo)
+    assert np.allclose(bo.download(), 2.0 * x - 0.5 * (y + 0.75 * x), rtol=1e-14, atol=1e-14)
+    res = _buf(gpu, np.array([1.5]))  # dotProduct ADDS to result[0]
+    gpu.dot_product(n, bx, bo, res)
+    d = res.download()[0]
+    ref = 1.5 + oracle.dot(x, bo.download())
+    assert abs(d - ref) <= 1e-13 * (1.5 + np.abs(x * bo.download()).sum())
+    res2 = _buf(gpu, np.array([1.5]))
+    gpu.dot_product(n, bx, bo, res2)
+    assert res2.download()[0] == d  # deterministic
+
+
+def _cavity_system(oracle, n=24, which="pressure_stiffness"):
+    from navier_stokes_fem_amd import mesh
+
+    m = mesh.cavity(n)
+    asm = oracle.Assembly(m, 0.01, 0.01)
+    return m, asm.csr[which]
+
+
+@pytest.mark.parametrize("which", ["pressure_stiffness", "diffusion", "velocity_mass"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+def test_cg_matches_oracle(gpu, oracle, which, precision):
+    m, A = _cavity_system(oracle, 24, which)
+    rs, col, val = A
+    n = len(rs) - 1
+    rng = np.random.default_rng(17)
+    b = rng.standard_normal(n)
+    x0 = rng.standard_normal(n) * 0.1
+    gpu.upload_csr(10, rs, col, val)
+    bx = _buf(gpu, x0)
+    st = gpu.cg_solve(10, _buf(gpu, b), bx, rtol=1e-10, precision=precision)[0]
+    x = bx.download()
+    ox, ost, ores = oracle.cg(A, b, x0, rtol=1e-10)
+    assert ost == 0 and st["status"] == 0
+    true_res = np.linalg.norm(oracle.spmv_sub(A, b, x))
+    assert true_res <= 1e-10 * np.linalg.norm(b) * (1 + 1e-9)
+    assert abs(st["final_residual"] - true_res) <= 1e-3 * true_res + 1e-300
+    Ad = dense(rs, col, val, (n, n))
+    xd = np.linalg.solve(Ad, b)
+    cond = np.linalg.cond(Ad)
+    assert np.linalg.norm(x - xd) <= 10 * cond * 1e-10 * np.linalg.norm(xd)
+    assert np.linalg.norm(x - ox) <= 10 * cond * 1e-10 * np.linalg.norm(ox)
+    if precision == "fp64":
+        assert abs(st["iterations"] - ores.iterations) <= 2
+    else:
+        assert st["refinements"] >= 1
+
+
+def test_cg_two_columns(gpu, oracle):
+    m, A = _cavity_system(oracle, 20, "diffusion")
+    rs, col, val = A
+    n = len(rs) - 1
+    rng = np.random.default_rng(23)
+    b = rng.standard_normal(2 * n)
+    b[n:] *= 3.0
+    gpu.upload_csr(11, rs, col, val)
+    for precision in ("fp64", "fp32_refined"):
+        bx = _buf(gpu, np.zeros(2 * n))
+        st = gpu.cg_solve(11, _buf(gpu, b), bx, columns=2, rtol=1e-10, precision=precision)
+        x = bx.download()
+        for c in range(2):
+            bc = b[c * n:(c + 1) * n]
+            xc = x[c * n:(c + 1) * n]
+            r = np.linalg.norm(oracle.spmv_sub(A, bc, xc))
+            assert st[c]["status"] == 0
+            assert r <= 1e-10 * np.linalg.norm(bc) * (1 + 1e-9)
+            if precision == "fp64":
+                single = _buf(gpu, np.zeros(n))
+                s1 = gpu.cg_solve(11, _buf(gpu, bc), single, rtol=1e-10)[0]
+                assert abs(s1["iterations"] - st[c]["iterations"]) <= 1
+
+
+def test_cg_reference_semantics(gpu, oracle):
+    """conjugateGradient: absolute eps, host arrays, x written on early exit,
+    max iterations reported with the last iterate (cpp/gpu_simulation_device.cpp:385-515)."""
+    from navier_stokes_fem_amd._native import NsfemError, MAX_ITERATIONS_REACHED
+
+    m, A = _cavity_system(oracle, 12, "pressure_stiffness")
+    rs, col, val = A
+    n = len(rs) - 1
+    gpu.upload_csr(12, rs, col, val)
+    rng = np.random.default_rng(29)
+    b = rng.standard_normal(n)
+    x = gpu.conjugate_gradient(12, b, np.zeros(n), -1, 1e-8)
+    assert np.linalg.norm(oracle.spmv_sub(A, b, x)) < 1e-8 * (1 + 1e-6)
+    # x0 already converged: x must be written (the reference returns early without it)
+    x2 = gpu.conjugate_gradient(12, b, x, -1, 1e-3)
+    assert np.array_equal(x2, x)
+    with pytest.raises(NsfemError) as ei:
+        gpu.conjugate_gradient(12, b, np.zeros(n), 3, 1e-12)
+    assert ei.value.code == MAX_ITERATIONS_REACHED
+    xo, _, _ = oracle.cg(A, b, np.zeros(n), max_iter=3, rtol=0.0, atol=1e-12)
+    assert np.allclose(ei.value.x, xo, rtol=1e-10, atol=1e-12)
+
+
+def test_cg_zero_rhs_and_breakdown(gpu):
+    from navier_stokes_fem_amd._native import NsfemError, BREAKDOWN
+
+    n = 50
+    rs = np.arange(n + 1, dtype=np.int32)
+    col = np.arange(n, dtype=np.int32)
+    gpu.upload_csr(13, rs, col, np.full(n, 2.0))
+    bx = _buf(gpu, np.zeros(n))
+    st = gpu.cg_solve(13, _buf(gpu, np.zeros(n)), bx)[0]
+    assert st["iterations"] == 0 and np.all(bx.download() == 0)
+    gpu.upload_csr(14, rs, col, -np.ones(n))  # negative definite
+    with pytest.raises(NsfemError) as ei:
+        gpu.cg_solve(14, _buf(gpu, np.ones(n)), _buf(gpu, np.zeros(n)))
+    assert ei.value.code == BREAKDOWN
+
+
+def test_cg_large_grid(gpu, oracle):
+    """A system big enough to run on many persistent blocks (grid barrier)."""
+    from navier_stokes_fem_amd import device as D
+    from navier_stokes_fem_amd import mesh
+
+    m = mesh.cavity(192)
+    a = D.assemble(m, 0.01, 0.01)
+    rs, col, val, _ = a["pressure_stiffness"]
+    n = len(rs) - 1
+    gpu.upload_csr(15, rs, col, val)
+    rng = np.random.default_rng(31)
+    b = rng.standard_normal(n)
+    for precision in ("fp64", "fp32_refined"):
+        bx = _buf(gpu, np.zeros(n))
+        st = gpu.cg_solve(15, _buf(gpu, b), bx, rtol=1e-8, precision=precision)[0]
+        x = bx.download()
+        r = np.linalg.norm(oracle.spmv_sub((rs, col, val), b, x))
+        assert st["status"] == 0 and r <= 1e-8 * np.linalg.norm(b) * (1 + 1e-9), (precision, st)

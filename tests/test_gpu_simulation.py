@@ -1,0 +1,61 @@
+"""Whole time steps on the GPU against the oracle's steps (same inputs).
+
+Bar: every CG solve reaches ||b - Ax|| <= 1e-8 ||b|| (true FP64 residual);
+fields after N steps agree with the oracle's FP64 fields to 1e-6 relative
+(the solves differ by at most cond * 1e-8 and advection is bit-exact given
+equal inputs).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_run(oracle, m, nu, dt, steps, rtol):
+    asm = oracle.Assembly(m, nu, dt)
+    tree = oracle.kd_build(m.xy(), m.elems())
+    u, v, p = oracle.initial_fields(m)
+    its = []
+    for _ in range(steps):
+        u, v, p, st = oracle.step(asm, tree, m, dt, u, v, p, rtol=rtol)
+        assert st["status"] == 0
+        its.append(st["iterations"])
+    return u, v, p, its
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("precision,pair", [("fp64", False), ("fp64", True),
+                                            ("fp32_refined", True)])
+def test_cavity32_steps(gpu, oracle, precision, pair):
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    m = mesh.cavity(32)
+    steps = 5
+    sim = Simulation(m, 0.01, 0.01, device=gpu, rtol=1e-8, precision=precision,
+                     pair_velocity=pair)
+    st = sim.step(steps)
+    u, v, p = sim.fields()
+    sim.close()
+    ou, ov, op, its = _oracle_run(oracle, m, 0.01, 0.01, steps, 1e-8)
+    assert max(st["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+    assert _rel(u, ou) < 1e-6 and _rel(v, ov) < 1e-6 and _rel(p, op) < 1e-6
+    if precision == "fp64":
+        tot = np.sum(its, axis=0)
+        assert abs(st["cg_iterations"][2] - tot[2]) <= 2 * steps
+
+
+def test_mesh_small_steps(gpu, oracle, mesh_small):
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    m = mesh_small
+    sim = Simulation(m, 0.001, 0.01, device=gpu, rtol=1e-8)
+    st = sim.step(10)
+    u, v, p = sim.fields()
+    sim.close()
+    ou, ov, op, _ = _oracle_run(oracle, m, 0.001, 0.01, 10, 1e-8)
+    assert max(st["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+    assert _rel(u, ou) < 1e-6 and _rel(v, ov) < 1e-6 and _rel(p, op) < 1e-6
