@@ -43,6 +43,8 @@ extern "C" {
 #define NSFEM_NOT_READY (-6)             /* slot / tree / buffer not uploaded */
 #define NSFEM_BREAKDOWN (-7)             /* (Ap, p) <= 0: matrix not SPD */
 #define NSFEM_BARRIER_TIMEOUT (-8)       /* persistent kernel barrier gave up */
+#define NSFEM_STAGNATION (-9)            /* FP32 iteration stopped reducing the FP64
+                                            residual (handled inside nsfem_cg_solve) */
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *nsfem_last_error(void);
@@ -275,6 +277,7 @@ typedef struct {
     double seconds;           /* device time of the steps (HIP events) */
     double advect_seconds;
     double cg_seconds[5];
+    int64_t cg_refinements[5];  /* FP64 reliable updates (refined precision) */
 } nsfem_step_stats;
 
 int nsfem_simulation_create(nsfem_device *dev, int node_count,
@@ -293,6 +296,12 @@ int nsfem_simulation_step(nsfem_simulation *sim, int steps, nsfem_step_stats *st
 int nsfem_simulation_get_fields(nsfem_simulation *sim, double *u, double *v, double *p);
 int nsfem_simulation_set_fields(nsfem_simulation *sim, const double *u,
                                 const double *v, const double *p);
+/* Change rtol / max_iterations / precision / pair_velocity of an existing
+ * simulation (viscosity, dt and threads are fixed at creation). */
+int nsfem_simulation_set_options(nsfem_simulation *sim, const nsfem_sim_options *options);
+/* Back to the initial state: zero fields with the velocity Dirichlet values
+ * imposed (include/assembly.h:1503-1507). */
+int nsfem_simulation_reset(nsfem_simulation *sim);
 
 #ifdef __cplusplus
 }

@@ -55,13 +55,15 @@ class SimOptions(C.Structure):
 class StepStats(C.Structure):
     _fields_ = [("steps", C.c_int32), ("cg_iterations", C.c_int64 * 5),
                 ("max_relative_residual", C.c_double * 5), ("seconds", C.c_double),
-                ("advect_seconds", C.c_double), ("cg_seconds", C.c_double * 5)]
+                ("advect_seconds", C.c_double), ("cg_seconds", C.c_double * 5),
+                ("cg_refinements", C.c_int64 * 5)]
 
     def as_dict(self):
         return {"steps": self.steps, "cg_iterations": list(self.cg_iterations),
                 "max_relative_residual": list(self.max_relative_residual),
                 "seconds": self.seconds, "advect_seconds": self.advect_seconds,
-                "cg_seconds": list(self.cg_seconds)}
+                "cg_seconds": list(self.cg_seconds),
+                "cg_refinements": list(self.cg_refinements)}
 
 
 _lib = None
@@ -122,6 +124,8 @@ _SIGNATURES = {
     "nsfem_simulation_step": ([vp, C.c_int, C.POINTER(StepStats)], C.c_int),
     "nsfem_simulation_get_fields": ([vp, vp, vp, vp], C.c_int),
     "nsfem_simulation_set_fields": ([vp, vp, vp, vp], C.c_int),
+    "nsfem_simulation_set_options": ([vp, C.POINTER(SimOptions)], C.c_int),
+    "nsfem_simulation_reset": ([vp], C.c_int),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -48,6 +48,13 @@ void upload_csr(Device &d, Matrix &m, int rows, int cols, int64_t nnz, const int
     if (nnz) {
         NSFEM_HIP(hipMemcpy(m.col.ptr, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
         NSFEM_HIP(hipMemcpy(m.val.ptr, val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    } else {
+        // kernels may read entry 0 of an empty matrix speculatively: keep it
+        // a valid column index with a zero value
+        const int32_t zc = 0;
+        const double zv = 0.0;
+        NSFEM_HIP(hipMemcpy(m.col.ptr, &zc, sizeof(int32_t), hipMemcpyHostToDevice));
+        NSFEM_HIP(hipMemcpy(m.val.ptr, &zv, sizeof(double), hipMemcpyHostToDevice));
     }
     std::vector<int32_t> hrs(rs, rs + rows + 1);
     build_tiles(m, hrs);
@@ -114,6 +121,8 @@ void upload_tree(Device &d, int32_t tn, const uint32_t *w, int64_t lc, const int
 
 static Buffer &buf(const nsfem_buffer *b) {
     NSFEM_REQUIRE(b != nullptr, NSFEM_INVALID_ARGUMENT, "null buffer");
+    NSFEM_REQUIRE(b->impl.dev != nullptr, NSFEM_INVALID_ARGUMENT,
+                  "buffer belongs to a destroyed device");
     return const_cast<nsfem_buffer *>(b)->impl;
 }
 
@@ -193,6 +202,16 @@ int nsfem_device_destroy(nsfem_device *h) {
         if (d.stream) (void)hipStreamSynchronize(d.stream);
         if (d.ev_start) (void)hipEventDestroy(d.ev_start);
         if (d.ev_stop) (void)hipEventDestroy(d.ev_stop);
+        {
+            std::lock_guard<std::mutex> lk(d.buf_mtx);
+            for (Buffer *b : d.buffers) {
+                b->mem.release();
+                b->dev = nullptr;
+            }
+            d.buffers.clear();
+            for (Simulation *sm : d.sims) simulation_detach(sm);
+            d.sims.clear();
+        }
         hipStream_t s = d.stream;
         delete h;  // frees every DeviceArray
         if (s) (void)hipStreamDestroy(s);
@@ -299,6 +318,7 @@ int nsfem_free_matrix(nsfem_device *h, int s) {
         m.val.release();
         m.val32.release();
         m.tiles.release();
+        m.wtiles.release();
         m.block_tiles.release();
         m.block_tiles_grid = 0;
         m.rows = m.cols = 0;
@@ -319,6 +339,10 @@ int nsfem_buffer_create(nsfem_device *h, int64_t bytes, nsfem_buffer **out) {
             delete b;
             throw;
         }
+        {
+            std::lock_guard<std::mutex> lk(d.buf_mtx);
+            d.buffers.insert(&b->impl);
+        }
         *out = b;
     });
 }
@@ -326,9 +350,12 @@ int nsfem_buffer_create(nsfem_device *h, int64_t bytes, nsfem_buffer **out) {
 int nsfem_buffer_destroy(nsfem_buffer *b) {
     return guard([&] {
         if (!b) return;
-        if (b->impl.dev) {
-            (void)hipSetDevice(b->impl.dev->index);
-            (void)hipStreamSynchronize(b->impl.dev->stream);
+        if (b->impl.dev) {  // still attached: its device is alive
+            Device &d = *b->impl.dev;
+            (void)hipSetDevice(d.index);
+            (void)hipStreamSynchronize(d.stream);
+            std::lock_guard<std::mutex> lk(d.buf_mtx);
+            d.buffers.erase(&b->impl);
         }
         delete b;
     });

@@ -17,6 +17,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <type_traits>
@@ -85,49 +87,70 @@ __device__ __forceinline__ float fma_t<float>(float a, float b, float c) {
 }  // namespace
 
 // ------------------------------------------------------------ tile tables
-void build_tiles(Matrix &m, const std::vector<int32_t> &rs) {
+namespace {
+// Greedy cut of the rows into runs of <= max_rows rows and <= max_nnz
+// nonzeros; a single longer row forms a run of its own.
+std::vector<int32_t> cut_rows(const std::vector<int32_t> &rs, int rows, int max_rows,
+                              int max_nnz) {
     std::vector<int32_t> t;
-    t.reserve((size_t)m.rows / 64 + 2);
+    t.reserve((size_t)rows / std::max(1, max_rows / 2) + 2);
     int r = 0;
     t.push_back(0);
-    while (r < m.rows) {
+    while (r < rows) {
         int end = r;
         int64_t nz = 0;
-        while (end < m.rows && end - r < kTileRows) {
+        while (end < rows && end - r < max_rows) {
             const int64_t rn = rs[end + 1] - rs[end];
-            if (end > r && nz + rn > kTileNnz) break;
+            if (end > r && nz + rn > max_nnz) break;
             nz += rn;
             ++end;
-            if (nz > kTileNnz) break;  // a single long row forms its own tile
+            if (nz > max_nnz) break;
         }
         t.push_back(end);
         r = end;
     }
+    return t;
+}
+}  // namespace
+
+void build_tiles(Matrix &m, const std::vector<int32_t> &rs) {
+    std::vector<int32_t> t = cut_rows(rs, m.rows, kTileRows, kTileNnz);
     m.n_tiles = (int)t.size() - 1;
-    m.host_tile_nnz_prefix.assign(t.size(), 0);
-    for (int i = 0; i < m.n_tiles; ++i)
-        m.host_tile_nnz_prefix[i + 1] =
-            m.host_tile_nnz_prefix[i] + (rs[t[i + 1]] - rs[t[i]]) + (t[i + 1] - t[i]);
     m.host_tiles = t;
     m.tiles.resize(t.size());
     NSFEM_HIP(hipMemcpy(m.tiles.ptr, t.data(), t.size() * sizeof(int32_t),
                         hipMemcpyHostToDevice));
+    std::vector<int32_t> w = cut_rows(rs, m.rows, kWaveRows, kWaveNnz);
+    m.n_wtiles = (int)w.size() - 1;
+    m.host_wtile_prefix.assign(w.size(), 0);
+    for (int i = 0; i < m.n_wtiles; ++i)
+        m.host_wtile_prefix[i + 1] =
+            m.host_wtile_prefix[i] + (rs[w[i + 1]] - rs[w[i]]) + (w[i + 1] - w[i]);
+    m.wtiles.resize(w.size());
+    NSFEM_HIP(hipMemcpy(m.wtiles.ptr, w.data(), w.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+    std::vector<int4> meta((size_t)std::max(m.n_wtiles, 1));
+    for (int i = 0; i < m.n_wtiles; ++i)
+        meta[i] = make_int4(w[i], w[i + 1], rs[w[i]], rs[w[i + 1]] - rs[w[i]]);
+    m.wmeta.resize(meta.size());
+    NSFEM_HIP(hipMemcpy(m.wmeta.ptr, meta.data(), meta.size() * sizeof(int4),
+                        hipMemcpyHostToDevice));
     m.block_tiles_grid = 0;
 }
 
-// Persistent blocks own contiguous tile ranges balanced by (nnz + rows).
+// Persistent blocks own contiguous wave-tile ranges balanced by (nnz + rows).
 void ensure_block_tiles(Device &d, Matrix &m, int grid) {
     if (m.block_tiles_grid == grid && m.block_tiles.ptr) return;
     std::vector<int32_t> bt(grid + 1);
-    const int64_t total = m.host_tile_nnz_prefix[m.n_tiles];
+    const int64_t total = m.host_wtile_prefix[m.n_wtiles];
     int t = 0;
     for (int b = 0; b <= grid; ++b) {
         const int64_t target = total * b / grid;
-        while (t < m.n_tiles && m.host_tile_nnz_prefix[t] < target) ++t;
+        while (t < m.n_wtiles && m.host_wtile_prefix[t] < target) ++t;
         bt[b] = t;
     }
     bt[0] = 0;
-    bt[grid] = m.n_tiles;
+    bt[grid] = m.n_wtiles;
     m.block_tiles.resize(grid + 1);
     NSFEM_HIP(hipMemcpyAsync(m.block_tiles.ptr, bt.data(), bt.size() * sizeof(int32_t),
                              hipMemcpyHostToDevice, d.stream));
@@ -392,8 +415,36 @@ struct alignas(256) SyncState {
     double partial[kMaxRed][kMaxGridBlocks];
 };
 
+#ifdef NSFEM_CG_TRACE
+// Phase timestamps of the first kTraceIters iterations (100 MHz wall clock):
+// [0] block 0 iteration start, [1] latest phase A end over all blocks,
+// [2] block 0 after the delta reduction, [3] latest phase B end,
+// [4] block 0 after the gamma reduction.
+constexpr int kTraceIters = 4096;
+__device__ unsigned long long g_cg_trace[kTraceIters][5];
+#define CG_TRACE_STAMP(it, slot)                                                    \
+    do {                                                                            \
+        if (threadIdx.x == 0 && (it) < kTraceIters && L == 0)                       \
+            g_cg_trace[(it)][(slot)] = wall_clock64();                              \
+    } while (0)
+#define CG_TRACE_MAX(it, slot)                                                      \
+    do {                                                                            \
+        __syncthreads();                                                            \
+        if (threadIdx.x == 0 && (it) < kTraceIters)                                 \
+            atomicMax(&g_cg_trace[(it)][(slot)], (unsigned long long)wall_clock64()); \
+    } while (0)
+#else
+#define CG_TRACE_STAMP(it, slot) \
+    do {                         \
+    } while (0)
+#define CG_TRACE_MAX(it, slot) \
+    do {                       \
+    } while (0)
+#endif
+
 struct CgArgs {
     int n;
+    const int4 *meta;  // wave tile descriptors
     const int32_t *tiles;
     const int32_t *block_tiles;
     const int32_t *rs;
@@ -405,6 +456,7 @@ struct CgArgs {
     void *r, *p0, *p1, *q, *d;  // interleaved [n][NC]
     double rtol, atol, rdelta;
     int max_iter, max_refine;
+    int ru_gap;  // mixed: force a reliable update after this many iterations
     unsigned long long timeout_ticks;
     SyncState *sync;
     CgColumnResult *res;
@@ -431,6 +483,16 @@ __device__ __forceinline__ double ld_relaxed(const double *p) {
 
 // Grid-wide sum of K values with barrier semantics.  Returns false if the
 // barrier timed out or another block aborted (the caller must return).
+// Memory ordering (gfx950: one L2 per XCD, not coherent across XCDs; an
+// agent-scope release writes the L2 back, an agent-scope acquire invalidates
+// it).  Every wave first waits for its own global stores to complete
+// (s_waitcnt 0: the stores have reached its XCD's L2), the block
+// synchronises, and then ONE thread per block performs the agent-scope
+// release as part of its arrival (a single L2 write-back per block).  Waiting
+// threads poll with relaxed agent-scope loads, which read the coherence
+// point without touching the caches, and perform one acquire fence after the
+// wait (a single invalidate per block), before the block reads data other
+// blocks wrote.
 template <int K, int BS>
 __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *lds,
                          unsigned long long timeout) {
@@ -445,19 +507,19 @@ __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *
     const int ng = (nb + kGroup - 1) / kGroup;
     const int gsize = min(kGroup, nb - g * kGroup);
     ++epoch;
-    __threadfence();  // publish this phase's global writes (release, agent)
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k) S->partial[k][b] = v[k];
-        const unsigned prev = __hip_atomic_fetch_add(&S->arrive_group[g][0], 1u, __ATOMIC_ACQ_REL,
+        const unsigned prev = __hip_atomic_fetch_add(&S->arrive_group[g][0], 1u, __ATOMIC_RELEASE,
                                                      __HIP_MEMORY_SCOPE_AGENT);
         s_group_last = (prev == epoch * (unsigned)gsize - 1u) ? 1 : 0;
     }
     __syncthreads();
     if (s_group_last) {
-        __threadfence();
-        // group sum, members in block order (lane i of wave 0 holds member i)
+        // group sum, members in block order (lane i of wave 0 holds member i);
+        // the partials are read at the coherence point
         double gs[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -467,14 +529,15 @@ __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *
         }
         if (threadIdx.x == 0) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) S->gpartial[k][g] = gs[k];
-            const unsigned prev = __hip_atomic_fetch_add(&S->arrive_top, 1u, __ATOMIC_ACQ_REL,
+            for (int k = 0; k < K; ++k)
+                __hip_atomic_store(&S->gpartial[k][g], gs[k], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned prev = __hip_atomic_fetch_add(&S->arrive_top, 1u, __ATOMIC_RELEASE,
                                                          __HIP_MEMORY_SCOPE_AGENT);
             s_top_last = (prev == epoch * (unsigned)ng - 1u) ? 1 : 0;
         }
         __syncthreads();
         if (s_top_last) {
-            __threadfence();
             double ts[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
@@ -485,12 +548,14 @@ __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *
             block_sum<K, BS>(ts, lds);
             if (threadIdx.x == 0) {
 #pragma unroll
-                for (int k = 0; k < K; ++k) S->result[k] = ts[k];
+                for (int k = 0; k < K; ++k)
+                    __hip_atomic_store(&S->result[k], ts[k], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             }
-            __threadfence();
             __syncthreads();
             for (int i = threadIdx.x; i < ng; i += BS)
-                __hip_atomic_store(&S->release_group[i][0], epoch, __ATOMIC_RELEASE,
+                __hip_atomic_store(&S->release_group[i][0], epoch, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -498,7 +563,8 @@ __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *
     if (threadIdx.x == 0) {
         int ok = 1;
         const unsigned long long t0 = wall_clock64();
-        while (ld_acquire(&S->release_group[g][0]) < epoch) {
+        while (__hip_atomic_load(&S->release_group[g][0], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT) < epoch) {
             if (__hip_atomic_load(&S->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 ok = 0;
                 break;
@@ -510,14 +576,13 @@ __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *
             }
             __builtin_amdgcn_s_sleep(2);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         s_ok = ok;
     }
     __syncthreads();
     const bool ok = s_ok != 0;
-    __threadfence();  // acquire for every thread of the block
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = ld_relaxed(&S->result[k]);
-    __syncthreads();
     return ok;
 }
 
@@ -554,112 +619,305 @@ __device__ __forceinline__ void stv(T *p, int i, const VecN<NC, T> &o) {
     }
 }
 
-// FP64 residual phase over the block's tiles: r = b - A x (x column-major).
+// ---------------------------------------------------------------------
+// Wave tiles: the persistent CG partitions rows into tiles of at most 64
+// rows and kWaveNnz nonzeros; each wavefront stages its own tile through
+// its own LDS region (no block-wide synchronisation inside a phase), so the
+// 16 waves of a block overlap their memory latencies independently.
+constexpr int kWItems = kWaveNnz / 64;  // nonzeros per lane per tile
+
+struct WTile {
+    int r0, r1, k0, nz;
+};
+
+__device__ __forceinline__ WTile wave_tile(const int4 *meta, int t) {
+    const int4 m = meta[t];
+    WTile w;
+    w.r0 = __builtin_amdgcn_readfirstlane(m.x);
+    w.r1 = __builtin_amdgcn_readfirstlane(m.y);
+    w.k0 = __builtin_amdgcn_readfirstlane(m.z);
+    w.nz = __builtin_amdgcn_readfirstlane(m.w);
+    return w;
+}
+
+// FP64 residual over the block's wave tiles: r = b - A x (x column-major).
 // SETUP: also p_old = 0 (and d = 0 for the mixed solver) and b.b.
 // Accumulates per column: acc[c] = r.r (FP64), acc[NC+c] = b.b,
 // acc[2NC+c] = (FP32 r).(FP32 r).
 template <int NC, int BS, bool MIXED, bool SETUP>
-__device__ void residual_phase(const CgArgs &A, int T0, int T1, double *prod,
+__device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave,
                                double (&acc)[3 * NC]) {
     using VT = typename std::conditional<MIXED, float, double>::type;
-    constexpr int Q = BS / kBlock;
-    const int quad = threadIdx.x / kBlock, qt = threadIdx.x % kBlock;
-    double *P = prod + quad * kTileNnz * NC;
+    constexpr int NW = BS / 64;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     VT *r = (VT *)A.r;
     VT *pold = (VT *)A.p0;
     VT *d = (VT *)A.d;
-    for (int base = T0; base < T1; base += Q) {
-        const int t = base + quad;
-        const bool act = t < T1;
-        int r0 = 0, r1 = 0, k0 = 0, nz = 0;
-        if (act) {
-            r0 = A.tiles[t];
-            r1 = A.tiles[t + 1];
-            k0 = A.rs[r0];
-            nz = A.rs[r1] - k0;
-        }
-        const bool small = nz <= kTileNnz;
-        if (act && small) {
-            int cj[kItems];
-            double vj[kItems];
+    for (int t = T0 + wave; t < T1; t += NW) {
+        const WTile w = wave_tile(A.meta, t);
+        const int row = w.r0 + lane;
+        const bool has_row = row < w.r1;
+        const int rowc = has_row ? row : w.r0;
+        double s[NC];
 #pragma unroll
-            for (int i = 0; i < kItems; ++i) {
-                const int k = qt + i * kBlock;
-                if (k < nz) {
-                    cj[i] = A.col[k0 + k];
-                    vj[i] = A.val64[k0 + k];
+        for (int c = 0; c < NC; ++c) s[c] = 0.0;
+        double bb[NC];
+        int ra = A.rs[rowc] - w.k0;
+        int rb = A.rs[rowc + 1] - w.k0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) bb[c] = A.b[(size_t)c * A.n + rowc];
+        if (!has_row) rb = ra;
+        if (w.nz <= kWaveNnz) {
+            // Branch-free staging: out-of-tile lanes read index 0 of the
+            // column/value arrays (always allocated) and gather x[0] (the
+            // matrix has >= 1 column); their products are zeroed and never
+            // summed.  All loads of a batch issue back to back.
+            constexpr int IB = (NC == 2 || MIXED) ? 4 : kWItems;  // register budget
+#pragma unroll
+            for (int i0 = 0; i0 < kWItems; i0 += IB) {
+                int cj[IB];
+                double vj[IB];
+#pragma unroll
+                for (int i = 0; i < IB; ++i) {
+                    const int k = lane + 64 * (i0 + i);
+                    const bool ok = k < w.nz;
+                    const int idx = ok ? w.k0 + k : 0;
+                    const int cc = A.col[idx];
+                    const double vv = A.val64[idx];
+                    cj[i] = ok ? cc : 0;
+                    vj[i] = ok ? vv : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < IB; ++i) {
+                    const int k = lane + 64 * (i0 + i);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+                        lds_wave[k * NC + c] = vj[i] * A.x[(size_t)c * A.n + cj[i]];
                 }
             }
+            __syncwarp();
+            if (has_row) {
+                for (int k = ra; k < rb; ++k) {
 #pragma unroll
-            for (int i = 0; i < kItems; ++i) {
-                const int k = qt + i * kBlock;
-                if (k < nz) {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) P[k * NC + c] = vj[i] * A.x[(size_t)c * A.n + cj[i]];
+                    for (int c = 0; c < NC; ++c) s[c] += lds_wave[k * NC + c];
                 }
             }
-        }
-        __syncthreads();
-        if (act) {
-            const int row = r0 + qt;
-            const bool mine = small ? row < r1 : qt == 0;
-            double s[NC];
+            __syncwarp();
+        } else {  // a single row longer than a tile: wave-wide sum
 #pragma unroll
-            for (int c = 0; c < NC; ++c) s[c] = 0.0;
-            if (small && row < r1) {
-                const int a = A.rs[row] - k0, e = A.rs[row + 1] - k0;
-                for (int k = a; k < e; ++k) {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) s[c] += P[k * NC + c];
-                }
-            } else if (!small && qt == 0) {  // long row: serial in row order
-                for (int k = 0; k < nz; ++k) {
-                    const double vv = A.val64[k0 + k];
-                    const int j = A.col[k0 + k];
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) s[c] += vv * A.x[(size_t)c * A.n + j];
-                }
-            }
-            if (mine) {
-                const int rw = small ? row : r0;
-                VecN<NC, VT> rv, zero;
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    const double bb = A.b[(size_t)c * A.n + rw];
-                    const double rr = bb - s[c];
-                    acc[c] += rr * rr;
-                    if (SETUP) acc[NC + c] += bb * bb;
-                    rv.v[c] = (VT)rr;
-                    if (MIXED) acc[2 * NC + c] += (double)rv.v[c] * (double)rv.v[c];
-                    zero.v[c] = (VT)0;
-                }
-                stv<NC, VT>(r, rw, rv);
-                if (SETUP) {
-                    stv<NC, VT>(pold, rw, zero);
-                    if (MIXED) stv<NC, VT>(d, rw, zero);
-                }
+            for (int c = 0; c < NC; ++c) {
+                double part = 0.0;
+                for (int k = lane; k < w.nz; k += 64)
+                    part += A.val64[w.k0 + k] * A.x[(size_t)c * A.n + A.col[w.k0 + k]];
+                s[c] = wave_sum(part);
             }
         }
-        __syncthreads();
+        if (has_row) {
+            VecN<NC, VT> rv, zero;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const double rr = bb[c] - s[c];
+                acc[c] += rr * rr;
+                if (SETUP) acc[NC + c] += bb[c] * bb[c];
+                rv.v[c] = (VT)rr;
+                if (MIXED) acc[2 * NC + c] += (double)rv.v[c] * (double)rv.v[c];
+                zero.v[c] = (VT)0;
+            }
+            stv<NC, VT>(r, row, rv);
+            if (SETUP) {
+                stv<NC, VT>(pold, row, zero);
+                if (MIXED) stv<NC, VT>(d, row, zero);
+            }
+        }
     }
 }
 
+// Phase A over the block's wave tiles:
+//   p_new = r + beta p_old (owner rows), q = A p_new (gathering r, p_old),
+//   deferred solution update with pend * p_old, delta += (p_new, q).
 template <int NC, int BS, bool MIXED>
-__global__ __launch_bounds__(BS) void k_cg(const CgArgs A) {
+__device__ void phase_a(const CgArgs &A, int T0, int T1,
+                        typename std::conditional<MIXED, float, double>::type *lds_wave,
+                        const typename std::conditional<MIXED, float, double>::type *pold,
+                        typename std::conditional<MIXED, float, double>::type *pnew,
+                        const typename std::conditional<MIXED, float, double>::type (&beta)[NC],
+                        const typename std::conditional<MIXED, float, double>::type (&pend)[NC],
+                        bool has_pend, double (&delta)[NC]) {
     using VT = typename std::conditional<MIXED, float, double>::type;
     using MT = VT;
-    constexpr int Q = BS / kBlock;
-    __shared__ double prod_raw[Q * kTileNnz * NC];
-    __shared__ double lds[kMaxRed * (BS / 64)];
+    constexpr int NW = BS / 64;
+    const MT *val = MIXED ? (const MT *)A.val32 : (const MT *)A.val64;
+    const VT *r = (const VT *)A.r;
+    VT *q = (VT *)A.q;
+    VT *dv = (VT *)A.d;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int t = T0 + wave;
+    if (t >= T1) return;
+    // Software pipeline: the column indices of the next tile load while the
+    // current tile's values and gathers are in flight.  Lanes beyond a
+    // tile's nonzeros read entry 0 of the column/value arrays (a valid
+    // column: upload makes col[0] = 0 for an empty matrix); their products
+    // land in LDS slots that no row sum reads, so no loaded value is masked
+    // and no load has to be waited for before the products are staged.
+    WTile w = wave_tile(A.meta, t);
+    int cn[kWItems];
+#pragma unroll
+    for (int i = 0; i < kWItems; ++i) {
+        const int k = lane + 64 * i;
+        cn[i] = A.col[k < w.nz ? w.k0 + k : 0];
+    }
+    for (; t < T1; t += NW) {
+        const WTile cur = w;
+        int cj[kWItems];
+#pragma unroll
+        for (int i = 0; i < kWItems; ++i) cj[i] = cn[i];
+        // 1. next tile: descriptor and columns
+        const int tn = t + NW;
+        if (tn < T1) {
+            w = wave_tile(A.meta, tn);
+#pragma unroll
+            for (int i = 0; i < kWItems; ++i) {
+                const int k = lane + 64 * i;
+                cn[i] = A.col[k < w.nz ? w.k0 + k : 0];
+            }
+        }
+        // 2. owner rows (clamped so the loads stay unconditional)
+        const int row = cur.r0 + lane;
+        const bool has_row = row < cur.r1;
+        const int rowc = has_row ? row : cur.r0;
+        const int rsa = A.rs[rowc];
+        const int rsb = A.rs[rowc + 1];
+        const VecN<NC, VT> rv = ldv<NC, VT>(r, rowc);
+        const VecN<NC, VT> pv = ldv<NC, VT>(pold, rowc);
+        VecN<NC, VT> dd;
+        double xo[NC];
+        if (has_pend) {
+            if (MIXED) {
+                dd = ldv<NC, VT>(dv, rowc);
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) xo[c] = A.x[(size_t)c * A.n + rowc];
+            }
+        }
+        VT s[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s[c] = (VT)0;
+        if (cur.nz <= kWaveNnz) {
+            // 3. values and gathers, then one wait for the whole tile
+            constexpr int IB = (NC == 2) ? 4 : kWItems;  // register budget
+#pragma unroll
+            for (int i0 = 0; i0 < kWItems; i0 += IB) {
+                MT vj[IB];
+                VecN<NC, VT> rj[IB], pj[IB];
+#pragma unroll
+                for (int i = 0; i < IB; ++i) {
+                    const int k = lane + 64 * (i0 + i);
+                    vj[i] = val[k < cur.nz ? cur.k0 + k : 0];
+                    rj[i] = ldv<NC, VT>(r, cj[i0 + i]);
+                    pj[i] = ldv<NC, VT>(pold, cj[i0 + i]);
+                }
+#pragma unroll
+                for (int i = 0; i < IB; ++i) {
+                    const int k = lane + 64 * (i0 + i);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+                        lds_wave[k * NC + c] = vj[i] * fma_t<VT>(beta[c], pj[i].v[c], rj[i].v[c]);
+                }
+            }
+            __syncwarp();
+            const int ra = rsa - cur.k0;
+            const int rb = has_row ? rsb - cur.k0 : ra;
+            for (int k = ra; k < rb; ++k) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s[c] += lds_wave[k * NC + c];
+            }
+            __syncwarp();
+        } else {
+            // a single row longer than a tile: wave-wide sum
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                VT part = (VT)0;
+                for (int k = lane; k < cur.nz; k += 64) {
+                    const int j = A.col[cur.k0 + k];
+                    const VecN<NC, VT> rjj = ldv<NC, VT>(r, j);
+                    const VecN<NC, VT> pjj = ldv<NC, VT>(pold, j);
+                    part += val[cur.k0 + k] * fma_t<VT>(beta[c], pjj.v[c], rjj.v[c]);
+                }
+                s[c] = wave_sum(part);
+            }
+        }
+        if (has_row) {
+            VecN<NC, VT> pn, qq;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                pn.v[c] = fma_t<VT>(beta[c], pv.v[c], rv.v[c]);
+                qq.v[c] = s[c];
+                delta[c] += (double)pn.v[c] * (double)s[c];
+            }
+            stv<NC, VT>(pnew, row, pn);
+            stv<NC, VT>(q, row, qq);
+            if (has_pend) {
+                if (MIXED) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) dd.v[c] = fma_t<VT>(pend[c], pv.v[c], dd.v[c]);
+                    stv<NC, VT>(dv, row, dd);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+                        A.x[(size_t)c * A.n + row] = __builtin_fma((double)pend[c], (double)pv.v[c], xo[c]);
+                }
+            }
+        }
+    }
+}
+
+// Phase B over the block's rows: r -= alpha q, g += (r, r); rows are
+// processed 4 per thread with all loads issued first.
+template <int NC, int BS, typename VT>
+__device__ void phase_b(const CgArgs &A, int R0, int R1, const VT (&alpha)[NC], double (&g)[NC]) {
+    VT *r = (VT *)A.r;
+    const VT *q = (const VT *)A.q;
+    constexpr int U = 4;
+    for (int base = R0 + (int)threadIdx.x; base < R1; base += U * BS) {
+        VecN<NC, VT> rv[U], qv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {  // clamped rows keep the loads unconditional
+            const int row = min(base + u * BS, R1 - 1);
+            rv[u] = ldv<NC, VT>(r, row);
+            qv[u] = ldv<NC, VT>(q, row);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int row = base + u * BS;
+            if (row < R1) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    rv[u].v[c] = fma_t<VT>(-alpha[c], qv[u].v[c], rv[u].v[c]);
+                    g[c] += (double)rv[u].v[c] * (double)rv[u].v[c];
+                }
+                stv<NC, VT>(r, row, rv[u]);
+            }
+        }
+    }
+}
+
+// One block per CU: 512 threads = 2 waves per SIMD, so each wave may use up
+// to 256 VGPRs (enough to keep a whole tile's loads in flight).
+template <int NC, int BS, bool MIXED>
+__global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
+    using VT = typename std::conditional<MIXED, float, double>::type;
+    constexpr int NW = BS / 64;
+    __shared__ double prod_raw[NW * kWaveNnz * NC];
+    __shared__ double lds[kMaxRed * NW];
     const int nb = gridDim.x;
     const int L = xcd_swizzle(blockIdx.x, nb);
     const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
     const int R0 = A.tiles[T0], R1 = A.tiles[T1];
-    const int quad = threadIdx.x / kBlock, qt = threadIdx.x % kBlock;
-    VT *prod = reinterpret_cast<VT *>(prod_raw) + quad * kTileNnz * NC;
-    const MT *val = MIXED ? (const MT *)A.val32 : (const MT *)A.val64;
-    VT *r = (VT *)A.r;
-    VT *q = (VT *)A.q;
+    const int wave = threadIdx.x >> 6;
+    double *lds64 = prod_raw + wave * kWaveNnz * NC;
+    VT *ldsv = reinterpret_cast<VT *>(prod_raw) + wave * kWaveNnz * NC;
     VT *dv = (VT *)A.d;
     VT *pbuf[2] = {(VT *)A.p0, (VT *)A.p1};
     unsigned epoch = 0;
@@ -669,7 +927,7 @@ __global__ __launch_bounds__(BS) void k_cg(const CgArgs A) {
     double acc[3 * NC];
 #pragma unroll
     for (int k = 0; k < 3 * NC; ++k) acc[k] = 0.0;
-    residual_phase<NC, BS, MIXED, true>(A, T0, T1, prod_raw, acc);
+    residual_phase<NC, BS, MIXED, true>(A, T0, T1, lds64, acc);
     if (!grid_sum<3 * NC, BS>(A.sync, acc, epoch, lds, A.timeout_ticks)) goto timeout;
     {
         double tol2[NC], gamma[NC], rmax2[NC];
@@ -702,102 +960,25 @@ __global__ __launch_bounds__(BS) void k_cg(const CgArgs A) {
         int k = 0;
         int cur = 0;  // pbuf[cur] = p_old
         bool has_pend = false;
+        int last_ru = 0;
+        int stall[NC];
+        double ru_rr[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            stall[c] = 0;
+            ru_rr[c] = acc[c];
+        }
         while (any) {
             VT *pold = pbuf[cur];
             VT *pnew = pbuf[cur ^ 1];
-            // ------------------------------------------------- phase A
             double delta[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) delta[c] = 0.0;
-            for (int base = T0; base < T1; base += Q) {
-                const int t = base + quad;
-                const bool act = t < T1;
-                int r0 = 0, r1 = 0, k0 = 0, nz = 0;
-                if (act) {
-                    r0 = A.tiles[t];
-                    r1 = A.tiles[t + 1];
-                    k0 = A.rs[r0];
-                    nz = A.rs[r1] - k0;
-                }
-                const bool small = nz <= kTileNnz;
-                if (act && small) {
-                    int cj[kItems];
-                    MT vj[kItems];
-#pragma unroll
-                    for (int i = 0; i < kItems; ++i) {
-                        const int kk = qt + i * kBlock;
-                        if (kk < nz) {
-                            cj[i] = A.col[k0 + kk];
-                            vj[i] = val[k0 + kk];
-                        }
-                    }
-#pragma unroll
-                    for (int i = 0; i < kItems; ++i) {
-                        const int kk = qt + i * kBlock;
-                        if (kk < nz) {
-                            const VecN<NC, VT> rj = ldv<NC, VT>(r, cj[i]);
-                            const VecN<NC, VT> pj = ldv<NC, VT>(pold, cj[i]);
-#pragma unroll
-                            for (int c = 0; c < NC; ++c)
-                                prod[kk * NC + c] = vj[i] * fma_t<VT>(beta[c], pj.v[c], rj.v[c]);
-                        }
-                    }
-                }
-                __syncthreads();
-                if (act) {
-                    const int row = r0 + qt;
-                    const bool mine = small ? row < r1 : qt == 0;
-                    VT s[NC];
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) s[c] = (VT)0;
-                    if (small && row < r1) {
-                        const int a = A.rs[row] - k0, e = A.rs[row + 1] - k0;
-                        for (int kk = a; kk < e; ++kk) {
-#pragma unroll
-                            for (int c = 0; c < NC; ++c) s[c] += prod[kk * NC + c];
-                        }
-                    } else if (!small && qt == 0) {
-                        for (int kk = 0; kk < nz; ++kk) {
-                            const int j = A.col[k0 + kk];
-                            const VecN<NC, VT> rj = ldv<NC, VT>(r, j);
-                            const VecN<NC, VT> pj = ldv<NC, VT>(pold, j);
-#pragma unroll
-                            for (int c = 0; c < NC; ++c)
-                                s[c] += val[k0 + kk] * fma_t<VT>(beta[c], pj.v[c], rj.v[c]);
-                        }
-                    }
-                    if (mine) {
-                        const int rw = small ? row : r0;
-                        const VecN<NC, VT> rv = ldv<NC, VT>(r, rw);
-                        const VecN<NC, VT> pv = ldv<NC, VT>(pold, rw);
-                        VecN<NC, VT> pn, qq;
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) {
-                            pn.v[c] = fma_t<VT>(beta[c], pv.v[c], rv.v[c]);
-                            qq.v[c] = s[c];
-                            delta[c] += (double)pn.v[c] * (double)s[c];
-                        }
-                        stv<NC, VT>(pnew, rw, pn);
-                        stv<NC, VT>(q, rw, qq);
-                        if (has_pend) {
-                            if (MIXED) {
-                                VecN<NC, VT> dd = ldv<NC, VT>(dv, rw);
-#pragma unroll
-                                for (int c = 0; c < NC; ++c) dd.v[c] = fma_t<VT>(pend[c], pv.v[c], dd.v[c]);
-                                stv<NC, VT>(dv, rw, dd);
-                            } else {
-#pragma unroll
-                                for (int c = 0; c < NC; ++c) {
-                                    double *xp = A.x + (size_t)c * A.n + rw;
-                                    *xp = __builtin_fma((double)pend[c], (double)pv.v[c], *xp);
-                                }
-                            }
-                        }
-                    }
-                }
-                __syncthreads();
-            }
+            CG_TRACE_STAMP(k, 0);
+            phase_a<NC, BS, MIXED>(A, T0, T1, ldsv, pold, pnew, beta, pend, has_pend, delta);
+            CG_TRACE_MAX(k, 1);
             if (!grid_sum<NC, BS>(A.sync, delta, epoch, lds, A.timeout_ticks)) goto timeout;
+            CG_TRACE_STAMP(k, 2);
             VT alpha[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
@@ -811,21 +992,13 @@ __global__ __launch_bounds__(BS) void k_cg(const CgArgs A) {
                     }
                 }
             }
-            // ------------------------------------------------- phase B
             double g[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) g[c] = 0.0;
-            for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) {
-                VecN<NC, VT> rv = ldv<NC, VT>(r, row);
-                const VecN<NC, VT> qv = ldv<NC, VT>(q, row);
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    rv.v[c] = fma_t<VT>(-alpha[c], qv.v[c], rv.v[c]);
-                    g[c] += (double)rv.v[c] * (double)rv.v[c];
-                }
-                stv<NC, VT>(r, row, rv);
-            }
+            phase_b<NC, BS, VT>(A, R0, R1, alpha, g);
+            CG_TRACE_MAX(k, 3);
             if (!grid_sum<NC, BS>(A.sync, g, epoch, lds, A.timeout_ticks)) goto timeout;
+            CG_TRACE_STAMP(k, 4);
             ++k;
             bool want_ru = false;
 #pragma unroll
@@ -843,7 +1016,9 @@ __global__ __launch_bounds__(BS) void k_cg(const CgArgs A) {
             }
             has_pend = true;
             const bool out_of_iters = k >= A.max_iter;
+            if (MIXED && k - last_ru >= A.ru_gap) want_ru = true;
             if (MIXED && (want_ru || out_of_iters)) {
+                last_ru = k;
                 // fold d + alpha*p_new into the FP64 solution, recompute r
                 for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) {
                     VecN<NC, VT> dd = ldv<NC, VT>(dv, row);
@@ -865,7 +1040,7 @@ __global__ __launch_bounds__(BS) void k_cg(const CgArgs A) {
                 double ra[3 * NC];
 #pragma unroll
                 for (int kk = 0; kk < 3 * NC; ++kk) ra[kk] = 0.0;
-                residual_phase<NC, BS, MIXED, false>(A, T0, T1, prod_raw, ra);
+                residual_phase<NC, BS, MIXED, false>(A, T0, T1, lds64, ra);
                 if (!grid_sum<3 * NC, BS>(A.sync, ra, epoch, lds, A.timeout_ticks)) goto timeout;
                 ++refines;
 #pragma unroll
@@ -877,6 +1052,15 @@ __global__ __launch_bounds__(BS) void k_cg(const CgArgs A) {
                         } else {
                             g[c] = ra[2 * NC + c];
                             rmax2[c] = ra[c];
+                            // the FP64 residual must at least halve between
+                            // reliable updates; three misses in a row end
+                            // the FP32 iteration for this column
+                            stall[c] = ra[c] > 0.25 * ru_rr[c] ? stall[c] + 1 : 0;
+                            ru_rr[c] = fmin(ru_rr[c], ra[c]);
+                            if (stall[c] >= 3) {
+                                status[c] = NSFEM_STAGNATION;
+                                active[c] = false;
+                            }
                         }
                     }
                 }
@@ -950,7 +1134,6 @@ int cg_max_grid(Device &d) {
 
 }  // namespace
 
-static constexpr int kCgBlock = 256;
 
 namespace {
 // true residual ||b_c - A x_c|| for each column (FP64), into host res[c]
@@ -970,26 +1153,56 @@ void true_residuals(Device &d, Matrix &m, const double *b, const double *x, int 
 }
 }  // namespace
 
-int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
+namespace {
+
+struct CgPass {
+    CgColumnResult res[2];
+    int iters[2] = {0, 0}, refines[2] = {0, 0};
+    double final_res[2] = {0, 0};
+    float ms = 0.f;
+};
+
+// One persistent-CG solve in the given precision.  The FP64 path ends on the
+// recursive residual; the true residual is recomputed and, should rounding
+// drift have left it above the tolerance, the solve restarts from the
+// current x (at most twice).
+void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
     Matrix &m = *req.m;
-    NSFEM_REQUIRE(m.ready, NSFEM_NOT_READY, "matrix slot is empty");
-    NSFEM_REQUIRE(m.rows == m.cols, NSFEM_INVALID_ARGUMENT, "CG needs a square matrix");
-    NSFEM_REQUIRE(req.ncol == 1 || req.ncol == 2, NSFEM_INVALID_ARGUMENT,
-                  "columns must be 1 or 2");
-    const bool mixed = req.precision == NSFEM_PRECISION_FP32_REFINED;
-    NSFEM_REQUIRE(mixed || req.precision == NSFEM_PRECISION_FP64, NSFEM_INVALID_ARGUMENT,
-                  "unknown precision");
     const int n = m.rows, nc = req.ncol;
     if (mixed) ensure_val32(d, m);
-
+    // block size: 1024 threads (16 waves per CU, 128 VGPRs each) by default.
+    // The loop is bound by gather latency, so waves in flight matter more
+    // than registers: 1024 beats 512 threads (8 waves, up to 256 VGPRs) on
+    // every measured solve.  NSFEM_CG_BLOCK=512 selects the smaller block.
+    static const int bs_env = [] {
+        const char *e = std::getenv("NSFEM_CG_BLOCK");
+        return e ? std::atoi(e) : 0;
+    }();
+    const int BSz = bs_env == 512 ? 512 : 1024;
     int maxg;
-    if (nc == 1)
-        maxg = mixed ? cg_max_grid<1, kCgBlock, true>(d) : cg_max_grid<1, kCgBlock, false>(d);
-    else
-        maxg = mixed ? cg_max_grid<2, kCgBlock, true>(d) : cg_max_grid<2, kCgBlock, false>(d);
-    int grid = req.grid > 0 ? std::min(req.grid, maxg)
-                            : std::min(maxg, std::max(1, m.n_tiles / 2));
-    grid = std::max(1, std::min(grid, std::max(1, m.n_tiles)));
+    if (BSz == 1024) {
+        if (nc == 1)
+            maxg = mixed ? cg_max_grid<1, 1024, true>(d) : cg_max_grid<1, 1024, false>(d);
+        else
+            maxg = mixed ? cg_max_grid<2, 1024, true>(d) : cg_max_grid<2, 1024, false>(d);
+    } else {
+        if (nc == 1)
+            maxg = mixed ? cg_max_grid<1, 512, true>(d) : cg_max_grid<1, 512, false>(d);
+        else
+            maxg = mixed ? cg_max_grid<2, 512, true>(d) : cg_max_grid<2, 512, false>(d);
+    }
+    int want = req.grid;
+    if (want <= 0) {
+        // tuning override: NSFEM_CG_GRID=<blocks> (capped by co-residency)
+        const char *env = std::getenv("NSFEM_CG_GRID");
+        if (env) want = std::atoi(env);
+    }
+    int grid = want > 0 ? std::min(want, maxg)
+                        : std::min(maxg, std::max(1, m.n_wtiles / (2 * (BSz / 64))));
+    grid = std::max(1, std::min(grid, std::max(1, m.n_wtiles)));
+    if (std::getenv("NSFEM_CG_VERBOSE"))
+        std::fprintf(stderr, "[nsfem] cg rows=%d cols=%d wave tiles=%d block=%d grid=%d maxgrid=%d mixed=%d\n",
+                     n, nc, m.n_wtiles, BSz, grid, maxg, (int)mixed);
     ensure_block_tiles(d, m, grid);
 
     const size_t vt = mixed ? sizeof(float) : sizeof(double);
@@ -1000,7 +1213,8 @@ int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
 
     CgArgs a{};
     a.n = n;
-    a.tiles = m.tiles.ptr;
+    a.meta = m.wmeta.ptr;
+    a.tiles = m.wtiles.ptr;
     a.block_tiles = m.block_tiles.ptr;
     a.rs = m.row_start.ptr;
     a.col = m.col.ptr;
@@ -1019,70 +1233,116 @@ int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
     a.rdelta = req.reliable_delta;
     a.max_iter = req.max_iter < 0 ? n : req.max_iter;
     a.max_refine = 400;
+    a.ru_gap = 1000;
     a.timeout_ticks = (unsigned long long)d.wall_khz * 1000ull * 20ull;  // 20 s per barrier
     unsigned char *sm = d.sync_mem.ptr;
     a.sync = reinterpret_cast<SyncState *>(sm + (256 - ((uintptr_t)sm % 256)) % 256);
     a.res = d.cg_res.ptr;
     const int ngroups = (grid + kGroup - 1) / kGroup;
 
-    CgColumnResult res[2];
-    int total_iters[2] = {0, 0}, total_ref[2] = {0, 0};
-    double final_res[2] = {0, 0};
-    float total_ms = 0.f;
-    // The FP64 path ends on the recursive residual; the true residual is
-    // recomputed and, should rounding drift have left it above the
-    // tolerance, the solve restarts from the current x (at most twice).
     for (int attempt = 0; attempt < 3; ++attempt) {
         NSFEM_HIP(hipEventRecord(d.ev_start, d.stream));
         k_sync_reset<<<(ngroups + 255) / 256, 256, 0, d.stream>>>(a.sync, ngroups);
         NSFEM_HIP(hipGetLastError());
-        if (nc == 1) {
-            if (mixed) cg_launch<1, kCgBlock, true>(d, a, grid);
-            else cg_launch<1, kCgBlock, false>(d, a, grid);
+        if (BSz == 1024) {
+            if (nc == 1) {
+                if (mixed) cg_launch<1, 1024, true>(d, a, grid);
+                else cg_launch<1, 1024, false>(d, a, grid);
+            } else {
+                if (mixed) cg_launch<2, 1024, true>(d, a, grid);
+                else cg_launch<2, 1024, false>(d, a, grid);
+            }
         } else {
-            if (mixed) cg_launch<2, kCgBlock, true>(d, a, grid);
-            else cg_launch<2, kCgBlock, false>(d, a, grid);
+            if (nc == 1) {
+                if (mixed) cg_launch<1, 512, true>(d, a, grid);
+                else cg_launch<1, 512, false>(d, a, grid);
+            } else {
+                if (mixed) cg_launch<2, 512, true>(d, a, grid);
+                else cg_launch<2, 512, false>(d, a, grid);
+            }
         }
         NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
-        NSFEM_HIP(hipMemcpyAsync(res, d.cg_res.ptr, sizeof(CgColumnResult) * nc,
+        NSFEM_HIP(hipMemcpyAsync(P.res, d.cg_res.ptr, sizeof(CgColumnResult) * nc,
                                  hipMemcpyDeviceToHost, d.stream));
         NSFEM_HIP(hipStreamSynchronize(d.stream));
         float ms = 0.f;
         NSFEM_HIP(hipEventElapsedTime(&ms, d.ev_start, d.ev_stop));
-        total_ms += ms;
+        P.ms += ms;
         bool drift = false;
-        if (!mixed) {
-            true_residuals(d, m, req.b, req.x, nc, (double *)a.q, final_res);
-        }
+        if (!mixed) true_residuals(d, m, req.b, req.x, nc, (double *)a.q, P.final_res);
         for (int c = 0; c < nc; ++c) {
-            total_iters[c] += res[c].iterations;
-            total_ref[c] += res[c].refinements;
-            if (mixed) final_res[c] = res[c].last_residual;
-            if (res[c].status == NSFEM_OK && final_res[c] > res[c].tolerance * (1.0 + 1e-12))
+            P.iters[c] += P.res[c].iterations;
+            P.refines[c] += P.res[c].refinements;
+            if (mixed) P.final_res[c] = P.res[c].last_residual;
+            if (P.res[c].status == NSFEM_OK && P.final_res[c] > P.res[c].tolerance * (1.0 + 1e-12))
                 drift = true;
-            if (res[c].status == NSFEM_BARRIER_TIMEOUT)
+            if (P.res[c].status == NSFEM_BARRIER_TIMEOUT)
                 fail(NSFEM_BARRIER_TIMEOUT, "CG kernel barrier timed out");
         }
         if (!drift || mixed) break;
     }
+}
 
+}  // namespace
+
+int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
+    Matrix &m = *req.m;
+    NSFEM_REQUIRE(m.ready, NSFEM_NOT_READY, "matrix slot is empty");
+    NSFEM_REQUIRE(m.rows == m.cols, NSFEM_INVALID_ARGUMENT, "CG needs a square matrix");
+    NSFEM_REQUIRE(req.ncol == 1 || req.ncol == 2, NSFEM_INVALID_ARGUMENT,
+                  "columns must be 1 or 2");
+    const bool mixed = req.precision == NSFEM_PRECISION_FP32_REFINED;
+    NSFEM_REQUIRE(mixed || req.precision == NSFEM_PRECISION_FP64, NSFEM_INVALID_ARGUMENT,
+                  "unknown precision");
+    const int nc = req.ncol;
+    CgPass P;
+    cg_pass(d, req, mixed, P);
+    CgPass F;  // FP64 continuation after an FP32 stagnation
+    bool fallback = false;
+    if (mixed) {
+        for (int c = 0; c < nc; ++c) fallback = fallback || P.res[c].status == NSFEM_STAGNATION;
+        if (fallback) {
+            CgRequest r2 = req;
+            if (r2.max_iter > 0) r2.max_iter = std::max(1, r2.max_iter - std::max(P.iters[0], P.iters[nc - 1]));
+            // the same stopping rule: ||b|| (or ||r0||) of the original solve
+            r2.rtol = 0.0;
+            r2.atol = std::max(P.res[0].tolerance, nc == 2 ? P.res[1].tolerance : 0.0);
+            if (nc == 2) r2.atol = std::min(P.res[0].tolerance, P.res[1].tolerance);
+            cg_pass(d, r2, false, F);
+        }
+    }
     int worst = NSFEM_OK;
     for (int c = 0; c < nc; ++c) {
-        if (res[c].status != NSFEM_OK && worst == NSFEM_OK) worst = res[c].status;
+        const int status = fallback ? F.res[c].status : P.res[c].status;
+        if (status != NSFEM_OK && worst == NSFEM_OK) worst = status;
         if (stats) {
             nsfem_cg_stats &s = stats[c];
-            s.iterations = total_iters[c];
-            s.refinements = total_ref[c];
-            s.status = res[c].status;
+            s.iterations = P.iters[c] + (fallback ? F.iters[c] : 0);
+            s.refinements = P.refines[c] + (fallback ? 1 : 0);
+            s.status = status;
             s.columns = nc;
-            s.rhs_norm = res[c].rhs_norm;
-            s.initial_residual = res[c].initial_residual;
-            s.final_residual = final_res[c];
-            s.tolerance = res[c].tolerance;
-            s.seconds = total_ms * 1e-3;
+            s.rhs_norm = P.res[c].rhs_norm;
+            s.initial_residual = P.res[c].initial_residual;
+            s.final_residual = fallback ? F.final_res[c] : P.final_res[c];
+            s.tolerance = P.res[c].tolerance;
+            s.seconds = (P.ms + (fallback ? F.ms : 0.f)) * 1e-3;
         }
     }
     return worst;
 }
 
 }  // namespace nsfem
+
+#ifdef NSFEM_CG_TRACE
+extern "C" int nsfem_trace_read(unsigned long long *out, int iters) {
+    const int n = iters < nsfem::kTraceIters ? iters : nsfem::kTraceIters;
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(nsfem::g_cg_trace), sizeof(unsigned long long) * 5 * n) !=
+        hipSuccess)
+        return -1;
+    return n;
+}
+extern "C" int nsfem_trace_clear(void) {
+    static unsigned long long zero[nsfem::kTraceIters][5];
+    return hipMemcpyToSymbol(HIP_SYMBOL(nsfem::g_cg_trace), zero, sizeof(zero)) == hipSuccess ? 0 : -1;
+}
+#endif

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -101,6 +102,8 @@ struct DeviceArray {
 constexpr int kBlock = 256;
 constexpr int kTileRows = 256;
 constexpr int kTileNnz = 2048;
+constexpr int kWaveRows = 64;   // wave tiles of the persistent CG
+constexpr int kWaveNnz = 512;
 constexpr int kMaxGridBlocks = 4096;
 
 struct Matrix {
@@ -112,8 +115,11 @@ struct Matrix {
     DeviceArray<int32_t> tiles;  // [n_tiles + 1] first row of each tile
     int n_tiles = 0;
     std::vector<int32_t> host_tiles;
-    std::vector<int64_t> host_tile_nnz_prefix;  // [n_tiles + 1]
-    // ownership of tiles by persistent blocks, cached per grid size
+    DeviceArray<int32_t> wtiles;  // [n_wtiles + 1] first row of each wave tile
+    DeviceArray<int4> wmeta;      // [n_wtiles] {first row, end row, first nnz, nnz}
+    int n_wtiles = 0;
+    std::vector<int64_t> host_wtile_prefix;  // [n_wtiles + 1] cumulative nnz + rows
+    // ownership of wave tiles by persistent blocks, cached per grid size
     DeviceArray<int32_t> block_tiles;
     int block_tiles_grid = 0;
     bool ready = false;
@@ -146,6 +152,9 @@ struct CgColumnResult {
     double last_residual;  // sqrt of the last FP64 (true) residual^2 computed in-kernel
 };
 
+struct Simulation;
+void simulation_detach(Simulation *s);  // frees its device memory, clears its device
+
 struct Device {
     int index = 0;
     hipStream_t stream = nullptr;
@@ -164,6 +173,11 @@ struct Device {
     int wall_khz = 100000;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::mutex mtx;  // serialises solves on this handle
+    // buffers created on this device; destroying the device releases their
+    // memory and detaches them, so a later nsfem_buffer_destroy stays safe
+    std::set<struct Buffer *> buffers;
+    std::set<Simulation *> sims;  // detached the same way
+    std::mutex buf_mtx;
     void set_current() const { NSFEM_HIP(hipSetDevice(index)); }
 };
 

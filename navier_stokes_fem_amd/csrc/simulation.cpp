@@ -22,6 +22,9 @@
 
 namespace nsfem {
 
+struct Simulation;
+void simulation_reset(Simulation &s);
+
 struct Simulation {
     Device *dev = nullptr;
     int V = 0, P = 0;
@@ -116,19 +119,30 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
     put(d, s->vb_v, vv.data(), vv.size());
     put(d, s->pb_idx, pi.data(), pi.size());
     put(d, s->pb_val, pv.data(), pv.size());
-    // zero initial field with the velocity Dirichlet values (:1503-1507)
-    launch_fill(d, 2 * V, 0.0, s->uv.ptr);
-    launch_fill(d, P, 0.0, s->p.ptr);
-    launch_scatter(d, s->nvb, s->vb_idx.ptr, s->vb_u.ptr, s->uv.ptr);
-    launch_scatter(d, s->nvb, s->vb_idx.ptr, s->vb_v.ptr, s->uv.ptr + V);
-    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    simulation_reset(*s);
+    {
+        std::lock_guard<std::mutex> lk(d.buf_mtx);
+        d.sims.insert(s);
+    }
     return s;
+}
+
+// zero initial field with the velocity Dirichlet values (:1503-1507)
+void simulation_reset(Simulation &s) {
+    Device &d = *s.dev;
+    d.set_current();
+    launch_fill(d, 2 * s.V, 0.0, s.uv.ptr);
+    launch_fill(d, s.P, 0.0, s.p.ptr);
+    launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, s.uv.ptr);
+    launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_v.ptr, s.uv.ptr + s.V);
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
 }
 
 namespace {
 
 struct SolveOut {
     int64_t iters[2];
+    int64_t refines[2];
     double rel[2];
     double seconds;
 };
@@ -153,6 +167,7 @@ void solve(Simulation &s, int slot, const double *b, double *x, int ncol, SolveO
     }
     for (int c = 0; c < ncol; ++c) {
         o.iters[c] = st[c].iterations;
+        o.refines[c] = st[c].refinements;
         o.rel[c] = st[c].rhs_norm > 0 ? st[c].final_residual / st[c].rhs_norm : st[c].final_residual;
     }
     o.seconds = st[0].seconds;
@@ -188,6 +203,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         if (s.opt.pair_velocity) {
             solve(s, NSFEM_DIFFUSION, s.rhs.ptr, s.uv.ptr, 2, o[0]);
             o[1].iters[0] = o[0].iters[1];
+            o[1].refines[0] = o[0].refines[1];
             o[1].rel[0] = o[0].rel[1];
         } else {
             solve(s, NSFEM_DIFFUSION, s.rhs.ptr, u, 1, o[0]);
@@ -203,6 +219,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         if (s.opt.pair_velocity) {
             solve(s, NSFEM_VELOCITY_MASS, s.rhs.ptr, s.du.ptr, 2, o[3]);
             o[4].iters[0] = o[3].iters[1];
+            o[4].refines[0] = o[3].refines[1];
             o[4].rel[0] = o[3].rel[1];
         } else {
             solve(s, NSFEM_VELOCITY_MASS, s.rhs.ptr, s.du.ptr, 1, o[3]);
@@ -223,6 +240,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
             stats->advect_seconds += ma * 1e-3;
             for (int k = 0; k < 5; ++k) {
                 stats->cg_iterations[k] += o[k].iters[0];
+                stats->cg_refinements[k] += o[k].refines[0];
                 stats->max_relative_residual[k] =
                     std::max(stats->max_relative_residual[k], o[k].rel[0]);
             }
@@ -260,7 +278,30 @@ void simulation_set(Simulation &s, const double *u, const double *v, const doubl
     NSFEM_HIP(hipStreamSynchronize(d.stream));
 }
 
-void simulation_destroy(Simulation *s) { delete s; }
+void simulation_detach(Simulation *s) {
+    s->uv.release();
+    s->utv.release();
+    s->rhs.release();
+    s->prhs.release();
+    s->p.release();
+    s->du.release();
+    s->c_uv.release();
+    s->c_p.release();
+    s->vb_idx.release();
+    s->pb_idx.release();
+    s->vb_u.release();
+    s->vb_v.release();
+    s->pb_val.release();
+    s->dev = nullptr;
+}
+
+void simulation_destroy(Simulation *s) {
+    if (s && s->dev) {
+        std::lock_guard<std::mutex> lk(s->dev->buf_mtx);
+        s->dev->sims.erase(s);
+    }
+    delete s;
+}
 
 }  // namespace nsfem
 
@@ -296,7 +337,7 @@ int nsfem_simulation_create(nsfem_device *h, int V, int P, const float *xy, int 
 int nsfem_simulation_destroy(nsfem_simulation *s) {
     return guard([&] {
         if (!s) return;
-        if (s->impl) {
+        if (s->impl && s->impl->dev) {
             (void)hipSetDevice(s->impl->dev->index);
             (void)hipStreamSynchronize(s->impl->dev->stream);
         }
@@ -308,6 +349,7 @@ int nsfem_simulation_destroy(nsfem_simulation *s) {
 int nsfem_simulation_step(nsfem_simulation *s, int steps, nsfem_step_stats *stats) {
     return guard([&] {
         NSFEM_REQUIRE(s && s->impl && steps >= 0, NSFEM_INVALID_ARGUMENT, "bad simulation call");
+        NSFEM_REQUIRE(s->impl->dev, NSFEM_INVALID_ARGUMENT, "simulation's device was destroyed");
         std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
         simulation_step(*s->impl, steps, stats);
     });
@@ -315,15 +357,40 @@ int nsfem_simulation_step(nsfem_simulation *s, int steps, nsfem_step_stats *stat
 
 int nsfem_simulation_get_fields(nsfem_simulation *s, double *u, double *v, double *p) {
     return guard([&] {
-        NSFEM_REQUIRE(s && s->impl, NSFEM_INVALID_ARGUMENT, "null simulation");
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
         simulation_get(*s->impl, u, v, p);
+    });
+}
+
+int nsfem_simulation_set_options(nsfem_simulation *s, const nsfem_sim_options *opt) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && opt, NSFEM_INVALID_ARGUMENT, "null argument");
+        NSFEM_REQUIRE(opt->precision == NSFEM_PRECISION_FP64 ||
+                          opt->precision == NSFEM_PRECISION_FP32_REFINED,
+                      NSFEM_INVALID_ARGUMENT, "unknown precision");
+        NSFEM_REQUIRE(opt->rtol >= 0, NSFEM_INVALID_ARGUMENT, "negative rtol");
+        s->impl->opt.rtol = opt->rtol;
+        s->impl->opt.max_iterations = opt->max_iterations;
+        s->impl->opt.precision = opt->precision;
+        s->impl->opt.pair_velocity = opt->pair_velocity;
+    });
+}
+
+int nsfem_simulation_reset(nsfem_simulation *s) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_reset(*s->impl);
     });
 }
 
 int nsfem_simulation_set_fields(nsfem_simulation *s, const double *u, const double *v,
                                 const double *p) {
     return guard([&] {
-        NSFEM_REQUIRE(s && s->impl, NSFEM_INVALID_ARGUMENT, "null simulation");
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
         simulation_set(*s->impl, u, v, p);
     });
 }

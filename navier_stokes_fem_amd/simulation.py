@@ -59,6 +59,23 @@ class Simulation:
         a = [None if x is None else np.ascontiguousarray(x, np.float64) for x in (u, v, p)]
         N.check(N.lib().nsfem_simulation_set_fields(self.handle, *[N.ptr(x) for x in a]))
 
+    def configure(self, rtol=None, precision=None, pair_velocity=None, max_iterations=None):
+        """Change solver options (viscosity and dt are fixed at creation)."""
+        o = self.options
+        if rtol is not None:
+            o.rtol = float(rtol)
+        if precision is not None:
+            o.precision = {"fp64": N.PRECISION_FP64, "fp32_refined": N.PRECISION_FP32_REFINED}[precision]
+        if pair_velocity is not None:
+            o.pair_velocity = int(bool(pair_velocity))
+        if max_iterations is not None:
+            o.max_iterations = int(max_iterations)
+        N.check(N.lib().nsfem_simulation_set_options(self.handle, C.byref(o)))
+
+    def reset(self):
+        """Zero fields with the velocity Dirichlet values imposed."""
+        N.check(N.lib().nsfem_simulation_reset(self.handle))
+
     def close(self):
         if getattr(self, "handle", None):
             N.lib().nsfem_simulation_destroy(self.handle)
