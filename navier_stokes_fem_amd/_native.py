@@ -21,6 +21,7 @@ OUT_OF_MEMORY = -5
 NOT_READY = -6
 BREAKDOWN = -7
 BARRIER_TIMEOUT = -8
+STAGNATION = -9
 
 PRECISION_FP64 = 0
 PRECISION_FP32_REFINED = 1

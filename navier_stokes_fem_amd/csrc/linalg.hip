@@ -453,7 +453,7 @@ struct CgArgs {
     const float *val32;
     const double *b;  // [NC][n]
     double *x;        // [NC][n]
-    void *r, *p0, *p1, *q, *d;  // interleaved [n][NC]
+    void *r, *p0, *p1, *q;  // interleaved [n][NC]
     double rtol, atol, rdelta;
     int max_iter, max_refine;
     int ru_gap;  // mixed: force a reliable update after this many iterations
@@ -641,7 +641,7 @@ __device__ __forceinline__ WTile wave_tile(const int4 *meta, int t) {
 }
 
 // FP64 residual over the block's wave tiles: r = b - A x (x column-major).
-// SETUP: also p_old = 0 (and d = 0 for the mixed solver) and b.b.
+// SETUP: also p_old = 0 and b.b.
 // Accumulates per column: acc[c] = r.r (FP64), acc[NC+c] = b.b,
 // acc[2NC+c] = (FP32 r).(FP32 r).
 template <int NC, int BS, bool MIXED, bool SETUP>
@@ -653,7 +653,6 @@ __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     VT *r = (VT *)A.r;
     VT *pold = (VT *)A.p0;
-    VT *d = (VT *)A.d;
     for (int t = T0 + wave; t < T1; t += NW) {
         const WTile w = wave_tile(A.meta, t);
         const int row = w.r0 + lane;
@@ -727,7 +726,6 @@ __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave
             stv<NC, VT>(r, row, rv);
             if (SETUP) {
                 stv<NC, VT>(pold, row, zero);
-                if (MIXED) stv<NC, VT>(d, row, zero);
             }
         }
     }
@@ -735,22 +733,21 @@ __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave
 
 // Phase A over the block's wave tiles:
 //   p_new = r + beta p_old (owner rows), q = A p_new (gathering r, p_old),
-//   deferred solution update with pend * p_old, delta += (p_new, q).
+//   deferred FP64 solution update x += pend * p_old, delta += (p_new, q).
+// The products and row sums accumulate in FP64 in both precisions (the
+// product of two FP32 numbers is exact in FP64); only q is stored as VT.
 template <int NC, int BS, bool MIXED>
-__device__ void phase_a(const CgArgs &A, int T0, int T1,
-                        typename std::conditional<MIXED, float, double>::type *lds_wave,
+__device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
                         const typename std::conditional<MIXED, float, double>::type *pold,
                         typename std::conditional<MIXED, float, double>::type *pnew,
                         const typename std::conditional<MIXED, float, double>::type (&beta)[NC],
-                        const typename std::conditional<MIXED, float, double>::type (&pend)[NC],
-                        bool has_pend, double (&delta)[NC]) {
+                        const double (&pend)[NC], bool has_pend, double (&delta)[NC]) {
     using VT = typename std::conditional<MIXED, float, double>::type;
     using MT = VT;
     constexpr int NW = BS / 64;
     const MT *val = MIXED ? (const MT *)A.val32 : (const MT *)A.val64;
     const VT *r = (const VT *)A.r;
     VT *q = (VT *)A.q;
-    VT *dv = (VT *)A.d;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int t = T0 + wave;
@@ -791,19 +788,14 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1,
         const int rsb = A.rs[rowc + 1];
         const VecN<NC, VT> rv = ldv<NC, VT>(r, rowc);
         const VecN<NC, VT> pv = ldv<NC, VT>(pold, rowc);
-        VecN<NC, VT> dd;
         double xo[NC];
         if (has_pend) {
-            if (MIXED) {
-                dd = ldv<NC, VT>(dv, rowc);
-            } else {
 #pragma unroll
-                for (int c = 0; c < NC; ++c) xo[c] = A.x[(size_t)c * A.n + rowc];
-            }
+            for (int c = 0; c < NC; ++c) xo[c] = A.x[(size_t)c * A.n + rowc];
         }
-        VT s[NC];
+        double s[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) s[c] = (VT)0;
+        for (int c = 0; c < NC; ++c) s[c] = 0.0;
         if (cur.nz <= kWaveNnz) {
             // 3. values and gathers, then one wait for the whole tile
             constexpr int IB = (NC == 2) ? 4 : kWItems;  // register budget
@@ -823,7 +815,8 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1,
                     const int k = lane + 64 * (i0 + i);
 #pragma unroll
                     for (int c = 0; c < NC; ++c)
-                        lds_wave[k * NC + c] = vj[i] * fma_t<VT>(beta[c], pj[i].v[c], rj[i].v[c]);
+                        lds_wave[k * NC + c] =
+                            (double)vj[i] * (double)fma_t<VT>(beta[c], pj[i].v[c], rj[i].v[c]);
                 }
             }
             __syncwarp();
@@ -838,12 +831,13 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1,
             // a single row longer than a tile: wave-wide sum
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                VT part = (VT)0;
+                double part = 0.0;
                 for (int k = lane; k < cur.nz; k += 64) {
                     const int j = A.col[cur.k0 + k];
                     const VecN<NC, VT> rjj = ldv<NC, VT>(r, j);
                     const VecN<NC, VT> pjj = ldv<NC, VT>(pold, j);
-                    part += val[cur.k0 + k] * fma_t<VT>(beta[c], pjj.v[c], rjj.v[c]);
+                    part += (double)val[cur.k0 + k] *
+                            (double)fma_t<VT>(beta[c], pjj.v[c], rjj.v[c]);
                 }
                 s[c] = wave_sum(part);
             }
@@ -853,21 +847,15 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1,
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 pn.v[c] = fma_t<VT>(beta[c], pv.v[c], rv.v[c]);
-                qq.v[c] = s[c];
-                delta[c] += (double)pn.v[c] * (double)s[c];
+                qq.v[c] = (VT)s[c];
+                delta[c] += (double)pn.v[c] * s[c];
             }
             stv<NC, VT>(pnew, row, pn);
             stv<NC, VT>(q, row, qq);
             if (has_pend) {
-                if (MIXED) {
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) dd.v[c] = fma_t<VT>(pend[c], pv.v[c], dd.v[c]);
-                    stv<NC, VT>(dv, row, dd);
-                } else {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c)
-                        A.x[(size_t)c * A.n + row] = __builtin_fma((double)pend[c], (double)pv.v[c], xo[c]);
-                }
+                for (int c = 0; c < NC; ++c)
+                    A.x[(size_t)c * A.n + row] = __builtin_fma(pend[c], (double)pv.v[c], xo[c]);
             }
         }
     }
@@ -903,8 +891,21 @@ __device__ void phase_b(const CgArgs &A, int R0, int R1, const VT (&alpha)[NC], 
     }
 }
 
-// One block per CU: 512 threads = 2 waves per SIMD, so each wave may use up
-// to 256 VGPRs (enough to keep a whole tile's loads in flight).
+// x += pend p over the block's rows (FP64 solution, p in VT)
+template <int NC, int BS, typename VT>
+__device__ void flush_x(const CgArgs &A, int R0, int R1, const VT *p, const double (&pend)[NC]) {
+    for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) {
+        const VecN<NC, VT> pv = ldv<NC, VT>(p, row);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            double *xp = A.x + (size_t)c * A.n + row;
+            *xp = __builtin_fma(pend[c], (double)pv.v[c], *xp);
+        }
+    }
+}
+
+// One block per CU: BS / 256 waves per SIMD (4 for 1024 threads, 128 VGPRs
+// per wave; 2 for 512 threads, up to 256 VGPRs).
 template <int NC, int BS, bool MIXED>
 __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
     using VT = typename std::conditional<MIXED, float, double>::type;
@@ -917,8 +918,6 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
     const int R0 = A.tiles[T0], R1 = A.tiles[T1];
     const int wave = threadIdx.x >> 6;
     double *lds64 = prod_raw + wave * kWaveNnz * NC;
-    VT *ldsv = reinterpret_cast<VT *>(prod_raw) + wave * kWaveNnz * NC;
-    VT *dv = (VT *)A.d;
     VT *pbuf[2] = {(VT *)A.p0, (VT *)A.p1};
     unsigned epoch = 0;
     const bool writer = (L == 0 && threadIdx.x == 0);
@@ -933,7 +932,8 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
         double tol2[NC], gamma[NC], rmax2[NC];
         bool active[NC];
         int iters[NC], status[NC], refines = 0;
-        VT beta[NC], pend[NC];
+        VT beta[NC];
+        double pend[NC];
         double last_rr[NC];
         bool any = false;
 #pragma unroll
@@ -950,7 +950,7 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
             iters[c] = 0;
             status[c] = NSFEM_OK;
             beta[c] = (VT)0;
-            pend[c] = (VT)0;
+            pend[c] = 0.0;
             if (writer) {
                 A.res[c].rhs_norm = bn;
                 A.res[c].initial_residual = r0n;
@@ -975,27 +975,29 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
 #pragma unroll
             for (int c = 0; c < NC; ++c) delta[c] = 0.0;
             CG_TRACE_STAMP(k, 0);
-            phase_a<NC, BS, MIXED>(A, T0, T1, ldsv, pold, pnew, beta, pend, has_pend, delta);
+            phase_a<NC, BS, MIXED>(A, T0, T1, lds64, pold, pnew, beta, pend, has_pend, delta);
             CG_TRACE_MAX(k, 1);
             if (!grid_sum<NC, BS>(A.sync, delta, epoch, lds, A.timeout_ticks)) goto timeout;
             CG_TRACE_STAMP(k, 2);
-            VT alpha[NC];
+            double alpha[NC];
+            VT alpha_v[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                alpha[c] = (VT)0;
+                alpha[c] = 0.0;
                 if (active[c]) {
                     if (!(delta[c] > 0.0)) {
                         status[c] = NSFEM_BREAKDOWN;
                         active[c] = false;
                     } else {
-                        alpha[c] = (VT)(gamma[c] / delta[c]);
+                        alpha[c] = gamma[c] / delta[c];
                     }
                 }
+                alpha_v[c] = (VT)alpha[c];
             }
             double g[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) g[c] = 0.0;
-            phase_b<NC, BS, VT>(A, R0, R1, alpha, g);
+            phase_b<NC, BS, VT>(A, R0, R1, alpha_v, g);
             CG_TRACE_MAX(k, 3);
             if (!grid_sum<NC, BS>(A.sync, g, epoch, lds, A.timeout_ticks)) goto timeout;
             CG_TRACE_STAMP(k, 4);
@@ -1019,22 +1021,11 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
             if (MIXED && k - last_ru >= A.ru_gap) want_ru = true;
             if (MIXED && (want_ru || out_of_iters)) {
                 last_ru = k;
-                // fold d + alpha*p_new into the FP64 solution, recompute r
-                for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) {
-                    VecN<NC, VT> dd = ldv<NC, VT>(dv, row);
-                    const VecN<NC, VT> pv = ldv<NC, VT>(pnew, row);
-                    VecN<NC, VT> zero;
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) {
-                        dd.v[c] = fma_t<VT>(pend[c], pv.v[c], dd.v[c]);
-                        A.x[(size_t)c * A.n + row] += (double)dd.v[c];
-                        zero.v[c] = (VT)0;
-                    }
-                    stv<NC, VT>(dv, row, zero);
-                }
+                // apply the pending x += alpha p_new (FP64), recompute r
+                flush_x<NC, BS, VT>(A, R0, R1, pnew, pend);
                 has_pend = false;
 #pragma unroll
-                for (int c = 0; c < NC; ++c) pend[c] = (VT)0;
+                for (int c = 0; c < NC; ++c) pend[c] = 0.0;
                 double dummy[1] = {0.0};
                 if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks)) goto timeout;
                 double ra[3 * NC];
@@ -1086,17 +1077,8 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
                 if (active[c]) gamma[c] = g[c];
             }
             if (!any) {
-                // flush the pending x update of the last iteration (FP64 path)
-                if (!MIXED && has_pend) {
-                    for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) {
-                        const VecN<NC, VT> pv = ldv<NC, VT>(pnew, row);
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) {
-                            double *xp = A.x + (size_t)c * A.n + row;
-                            *xp = __builtin_fma((double)pend[c], (double)pv.v[c], *xp);
-                        }
-                    }
-                }
+                // flush the pending x update of the last iteration
+                if (has_pend) flush_x<NC, BS, VT>(A, R0, R1, pnew, pend);
                 break;
             }
             cur ^= 1;
@@ -1207,7 +1189,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
 
     const size_t vt = mixed ? sizeof(float) : sizeof(double);
     const size_t vec_bytes = ((size_t)n * nc * vt + 255) / 256 * 256;
-    d.work.ensure(vec_bytes * 5);
+    d.work.ensure(vec_bytes * 4);
     d.sync_mem.ensure(sizeof(SyncState) + 256);
     d.cg_res.ensure(2);
 
@@ -1227,7 +1209,6 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
     a.p0 = w + vec_bytes;
     a.p1 = w + 2 * vec_bytes;
     a.q = w + 3 * vec_bytes;
-    a.d = w + 4 * vec_bytes;
     a.rtol = req.rtol;
     a.atol = req.atol;
     a.rdelta = req.reliable_delta;
