@@ -208,7 +208,7 @@ struct CgRequest {
     int max_iter;
     double rtol, atol;
     int precision;
-    double reliable_delta = 0.1;
+    double reliable_delta = 0.5;
     int grid = 0;     // 0 = automatic
 };
 // Runs the persistent CG; fills stats[ncol].  Returns the worst status.
