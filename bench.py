@@ -59,6 +59,37 @@ def cg_bytes(nnz, n, iters, refinements, mixed, columns=1):
     return iters * it + (refinements + 1) * res
 
 
+# HBM bytes per pressure-CG iteration measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (scaled by a k_axpy calibration) on this workload; source:
+# profiles/k_cg_cavity2048_hbm_traffic.md.  roofline.traffic = this figure
+# x the iterations of the launch.
+MEASURED_BYTES_PER_ITERATION = {False: 809.8e6, True: 520.7e6}
+
+
+def spmv_bench(dev, slot, n_rows, n_cols, nnz, reps=100):
+    """Median-free throughput of `reps` back-to-back nsfem_sp_r_mult launches
+    (x ~ U(-1, 1), seed 7): wall time between two device synchronisations,
+    so launch gaps count against the kernel.  Algorithmic bytes per product:
+    (4+8) nnz + 4 (rows+1) + 8 cols (x once) + 8 rows (y)."""
+    import numpy as np
+
+    x = dev.buffer(np.random.default_rng(7).uniform(-1, 1, n_cols))
+    y = dev.buffer(np.zeros(n_rows))
+    for _ in range(3):
+        dev.sp_r_mult(slot, x, y)
+    dev.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        dev.sp_r_mult(slot, x, y)
+    dev.synchronize()
+    sec = (time.perf_counter() - t) / reps
+    x.close()
+    y.close()
+    byt = 12 * nnz + 4 * (n_rows + 1) + 8 * n_cols + 8 * n_rows
+    return {"rows": n_rows, "nnz": nnz, "us": round(sec * 1e6, 1),
+            "achieved_GBs": round(byt / sec / 1e9, 1), "frac": round(byt / sec / 1e9 / PEAK_HBM_GBS, 4)}
+
+
 def run_form(sim, form, warmup, steps, n_p, nnz_p):
     sim.reset()
     sim.configure(precision=form, rtol=1e-8, pair_velocity=True)
@@ -227,8 +258,10 @@ def main():
     p_secs = st["cg_seconds"][2]
     bytes_p = cg_bytes(nnz_p, n_p, p_iters, st["cg_refinements"][2] + args.steps - 1, mixed)
     achieved = bytes_p / max(p_secs, 1e-12) / 1e9
+    traffic = (MEASURED_BYTES_PER_ITERATION[mixed] * p_iters / args.steps
+               if args.n == 2048 else None)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                 "kernel": "k_cg<1,1024,%s> (pressure CG, persistent)" % ("true" if mixed else "false"),
                 "launches": args.steps, "bytes_per_launch": bytes_p / args.steps,
                 "seconds_per_launch": p_secs / args.steps}
@@ -258,10 +291,17 @@ def main():
                      "cg_seconds": [round(x, 4) for x in s["cg_seconds"]],
                      "cg_refinements": s["cg_refinements"],
                      "advect_seconds": round(s["advect_seconds"], 4)}
+    # SpMV micro-benchmark on the assembled pressure and diffusion matrices
+    out["spmv"] = {}
+    for name, slot in (("pressure_stiffness", 0), ("diffusion", 2)):
+        r_, c_, z_ = dev.matrix_info(slot)
+        out["spmv"][name] = spmv_bench(dev, slot, r_, c_, z_)
+        log(f"spmv {name}: {out['spmv'][name]}")
     if "fp64" in results and "fp32_refined" in results:
         out["speedup_vs_fp64"] = round((args.steps / results["fp32_refined"][0]) /
                                        (args.steps / results["fp64"][0]), 3)
     sim.close()
+    dev.close()  # release the device before interpreter teardown
     if not args.no_cpu:
         ref = results.get("fp64", results[chosen])[1]
         its = [x / args.steps for x in ref["cg_iterations"]]

@@ -236,3 +236,14 @@ def test_bench_refuses_several_gpus():
     assert out.returncode == 2
     line = json.loads(out.stdout.strip().splitlines()[-1])
     assert "one GPU" in line.get("error", "") or "single GPU" in line.get("error", "")
+
+
+def test_cpp_device_header_compiles(tmp_path):
+    """include/nsfem_gpu_device.hpp is self-contained C++17 over the C ABI."""
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "nsfem_gpu_device.hpp"\n'
+                   "int main() { nsfem::GPUSimulationDevice d; (void)d; return 0; }\n")
+    out = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                          "-I" + os.path.join(ROOT, "include"), str(src)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
