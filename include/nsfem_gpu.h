@@ -303,6 +303,65 @@ int nsfem_simulation_set_options(nsfem_simulation *sim, const nsfem_sim_options 
  * imposed (include/assembly.h:1503-1507). */
 int nsfem_simulation_reset(nsfem_simulation *sim);
 
+/* ------------------------------------------------------------------ */
+/* Learned warm start of the pressure solve.  No counterpart in the     */
+/* reference (its pressure CG starts from the previous pressure,       */
+/* include/assembly.h:608-615).  A 3-layer message-passing network     */
+/* over the pressure-matrix graph (hidden width 32, BF16 weights and   */
+/* MFMA operands, FP32 accumulation) maps the right-hand side b to an  */
+/* initial guess; CG then solves to the same tolerance as without it.  */
+/* ------------------------------------------------------------------ */
+typedef struct nsfem_gnn nsfem_gnn;
+typedef struct nsfem_gnn_graph nsfem_gnn_graph;
+
+/* Parameters, flat FP32 in this order: W1 [8][32], c1 [32], W2 [64][32],
+ * c2 [32], W3 [64][32], c3 [32], w4 [32], c4 [1]. */
+#define NSFEM_GNN_PARAM_COUNT 4481
+
+/* A network with He-initialised weights (seeded), Adam state zero. */
+int nsfem_gnn_create(nsfem_device *dev, uint64_t seed, nsfem_gnn **out);
+int nsfem_gnn_destroy(nsfem_gnn *gnn);
+int nsfem_gnn_get_params(nsfem_gnn *gnn, float *params);
+/* Replaces the parameters and resets the Adam state. */
+int nsfem_gnn_set_params(nsfem_gnn *gnn, const float *params);
+
+/* The graph of a square pressure matrix (CSR, every row with a positive
+ * diagonal), the area of each node (> 0; the lumped P1 mass) and the node
+ * coordinates (2 floats per node). */
+int nsfem_gnn_graph_create(nsfem_device *dev, int n, int64_t nnz,
+                           const int32_t *row_start, const int32_t *col,
+                           const double *val, const double *area,
+                           const float *xy, nsfem_gnn_graph **out);
+int nsfem_gnn_graph_destroy(nsfem_gnn_graph *graph);
+
+/* p_hat = network(b) on device buffers of graph-n doubles. */
+int nsfem_gnn_predict(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
+                      const nsfem_buffer *b, nsfem_buffer *p_hat);
+/* Loss sum (p_hat - p)^2 / sum p^2 and its gradient (host array of
+ * NSFEM_GNN_PARAM_COUNT floats, nullable); parameters unchanged. */
+int nsfem_gnn_gradients(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
+                        const nsfem_buffer *b, const nsfem_buffer *p,
+                        float *grads, double *loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on one (b, p) pair; the loss
+ * before the step is written to *loss when loss is not NULL (that
+ * synchronises; NULL keeps the step asynchronous). */
+int nsfem_gnn_train_step(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
+                         const nsfem_buffer *b, const nsfem_buffer *p,
+                         float learning_rate, double *loss);
+
+/* Initial guess of a simulation's pressure solve. */
+#define NSFEM_WARM_PREVIOUS 0            /* previous pressure (the reference) */
+#define NSFEM_WARM_ZERO 1                /* zero */
+#define NSFEM_WARM_LEARNED 2             /* network(b) */
+#define NSFEM_WARM_LEARNED_OR_PREVIOUS 3 /* the one with the smaller residual */
+#define NSFEM_WARM_LEARNED_CORRECTION 4  /* p_prev + network(b - A p_prev) if its
+                                            residual is smaller, else p_prev */
+int nsfem_simulation_set_warm_start(nsfem_simulation *sim, int mode, nsfem_gnn *gnn);
+/* Copies the last step's pressure right-hand side and pressure into device
+ * buffers (either may be NULL): training pairs for the network. */
+int nsfem_simulation_copy_pressure_system(nsfem_simulation *sim, nsfem_buffer *rhs,
+                                          nsfem_buffer *p);
+
 #ifdef __cplusplus
 }
 #endif

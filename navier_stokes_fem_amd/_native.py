@@ -26,6 +26,13 @@ STAGNATION = -9
 PRECISION_FP64 = 0
 PRECISION_FP32_REFINED = 1
 
+WARM_PREVIOUS = 0
+WARM_ZERO = 1
+WARM_LEARNED = 2
+WARM_LEARNED_OR_PREVIOUS = 3
+WARM_LEARNED_CORRECTION = 4
+GNN_PARAM_COUNT = 4481
+
 
 class NsfemError(RuntimeError):
     """A failed library call; ``code`` is the nsfem status (EC::ErrorCode
@@ -127,6 +134,17 @@ _SIGNATURES = {
     "nsfem_simulation_set_fields": ([vp, vp, vp, vp], C.c_int),
     "nsfem_simulation_set_options": ([vp, C.POINTER(SimOptions)], C.c_int),
     "nsfem_simulation_reset": ([vp], C.c_int),
+    "nsfem_simulation_set_warm_start": ([vp, C.c_int, vp], C.c_int),
+    "nsfem_simulation_copy_pressure_system": ([vp, vp, vp], C.c_int),
+    "nsfem_gnn_create": ([vp, C.c_uint64, PP], C.c_int),
+    "nsfem_gnn_destroy": ([vp], C.c_int),
+    "nsfem_gnn_get_params": ([vp, vp], C.c_int),
+    "nsfem_gnn_set_params": ([vp, vp], C.c_int),
+    "nsfem_gnn_graph_create": ([vp, C.c_int, i64, vp, vp, vp, vp, vp, PP], C.c_int),
+    "nsfem_gnn_graph_destroy": ([vp], C.c_int),
+    "nsfem_gnn_predict": ([vp, vp, vp, vp], C.c_int),
+    "nsfem_gnn_gradients": ([vp, vp, vp, vp, vp, C.POINTER(dbl)], C.c_int),
+    "nsfem_gnn_train_step": ([vp, vp, vp, vp, flt, C.POINTER(dbl)], C.c_int),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -211,6 +211,16 @@ int nsfem_device_destroy(nsfem_device *h) {
             d.buffers.clear();
             for (Simulation *sm : d.sims) simulation_detach(sm);
             d.sims.clear();
+            for (GnnNet *g : d.gnns) {
+                gnn_release(*g);
+                g->dev = nullptr;
+            }
+            d.gnns.clear();
+            for (GnnGraph *g : d.graphs) {
+                gnn_graph_release(*g);
+                g->dev = nullptr;
+            }
+            d.graphs.clear();
         }
         hipStream_t s = d.stream;
         delete h;  // frees every DeviceArray
@@ -703,6 +713,152 @@ int nsfem_assembly_bc_copy(const nsfem_assembly *a, double *cu, double *cv, doub
         if (cu) std::memcpy(cu, a->impl.c_u.data(), sizeof(double) * a->impl.c_u.size());
         if (cv) std::memcpy(cv, a->impl.c_v.data(), sizeof(double) * a->impl.c_v.size());
         if (cp) std::memcpy(cp, a->impl.c_p.data(), sizeof(double) * a->impl.c_p.size());
+    });
+}
+
+// ------------------------------------------------------ learned warm start
+static GnnNet &net(nsfem_gnn *g) {
+    NSFEM_REQUIRE(g && g->impl, NSFEM_INVALID_ARGUMENT, "null network");
+    NSFEM_REQUIRE(g->impl->dev, NSFEM_INVALID_ARGUMENT, "network belongs to a destroyed device");
+    return *g->impl;
+}
+
+static GnnGraph &graph(nsfem_gnn_graph *g, const GnnNet &n) {
+    NSFEM_REQUIRE(g && g->impl && g->impl->dev, NSFEM_INVALID_ARGUMENT,
+                  "null graph or graph of a destroyed device");
+    NSFEM_REQUIRE(g->impl->dev == n.dev, NSFEM_INVALID_ARGUMENT,
+                  "graph and network live on different devices");
+    return *g->impl;
+}
+
+static const double *vec(const nsfem_buffer *b, const GnnGraph &g, const char *what) {
+    Buffer &bb = buf(b);
+    NSFEM_REQUIRE(bb.dev == g.dev, NSFEM_INVALID_ARGUMENT, std::string(what) + " is on another device");
+    NSFEM_REQUIRE((int64_t)bb.mem.bytes() >= (int64_t)g.n * (int64_t)sizeof(double),
+                  NSFEM_INVALID_ARGUMENT, std::string(what) + " is shorter than the graph");
+    return (const double *)bb.mem.ptr;
+}
+
+int nsfem_gnn_create(nsfem_device *h, uint64_t seed, nsfem_gnn **out) {
+    return guard([&] {
+        NSFEM_REQUIRE(h && out, NSFEM_INVALID_ARGUMENT, "null argument");
+        Device &d = h->impl;
+        std::lock_guard<std::mutex> lk(d.mtx);
+        auto *w = new nsfem_gnn();
+        try {
+            w->impl = gnn_create(d, seed);
+        } catch (...) {
+            delete w;
+            throw;
+        }
+        std::lock_guard<std::mutex> lb(d.buf_mtx);
+        d.gnns.insert(w->impl);
+        *out = w;
+    });
+}
+
+int nsfem_gnn_destroy(nsfem_gnn *g) {
+    return guard([&] {
+        if (!g) return;
+        if (g->impl && g->impl->dev) {
+            Device &d = *g->impl->dev;
+            (void)hipSetDevice(d.index);
+            (void)hipStreamSynchronize(d.stream);
+            std::lock_guard<std::mutex> lk(d.buf_mtx);
+            for (Simulation *s : d.sims) simulation_forget_gnn(s, g->impl);
+            d.gnns.erase(g->impl);
+        }
+        delete g->impl;
+        delete g;
+    });
+}
+
+int nsfem_gnn_get_params(nsfem_gnn *g, float *params) {
+    return guard([&] {
+        GnnNet &n = net(g);
+        NSFEM_REQUIRE(params, NSFEM_INVALID_ARGUMENT, "null params");
+        gnn_get(n, n.params.ptr, params);
+    });
+}
+
+int nsfem_gnn_set_params(nsfem_gnn *g, const float *params) {
+    return guard([&] {
+        GnnNet &n = net(g);
+        NSFEM_REQUIRE(params, NSFEM_INVALID_ARGUMENT, "null params");
+        std::lock_guard<std::mutex> lk(n.dev->mtx);
+        gnn_set_params(n, params);
+    });
+}
+
+int nsfem_gnn_graph_create(nsfem_device *h, int n, int64_t nnz, const int32_t *rs,
+                           const int32_t *col, const double *val, const double *area,
+                           const float *xy, nsfem_gnn_graph **out) {
+    return guard([&] {
+        NSFEM_REQUIRE(h && out, NSFEM_INVALID_ARGUMENT, "null argument");
+        Device &d = h->impl;
+        auto *w = new nsfem_gnn_graph();
+        try {
+            w->impl = gnn_graph_create(d, n, nnz, rs, col, val, area, xy);
+        } catch (...) {
+            delete w;
+            throw;
+        }
+        std::lock_guard<std::mutex> lb(d.buf_mtx);
+        d.graphs.insert(w->impl);
+        *out = w;
+    });
+}
+
+int nsfem_gnn_graph_destroy(nsfem_gnn_graph *g) {
+    return guard([&] {
+        if (!g) return;
+        if (g->impl && g->impl->dev) {
+            Device &d = *g->impl->dev;
+            (void)hipSetDevice(d.index);
+            (void)hipStreamSynchronize(d.stream);
+            std::lock_guard<std::mutex> lk(d.buf_mtx);
+            d.graphs.erase(g->impl);
+        }
+        delete g->impl;
+        delete g;
+    });
+}
+
+int nsfem_gnn_predict(nsfem_gnn *g, nsfem_gnn_graph *gr, const nsfem_buffer *b,
+                      nsfem_buffer *p_hat) {
+    return guard([&] {
+        GnnNet &n = net(g);
+        GnnGraph &G = graph(gr, n);
+        const double *bb = vec(b, G, "b");
+        double *out = const_cast<double *>(vec(p_hat, G, "p_hat"));
+        std::lock_guard<std::mutex> lk(n.dev->mtx);
+        gnn_predict(n, G, bb, out);
+        NSFEM_HIP(hipStreamSynchronize(n.dev->stream));
+    });
+}
+
+int nsfem_gnn_gradients(nsfem_gnn *g, nsfem_gnn_graph *gr, const nsfem_buffer *b,
+                        const nsfem_buffer *p, float *grads, double *loss) {
+    return guard([&] {
+        GnnNet &n = net(g);
+        GnnGraph &G = graph(gr, n);
+        const double *bb = vec(b, G, "b");
+        const double *pp = vec(p, G, "p");
+        std::lock_guard<std::mutex> lk(n.dev->mtx);
+        gnn_gradients(n, G, bb, pp, grads, loss);
+    });
+}
+
+int nsfem_gnn_train_step(nsfem_gnn *g, nsfem_gnn_graph *gr, const nsfem_buffer *b,
+                         const nsfem_buffer *p, float learning_rate, double *loss) {
+    return guard([&] {
+        GnnNet &n = net(g);
+        GnnGraph &G = graph(gr, n);
+        NSFEM_REQUIRE(learning_rate > 0.f, NSFEM_INVALID_ARGUMENT, "learning rate must be > 0");
+        const double *bb = vec(b, G, "b");
+        const double *pp = vec(p, G, "p");
+        std::lock_guard<std::mutex> lk(n.dev->mtx);
+        gnn_train_step(n, G, bb, pp, learning_rate, loss);
     });
 }
 

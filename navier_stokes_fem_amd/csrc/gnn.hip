@@ -1,0 +1,729 @@
+// Learned warm start for the pressure CG: a 3-layer message-passing network
+// over the pressure-matrix graph that maps the divergence right-hand side to
+// an initial guess.  Forward and backward passes, the loss and an Adam step
+// run here, on the GPU; CG still solves to the same tolerance afterwards, so
+// a poor guess costs iterations, never accuracy.  (The reference has no
+// learned component: the pressure solve is warm-started from the previous
+// pressure, include/assembly.h:608-615.)
+//
+// Network (hidden width H = 32):
+//   graph     w_ij = -A_ij / sqrt(A_ii A_jj), j != i (symmetric), a_i = node
+//             area (lumped P1 mass), (X_i, Y_i) = coordinates scaled to the
+//             unit box
+//   input     d_i = b_i / a_i, s = sqrt(mean d^2), x0_i = [d_i/s, X_i, Y_i, 0]
+//   layer l   m = W_hat h, z = [h | m] W_l + c_l, h' = relu(z)
+//   readout   y_i = h3_i . w4 + c4, p_hat_i = s y_i
+//   loss      sum (p_hat - p)^2 / sum p^2
+// Storage: weights FP32 (master) with BF16 copies; [h | m] and dz in BF16 as
+// MFMA operands; z, dX in FP32; v_mfma_f32_16x16x16_bf16 with FP32
+// accumulation for every dense product (forward, dW = X^T dz, dX = dz W^T).
+// Every reduction (sum of d^2, loss, dW, dc) uses per-block partials added
+// in block order, so a given input gives bit-identical results run to run.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "nsfem_internal.hpp"
+
+namespace nsfem {
+namespace gnn {
+
+constexpr int H = 32;
+constexpr int K1 = 16;  // layer-1 operand width: x0 (4) | m0 (4) | zero padding
+constexpr int K2 = 64;  // layers 2 and 3: h (32) | m (32)
+constexpr int OFF_W1 = 0;                // [8][H]
+constexpr int OFF_C1 = OFF_W1 + 8 * H;   // [H]
+constexpr int OFF_W2 = OFF_C1 + H;       // [K2][H]
+constexpr int OFF_C2 = OFF_W2 + K2 * H;
+constexpr int OFF_W3 = OFF_C2 + H;       // [K2][H]
+constexpr int OFF_C3 = OFF_W3 + K2 * H;
+constexpr int OFF_W4 = OFF_C3 + H;       // [H]
+constexpr int OFF_C4 = OFF_W4 + H;       // [1]
+constexpr int NPARAM = OFF_C4 + 1;
+constexpr int kRB = 256;  // threads per block of the elementwise kernels
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint32_t b) { return __uint_as_float((b & 0xFFFFu) << 16); }
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// d = b / area
+__global__ void k_ratio(int n, const double *__restrict__ b, const double *__restrict__ area,
+                        double *__restrict__ d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = b[i] / area[i];
+}
+
+// x0 = [d/s, X, Y, 0] into columns 0..3 of X1; columns 8..15 are padding
+__global__ void k_input(int n, const double *__restrict__ d, const double *__restrict__ sumsq,
+                        const float2 *__restrict__ xy, uint16_t *__restrict__ X1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = sqrt(sumsq[0] / n) + 1e-30;
+    const float2 c = xy[i];
+    uint2 lo, hi;
+    lo.x = (uint32_t)f2bf((float)(d[i] / s)) | ((uint32_t)f2bf(c.x) << 16);
+    lo.y = (uint32_t)f2bf(c.y);
+    hi.x = 0;
+    hi.y = 0;
+    uint2 *row = reinterpret_cast<uint2 *>(X1 + (size_t)i * K1);
+    row[0] = lo;  // columns 0..3
+    row[2] = hi;  // columns 8..11
+    row[3] = hi;  // columns 12..15
+}
+
+// m_i = sum_j w_ij h_j (FP64 sums in CSR order) into columns KS..2KS-1
+template <int KS, int STRIDE>
+__global__ void k_agg_fwd(int n, const int32_t *__restrict__ rs, const int32_t *__restrict__ col,
+                          const double *__restrict__ w, uint16_t *__restrict__ X) {
+    constexpr int CH = KS / 4;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * CH) return;
+    const int i = (int)(t / CH), c = (int)(t % CH);
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int k = rs[i]; k < rs[i + 1]; ++k) {
+        const int j = col[k];
+        const double wk = w[k];
+        const uint2 v = *reinterpret_cast<const uint2 *>(X + (size_t)j * STRIDE + 4 * c);
+        a0 += wk * (double)bf2f(v.x);
+        a1 += wk * (double)bf2f(v.x >> 16);
+        a2 += wk * (double)bf2f(v.y);
+        a3 += wk * (double)bf2f(v.y >> 16);
+    }
+    uint2 o;
+    o.x = (uint32_t)f2bf((float)a0) | ((uint32_t)f2bf((float)a1) << 16);
+    o.y = (uint32_t)f2bf((float)a2) | ((uint32_t)f2bf((float)a3) << 16);
+    *reinterpret_cast<uint2 *>(X + (size_t)i * STRIDE + KS + 4 * c) = o;
+}
+
+// z = X W + c, h = relu(z): one wave per 16 rows, two 16x16 output tiles.
+// MFMA 16x16x16 operand layout: lane l holds A[l%16][4(l/16)+i],
+// B[4(l/16)+i][l%16]; the result D[4(l/16)+j][l%16].
+template <int KP>
+__global__ __launch_bounds__(kRB) void k_dense_fwd(int n, const uint16_t *__restrict__ X,
+                                                   const uint16_t *__restrict__ Wt,
+                                                   const float *__restrict__ c,
+                                                   float *__restrict__ z, uint16_t *__restrict__ hout,
+                                                   int hstride) {
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6) * 16;
+    if (r0 >= n) return;  // wave-uniform
+    const int arow = (int)min((int64_t)n - 1, r0 + (lane & 15));
+    const int kq = 4 * (lane >> 4);
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KP / 16; ++ks) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(X + (size_t)arow * KP + 16 * ks + kq);
+        const bf16x4 b0 =
+            *reinterpret_cast<const bf16x4 *>(Wt + (size_t)(lane & 15) * KP + 16 * ks + kq);
+        const bf16x4 b1 =
+            *reinterpret_cast<const bf16x4 *>(Wt + (size_t)(16 + (lane & 15)) * KP + 16 * ks + kq);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1, acc1, 0, 0, 0);
+    }
+    const int col = lane & 15;
+    const float c0 = c[col], c1 = c[col + 16];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t row = r0 + kq + j;
+        if (row < n) {
+            const float v0 = acc0[j] + c0, v1 = acc1[j] + c1;
+            if (z) {
+                z[row * H + col] = v0;
+                z[row * H + col + 16] = v1;
+            }
+            hout[row * hstride + col] = f2bf(fmaxf(v0, 0.f));
+            hout[row * hstride + col + 16] = f2bf(fmaxf(v1, 0.f));
+        }
+    }
+}
+
+// y = h3 . w4 + c4 (FP64 sum, rounded to FP32), p_hat = s y
+__global__ void k_readout(int n, const uint16_t *__restrict__ h3, const float *__restrict__ prm,
+                          const double *__restrict__ sumsq, double *__restrict__ p_hat) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = sqrt(sumsq[0] / n) + 1e-30;
+    const uint4 *row = reinterpret_cast<const uint4 *>(h3 + (size_t)i * H);
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = row[q];
+        const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            acc += (double)bf2f(wv[e]) * (double)prm[OFF_W4 + 8 * q + 2 * e];
+            acc += (double)bf2f(wv[e] >> 16) * (double)prm[OFF_W4 + 8 * q + 2 * e + 1];
+        }
+    }
+    const float y = (float)(acc + (double)prm[OFF_C4]);
+    p_hat[i] = s * (double)y;
+}
+
+// block sum of K doubles into part[blockIdx.x * K + k]
+template <int K>
+__device__ void block_partial(double (&v)[K], double *part) {
+    __shared__ double red[K][kRB / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) red[k][wv] = s;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        double s = 0.0;
+        for (int w = 0; w < kRB / 64; ++w) s += red[k][w];
+        part[(size_t)blockIdx.x * K + k] = s;
+    }
+}
+
+// dy = 2 s (p_hat - p) / sum p^2 (FP32); partial sums of (p_hat - p)^2
+__global__ __launch_bounds__(kRB) void k_loss(int n, const double *__restrict__ p_hat,
+                                              const double *__restrict__ p,
+                                              const double *__restrict__ scal,
+                                              float *__restrict__ dy, double *__restrict__ part) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double s = sqrt(scal[0] / n) + 1e-30;
+    const double den = scal[1] + 1e-300;
+    double v[1] = {0.0};
+    if (i < n) {
+        const double diff = p_hat[i] - p[i];
+        v[0] = diff * diff;
+        dy[i] = (float)(2.0 * s * diff / den);
+    }
+    block_partial<1>(v, part);
+}
+
+// dh3 = dy w4 (FP32), dz3 = bf16(dh3 [z3 > 0]); partials of h3 dy and dy
+__global__ __launch_bounds__(kRB) void k_readout_bwd(int n, const uint16_t *__restrict__ h3,
+                                                     const float *__restrict__ dy,
+                                                     const float *__restrict__ prm,
+                                                     const float *__restrict__ z3,
+                                                     uint16_t *__restrict__ dz3,
+                                                     double *__restrict__ part) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[H + 1];
+#pragma unroll
+    for (int k = 0; k <= H; ++k) v[k] = 0.0;
+    if (i < n) {
+        const float d = dy[i];
+#pragma unroll
+        for (int f = 0; f < H; ++f) {
+            const size_t e = (size_t)i * H + f;
+            v[f] = (double)bf2f(h3[e]) * (double)d;
+            const float dh = (float)((double)d * (double)prm[OFF_W4 + f]);
+            dz3[e] = z3[e] > 0.f ? f2bf(dh) : (uint16_t)0;
+        }
+        v[H] = (double)d;
+    }
+    block_partial<H + 1>(v, part);
+}
+
+// Partials of dW = X^T dz ([KP][H]) and dc = sum dz ([H]) per block:
+// part[block][KP * H + H].  Waves stride over 16-node groups; rows past n
+// contribute zeros.
+template <int KP>
+__global__ __launch_bounds__(kRB) void k_dense_bwd_w(int n, const uint16_t *__restrict__ X,
+                                                     const uint16_t *__restrict__ dz,
+                                                     double *__restrict__ part) {
+    constexpr int MT = KP / 16;
+    constexpr int NW = kRB / 64;
+    __shared__ float lds[NW][KP * H];
+    __shared__ float ldc[NW][H];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15;
+    f32x4 acc[MT][2];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float dc0 = 0.f, dc1 = 0.f;
+    const int64_t groups = ((int64_t)n + 15) / 16;
+    for (int64_t g = (int64_t)blockIdx.x * NW + wave; g < groups; g += (int64_t)gridDim.x * NW) {
+        const int64_t base = g * 16 + 4 * (lane >> 4);
+        bf16x4 b0, b1;
+        int64_t nodec[4];
+        bool ok[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ok[i] = base + i < n;
+            nodec[i] = ok[i] ? base + i : (int64_t)n - 1;
+            const uint16_t v0 = dz[nodec[i] * H + col];
+            const uint16_t v1 = dz[nodec[i] * H + 16 + col];
+            b0[i] = ok[i] ? (short)v0 : (short)0;
+            b1[i] = ok[i] ? (short)v1 : (short)0;
+            dc0 += ok[i] ? bf2f(v0) : 0.f;
+            dc1 += ok[i] ? bf2f(v1) : 0.f;
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            bf16x4 a;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint16_t xv = X[nodec[i] * KP + 16 * mt + col];
+                a[i] = ok[i] ? (short)xv : (short)0;
+            }
+            acc[mt][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b0, acc[mt][0], 0, 0, 0);
+            acc[mt][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1, acc[mt][1], 0, 0, 0);
+        }
+    }
+    // lanes l, l+16, l+32, l+48 hold the same columns
+    dc0 += __shfl_xor(dc0, 16, 64);
+    dc0 += __shfl_xor(dc0, 32, 64);
+    dc1 += __shfl_xor(dc1, 16, 64);
+    dc1 += __shfl_xor(dc1, 32, 64);
+    const int kq = 4 * (lane >> 4);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) lds[wave][(16 * mt + kq + j) * H + 16 * nt + col] = acc[mt][nt][j];
+    if (lane < 16) {
+        ldc[wave][col] = dc0;
+        ldc[wave][col + 16] = dc1;
+    }
+    __syncthreads();
+    double *out = part + (size_t)blockIdx.x * (KP * H + H);
+    for (int e = threadIdx.x; e < KP * H + H; e += blockDim.x) {
+        double s = 0.0;
+        for (int w = 0; w < NW; ++w) s += e < KP * H ? (double)lds[w][e] : (double)ldc[w][e - KP * H];
+        out[e] = s;
+    }
+}
+
+// dX = dz W^T: one wave per 16 rows, KP/16 output tiles, K = H
+template <int KP>
+__global__ __launch_bounds__(kRB) void k_dense_bwd_x(int n, const uint16_t *__restrict__ dz,
+                                                     const uint16_t *__restrict__ Wr,
+                                                     float *__restrict__ dX) {
+    constexpr int NT = KP / 16;
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6) * 16;
+    if (r0 >= n) return;
+    const int arow = (int)min((int64_t)n - 1, r0 + (lane & 15));
+    const int kq = 4 * (lane >> 4);
+    const int col = lane & 15;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < H / 16; ++ks) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(dz + (size_t)arow * H + 16 * ks + kq);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const bf16x4 b =
+                *reinterpret_cast<const bf16x4 *>(Wr + (size_t)(16 * nt + col) * H + 16 * ks + kq);
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t row = r0 + kq + j;
+        if (row < n) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) dX[row * KP + 16 * nt + col] = acc[nt][j];
+        }
+    }
+}
+
+// dh = dX[:, :H] + W_hat dX[:, H:] (FP64 sums in CSR order, rounded to FP32),
+// dz_prev = bf16(dh [z_prev > 0]); one thread per (node, 4 features)
+__global__ void k_agg_bwd(int n, const int32_t *__restrict__ rs, const int32_t *__restrict__ col,
+                          const double *__restrict__ w, const float *__restrict__ dX,
+                          const float *__restrict__ zp, uint16_t *__restrict__ dzp) {
+    constexpr int CH = H / 4;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * CH) return;
+    const int i = (int)(t / CH), c = (int)(t % CH);
+    double a[4] = {0, 0, 0, 0};
+    for (int k = rs[i]; k < rs[i + 1]; ++k) {
+        const int j = col[k];
+        const double wk = w[k];
+        const float4 v = *reinterpret_cast<const float4 *>(dX + (size_t)j * K2 + H + 4 * c);
+        a[0] += wk * (double)v.x;
+        a[1] += wk * (double)v.y;
+        a[2] += wk * (double)v.z;
+        a[3] += wk * (double)v.w;
+    }
+    const float4 own = *reinterpret_cast<const float4 *>(dX + (size_t)i * K2 + 4 * c);
+    const float4 z = *reinterpret_cast<const float4 *>(zp + (size_t)i * H + 4 * c);
+    const float dh[4] = {(float)((double)own.x + a[0]), (float)((double)own.y + a[1]),
+                         (float)((double)own.z + a[2]), (float)((double)own.w + a[3])};
+    const float zz[4] = {z.x, z.y, z.z, z.w};
+    uint2 o;
+    uint16_t r[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) r[f] = zz[f] > 0.f ? f2bf(dh[f]) : (uint16_t)0;
+    o.x = (uint32_t)r[0] | ((uint32_t)r[1] << 16);
+    o.y = (uint32_t)r[2] | ((uint32_t)r[3] << 16);
+    *reinterpret_cast<uint2 *>(dzp + (size_t)i * H + 4 * c) = o;
+}
+
+// out[k] (+)= sum_b part[b][k], blocks in order (FP64), k < count
+__global__ void k_reduce(int nb, int count, const double *__restrict__ part, float *__restrict__ out,
+                         double *__restrict__ out64) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[(size_t)b * count + k];
+    if (out) out[k] = (float)s;
+    if (out64) out64[k] = s;
+}
+
+// grads of one dense layer from its partial buffer [nb][KP*H + H]:
+// rows < kin of dW, then dc
+__global__ void k_reduce_layer(int nb, int kp, int kin, const double *__restrict__ part,
+                               float *__restrict__ gW, float *__restrict__ gc) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int count = kp * H + H;
+    if (e >= count) return;
+    const bool is_w = e < kp * H;
+    if (is_w && e / H >= kin) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[(size_t)b * count + e];
+    if (is_w) gW[e] = (float)s;
+    else gc[e - kp * H] = (float)s;
+}
+
+__global__ void k_adam(int count, float *__restrict__ w, const float *__restrict__ g,
+                       float *__restrict__ m, float *__restrict__ v, float lr, float b1, float b2,
+                       float eps, float bc1, float bc2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.f - b1) * gk;
+    const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    w[k] -= lr * (mk / bc1) / (sqrtf(vk / bc2) + eps);
+}
+
+// BF16 copies: Wt[o][k] (forward operand, K padded) and Wr[k][o] (dX operand)
+__global__ void k_pack(const float *__restrict__ prm, uint16_t *wt1, uint16_t *wt2,
+                       uint16_t *wr2, uint16_t *wt3, uint16_t *wr3) {
+    for (int e = threadIdx.x; e < H * K1; e += blockDim.x) {
+        const int o = e / K1, k = e % K1;
+        wt1[e] = k < 8 ? f2bf(prm[OFF_W1 + k * H + o]) : (uint16_t)0;
+    }
+    for (int e = threadIdx.x; e < H * K2; e += blockDim.x) {
+        const int o = e / K2, k = e % K2;
+        wt2[e] = f2bf(prm[OFF_W2 + k * H + o]);
+        wt3[e] = f2bf(prm[OFF_W3 + k * H + o]);
+        wr2[e] = f2bf(prm[OFF_W2 + e]);
+        wr3[e] = f2bf(prm[OFF_W3 + e]);
+    }
+}
+
+// copy src into dst where ||b - A src|| < ||b - A dst||: scal[0] = the
+// squared residual of dst, scal[1] that of src
+__global__ void k_select(int n, const double *__restrict__ scal, const double *__restrict__ src,
+                         double *__restrict__ dst) {
+    if (!(scal[1] < scal[0])) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+inline int blocks_for(int64_t work, int bs = kRB) { return (int)((work + bs - 1) / bs); }
+
+}  // namespace gnn
+
+// ------------------------------------------------------------------- host
+namespace {
+
+void gnn_pack(GnnNet &g) {
+    Device &d = *g.dev;
+    gnn::k_pack<<<1, 256, 0, d.stream>>>(g.params.ptr, g.wt1.ptr, g.wt2.ptr, g.wr2.ptr, g.wt3.ptr,
+                                         g.wr3.ptr);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void gnn_reserve(GnnNet &g, int n) {
+    if (n <= g.cap) return;
+    using namespace gnn;
+    g.X1.resize((size_t)n * K1);
+    g.X2.resize((size_t)n * K2);
+    g.X3.resize((size_t)n * K2);
+    g.h3.resize((size_t)n * H);
+    g.dz.resize((size_t)n * H);
+    g.z1.resize((size_t)n * H);
+    g.z2.resize((size_t)n * H);
+    g.z3.resize((size_t)n * H);
+    g.dX.resize((size_t)n * K2);
+    g.dy.resize(n);
+    g.d.resize(n);
+    g.phat.resize(n);
+    const size_t nb = (size_t)blocks_for(n);
+    const size_t groups = ((size_t)n + 15) / 16;
+    const size_t nbw = std::max<size_t>(1, std::min<size_t>(256, (groups + 3) / 4));
+    g.part.resize(std::max(nb * (H + 1), nbw * (K2 * H + H)));
+    g.cap = n;
+}
+
+int bwd_w_blocks(int n) {
+    const int64_t groups = ((int64_t)n + 15) / 16;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(256, (groups + 3) / 4));
+}
+
+// forward pass; stores z for the backward pass when `train`.  p_hat ==
+// nullptr writes the prediction to the workspace (g.phat), resolved after
+// the workspace has been sized: a pointer taken before gnn_reserve may be
+// null or freed.
+void gnn_forward(GnnNet &g, GnnGraph &gr, const double *b, double *p_hat, bool train) {
+    using namespace gnn;
+    Device &d = *g.dev;
+    const int n = gr.n;
+    gnn_reserve(g, n);
+    if (!p_hat) p_hat = g.phat.ptr;
+    const int nb = blocks_for(n);
+    g.scal.ensure(4);
+    launch_fill(d, 4, 0.0, g.scal.ptr);
+    k_ratio<<<nb, kRB, 0, d.stream>>>(n, b, gr.area.ptr, g.d.ptr);
+    launch_dot_accumulate(d, n, g.d.ptr, g.d.ptr, g.scal.ptr);
+    k_input<<<nb, kRB, 0, d.stream>>>(n, g.d.ptr, g.scal.ptr, gr.xy.ptr, g.X1.ptr);
+    k_agg_fwd<4, K1><<<blocks_for((int64_t)n * 1), kRB, 0, d.stream>>>(n, gr.rs.ptr, gr.col.ptr,
+                                                                       gr.w.ptr, g.X1.ptr);
+    const int nbt = blocks_for(((int64_t)n + 15) / 16 * 64);  // one wave per 16 rows
+    k_dense_fwd<K1><<<nbt, kRB, 0, d.stream>>>(n, g.X1.ptr, g.wt1.ptr, g.params.ptr + OFF_C1,
+                                              train ? g.z1.ptr : nullptr, g.X2.ptr, K2);
+    k_agg_fwd<H, K2><<<blocks_for((int64_t)n * (H / 4)), kRB, 0, d.stream>>>(
+        n, gr.rs.ptr, gr.col.ptr, gr.w.ptr, g.X2.ptr);
+    k_dense_fwd<K2><<<nbt, kRB, 0, d.stream>>>(n, g.X2.ptr, g.wt2.ptr, g.params.ptr + OFF_C2,
+                                              train ? g.z2.ptr : nullptr, g.X3.ptr, K2);
+    k_agg_fwd<H, K2><<<blocks_for((int64_t)n * (H / 4)), kRB, 0, d.stream>>>(
+        n, gr.rs.ptr, gr.col.ptr, gr.w.ptr, g.X3.ptr);
+    k_dense_fwd<K2><<<nbt, kRB, 0, d.stream>>>(n, g.X3.ptr, g.wt3.ptr, g.params.ptr + OFF_C3,
+                                              train ? g.z3.ptr : nullptr, g.h3.ptr, H);
+    k_readout<<<nb, kRB, 0, d.stream>>>(n, g.h3.ptr, g.params.ptr, g.scal.ptr, p_hat);
+    NSFEM_HIP(hipGetLastError());
+}
+
+// forward + loss + backward into g.grads; the loss lands in scal[2]
+void gnn_backward(GnnNet &g, GnnGraph &gr, const double *b, const double *p) {
+    using namespace gnn;
+    Device &d = *g.dev;
+    const int n = gr.n;
+    gnn_forward(g, gr, b, nullptr, true);  // prediction into g.phat
+    const int nb = blocks_for(n);
+    launch_dot_accumulate(d, n, p, p, g.scal.ptr + 1);
+    k_loss<<<nb, kRB, 0, d.stream>>>(n, g.phat.ptr, p, g.scal.ptr, g.dy.ptr, g.part.ptr);
+    k_reduce<<<1, 64, 0, d.stream>>>(nb, 1, g.part.ptr, nullptr, g.scal.ptr + 2);
+    k_readout_bwd<<<nb, kRB, 0, d.stream>>>(n, g.h3.ptr, g.dy.ptr, g.params.ptr, g.z3.ptr, g.dz.ptr,
+                                            g.part.ptr);
+    k_reduce<<<1, 64, 0, d.stream>>>(nb, H + 1, g.part.ptr, g.grads.ptr + OFF_W4, nullptr);
+    const int nbw = bwd_w_blocks(n);
+    const int nbt = blocks_for(((int64_t)n + 15) / 16 * 64);
+    // layer 3
+    k_dense_bwd_w<K2><<<nbw, kRB, 0, d.stream>>>(n, g.X3.ptr, g.dz.ptr, g.part.ptr);
+    k_reduce_layer<<<blocks_for(K2 * H + H), kRB, 0, d.stream>>>(
+        nbw, K2, K2, g.part.ptr, g.grads.ptr + OFF_W3, g.grads.ptr + OFF_C3);
+    k_dense_bwd_x<K2><<<nbt, kRB, 0, d.stream>>>(n, g.dz.ptr, g.wr3.ptr, g.dX.ptr);
+    k_agg_bwd<<<blocks_for((int64_t)n * (H / 4)), kRB, 0, d.stream>>>(n, gr.rs.ptr, gr.col.ptr,
+                                                                      gr.w.ptr, g.dX.ptr, g.z2.ptr,
+                                                                      g.dz.ptr);
+    // layer 2
+    k_dense_bwd_w<K2><<<nbw, kRB, 0, d.stream>>>(n, g.X2.ptr, g.dz.ptr, g.part.ptr);
+    k_reduce_layer<<<blocks_for(K2 * H + H), kRB, 0, d.stream>>>(
+        nbw, K2, K2, g.part.ptr, g.grads.ptr + OFF_W2, g.grads.ptr + OFF_C2);
+    k_dense_bwd_x<K2><<<nbt, kRB, 0, d.stream>>>(n, g.dz.ptr, g.wr2.ptr, g.dX.ptr);
+    k_agg_bwd<<<blocks_for((int64_t)n * (H / 4)), kRB, 0, d.stream>>>(n, gr.rs.ptr, gr.col.ptr,
+                                                                      gr.w.ptr, g.dX.ptr, g.z1.ptr,
+                                                                      g.dz.ptr);
+    // layer 1 (8 real input rows of the 16-row padded operand)
+    k_dense_bwd_w<K1><<<nbw, kRB, 0, d.stream>>>(n, g.X1.ptr, g.dz.ptr, g.part.ptr);
+    k_reduce_layer<<<blocks_for(K1 * H + H), kRB, 0, d.stream>>>(
+        nbw, K1, 8, g.part.ptr, g.grads.ptr + OFF_W1, g.grads.ptr + OFF_C1);
+    NSFEM_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+GnnNet *gnn_create(Device &d, uint64_t seed) {
+    using namespace gnn;
+    auto *g = new GnnNet();
+    g->dev = &d;
+    d.set_current();
+    std::vector<float> h(NPARAM, 0.f);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    auto fill = [&](int off, int rows, int cols) {
+        const double sc = std::sqrt(2.0 / rows);
+        for (int e = 0; e < rows * cols; ++e) h[off + e] = (float)(nd(rng) * sc);
+    };
+    fill(OFF_W1, 8, H);
+    fill(OFF_W2, K2, H);
+    fill(OFF_W3, K2, H);
+    // the readout starts at zero: an untrained network predicts 0 (loss 1),
+    // and training only moves it from there
+    for (int e = 0; e < H; ++e) h[OFF_W4 + e] = 0.f;
+    g->params.resize(NPARAM);
+    g->grads.resize(NPARAM);
+    g->m.resize(NPARAM);
+    g->v.resize(NPARAM);
+    g->wt1.resize(H * K1);
+    g->wt2.resize(H * K2);
+    g->wr2.resize(H * K2);
+    g->wt3.resize(H * K2);
+    g->wr3.resize(H * K2);
+    NSFEM_HIP(hipMemcpy(g->params.ptr, h.data(), NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    NSFEM_HIP(hipMemset(g->grads.ptr, 0, NPARAM * sizeof(float)));
+    NSFEM_HIP(hipMemset(g->m.ptr, 0, NPARAM * sizeof(float)));
+    NSFEM_HIP(hipMemset(g->v.ptr, 0, NPARAM * sizeof(float)));
+    gnn_pack(*g);
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    return g;
+}
+
+void gnn_set_params(GnnNet &g, const float *h) {
+    using namespace gnn;
+    Device &d = *g.dev;
+    d.set_current();
+    NSFEM_HIP(hipMemcpyAsync(g.params.ptr, h, NPARAM * sizeof(float), hipMemcpyHostToDevice,
+                             d.stream));
+    NSFEM_HIP(hipMemsetAsync(g.m.ptr, 0, NPARAM * sizeof(float), d.stream));
+    NSFEM_HIP(hipMemsetAsync(g.v.ptr, 0, NPARAM * sizeof(float), d.stream));
+    g.adam_t = 0;
+    gnn_pack(g);
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+}
+
+void gnn_get(GnnNet &g, const float *src, float *h) {
+    Device &d = *g.dev;
+    d.set_current();
+    NSFEM_HIP(hipMemcpyAsync(h, src, gnn::NPARAM * sizeof(float), hipMemcpyDeviceToHost, d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+}
+
+// Graph of a pressure matrix: CSR without the diagonal, symmetric weights.
+GnnGraph *gnn_graph_create(Device &d, int n, int64_t nnz, const int32_t *rs, const int32_t *col,
+                           const double *val, const double *area, const float *xy) {
+    NSFEM_REQUIRE(n > 0 && rs && col && val && area && xy, NSFEM_INVALID_ARGUMENT,
+                  "gnn graph: empty or null input");
+    NSFEM_REQUIRE(rs[0] == 0 && rs[n] == nnz, NSFEM_INVALID_ARGUMENT, "gnn graph: bad row_start");
+    std::vector<double> diag(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        NSFEM_REQUIRE(rs[i + 1] >= rs[i], NSFEM_INVALID_ARGUMENT, "gnn graph: bad row_start");
+        for (int k = rs[i]; k < rs[i + 1]; ++k) {
+            NSFEM_REQUIRE(col[k] >= 0 && col[k] < n, NSFEM_INVALID_ARGUMENT,
+                          "gnn graph: matrix must be square, column out of range");
+            if (col[k] == i) diag[i] = val[k];
+        }
+        NSFEM_REQUIRE(diag[i] > 0 && area[i] > 0, NSFEM_INVALID_ARGUMENT,
+                      "gnn graph: every row needs a positive diagonal and a positive area");
+    }
+    std::vector<int32_t> grs(n + 1, 0), gcol;
+    std::vector<double> gw;
+    gcol.reserve(nnz);
+    gw.reserve(nnz);
+    for (int i = 0; i < n; ++i) {
+        for (int k = rs[i]; k < rs[i + 1]; ++k) {
+            if (col[k] == i) continue;
+            gcol.push_back(col[k]);
+            gw.push_back(-val[k] / std::sqrt(diag[i] * diag[col[k]]));
+        }
+        grs[i + 1] = (int32_t)gcol.size();
+    }
+    double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 2; ++a) {
+            lo[a] = std::min(lo[a], (double)xy[2 * i + a]);
+            hi[a] = std::max(hi[a], (double)xy[2 * i + a]);
+        }
+    const double ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), 1e-30);
+    std::vector<float2> nx(n);
+    for (int i = 0; i < n; ++i)
+        nx[i] = make_float2((float)(((double)xy[2 * i] - lo[0]) / ext),
+                            (float)(((double)xy[2 * i + 1] - lo[1]) / ext));
+    auto *g = new GnnGraph();
+    g->dev = &d;
+    g->n = n;
+    g->ne = (int64_t)gcol.size();
+    d.set_current();
+    g->rs.resize(n + 1);
+    g->col.resize(std::max<size_t>(1, gcol.size()));
+    g->w.resize(std::max<size_t>(1, gw.size()));
+    g->area.resize(n);
+    g->xy.resize(n);
+    NSFEM_HIP(hipMemcpy(g->rs.ptr, grs.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!gcol.empty()) {
+        NSFEM_HIP(hipMemcpy(g->col.ptr, gcol.data(), gcol.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice));
+        NSFEM_HIP(hipMemcpy(g->w.ptr, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    NSFEM_HIP(hipMemcpy(g->area.ptr, area, n * sizeof(double), hipMemcpyHostToDevice));
+    NSFEM_HIP(hipMemcpy(g->xy.ptr, nx.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    return g;
+}
+
+void gnn_predict(GnnNet &g, GnnGraph &gr, const double *b, double *p_hat) {
+    g.dev->set_current();
+    gnn_forward(g, gr, b, p_hat, false);
+}
+
+// gradients (device) and the loss; host copies when requested
+void gnn_gradients(GnnNet &g, GnnGraph &gr, const double *b, const double *p, float *host_grads,
+                   double *loss) {
+    Device &d = *g.dev;
+    d.set_current();
+    gnn_backward(g, gr, b, p);
+    if (host_grads)
+        NSFEM_HIP(hipMemcpyAsync(host_grads, g.grads.ptr, gnn::NPARAM * sizeof(float),
+                                 hipMemcpyDeviceToHost, d.stream));
+    double h[3] = {0, 0, 0};
+    if (loss)
+        NSFEM_HIP(hipMemcpyAsync(h, g.scal.ptr, 3 * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    if (loss) *loss = h[2] / (h[1] + 1e-300);
+}
+
+void gnn_train_step(GnnNet &g, GnnGraph &gr, const double *b, const double *p, float lr,
+                    double *loss) {
+    Device &d = *g.dev;
+    d.set_current();
+    gnn_backward(g, gr, b, p);
+    ++g.adam_t;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float bc1 = 1.f - std::pow(b1, (float)g.adam_t), bc2 = 1.f - std::pow(b2, (float)g.adam_t);
+    gnn::k_adam<<<gnn::blocks_for(gnn::NPARAM), gnn::kRB, 0, d.stream>>>(
+        gnn::NPARAM, g.params.ptr, g.grads.ptr, g.m.ptr, g.v.ptr, lr, b1, b2, 1e-8f, bc1, bc2);
+    NSFEM_HIP(hipGetLastError());
+    gnn_pack(g);
+    if (loss) {
+        double h[3];
+        NSFEM_HIP(hipMemcpyAsync(h, g.scal.ptr, 3 * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+        NSFEM_HIP(hipStreamSynchronize(d.stream));
+        *loss = h[2] / (h[1] + 1e-300);
+    }
+}
+
+// dst = src where src has the smaller residual ||b - A x|| (device-side,
+// no host synchronisation); tmp: n doubles of scratch
+void select_better_guess(Device &d, Matrix &A, const double *b, const double *src, double *dst,
+                         double *tmp, double *scal2) {
+    const int n = A.rows;
+    launch_fill(d, 2, 0.0, scal2);
+    launch_spmv(d, A, dst, b, tmp, 1);
+    launch_dot_accumulate(d, n, tmp, tmp, scal2);
+    launch_spmv(d, A, src, b, tmp, 1);
+    launch_dot_accumulate(d, n, tmp, tmp, scal2 + 1);
+    gnn::k_select<<<gnn::blocks_for(n), gnn::kRB, 0, d.stream>>>(n, scal2, src, dst);
+    NSFEM_HIP(hipGetLastError());
+}
+
+}  // namespace nsfem

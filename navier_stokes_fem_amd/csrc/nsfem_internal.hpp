@@ -154,6 +154,9 @@ struct CgColumnResult {
 
 struct Simulation;
 void simulation_detach(Simulation *s);  // frees its device memory, clears its device
+struct GnnNet;
+struct GnnGraph;
+void simulation_forget_gnn(Simulation *s, GnnNet *g);  // a network is being destroyed
 
 struct Device {
     int index = 0;
@@ -177,6 +180,8 @@ struct Device {
     // memory and detaches them, so a later nsfem_buffer_destroy stays safe
     std::set<struct Buffer *> buffers;
     std::set<Simulation *> sims;  // detached the same way
+    std::set<GnnNet *> gnns;
+    std::set<GnnGraph *> graphs;
     std::mutex buf_mtx;
     void set_current() const { NSFEM_HIP(hipSetDevice(index)); }
 };
@@ -266,6 +271,56 @@ void upload_tree(Device &d, int32_t tree_nodes, const uint32_t *words, int64_t l
                  const int32_t *elems);
 void launch_fill(Device &d, int n, double v, double *dst);
 
+// ---------------------------------------------------- learned warm start
+// Pressure-matrix graph (CSR without the diagonal, symmetric weights
+// -A_ij / sqrt(A_ii A_jj)), node areas and coordinates scaled to the unit box.
+struct GnnGraph {
+    Device *dev = nullptr;
+    int n = 0;
+    int64_t ne = 0;
+    DeviceArray<int32_t> rs, col;
+    DeviceArray<double> w, area;
+    DeviceArray<float2> xy;
+};
+
+// Network parameters (FP32 master + BF16 copies), Adam state, workspace.
+struct GnnNet {
+    Device *dev = nullptr;
+    DeviceArray<float> params, grads, m, v;
+    DeviceArray<uint16_t> wt1, wt2, wr2, wt3, wr3;
+    int adam_t = 0;
+    int cap = 0;  // workspace rows
+    DeviceArray<uint16_t> X1, X2, X3, h3, dz;
+    DeviceArray<float> z1, z2, z3, dX, dy;
+    DeviceArray<double> d, phat, scal, part;
+};
+
+GnnNet *gnn_create(Device &d, uint64_t seed);
+void gnn_set_params(GnnNet &g, const float *host);
+void gnn_get(GnnNet &g, const float *dev_src, float *host);
+GnnGraph *gnn_graph_create(Device &d, int n, int64_t nnz, const int32_t *rs, const int32_t *col,
+                           const double *val, const double *area, const float *xy);
+void gnn_predict(GnnNet &g, GnnGraph &gr, const double *b, double *p_hat);
+void gnn_gradients(GnnNet &g, GnnGraph &gr, const double *b, const double *p, float *host_grads,
+                   double *loss);
+void gnn_train_step(GnnNet &g, GnnGraph &gr, const double *b, const double *p, float lr,
+                    double *loss);
+// dst = src if src has the smaller residual ||b - A x|| (no host sync);
+// tmp: A.rows doubles, scal2: 2 doubles
+void select_better_guess(Device &d, Matrix &A, const double *b, const double *src, double *dst,
+                         double *tmp, double *scal2);
+inline void gnn_release(GnnNet &g) {
+    g.params.release(); g.grads.release(); g.m.release(); g.v.release();
+    g.wt1.release(); g.wt2.release(); g.wr2.release(); g.wt3.release(); g.wr3.release();
+    g.X1.release(); g.X2.release(); g.X3.release(); g.h3.release(); g.dz.release();
+    g.z1.release(); g.z2.release(); g.z3.release(); g.dX.release(); g.dy.release();
+    g.d.release(); g.phat.release(); g.scal.release(); g.part.release();
+    g.cap = 0;
+}
+inline void gnn_graph_release(GnnGraph &g) {
+    g.rs.release(); g.col.release(); g.w.release(); g.area.release(); g.xy.release();
+}
+
 }  // namespace nsfem
 
 struct nsfem_device {
@@ -279,4 +334,10 @@ struct nsfem_kdtree {
 };
 struct nsfem_assembly {
     nsfem::HostAssembly impl;
+};
+struct nsfem_gnn {
+    nsfem::GnnNet *impl;
+};
+struct nsfem_gnn_graph {
+    nsfem::GnnGraph *impl;
 };

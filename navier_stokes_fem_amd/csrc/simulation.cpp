@@ -14,6 +14,7 @@
 // pair_velocity they run as one 2-column CG (same iterates per column,
 // the matrix is read once for both).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <unordered_map>
 #include <vector>
@@ -33,6 +34,13 @@ struct Simulation {
     DeviceArray<int32_t> vb_idx, pb_idx;
     DeviceArray<double> vb_u, vb_v, pb_val;
     int nvb = 0, npb = 0;
+    // initial guess of the pressure solve (NSFEM_WARM_*)
+    int warm_mode = NSFEM_WARM_PREVIOUS;
+    GnnNet *gnn = nullptr;      // not owned
+    GnnGraph *graph = nullptr;  // owned, built on first use of a learned mode
+    std::vector<double> p_area;  // lumped P1 area of each pressure node
+    std::vector<float> p_xy;     // coordinates of the pressure nodes
+    DeviceArray<double> pguess, ptmp, pscal;
 };
 
 namespace {
@@ -119,6 +127,15 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
     put(d, s->vb_v, vv.data(), vv.size());
     put(d, s->pb_idx, pi.data(), pi.size());
     put(d, s->pb_val, pv.data(), pv.size());
+    s->p_area.assign(P, 0.0);
+    for (int e = 0; e < E; ++e) {
+        const int32_t *t = el + 6 * (size_t)e;
+        const double ax = xy[2 * t[0]], ay = xy[2 * t[0] + 1];
+        const double area = 0.5 * std::fabs((xy[2 * t[1]] - ax) * (xy[2 * t[2] + 1] - ay) -
+                                            (xy[2 * t[2]] - ax) * (xy[2 * t[1] + 1] - ay));
+        for (int k = 0; k < 3; ++k) s->p_area[t[k]] += area / 3.0;
+    }
+    s->p_xy.assign(xy, xy + 2 * (size_t)P);
     simulation_reset(*s);
     {
         std::lock_guard<std::mutex> lk(d.buf_mtx);
@@ -173,7 +190,90 @@ void solve(Simulation &s, int slot, const double *b, double *x, int ncol, SolveO
     o.seconds = st[0].seconds;
 }
 
+// Initial guess of the pressure solve.  The reference keeps the previous
+// step's pressure (include/assembly.h:608-615); the learned modes replace it
+// with the network's prediction from the right-hand side (or keep whichever
+// of the two has the smaller residual), or add the network's correction,
+// predicted from the previous pressure's residual, to the previous pressure.
+void pressure_guess(Simulation &s) {
+    Device &d = *s.dev;
+    switch (s.warm_mode) {
+    case NSFEM_WARM_PREVIOUS:
+        return;
+    case NSFEM_WARM_ZERO:
+        launch_fill(d, s.P, 0.0, s.p.ptr);
+        return;
+    case NSFEM_WARM_LEARNED:
+        gnn_predict(*s.gnn, *s.graph, s.prhs.ptr, s.p.ptr);
+        return;
+    case NSFEM_WARM_LEARNED_OR_PREVIOUS:
+        gnn_predict(*s.gnn, *s.graph, s.prhs.ptr, s.pguess.ptr);
+        select_better_guess(d, d.slots[NSFEM_PRESSURE_STIFFNESS], s.prhs.ptr, s.pguess.ptr, s.p.ptr,
+                            s.ptmp.ptr, s.pscal.ptr);
+        return;
+    case NSFEM_WARM_LEARNED_CORRECTION: {
+        // r = b - A p_prev; guess = p_prev + network(r), kept only if its
+        // residual is smaller than the previous pressure's
+        Matrix &A = d.slots[NSFEM_PRESSURE_STIFFNESS];
+        launch_spmv(d, A, s.p.ptr, s.prhs.ptr, s.ptmp.ptr, 1);
+        gnn_predict(*s.gnn, *s.graph, s.ptmp.ptr, s.pguess.ptr);
+        launch_add(d, s.P, s.p.ptr, s.pguess.ptr);
+        select_better_guess(d, A, s.prhs.ptr, s.pguess.ptr, s.p.ptr, s.ptmp.ptr, s.pscal.ptr);
+        return;
+    }
+    }
+}
+
 }  // namespace
+
+void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
+    NSFEM_REQUIRE(mode >= NSFEM_WARM_PREVIOUS && mode <= NSFEM_WARM_LEARNED_CORRECTION,
+                  NSFEM_INVALID_ARGUMENT, "unknown warm-start mode");
+    const bool learned = mode == NSFEM_WARM_LEARNED || mode == NSFEM_WARM_LEARNED_OR_PREVIOUS ||
+                         mode == NSFEM_WARM_LEARNED_CORRECTION;
+    NSFEM_REQUIRE(!learned || (g && g->dev == s.dev), NSFEM_INVALID_ARGUMENT,
+                  "a learned warm start needs a network on the simulation's device");
+    Device &d = *s.dev;
+    d.set_current();
+    if (learned && !s.graph) {
+        // the graph of the simulation's pressure matrix, read back once
+        Matrix &A = d.slots[NSFEM_PRESSURE_STIFFNESS];
+        NSFEM_REQUIRE(A.ready && A.rows == s.P, NSFEM_NOT_READY,
+                      "the simulation's pressure matrix is no longer in its slot");
+        std::vector<int32_t> rs(A.rows + 1), col((size_t)A.nnz);
+        std::vector<double> val((size_t)A.nnz);
+        NSFEM_HIP(hipMemcpy(rs.data(), A.row_start.ptr, rs.size() * sizeof(int32_t),
+                            hipMemcpyDeviceToHost));
+        NSFEM_HIP(hipMemcpy(col.data(), A.col.ptr, col.size() * sizeof(int32_t),
+                            hipMemcpyDeviceToHost));
+        NSFEM_HIP(hipMemcpy(val.data(), A.val.ptr, val.size() * sizeof(double),
+                            hipMemcpyDeviceToHost));
+        s.graph = gnn_graph_create(d, s.P, A.nnz, rs.data(), col.data(), val.data(),
+                                   s.p_area.data(), s.p_xy.data());
+    }
+    if (mode == NSFEM_WARM_LEARNED_OR_PREVIOUS || mode == NSFEM_WARM_LEARNED_CORRECTION) {
+        s.pguess.resize(s.P);
+        s.ptmp.resize(s.P);
+        s.pscal.resize(2);
+    }
+    s.gnn = learned ? g : nullptr;
+    s.warm_mode = mode;
+}
+
+void simulation_forget_gnn(Simulation *s, GnnNet *g) {
+    if (s->gnn == g) {
+        s->gnn = nullptr;
+        s->warm_mode = NSFEM_WARM_PREVIOUS;
+    }
+}
+
+void simulation_copy_pressure(Simulation &s, double *rhs, double *p) {
+    Device &d = *s.dev;
+    d.set_current();
+    if (rhs) launch_copy(d, s.P, s.prhs.ptr, rhs);
+    if (p) launch_copy(d, s.P, s.p.ptr, p);
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+}
 
 void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
     Device &d = *s.dev;
@@ -212,6 +312,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         // 3. pressure
         launch_spmv(d, BD, s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
         launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.prhs.ptr);
+        pressure_guess(s);
         solve(s, NSFEM_PRESSURE_STIFFNESS, s.prhs.ptr, s.p.ptr, 1, o[2]);
         // 4. velocity correction
         launch_spmv(d, PD, s.p.ptr, nullptr, s.rhs.ptr, 0);
@@ -279,6 +380,12 @@ void simulation_set(Simulation &s, const double *u, const double *v, const doubl
 }
 
 void simulation_detach(Simulation *s) {
+    if (s->graph) gnn_graph_release(*s->graph);
+    s->gnn = nullptr;
+    s->warm_mode = NSFEM_WARM_PREVIOUS;
+    s->pguess.release();
+    s->ptmp.release();
+    s->pscal.release();
     s->uv.release();
     s->utv.release();
     s->rhs.release();
@@ -300,6 +407,7 @@ void simulation_destroy(Simulation *s) {
         std::lock_guard<std::mutex> lk(s->dev->buf_mtx);
         s->dev->sims.erase(s);
     }
+    delete s->graph;
     delete s;
 }
 
@@ -383,6 +491,32 @@ int nsfem_simulation_reset(nsfem_simulation *s) {
                       "null simulation or destroyed device");
         std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
         simulation_reset(*s->impl);
+    });
+}
+
+int nsfem_simulation_set_warm_start(nsfem_simulation *s, int mode, nsfem_gnn *g) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        NSFEM_REQUIRE(!g || g->impl, NSFEM_INVALID_ARGUMENT, "network of a destroyed device");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_set_warm_start(*s->impl, mode, g ? g->impl : nullptr);
+    });
+}
+
+int nsfem_simulation_copy_pressure_system(nsfem_simulation *s, nsfem_buffer *rhs,
+                                          nsfem_buffer *p) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        const int64_t need = (int64_t)s->impl->P * (int64_t)sizeof(double);
+        for (nsfem_buffer *b : {rhs, p})
+            NSFEM_REQUIRE(!b || (b->impl.dev == s->impl->dev && (int64_t)b->impl.mem.bytes() >= need),
+                          NSFEM_INVALID_ARGUMENT,
+                          "buffer too small or on another device (needs pressure-count doubles)");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_copy_pressure(*s->impl, rhs ? (double *)rhs->impl.mem.ptr : nullptr,
+                                 p ? (double *)p->impl.mem.ptr : nullptr);
     });
 }
 

@@ -76,6 +76,26 @@ class Simulation:
         """Zero fields with the velocity Dirichlet values imposed."""
         N.check(N.lib().nsfem_simulation_reset(self.handle))
 
+    WARM_MODES = {"previous": N.WARM_PREVIOUS, "zero": N.WARM_ZERO, "learned": N.WARM_LEARNED,
+                  "learned_or_previous": N.WARM_LEARNED_OR_PREVIOUS,
+                  "learned_correction": N.WARM_LEARNED_CORRECTION}
+
+    def set_warm_start(self, mode: str = "previous", net=None):
+        """Initial guess of the pressure solve: ``previous`` (the reference's
+        choice), ``zero``, ``learned`` (``net``'s prediction from the
+        right-hand side), ``learned_or_previous`` (the smaller residual) or
+        ``learned_correction`` (previous pressure + the network's correction
+        from the previous pressure's residual, if that lowers the residual)."""
+        N.check(N.lib().nsfem_simulation_set_warm_start(
+            self.handle, self.WARM_MODES[mode], net.handle if net is not None else None))
+
+    def copy_pressure_system(self, rhs=None, p=None):
+        """Copy the last step's pressure right-hand side and solution into
+        device buffers (``DeviceBuffer`` of pressure-count doubles)."""
+        N.check(N.lib().nsfem_simulation_copy_pressure_system(
+            self.handle, rhs.handle if rhs is not None else None,
+            p.handle if p is not None else None))
+
     def close(self):
         if getattr(self, "handle", None):
             N.lib().nsfem_simulation_destroy(self.handle)
