@@ -66,7 +66,7 @@ def cg_bytes(nnz, n, iters, refinements, mixed, columns=1):
 MEASURED_BYTES_PER_ITERATION = {False: 809.8e6, True: 520.7e6}
 
 
-def spmv_bench(dev, slot, n_rows, n_cols, nnz, reps=100):
+def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
     """Median-free throughput of `reps` back-to-back nsfem_sp_r_mult launches
     (x ~ U(-1, 1), seed 7): wall time between two device synchronisations,
     so launch gaps count against the kernel.  Algorithmic bytes per product:
@@ -76,11 +76,11 @@ def spmv_bench(dev, slot, n_rows, n_cols, nnz, reps=100):
     x = dev.buffer(np.random.default_rng(7).uniform(-1, 1, n_cols))
     y = dev.buffer(np.zeros(n_rows))
     for _ in range(3):
-        dev.sp_r_mult(slot, x, y)
+        sim.sp_r_mult(slot, x, y)
     dev.synchronize()
     t = time.perf_counter()
     for _ in range(reps):
-        dev.sp_r_mult(slot, x, y)
+        sim.sp_r_mult(slot, x, y)
     dev.synchronize()
     sec = (time.perf_counter() - t) / reps
     x.close()
@@ -239,7 +239,7 @@ def main():
     sim = Simulation(m, nu, dt, device=dev, rtol=1e-8, precision="fp64", pair_velocity=True)
     log(f"simulation set-up (assembly + KD tree + upload) {time.perf_counter() - t:.1f} s")
     n_p = m.pressure_count
-    nnz_p = dev.matrix_info(0)[2]
+    nnz_p = sim.matrix_info(0)[2]
 
     results = {}
     forms = ["fp64", "fp32_refined"] if args.form == "both" else [args.form]
@@ -294,8 +294,8 @@ def main():
     # SpMV micro-benchmark on the assembled pressure and diffusion matrices
     out["spmv"] = {}
     for name, slot in (("pressure_stiffness", 0), ("diffusion", 2)):
-        r_, c_, z_ = dev.matrix_info(slot)
-        out["spmv"][name] = spmv_bench(dev, slot, r_, c_, z_)
+        r_, c_, z_ = sim.matrix_info(slot)
+        out["spmv"][name] = spmv_bench(dev, sim, slot, r_, c_, z_)
         log(f"spmv {name}: {out['spmv'][name]}")
     if "fp64" in results and "fp32_refined" in results:
         out["speedup_vs_fp64"] = round((args.steps / results["fp32_refined"][0]) /

@@ -302,6 +302,32 @@ int nsfem_simulation_set_options(nsfem_simulation *sim, const nsfem_sim_options 
 /* Back to the initial state: zero fields with the velocity Dirichlet values
  * imposed (include/assembly.h:1503-1507). */
 int nsfem_simulation_reset(nsfem_simulation *sim);
+/* Batches (north_star C3): `count` simulations of one device and one
+ * precision advance `steps` steps together; each solve kind (u/v diffusion
+ * as 2 columns, pressure, u/v mass as 2 columns) runs for all of them in
+ * one persistent launch.  stats: `count` entries (nullable), accumulated;
+ * seconds and cg_seconds are the whole batch's. */
+int nsfem_simulation_step_batch(nsfem_simulation *const *sims, int count, int steps,
+                                nsfem_step_stats *stats);
+typedef struct {
+    int32_t rows_per_wave;          /* wave-tile rows chosen (64, 32 or 16) */
+    int32_t nnz_per_wave;           /* wave-tile nonzero cap chosen */
+    int32_t block_scale;            /* blocks per instance multiplier (1, 2, 4) */
+    int32_t pad;
+    double seconds_per_iteration;   /* batched pressure iteration, best candidate */
+} nsfem_batch_tuning;
+/* Times the candidate wave-tile shapes and block splits on the batch's
+ * pressure systems and applies the fastest to all of the batch's matrices. */
+int nsfem_simulation_autotune_batch(nsfem_simulation *const *sims, int count,
+                                    nsfem_batch_tuning *out);
+
+/* The simulation's own matrices (ids as the device slots:
+ * NSFEM_PRESSURE_STIFFNESS ... NSFEM_PRESSURE_DIVERGENCE): shape, and
+ * y = A x (lhs == NULL) or y = lhs - A x on device buffers. */
+int nsfem_simulation_matrix_info(nsfem_simulation *sim, int matrix, int *rows, int *cols,
+                                 int64_t *nnz);
+int nsfem_simulation_sp_r_mult(nsfem_simulation *sim, int matrix, const nsfem_buffer *lhs,
+                               const nsfem_buffer *x, nsfem_buffer *y);
 
 /* ------------------------------------------------------------------ */
 /* Learned warm start of the pressure solve.  No counterpart in the     */

@@ -74,6 +74,17 @@ class StepStats(C.Structure):
                 "cg_refinements": list(self.cg_refinements)}
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("rows_per_wave", C.c_int32), ("nnz_per_wave", C.c_int32),
+                ("block_scale", C.c_int32), ("pad", C.c_int32),
+                ("seconds_per_iteration", C.c_double)]
+
+    def as_dict(self):
+        return {"rows_per_wave": self.rows_per_wave, "nnz_per_wave": self.nnz_per_wave,
+                "block_scale": self.block_scale,
+                "seconds_per_iteration": self.seconds_per_iteration}
+
+
 _lib = None
 vp = C.c_void_p
 i32 = C.c_int32
@@ -134,6 +145,11 @@ _SIGNATURES = {
     "nsfem_simulation_set_fields": ([vp, vp, vp, vp], C.c_int),
     "nsfem_simulation_set_options": ([vp, C.POINTER(SimOptions)], C.c_int),
     "nsfem_simulation_reset": ([vp], C.c_int),
+    "nsfem_simulation_step_batch": ([vp, C.c_int, C.c_int, vp], C.c_int),
+    "nsfem_simulation_autotune_batch": ([vp, C.c_int, C.POINTER(BatchTuning)], C.c_int),
+    "nsfem_simulation_matrix_info": ([vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(i64)], C.c_int),
+    "nsfem_simulation_sp_r_mult": ([vp, C.c_int, vp, vp, vp], C.c_int),
     "nsfem_simulation_set_warm_start": ([vp, C.c_int, vp], C.c_int),
     "nsfem_simulation_copy_pressure_system": ([vp, vp, vp], C.c_int),
     "nsfem_gnn_create": ([vp, C.c_uint64, PP], C.c_int),

@@ -252,8 +252,7 @@ __global__ void k_tri_xy(int ne, const int32_t *__restrict__ elem, const float2 
     tri[3 * e + 2] = xy[elem[6 * e + 2]];
 }
 
-TreeView view(Device &d) {
-    KdTreeGpu &t = d.tree;
+TreeView view(const KdTreeGpu &t) {
     NSFEM_REQUIRE(t.ready, NSFEM_NOT_READY, "KD tree not uploaded");
     return TreeView{reinterpret_cast<const uint2 *>(t.nodes.ptr), t.leaf.ptr,
                     reinterpret_cast<const float2 *>(t.xy.ptr), t.elements.ptr,
@@ -262,8 +261,7 @@ TreeView view(Device &d) {
 
 }  // namespace
 
-void build_tri_xy(Device &d) {
-    KdTreeGpu &t = d.tree;
+void build_tri_xy(Device &d, KdTreeGpu &t) {
     t.tri_xy.resize((size_t)t.elements_count * 6);
     if (t.elements_count == 0) return;
     k_tri_xy<<<(t.elements_count + 255) / 256, 256, 0, d.stream>>>(
@@ -272,29 +270,29 @@ void build_tri_xy(Device &d) {
     NSFEM_HIP(hipGetLastError());
 }
 
-void launch_advect_f32(Device &d, int n, const float *u, const float *v, float dt, float *uo,
-                       float *vo) {
+void launch_advect_f32(Device &d, const KdTreeGpu &t, int n, const float *u, const float *v,
+                       float dt, float *uo, float *vo) {
     if (n <= 0) return;
-    NSFEM_REQUIRE(n <= d.tree.mesh_nodes, NSFEM_INVALID_ARGUMENT,
+    NSFEM_REQUIRE(n <= t.mesh_nodes, NSFEM_INVALID_ARGUMENT,
                   "advect: more velocity nodes than mesh nodes");
-    k_advect_f32<<<(n + 255) / 256, 256, 0, d.stream>>>(view(d), n, u, v, dt, uo, vo);
+    k_advect_f32<<<(n + 255) / 256, 256, 0, d.stream>>>(view(t), n, u, v, dt, uo, vo);
     NSFEM_HIP(hipGetLastError());
 }
 
-void launch_advect_f64(Device &d, int n, const double *u, const double *v, double dt,
-                       double *uo, double *vo) {
+void launch_advect_f64(Device &d, const KdTreeGpu &t, int n, const double *u, const double *v,
+                       double dt, double *uo, double *vo) {
     if (n <= 0) return;
-    NSFEM_REQUIRE(n <= d.tree.mesh_nodes, NSFEM_INVALID_ARGUMENT,
+    NSFEM_REQUIRE(n <= t.mesh_nodes, NSFEM_INVALID_ARGUMENT,
                   "advect: more velocity nodes than mesh nodes");
-    k_advect_f64<<<(n + 255) / 256, 256, 0, d.stream>>>(view(d), n, u, v, (float)dt, uo, vo);
+    k_advect_f64<<<(n + 255) / 256, 256, 0, d.stream>>>(view(t), n, u, v, (float)dt, uo, vo);
     NSFEM_HIP(hipGetLastError());
 }
 
-void launch_locate(Device &d, int n, const float *pts, int32_t *elem, float *xi, float *eta,
-                   int32_t *nearest) {
+void launch_locate(Device &d, const KdTreeGpu &t, int n, const float *pts, int32_t *elem,
+                   float *xi, float *eta, int32_t *nearest) {
     if (n <= 0) return;
     k_locate<<<(n + 255) / 256, 256, 0, d.stream>>>(
-        view(d), n, reinterpret_cast<const float2 *>(pts), elem, xi, eta, nearest);
+        view(t), n, reinterpret_cast<const float2 *>(pts), elem, xi, eta, nearest);
     NSFEM_HIP(hipGetLastError());
 }
 

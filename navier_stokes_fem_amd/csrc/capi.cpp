@@ -61,8 +61,8 @@ void upload_csr(Device &d, Matrix &m, int rows, int cols, int64_t nnz, const int
     m.ready = true;
 }
 
-void upload_tree(Device &d, int32_t tn, const uint32_t *w, int64_t lc, const int32_t *leaf,
-                 int V, const float *xy, int E, const int32_t *el) {
+void upload_tree(Device &d, KdTreeGpu &t, int32_t tn, const uint32_t *w, int64_t lc,
+                 const int32_t *leaf, int V, const float *xy, int E, const int32_t *el) {
     NSFEM_REQUIRE(tn > 0 && V > 0 && E > 0 && lc >= 0, NSFEM_INVALID_ARGUMENT,
                   "KD tree upload: empty tree or mesh");
     NSFEM_REQUIRE(tn < (1 << 30), NSFEM_INVALID_ARGUMENT, "KD tree: too many nodes");
@@ -97,7 +97,6 @@ void upload_tree(Device &d, int32_t tn, const uint32_t *w, int64_t lc, const int
         }
     }
     d.set_current();
-    KdTreeGpu &t = d.tree;
     t.ready = false;
     t.nodes.resize(2 * (size_t)tn);
     t.leaf.resize((size_t)std::max<int64_t>(lc, 1));
@@ -111,12 +110,14 @@ void upload_tree(Device &d, int32_t tn, const uint32_t *w, int64_t lc, const int
     t.leaf_count = lc;
     t.mesh_nodes = V;
     t.elements_count = E;
-    build_tri_xy(d);
+    build_tri_xy(d, t);
     NSFEM_HIP(hipStreamSynchronize(d.stream));
     t.ready = true;
-    // initVelocityBuffers (cpp/gpu_simulation_device.cpp:76)
-    for (auto &b : d.vel32) b.resize((size_t)V);
-    d.vel_n = V;
+    if (&t == &d.tree) {
+        // initVelocityBuffers (cpp/gpu_simulation_device.cpp:76)
+        for (auto &b : d.vel32) b.resize((size_t)V);
+        d.vel_n = V;
+    }
 }
 
 static Buffer &buf(const nsfem_buffer *b) {
@@ -571,7 +572,7 @@ int nsfem_upload_kdtree(nsfem_device *h, const nsfem_kdtree *t, int V, const flo
         Device &d = dev(h);
         NSFEM_REQUIRE(t && xy && el, NSFEM_INVALID_ARGUMENT, "null argument");
         std::lock_guard<std::mutex> lk(d.mtx);
-        upload_tree(d, (int32_t)(t->impl.nodes.size() / 2), t->impl.nodes.data(),
+        upload_tree(d, d.tree, (int32_t)(t->impl.nodes.size() / 2), t->impl.nodes.data(),
                     (int64_t)t->impl.leaf.size(), t->impl.leaf.data(), V, xy, E, el);
     });
 }
@@ -583,7 +584,7 @@ int nsfem_upload_kdtree_arrays(nsfem_device *h, int32_t tn, const uint32_t *w, i
         Device &d = dev(h);
         NSFEM_REQUIRE(w && xy && el && (lc == 0 || leaf), NSFEM_INVALID_ARGUMENT, "null argument");
         std::lock_guard<std::mutex> lk(d.mtx);
-        upload_tree(d, tn, w, lc, leaf, V, xy, E, el);
+        upload_tree(d, d.tree, tn, w, lc, leaf, V, xy, E, el);
     });
 }
 
@@ -609,7 +610,7 @@ int nsfem_advect(nsfem_device *h, int n, const float *u, const float *v, float d
         const size_t bytes = sizeof(float) * (size_t)n;
         NSFEM_HIP(hipMemcpyAsync(d.vel32[0].ptr, u, bytes, hipMemcpyHostToDevice, d.stream));
         NSFEM_HIP(hipMemcpyAsync(d.vel32[1].ptr, v, bytes, hipMemcpyHostToDevice, d.stream));
-        launch_advect_f32(d, n, d.vel32[0].ptr, d.vel32[1].ptr, dt, d.vel32[2].ptr, d.vel32[3].ptr);
+        launch_advect_f32(d, d.tree, n, d.vel32[0].ptr, d.vel32[1].ptr, dt, d.vel32[2].ptr, d.vel32[3].ptr);
         NSFEM_HIP(hipMemcpyAsync(uo, d.vel32[2].ptr, bytes, hipMemcpyDeviceToHost, d.stream));
         NSFEM_HIP(hipMemcpyAsync(vo, d.vel32[3].ptr, bytes, hipMemcpyDeviceToHost, d.stream));
         NSFEM_HIP(hipStreamSynchronize(d.stream));
@@ -629,7 +630,7 @@ int nsfem_advect_f64(nsfem_device *h, int n, const nsfem_buffer *u, const nsfem_
         double *uop = dptr(uo, n, "advect uOut");
         double *vop = dptr(vo, n, "advect vOut");
         d.set_current();
-        launch_advect_f64(d, n, up, vp, dt, uop, vop);
+        launch_advect_f64(d, d.tree, n, up, vp, dt, uop, vop);
     });
 }
 
@@ -652,7 +653,7 @@ int nsfem_locate(nsfem_device *h, int n, const float *pts, int32_t *elem, float 
         de.resize(n);
         dn.resize(n);
         NSFEM_HIP(hipMemcpyAsync(dp.ptr, pts, sizeof(float) * 2 * n, hipMemcpyHostToDevice, d.stream));
-        launch_locate(d, n, dp.ptr, de.ptr, dxi.ptr, deta.ptr, dn.ptr);
+        launch_locate(d, d.tree, n, dp.ptr, de.ptr, dxi.ptr, deta.ptr, dn.ptr);
         NSFEM_HIP(hipMemcpyAsync(elem, de.ptr, sizeof(int32_t) * n, hipMemcpyDeviceToHost, d.stream));
         NSFEM_HIP(hipMemcpyAsync(xi, dxi.ptr, sizeof(float) * n, hipMemcpyDeviceToHost, d.stream));
         NSFEM_HIP(hipMemcpyAsync(eta, deta.ptr, sizeof(float) * n, hipMemcpyDeviceToHost, d.stream));

@@ -120,7 +120,17 @@ void build_tiles(Matrix &m, const std::vector<int32_t> &rs) {
     m.tiles.resize(t.size());
     NSFEM_HIP(hipMemcpy(m.tiles.ptr, t.data(), t.size() * sizeof(int32_t),
                         hipMemcpyHostToDevice));
-    std::vector<int32_t> w = cut_rows(rs, m.rows, kWaveRows, kWaveNnz);
+    m.host_rs = rs;
+    build_wave_tiles(m, kWaveRows, kWaveNnz);
+}
+
+void build_wave_tiles(Matrix &m, int rows, int nnz) {
+    NSFEM_REQUIRE(rows >= 1 && rows <= kWaveRows && nnz >= 64 && nnz <= kWaveNnz,
+                  NSFEM_INVALID_ARGUMENT, "wave tile shape out of range");
+    const std::vector<int32_t> &rs = m.host_rs;
+    m.wave_rows = rows;
+    m.wave_nnz = nnz;
+    std::vector<int32_t> w = cut_rows(rs, m.rows, rows, nnz);
     m.n_wtiles = (int)w.size() - 1;
     m.host_wtile_prefix.assign(w.size(), 0);
     for (int i = 0; i < m.n_wtiles; ++i)
@@ -467,6 +477,7 @@ struct CgArgs {
     unsigned long long timeout_ticks;
     SyncState *sync;
     CgColumnResult *res;
+    int nblocks;  // blocks of this solve (batched launches)
 };
 
 __global__ void k_sync_reset(SyncState *s, int ngroups) {
@@ -502,14 +513,12 @@ __device__ __forceinline__ double ld_relaxed(const double *p) {
 // blocks wrote.
 template <int K, int BS>
 __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *lds,
-                         unsigned long long timeout) {
+                         unsigned long long timeout, const int b, const int nb) {
     // Each flag is written by thread 0 before a __syncthreads and read by
     // the block after it; none is rewritten until the next call, which
     // every thread reaches only after passing several more barriers.
     __shared__ int s_group_last, s_top_last, s_ok;
     block_sum<K, BS>(v, lds);
-    const int nb = gridDim.x;
-    const int b = blockIdx.x;
     const int g = b / kGroup;
     const int ng = (nb + kGroup - 1) / kGroup;
     const int gsize = min(kGroup, nb - g * kGroup);
@@ -911,16 +920,13 @@ __device__ void flush_x(const CgArgs &A, int R0, int R1, const VT *p, const doub
     }
 }
 
-// One block per CU: BS / 256 waves per SIMD (4 for 1024 threads, 128 VGPRs
-// per wave; 2 for 512 threads, up to 256 VGPRs).
+// The persistent CG run by one block of a solve that owns `nb` blocks: `bid`
+// is the block's index among them (reduction slots), `L` its logical index
+// (tile ownership).  prod_raw / lds are the block's LDS.
 template <int NC, int BS, bool MIXED>
-__global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
+__device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const int nb, const int L,
+                                        double *prod_raw, double *lds) {
     using VT = typename std::conditional<MIXED, float, double>::type;
-    constexpr int NW = BS / 64;
-    __shared__ double prod_raw[NW * kWaveNnz * NC];
-    __shared__ double lds[kMaxRed * NW];
-    const int nb = gridDim.x;
-    const int L = xcd_swizzle(blockIdx.x, nb);
     const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
     const int R0 = A.tiles[T0], R1 = A.tiles[T1];
     const int wave = threadIdx.x >> 6;
@@ -934,7 +940,7 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
 #pragma unroll
     for (int k = 0; k < 3 * NC; ++k) acc[k] = 0.0;
     residual_phase<NC, BS, MIXED, true>(A, T0, T1, lds64, acc);
-    if (!grid_sum<3 * NC, BS>(A.sync, acc, epoch, lds, A.timeout_ticks)) goto timeout;
+    if (!grid_sum<3 * NC, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
     {
         double tol2[NC], gamma[NC], rmax2[NC];
         bool active[NC];
@@ -984,7 +990,7 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
             CG_TRACE_STAMP(k, 0);
             phase_a<NC, BS, MIXED>(A, T0, T1, lds64, pold, pnew, beta, pend, has_pend, delta);
             CG_TRACE_MAX(k, 1);
-            if (!grid_sum<NC, BS>(A.sync, delta, epoch, lds, A.timeout_ticks)) goto timeout;
+            if (!grid_sum<NC, BS>(A.sync, delta, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
             CG_TRACE_STAMP(k, 2);
             double alpha[NC];
             VT alpha_v[NC];
@@ -1006,7 +1012,7 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
             for (int c = 0; c < NC; ++c) g[c] = 0.0;
             phase_b<NC, BS, VT>(A, R0, R1, alpha_v, g);
             CG_TRACE_MAX(k, 3);
-            if (!grid_sum<NC, BS>(A.sync, g, epoch, lds, A.timeout_ticks)) goto timeout;
+            if (!grid_sum<NC, BS>(A.sync, g, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
             CG_TRACE_STAMP(k, 4);
             ++k;
             bool want_ru = false;
@@ -1039,12 +1045,12 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) pend[c] = 0.0;
                 double dummy[1] = {0.0};
-                if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks)) goto timeout;
+                if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
                 double ra[3 * NC];
 #pragma unroll
                 for (int kk = 0; kk < 3 * NC; ++kk) ra[kk] = 0.0;
                 residual_phase<NC, BS, MIXED, false>(A, T0, T1, lds64, ra);
-                if (!grid_sum<3 * NC, BS>(A.sync, ra, epoch, lds, A.timeout_ticks)) goto timeout;
+                if (!grid_sum<3 * NC, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
                 ++refines;
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
@@ -1118,6 +1124,34 @@ timeout:
     }
 }
 
+// One block per CU: BS / 256 waves per SIMD (4 for 1024 threads, 128 VGPRs
+// per wave; 2 for 512 threads, up to 256 VGPRs).
+template <int NC, int BS, bool MIXED>
+__global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
+    constexpr int NW = BS / 64;
+    __shared__ double prod_raw[NW * kWaveNnz * NC];
+    __shared__ double lds[kMaxRed * NW];
+    cg_body<NC, BS, MIXED>(A, blockIdx.x, gridDim.x, xcd_swizzle(blockIdx.x, gridDim.x), prod_raw,
+                           lds);
+}
+
+// Many independent systems in one persistent launch: block i works for
+// instance map[i].x as its block map[i].y of args[instance].nblocks; each
+// instance reduces over its own blocks with its own SyncState, and its blocks
+// leave the kernel when it converges.
+template <int NC, int BS, bool MIXED>
+__global__ __launch_bounds__(BS, BS / 256) void k_cg_batch(const CgArgs *__restrict__ args,
+                                                           const int2 *__restrict__ map) {
+    constexpr int NW = BS / 64;
+    __shared__ double prod_raw[NW * kWaveNnz * NC];
+    __shared__ double lds[kMaxRed * NW];
+    const int2 m = map[blockIdx.x];
+    const int inst = __builtin_amdgcn_readfirstlane(m.x);
+    const int lb = __builtin_amdgcn_readfirstlane(m.y);
+    const CgArgs A = args[inst];
+    cg_body<NC, BS, MIXED>(A, lb, A.nblocks, lb, prod_raw, lds);
+}
+
 template <int NC, int BS, bool MIXED>
 void cg_launch(Device &d, const CgArgs &args, int grid) {
     void *kargs[] = {(void *)&args};
@@ -1154,6 +1188,37 @@ void true_residuals(Device &d, Matrix &m, const double *b, const double *x, int 
 }  // namespace
 
 namespace {
+
+// Tolerances, iteration limits and reliable-update knobs of one solve.
+void set_cg_params(const Device &d, const CgRequest &req, int n, CgArgs &a) {
+    a.rtol = req.rtol;
+    a.atol = req.atol;
+    // reliable updates of the mixed solver: when the recursive residual has
+    // fallen by rdelta since the last one (but not sooner than ru_min_gap
+    // iterations after it, which matters on fast-converging systems), or
+    // after ru_gap iterations.  Forced updates every 1000 iterations cost the
+    // 2048^2 cavity pressure solve 26% more iterations, so ru_gap is only a
+    // backstop.  Tuning overrides: NSFEM_CG_RDELTA, NSFEM_CG_RU_GAP,
+    // NSFEM_CG_RU_MIN_GAP.
+    static const double rdelta_env = [] {
+        const char *e = std::getenv("NSFEM_CG_RDELTA");
+        return e ? std::atof(e) : 0.0;
+    }();
+    static const int ru_gap_env = [] {
+        const char *e = std::getenv("NSFEM_CG_RU_GAP");
+        return e ? std::atoi(e) : 0;
+    }();
+    a.rdelta = rdelta_env > 0.0 && rdelta_env < 1.0 ? rdelta_env : req.reliable_delta;
+    a.max_iter = req.max_iter < 0 ? n : req.max_iter;
+    a.max_refine = 400;
+    static const int ru_min_gap_env = [] {
+        const char *e = std::getenv("NSFEM_CG_RU_MIN_GAP");
+        return e ? std::atoi(e) : -1;
+    }();
+    a.ru_gap = ru_gap_env > 0 ? ru_gap_env : 20000;
+    a.ru_min_gap = ru_min_gap_env >= 0 ? ru_min_gap_env : 8;
+    a.timeout_ticks = (unsigned long long)d.wall_khz * 1000ull * 20ull;  // 20 s per barrier
+}
 
 struct CgPass {
     CgColumnResult res[2];
@@ -1227,33 +1292,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
     a.p0 = w + vec_bytes;
     a.p1 = w + 2 * vec_bytes;
     a.q = w + 3 * vec_bytes;
-    a.rtol = req.rtol;
-    a.atol = req.atol;
-    // reliable updates of the mixed solver: when the recursive residual has
-    // fallen by rdelta since the last one (but not sooner than ru_min_gap
-    // iterations after it, which matters on fast-converging systems), or
-    // after ru_gap iterations.  Forced updates every 1000 iterations cost the
-    // 2048^2 cavity pressure solve 26% more iterations, so ru_gap is only a
-    // backstop.  Tuning overrides: NSFEM_CG_RDELTA, NSFEM_CG_RU_GAP,
-    // NSFEM_CG_RU_MIN_GAP.
-    static const double rdelta_env = [] {
-        const char *e = std::getenv("NSFEM_CG_RDELTA");
-        return e ? std::atof(e) : 0.0;
-    }();
-    static const int ru_gap_env = [] {
-        const char *e = std::getenv("NSFEM_CG_RU_GAP");
-        return e ? std::atoi(e) : 0;
-    }();
-    a.rdelta = rdelta_env > 0.0 && rdelta_env < 1.0 ? rdelta_env : req.reliable_delta;
-    a.max_iter = req.max_iter < 0 ? n : req.max_iter;
-    a.max_refine = 400;
-    static const int ru_min_gap_env = [] {
-        const char *e = std::getenv("NSFEM_CG_RU_MIN_GAP");
-        return e ? std::atoi(e) : -1;
-    }();
-    a.ru_gap = ru_gap_env > 0 ? ru_gap_env : 20000;
-    a.ru_min_gap = ru_min_gap_env >= 0 ? ru_min_gap_env : 8;
-    a.timeout_ticks = (unsigned long long)d.wall_khz * 1000ull * 20ull;  // 20 s per barrier
+    set_cg_params(d, req, n, a);
     unsigned char *sm = d.sync_mem.ptr;
     a.sync = reinterpret_cast<SyncState *>(sm + (256 - ((uintptr_t)sm % 256)) % 256);
     a.res = d.cg_res.ptr;
@@ -1350,18 +1389,215 @@ int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
     return worst;
 }
 
-}  // namespace nsfem
+// ------------------------------------------------------------ batched CG
+namespace {
 
-#ifdef NSFEM_CG_TRACE
-extern "C" int nsfem_trace_read(unsigned long long *out, int iters) {
-    const int n = iters < nsfem::kTraceIters ? iters : nsfem::kTraceIters;
-    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(nsfem::g_cg_trace), sizeof(unsigned long long) * 5 * n) !=
-        hipSuccess)
-        return -1;
-    return n;
+template <int NC, bool MIXED>
+void cg_batch_launch(Device &d, const CgArgs *args, const int2 *map, int grid) {
+    void *kargs[] = {(void *)&args, (void *)&map};
+    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg_batch<NC, 1024, MIXED>, dim3(grid),
+                                         dim3(1024), kargs, 0, d.stream));
 }
-extern "C" int nsfem_trace_clear(void) {
-    static unsigned long long zero[nsfem::kTraceIters][5];
-    return hipMemcpyToSymbol(HIP_SYMBOL(nsfem::g_cg_trace), zero, sizeof(zero)) == hipSuccess ? 0 : -1;
+
+template <int NC, bool MIXED>
+int cg_batch_max_grid(Device &d) {
+    int per_cu = 0;
+    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_batch<NC, 1024, MIXED>,
+                                                           1024, 0));
+    return std::max(1, std::min(per_cu * d.cu_count, kMaxGridBlocks));
 }
-#endif
+
+inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Blocks per instance: proportional to the instance's wave tiles (one block
+// per 32 tiles, as a single solve takes), at least one each, at most `maxg`
+// in total.
+std::vector<int> split_blocks(const std::vector<const CgRequest *> &rq, int maxg, int scale) {
+    const size_t I = rq.size();
+    std::vector<int> nb(I);
+    int64_t total = 0;
+    for (size_t i = 0; i < I; ++i) {
+        const int want = std::max(1, rq[i]->m->n_wtiles * scale / 32);
+        nb[i] = std::min(want, std::max(1, rq[i]->m->n_wtiles));
+        total += nb[i];
+    }
+    while (total > maxg) {  // trim the largest share until the batch fits
+        size_t big = 0;
+        for (size_t i = 1; i < I; ++i)
+            if (nb[i] > nb[big]) big = i;
+        if (nb[big] <= 1) break;
+        const int cut = std::max<int64_t>(1, std::min<int64_t>(nb[big] - 1, total - maxg));
+        nb[big] -= cut;
+        total -= cut;
+    }
+    return nb;
+}
+
+}  // namespace
+
+int run_cg_batch(Device &d, const std::vector<CgRequest> &reqs, nsfem_cg_stats *stats,
+                 int block_scale, double *batch_seconds, bool finish) {
+    if (batch_seconds) *batch_seconds = 0.0;
+    if (reqs.empty()) return NSFEM_OK;
+    const int nc = reqs[0].ncol;
+    const bool mixed = reqs[0].precision == NSFEM_PRECISION_FP32_REFINED;
+    for (const CgRequest &r : reqs) {
+        NSFEM_REQUIRE(r.m && r.m->ready && r.m->rows == r.m->cols, NSFEM_INVALID_ARGUMENT,
+                      "batched CG needs square uploaded matrices");
+        NSFEM_REQUIRE(r.ncol == nc && (r.precision == NSFEM_PRECISION_FP32_REFINED) == mixed &&
+                          (mixed || r.precision == NSFEM_PRECISION_FP64),
+                      NSFEM_INVALID_ARGUMENT,
+                      "a batch needs one column count and one precision");
+        NSFEM_REQUIRE(nc == 1 || nc == 2, NSFEM_INVALID_ARGUMENT, "columns must be 1 or 2");
+    }
+    const int maxg = nc == 1 ? (mixed ? cg_batch_max_grid<1, true>(d) : cg_batch_max_grid<1, false>(d))
+                             : (mixed ? cg_batch_max_grid<2, true>(d) : cg_batch_max_grid<2, false>(d));
+    int worst = NSFEM_OK;
+    const size_t vt = mixed ? sizeof(float) : sizeof(double);
+    // instances beyond the co-resident block count run in further launches
+    for (size_t first = 0; first < reqs.size(); first += (size_t)maxg) {
+        const size_t I = std::min(reqs.size() - first, (size_t)maxg);
+        std::vector<const CgRequest *> rq(I);
+        for (size_t i = 0; i < I; ++i) rq[i] = &reqs[first + i];
+        const std::vector<int> nb = split_blocks(rq, maxg, std::max(1, block_scale));
+        // per-instance memory
+        std::vector<size_t> voff(I), vbytes(I);
+        size_t work = 0;
+        for (size_t i = 0; i < I; ++i) {
+            if (mixed) ensure_val32(d, *rq[i]->m);
+            ensure_block_tiles(d, *rq[i]->m, nb[i]);
+            vbytes[i] = round256((size_t)rq[i]->m->rows * nc * vt);
+            voff[i] = work;
+            work += 4 * vbytes[i];
+        }
+        const size_t sync_bytes = round256(sizeof(SyncState));
+        d.batch_work.ensure(std::max<size_t>(work, 256));
+        d.batch_sync.ensure(sync_bytes * I + 256);
+        d.batch_res.ensure(2 * I);
+        d.batch_args.ensure(sizeof(CgArgs) * I);
+        d.batch_scal.ensure(2 * I);
+        unsigned char *sm = d.batch_sync.ptr + (256 - ((uintptr_t)d.batch_sync.ptr % 256)) % 256;
+        std::vector<CgArgs> args(I);
+        std::vector<int2> map;
+        for (size_t i = 0; i < I; ++i) {
+            const CgRequest &r = *rq[i];
+            Matrix &m = *r.m;
+            CgArgs &a = args[i];
+            a = CgArgs{};
+            a.n = m.rows;
+            a.meta = m.wmeta.ptr;
+            a.tiles = m.wtiles.ptr;
+            a.block_tiles = m.block_tiles.ptr;
+            a.rs = m.row_start.ptr;
+            a.col = m.col.ptr;
+            a.val64 = m.val.ptr;
+            a.val32 = m.val32.ptr;
+            a.b = r.b;
+            a.x = r.x;
+            unsigned char *w = d.batch_work.ptr + voff[i];
+            a.r = w;
+            a.p0 = w + vbytes[i];
+            a.p1 = w + 2 * vbytes[i];
+            a.q = w + 3 * vbytes[i];
+            set_cg_params(d, r, m.rows, a);
+            a.sync = reinterpret_cast<SyncState *>(sm + i * sync_bytes);
+            a.res = d.batch_res.ptr + 2 * i;
+            a.nblocks = nb[i];
+            for (int k = 0; k < nb[i]; ++k) map.push_back(make_int2((int)i, k));
+        }
+        const int grid = (int)map.size();
+        NSFEM_REQUIRE(grid <= maxg, NSFEM_ERROR, "batched CG: block split exceeds co-residency");
+        d.batch_map.ensure(map.size());
+        NSFEM_HIP(hipMemcpyAsync(d.batch_args.ptr, args.data(), sizeof(CgArgs) * I,
+                                 hipMemcpyHostToDevice, d.stream));
+        NSFEM_HIP(hipMemcpyAsync(d.batch_map.ptr, map.data(), sizeof(int2) * map.size(),
+                                 hipMemcpyHostToDevice, d.stream));
+        for (size_t i = 0; i < I; ++i) {
+            const int ng = (nb[i] + kGroup - 1) / kGroup;
+            k_sync_reset<<<(ng + 255) / 256, 256, 0, d.stream>>>(args[i].sync, ng);
+        }
+        NSFEM_HIP(hipGetLastError());
+        NSFEM_HIP(hipEventRecord(d.ev_start, d.stream));
+        const CgArgs *dargs = reinterpret_cast<const CgArgs *>(d.batch_args.ptr);
+        if (nc == 1) {
+            if (mixed) cg_batch_launch<1, true>(d, dargs, d.batch_map.ptr, grid);
+            else cg_batch_launch<1, false>(d, dargs, d.batch_map.ptr, grid);
+        } else {
+            if (mixed) cg_batch_launch<2, true>(d, dargs, d.batch_map.ptr, grid);
+            else cg_batch_launch<2, false>(d, dargs, d.batch_map.ptr, grid);
+        }
+        NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
+        // FP64 true residuals of every instance, one host synchronisation
+        if (!mixed) {
+            launch_fill(d, (int)(nc * I), 0.0, d.batch_scal.ptr);
+            for (size_t i = 0; i < I; ++i) {
+                const CgRequest &r = *rq[i];
+                double *tmp = (double *)args[i].q;  // n * nc doubles of scratch
+                for (int c = 0; c < nc; ++c) {
+                    const size_t off = (size_t)c * r.m->rows;
+                    launch_spmv(d, *r.m, r.x + off, r.b + off, tmp, 1);
+                    launch_dot_accumulate(d, r.m->rows, tmp, tmp, d.batch_scal.ptr + nc * i + c);
+                }
+            }
+        }
+        std::vector<CgColumnResult> res(2 * I);
+        std::vector<double> tres(nc * I, 0.0);
+        NSFEM_HIP(hipMemcpyAsync(res.data(), d.batch_res.ptr, sizeof(CgColumnResult) * 2 * I,
+                                 hipMemcpyDeviceToHost, d.stream));
+        if (!mixed)
+            NSFEM_HIP(hipMemcpyAsync(tres.data(), d.batch_scal.ptr, sizeof(double) * nc * I,
+                                     hipMemcpyDeviceToHost, d.stream));
+        NSFEM_HIP(hipStreamSynchronize(d.stream));
+        float ms = 0.f;
+        NSFEM_HIP(hipEventElapsedTime(&ms, d.ev_start, d.ev_stop));
+        if (batch_seconds) *batch_seconds += ms * 1e-3;
+        for (size_t i = 0; i < I; ++i) {
+            const CgRequest &r = *rq[i];
+            bool redo = false;
+            for (int c = 0; c < nc; ++c) {
+                const CgColumnResult &cr = res[2 * i + c];
+                if (cr.status == NSFEM_BARRIER_TIMEOUT)
+                    fail(NSFEM_BARRIER_TIMEOUT, "batched CG kernel barrier timed out");
+                const double fin = mixed ? cr.last_residual : std::sqrt(tres[nc * i + c]);
+                if (cr.status != NSFEM_OK || fin > cr.tolerance * (1.0 + 1e-12)) redo = true;
+                nsfem_cg_stats &st = stats[2 * (first + i) + c];
+                st.iterations = cr.iterations;
+                st.refinements = cr.refinements;
+                st.status = cr.status;
+                st.columns = nc;
+                st.rhs_norm = cr.rhs_norm;
+                st.initial_residual = cr.initial_residual;
+                st.final_residual = fin;
+                st.tolerance = cr.tolerance;
+                st.seconds = ms * 1e-3;
+            }
+            if (redo && finish) {
+                // stagnation, the iteration limit or FP64 drift: finish this
+                // instance alone from its current x, to its original
+                // absolute tolerance
+                CgRequest r2 = r;
+                r2.rtol = 0.0;
+                r2.atol = res[2 * i].tolerance;
+                if (nc == 2) r2.atol = std::min(r2.atol, res[2 * i + 1].tolerance);
+                if (r2.max_iter > 0)
+                    r2.max_iter = std::max(1, r2.max_iter - std::max(res[2 * i].iterations,
+                                                                     res[2 * i + nc - 1].iterations));
+                nsfem_cg_stats s2[2]{};
+                const int st2 = run_cg(d, r2, s2);
+                for (int c = 0; c < nc; ++c) {
+                    nsfem_cg_stats &st = stats[2 * (first + i) + c];
+                    st.iterations += s2[c].iterations;
+                    st.refinements += s2[c].refinements + 1;
+                    st.status = s2[c].status;
+                    st.final_residual = s2[c].final_residual;
+                    st.seconds += s2[c].seconds;
+                }
+                if (batch_seconds) *batch_seconds += s2[0].seconds;
+                if (st2 != NSFEM_OK && worst == NSFEM_OK) worst = st2;
+            }
+        }
+    }
+    return worst;
+}
+
+}  // namespace nsfem

@@ -119,6 +119,8 @@ struct Matrix {
     DeviceArray<int4> wmeta;      // [n_wtiles] {first row, end row, first nnz, nnz}
     int n_wtiles = 0;
     std::vector<int64_t> host_wtile_prefix;  // [n_wtiles + 1] cumulative nnz + rows
+    std::vector<int32_t> host_rs;            // row_start (wave tiles are rebuilt from it)
+    int wave_rows = kWaveRows, wave_nnz = kWaveNnz;  // current wave-tile shape
     // ownership of wave tiles by persistent blocks, cached per grid size
     DeviceArray<int32_t> block_tiles;
     int block_tiles_grid = 0;
@@ -126,6 +128,15 @@ struct Matrix {
     std::mutex mtx;
 };
 
+inline void matrix_release(Matrix &m) {
+    m.row_start.release(); m.col.release(); m.val.release(); m.val32.release();
+    m.tiles.release(); m.wtiles.release(); m.wmeta.release(); m.block_tiles.release();
+    m.block_tiles_grid = 0;
+    m.ready = false;
+}
+
+struct KdTreeGpu;
+inline void kdtree_release(KdTreeGpu &t);
 struct KdTreeGpu {
     DeviceArray<uint32_t> nodes;     // 2 words per node
     DeviceArray<int32_t> leaf;       // leaf triangle indices
@@ -138,6 +149,10 @@ struct KdTreeGpu {
     int elements_count = 0;
     bool ready = false;
 };
+inline void kdtree_release(KdTreeGpu &t) {
+    t.nodes.release(); t.leaf.release(); t.xy.release(); t.elements.release(); t.tri_xy.release();
+    t.ready = false;
+}
 
 constexpr int kMaxRed = 8;
 
@@ -180,6 +195,13 @@ struct Device {
     // memory and detaches them, so a later nsfem_buffer_destroy stays safe
     std::set<struct Buffer *> buffers;
     std::set<Simulation *> sims;  // detached the same way
+    // batched persistent CG: per-instance vectors, barrier state, results,
+    // arguments and the block -> instance map
+    DeviceArray<unsigned char> batch_work, batch_sync;
+    DeviceArray<CgColumnResult> batch_res;
+    DeviceArray<unsigned char> batch_args;
+    DeviceArray<int2> batch_map;
+    DeviceArray<double> batch_scal;
     std::set<GnnNet *> gnns;
     std::set<GnnGraph *> graphs;
     std::mutex buf_mtx;
@@ -194,6 +216,9 @@ struct Buffer {
 // ------------------------------------------------------------ entry points
 // (implemented in the .hip translation units, called by the C ABI layer)
 void build_tiles(Matrix &m, const std::vector<int32_t> &row_start);
+// Re-cut the rows into wave tiles of <= rows (<= 64) rows and <= nnz (<= 512)
+// nonzeros (the persistent CG's unit of work).
+void build_wave_tiles(Matrix &m, int rows, int nnz);
 void ensure_block_tiles(Device &d, Matrix &m, int grid);
 void ensure_val32(Device &d, Matrix &m);
 
@@ -218,14 +243,22 @@ struct CgRequest {
 };
 // Runs the persistent CG; fills stats[ncol].  Returns the worst status.
 int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats);
+// Many independent solves (same column count and precision) in one
+// persistent launch; stats[2 * i + c] for request i, column c.  block_scale
+// multiplies the blocks each instance gets (autotuned).  With `finish`, an
+// instance that stagnated, drifted or reached its iteration limit is
+// completed alone by run_cg.  Returns the worst status; *batch_seconds =
+// device time of the batched launches.
+int run_cg_batch(Device &d, const std::vector<CgRequest> &reqs, nsfem_cg_stats *stats,
+                 int block_scale, double *batch_seconds, bool finish = true);
 
-void launch_advect_f32(Device &d, int n, const float *u, const float *v, float dt,
+void launch_advect_f32(Device &d, const KdTreeGpu &t, int n, const float *u, const float *v, float dt,
                        float *uo, float *vo);
-void launch_advect_f64(Device &d, int n, const double *u, const double *v, double dt,
+void launch_advect_f64(Device &d, const KdTreeGpu &t, int n, const double *u, const double *v, double dt,
                        double *uo, double *vo);
-void launch_locate(Device &d, int n, const float *pts, int32_t *elem, float *xi,
+void launch_locate(Device &d, const KdTreeGpu &t, int n, const float *pts, int32_t *elem, float *xi,
                    float *eta, int32_t *nearest);
-void build_tri_xy(Device &d);
+void build_tri_xy(Device &d, KdTreeGpu &t);
 
 // elementwise helpers for the step driver
 void launch_scatter(Device &d, int count, const int32_t *idx, const double *val,
@@ -265,8 +298,9 @@ void assemble_host(int V, int P, const float *xy, int E, const int32_t *el, int 
 // Validate and upload a host CSR into a device slot (builds the tile table).
 void upload_csr(Device &d, Matrix &m, int rows, int cols, int64_t nnz, const int32_t *rs,
                 const int32_t *col, const double *val);
-// Validate and upload a KD tree + mesh.
-void upload_tree(Device &d, int32_t tree_nodes, const uint32_t *words, int64_t leaf_count,
+// Validate and upload a KD tree + mesh into `t` (the device's own tree or a
+// simulation's).
+void upload_tree(Device &d, KdTreeGpu &t, int32_t tree_nodes, const uint32_t *words, int64_t leaf_count,
                  const int32_t *leaf, int mesh_nodes, const float *xy, int elem_count,
                  const int32_t *elems);
 void launch_fill(Device &d, int n, double v, double *dst);

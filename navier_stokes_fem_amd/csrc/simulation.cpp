@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <unordered_map>
 #include <vector>
 
@@ -34,6 +35,10 @@ struct Simulation {
     DeviceArray<int32_t> vb_idx, pb_idx;
     DeviceArray<double> vb_u, vb_v, pb_val;
     int nvb = 0, npb = 0;
+    // the simulation's own matrices, indexed like the device slots
+    // (NSFEM_PRESSURE_STIFFNESS ... NSFEM_PRESSURE_DIVERGENCE)
+    Matrix mats[5];
+    KdTreeGpu tree;  // the simulation's own KD tree and mesh
     // initial guess of the pressure solve (NSFEM_WARM_*)
     int warm_mode = NSFEM_WARM_PREVIOUS;
     GnnNet *gnn = nullptr;      // not owned
@@ -41,6 +46,7 @@ struct Simulation {
     std::vector<double> p_area;  // lumped P1 area of each pressure node
     std::vector<float> p_xy;     // coordinates of the pressure nodes
     DeviceArray<double> pguess, ptmp, pscal;
+    int batch_block_scale = 1;  // blocks per instance in batched solves (autotuned)
 };
 
 namespace {
@@ -87,21 +93,22 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
     HostKdTree tree;
     kd_build_host(V, xy, E, el, -1, -1, tree);
     d.set_current();
+    std::unique_ptr<Simulation> owner(new Simulation());  // freed if an upload throws
+    Simulation *s = owner.get();
+    s->dev = &d;
     const int slot_of[5] = {NSFEM_VELOCITY_MASS, NSFEM_DIFFUSION, NSFEM_PRESSURE_STIFFNESS,
                             NSFEM_VELOCITY_DIVERGENCE, NSFEM_PRESSURE_DIVERGENCE};
     for (int k = 0; k < 5; ++k) {
         HostCsr &c = A.m[k];
-        upload_csr(d, d.slots[slot_of[k]], c.rows, c.cols, (int64_t)c.col.size(), c.rs.data(),
+        upload_csr(d, s->mats[slot_of[k]], c.rows, c.cols, (int64_t)c.col.size(), c.rs.data(),
                    c.col.data(), c.val.data());
         HostCsr().rs.swap(c.rs);
         std::vector<int32_t>().swap(c.col);
         std::vector<double>().swap(c.val);
     }
-    upload_tree(d, (int32_t)(tree.nodes.size() / 2), tree.nodes.data(), (int64_t)tree.leaf.size(),
+    upload_tree(d, s->tree, (int32_t)(tree.nodes.size() / 2), tree.nodes.data(), (int64_t)tree.leaf.size(),
                 tree.leaf.data(), V, xy, E, el);
 
-    auto *s = new Simulation();
-    s->dev = &d;
     s->V = V;
     s->P = P;
     s->opt = opt;
@@ -141,7 +148,7 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
         std::lock_guard<std::mutex> lk(d.buf_mtx);
         d.sims.insert(s);
     }
-    return s;
+    return owner.release();
 }
 
 // zero initial field with the velocity Dirichlet values (:1503-1507)
@@ -167,7 +174,7 @@ struct SolveOut {
 void solve(Simulation &s, int slot, const double *b, double *x, int ncol, SolveOut &o) {
     Device &d = *s.dev;
     CgRequest rq{};
-    rq.m = &d.slots[slot];
+    rq.m = &s.mats[slot];
     rq.b = b;
     rq.x = x;
     rq.ncol = ncol;
@@ -208,13 +215,13 @@ void pressure_guess(Simulation &s) {
         return;
     case NSFEM_WARM_LEARNED_OR_PREVIOUS:
         gnn_predict(*s.gnn, *s.graph, s.prhs.ptr, s.pguess.ptr);
-        select_better_guess(d, d.slots[NSFEM_PRESSURE_STIFFNESS], s.prhs.ptr, s.pguess.ptr, s.p.ptr,
+        select_better_guess(d, s.mats[NSFEM_PRESSURE_STIFFNESS], s.prhs.ptr, s.pguess.ptr, s.p.ptr,
                             s.ptmp.ptr, s.pscal.ptr);
         return;
     case NSFEM_WARM_LEARNED_CORRECTION: {
         // r = b - A p_prev; guess = p_prev + network(r), kept only if its
         // residual is smaller than the previous pressure's
-        Matrix &A = d.slots[NSFEM_PRESSURE_STIFFNESS];
+        Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
         launch_spmv(d, A, s.p.ptr, s.prhs.ptr, s.ptmp.ptr, 1);
         gnn_predict(*s.gnn, *s.graph, s.ptmp.ptr, s.pguess.ptr);
         launch_add(d, s.P, s.p.ptr, s.pguess.ptr);
@@ -237,9 +244,7 @@ void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
     d.set_current();
     if (learned && !s.graph) {
         // the graph of the simulation's pressure matrix, read back once
-        Matrix &A = d.slots[NSFEM_PRESSURE_STIFFNESS];
-        NSFEM_REQUIRE(A.ready && A.rows == s.P, NSFEM_NOT_READY,
-                      "the simulation's pressure matrix is no longer in its slot");
+        Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
         std::vector<int32_t> rs(A.rows + 1), col((size_t)A.nnz);
         std::vector<double> val((size_t)A.nnz);
         NSFEM_HIP(hipMemcpy(rs.data(), A.row_start.ptr, rs.size() * sizeof(int32_t),
@@ -281,9 +286,9 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
     const int V = s.V, P = s.P;
     double *u = s.uv.ptr, *v = s.uv.ptr + V;
     double *ut = s.utv.ptr, *vt = s.utv.ptr + V;
-    Matrix &M = d.slots[NSFEM_VELOCITY_MASS];
-    Matrix &BD = d.slots[NSFEM_VELOCITY_DIVERGENCE];
-    Matrix &PD = d.slots[NSFEM_PRESSURE_DIVERGENCE];
+    Matrix &M = s.mats[NSFEM_VELOCITY_MASS];
+    Matrix &BD = s.mats[NSFEM_VELOCITY_DIVERGENCE];
+    Matrix &PD = s.mats[NSFEM_PRESSURE_DIVERGENCE];
     hipEvent_t e0, e1, ea;
     NSFEM_HIP(hipEventCreate(&e0));
     NSFEM_HIP(hipEventCreate(&e1));
@@ -291,7 +296,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
     for (int step = 0; step < steps; ++step) {
         NSFEM_HIP(hipEventRecord(e0, d.stream));
         // 1. advection
-        launch_advect_f64(d, V, u, v, s.opt.dt, ut, vt);
+        launch_advect_f64(d, s.tree, V, u, v, s.opt.dt, ut, vt);
         NSFEM_HIP(hipEventRecord(ea, d.stream));
         // 2. diffusion
         launch_spmv(d, M, ut, s.c_uv.ptr, s.rhs.ptr, 2);
@@ -357,6 +362,190 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
     (void)hipEventDestroy(ea);
 }
 
+// ------------------------------------------------------------ batches
+// Several simulations stepped together: the per-instance stages (advection,
+// right-hand sides, boundary values) are launched instance by instance, and
+// each of the three solve kinds (u/v diffusion as a 2-column solve, pressure,
+// u/v mass as a 2-column solve) runs for every instance in one persistent
+// launch (run_cg_batch).
+void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
+                           nsfem_step_stats *stats) {
+    NSFEM_REQUIRE(!S.empty(), NSFEM_INVALID_ARGUMENT, "empty batch");
+    Device &d = *S[0]->dev;
+    for (Simulation *s : S) {
+        NSFEM_REQUIRE(s->dev == &d, NSFEM_INVALID_ARGUMENT,
+                      "a batch needs simulations of one device");
+        NSFEM_REQUIRE(s->opt.precision == S[0]->opt.precision, NSFEM_INVALID_ARGUMENT,
+                      "a batch needs one precision");
+    }
+    d.set_current();
+    const size_t I = S.size();
+    const int scale = S[0]->batch_block_scale;
+    std::vector<nsfem_cg_stats> st(2 * I);
+    std::vector<CgRequest> rq(I);
+    auto request = [&](Simulation &s, int slot, const double *b, double *x, int ncol) {
+        CgRequest r{};
+        r.m = &s.mats[slot];
+        r.b = b;
+        r.x = x;
+        r.ncol = ncol;
+        r.max_iter = s.opt.max_iterations;
+        r.rtol = s.opt.rtol;
+        r.atol = 0.0;
+        r.precision = s.opt.precision;
+        return r;
+    };
+    auto check = [&](int status, const char *what) {
+        if (status != NSFEM_OK)
+            fail(status, std::string("batched CG on the ") + what + " matrices did not converge");
+    };
+    hipEvent_t e0, e1;
+    NSFEM_HIP(hipEventCreate(&e0));
+    NSFEM_HIP(hipEventCreate(&e1));
+    for (int step = 0; step < steps; ++step) {
+        NSFEM_HIP(hipEventRecord(e0, d.stream));
+        double sec[3] = {0, 0, 0};
+        for (Simulation *sp : S) {
+            Simulation &s = *sp;
+            const int V = s.V;
+            Matrix &M = s.mats[NSFEM_VELOCITY_MASS];
+            launch_advect_f64(d, s.tree, V, s.uv.ptr, s.uv.ptr + V, s.opt.dt, s.utv.ptr,
+                              s.utv.ptr + V);
+            launch_spmv(d, M, s.utv.ptr, s.c_uv.ptr, s.rhs.ptr, 2);
+            launch_spmv(d, M, s.utv.ptr + V, s.c_uv.ptr + V, s.rhs.ptr + V, 2);
+            launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, s.rhs.ptr);
+            launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_v.ptr, s.rhs.ptr + V);
+            launch_copy(d, 2 * V, s.utv.ptr, s.uv.ptr);
+        }
+        for (size_t i = 0; i < I; ++i)
+            rq[i] = request(*S[i], NSFEM_DIFFUSION, S[i]->rhs.ptr, S[i]->uv.ptr, 2);
+        std::vector<nsfem_cg_stats> sd(2 * I);
+        check(run_cg_batch(d, rq, sd.data(), scale, &sec[0]), "diffusion");
+        for (Simulation *sp : S) {
+            Simulation &s = *sp;
+            launch_spmv(d, s.mats[NSFEM_VELOCITY_DIVERGENCE], s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
+            launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.prhs.ptr);
+            pressure_guess(s);
+        }
+        for (size_t i = 0; i < I; ++i)
+            rq[i] = request(*S[i], NSFEM_PRESSURE_STIFFNESS, S[i]->prhs.ptr, S[i]->p.ptr, 1);
+        std::vector<nsfem_cg_stats> sp_(2 * I);
+        check(run_cg_batch(d, rq, sp_.data(), scale, &sec[1]), "pressure");
+        for (Simulation *sp : S) {
+            Simulation &s = *sp;
+            launch_spmv(d, s.mats[NSFEM_PRESSURE_DIVERGENCE], s.p.ptr, nullptr, s.rhs.ptr, 0);
+            launch_copy(d, 2 * s.V, s.uv.ptr, s.du.ptr);
+        }
+        for (size_t i = 0; i < I; ++i)
+            rq[i] = request(*S[i], NSFEM_VELOCITY_MASS, S[i]->rhs.ptr, S[i]->du.ptr, 2);
+        std::vector<nsfem_cg_stats> sm(2 * I);
+        check(run_cg_batch(d, rq, sm.data(), scale, &sec[2]), "velocity mass");
+        for (Simulation *sp : S) {
+            Simulation &s = *sp;
+            launch_add(d, 2 * s.V, s.du.ptr, s.uv.ptr);
+            launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, s.uv.ptr);
+            launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_v.ptr, s.uv.ptr + s.V);
+        }
+        NSFEM_HIP(hipEventRecord(e1, d.stream));
+        NSFEM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        NSFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (stats) {
+            for (size_t i = 0; i < I; ++i) {
+                nsfem_step_stats &o = stats[i];
+                o.steps += 1;
+                o.seconds += ms * 1e-3;  // the whole batch's step
+                const nsfem_cg_stats *kinds[5] = {&sd[2 * i], &sd[2 * i + 1], &sp_[2 * i],
+                                                  &sm[2 * i], &sm[2 * i + 1]};
+                for (int k = 0; k < 5; ++k) {
+                    const nsfem_cg_stats &c = *kinds[k];
+                    o.cg_iterations[k] += c.iterations;
+                    o.cg_refinements[k] += c.refinements;
+                    const double rel = c.rhs_norm > 0 ? c.final_residual / c.rhs_norm
+                                                      : c.final_residual;
+                    o.max_relative_residual[k] = std::max(o.max_relative_residual[k], rel);
+                }
+                o.cg_seconds[0] += sec[0];
+                o.cg_seconds[2] += sec[1];
+                o.cg_seconds[3] += sec[2];
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+// Picks the wave-tile shape and the blocks per instance of batched solves:
+// every candidate runs `iters` iterations of the batched pressure solve on a
+// fixed right-hand side (b = 1, x0 = 0), and the fastest is applied to all
+// matrices of the batch.
+void simulation_autotune_batch(const std::vector<Simulation *> &S, nsfem_batch_tuning *out) {
+    NSFEM_REQUIRE(!S.empty(), NSFEM_INVALID_ARGUMENT, "empty batch");
+    Device &d = *S[0]->dev;
+    for (Simulation *s : S)
+        NSFEM_REQUIRE(s->dev == &d, NSFEM_INVALID_ARGUMENT, "a batch needs simulations of one device");
+    d.set_current();
+    const size_t I = S.size();
+    size_t total = 0;
+    std::vector<size_t> off(I);
+    for (size_t i = 0; i < I; ++i) {
+        off[i] = total;
+        total += (size_t)S[i]->P;
+    }
+    DeviceArray<double> b, x;
+    b.resize(total);
+    x.resize(total);
+    launch_fill(d, (int)total, 1.0, b.ptr);
+    const int iters = 40;
+    struct Shape { int rows, nnz; };
+    const Shape shapes[3] = {{64, 512}, {32, 256}, {16, 128}};
+    const int scales[3] = {1, 2, 4};
+    double best = 1e300;
+    Shape best_shape = shapes[0];
+    int best_scale = 1;
+    std::vector<nsfem_cg_stats> st(2 * I);
+    for (const Shape &sh : shapes) {
+        for (Simulation *s : S)
+            for (Matrix &m : s->mats)
+                if (m.ready && m.rows == m.cols) build_wave_tiles(m, sh.rows, sh.nnz);
+        for (int sc : scales) {
+            std::vector<CgRequest> rq(I);
+            for (size_t i = 0; i < I; ++i) {
+                CgRequest r{};
+                r.m = &S[i]->mats[NSFEM_PRESSURE_STIFFNESS];
+                r.b = b.ptr + off[i];
+                r.x = x.ptr + off[i];
+                r.ncol = 1;
+                r.max_iter = iters;
+                r.rtol = 0.0;
+                r.atol = 0.0;
+                r.precision = S[0]->opt.precision;
+                rq[i] = r;
+            }
+            launch_fill(d, (int)total, 0.0, x.ptr);
+            double sec = 0.0;
+            run_cg_batch(d, rq, st.data(), sc, &sec, false);
+            if (sec < best) {
+                best = sec;
+                best_shape = sh;
+                best_scale = sc;
+            }
+        }
+    }
+    for (Simulation *s : S) {
+        for (Matrix &m : s->mats)
+            if (m.ready && m.rows == m.cols) build_wave_tiles(m, best_shape.rows, best_shape.nnz);
+        s->batch_block_scale = best_scale;
+    }
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    if (out) {
+        out->rows_per_wave = best_shape.rows;
+        out->nnz_per_wave = best_shape.nnz;
+        out->block_scale = best_scale;
+        out->seconds_per_iteration = best / iters;
+    }
+}
+
 void simulation_get(Simulation &s, double *u, double *v, double *p) {
     Device &d = *s.dev;
     d.set_current();
@@ -380,6 +569,8 @@ void simulation_set(Simulation &s, const double *u, const double *v, const doubl
 }
 
 void simulation_detach(Simulation *s) {
+    for (Matrix &m : s->mats) matrix_release(m);
+    kdtree_release(s->tree);
     if (s->graph) gnn_graph_release(*s->graph);
     s->gnn = nullptr;
     s->warm_mode = NSFEM_WARM_PREVIOUS;
@@ -517,6 +708,77 @@ int nsfem_simulation_copy_pressure_system(nsfem_simulation *s, nsfem_buffer *rhs
         std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
         simulation_copy_pressure(*s->impl, rhs ? (double *)rhs->impl.mem.ptr : nullptr,
                                  p ? (double *)p->impl.mem.ptr : nullptr);
+    });
+}
+
+static std::vector<Simulation *> batch_of(nsfem_simulation *const *sims, int count) {
+    NSFEM_REQUIRE(sims && count > 0, NSFEM_INVALID_ARGUMENT, "empty batch");
+    std::vector<Simulation *> S;
+    for (int i = 0; i < count; ++i) {
+        NSFEM_REQUIRE(sims[i] && sims[i]->impl && sims[i]->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device in the batch");
+        for (Simulation *o : S)
+            NSFEM_REQUIRE(o != sims[i]->impl, NSFEM_INVALID_ARGUMENT,
+                          "a simulation appears twice in the batch");
+        S.push_back(sims[i]->impl);
+    }
+    return S;
+}
+
+int nsfem_simulation_step_batch(nsfem_simulation *const *sims, int count, int steps,
+                                nsfem_step_stats *stats) {
+    return guard([&] {
+        std::vector<Simulation *> S = batch_of(sims, count);
+        NSFEM_REQUIRE(steps >= 0, NSFEM_INVALID_ARGUMENT, "negative step count");
+        std::lock_guard<std::mutex> lk(S[0]->dev->mtx);
+        simulation_step_batch(S, steps, stats);
+    });
+}
+
+int nsfem_simulation_autotune_batch(nsfem_simulation *const *sims, int count,
+                                    nsfem_batch_tuning *out) {
+    return guard([&] {
+        std::vector<Simulation *> S = batch_of(sims, count);
+        std::lock_guard<std::mutex> lk(S[0]->dev->mtx);
+        simulation_autotune_batch(S, out);
+    });
+}
+
+int nsfem_simulation_matrix_info(nsfem_simulation *s, int slot, int *rows, int *cols,
+                                 int64_t *nnz) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        NSFEM_REQUIRE(slot >= 0 && slot < 5, NSFEM_INVALID_ARGUMENT, "matrix id out of range");
+        const Matrix &m = s->impl->mats[slot];
+        if (rows) *rows = m.rows;
+        if (cols) *cols = m.cols;
+        if (nnz) *nnz = m.nnz;
+    });
+}
+
+int nsfem_simulation_sp_r_mult(nsfem_simulation *s, int slot, const nsfem_buffer *lhs,
+                               const nsfem_buffer *x, nsfem_buffer *y) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        NSFEM_REQUIRE(slot >= 0 && slot < 5, NSFEM_INVALID_ARGUMENT, "matrix id out of range");
+        Simulation &sim = *s->impl;
+        Matrix &m = sim.mats[slot];
+        auto ok = [&](const nsfem_buffer *b, int64_t n) {
+            return b && b->impl.dev == sim.dev && (int64_t)b->impl.mem.bytes() >= n * 8;
+        };
+        NSFEM_REQUIRE(ok(x, m.cols) && ok(y, m.rows) && (!lhs || ok(lhs, m.rows)),
+                      NSFEM_INVALID_ARGUMENT,
+                      "buffers must be on the simulation's device and hold cols (x) / rows doubles");
+        NSFEM_REQUIRE(x->impl.mem.ptr != y->impl.mem.ptr, NSFEM_INVALID_ARGUMENT,
+                      "x must not alias y");
+        std::lock_guard<std::mutex> lk(sim.dev->mtx);
+        sim.dev->set_current();
+        // stream-ordered like nsfem_sp_r_mult: no host synchronisation
+        launch_spmv(*sim.dev, m, (const double *)x->impl.mem.ptr,
+                    lhs ? (const double *)lhs->impl.mem.ptr : nullptr, (double *)y->impl.mem.ptr,
+                    lhs ? 1 : 0);
     });
 }
 

@@ -184,8 +184,7 @@ def collect_pairs(device, meshes, viscosity=1e-3, dt=5e-4, steps=40, warmup=5, e
             p = DeviceBuffer(device, 8 * P)
             sim.copy_pressure_system(b, p)
             if prev is not None:
-                # the simulation's pressure matrix is in slot 0 while it lives
-                device.sp_r_mult_sub(0, b, prev, b)     # b <- b - A p_prev
+                sim.sp_r_mult(0, prev, b, lhs=b)         # b <- b - A p_prev
                 device.saxpby(P, 1.0, -1.0, p, prev, p)  # p <- p - p_prev
             pairs.append((graph, b, p))
         if prev is not None:

@@ -140,6 +140,22 @@ def cavity(n: int, lid_speed: float = 1.0) -> Mesh:
     return m
 
 
+def cavity_batch(count: int = 64, seed: int = 3, n_min: int = 16, n_max: int = 256):
+    """A batch of independent lid-driven cavities of uneven size: side
+    ``n ~ U{n_min..n_max}``, lid speed ``U ~ U(0.5, 2)``, viscosity
+    ``nu ~ logU(1e-3, 1e-2)`` (seeded).  Returns ``[(mesh, nu), ...]``."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(n_min, n_max + 1))
+        lid = float(rng.uniform(0.5, 2.0))
+        nu = float(np.exp(rng.uniform(np.log(1e-3), np.log(1e-2))))
+        m = cavity(n, lid_speed=lid)
+        m.name = f"cavity{n}_lid{lid:.3f}"
+        out.append((m, nu))
+    return out
+
+
 def _inflow(y: np.ndarray, umax: float, height: float) -> np.ndarray:
     # 4*Um*y*(H-y)/H^2, the inflow profile of assets/mesh_small.json
     return 4.0 * umax * y * (height - y) / (height * height)

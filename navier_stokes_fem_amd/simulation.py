@@ -14,7 +14,7 @@ import numpy as np
 from . import _native as N
 from .device import GPUSimulationDevice
 
-__all__ = ["Simulation"]
+__all__ = ["Simulation", "step_batch", "autotune_batch"]
 
 
 class Simulation:
@@ -76,6 +76,21 @@ class Simulation:
         """Zero fields with the velocity Dirichlet values imposed."""
         N.check(N.lib().nsfem_simulation_reset(self.handle))
 
+    def matrix_info(self, matrix: int):
+        """(rows, cols, nnz) of one of the simulation's matrices (ids as the
+        device slots: 0 pressure stiffness, 1 velocity mass, 2 diffusion,
+        3 velocity divergence, 4 pressure divergence)."""
+        r, c, z = C.c_int(0), C.c_int(0), C.c_int64(0)
+        N.check(N.lib().nsfem_simulation_matrix_info(self.handle, int(matrix), C.byref(r),
+                                                       C.byref(c), C.byref(z)))
+        return r.value, c.value, z.value
+
+    def sp_r_mult(self, matrix: int, x, y, lhs=None):
+        """y = A x, or y = lhs - A x, with one of the simulation's matrices."""
+        N.check(N.lib().nsfem_simulation_sp_r_mult(self.handle, int(matrix),
+                                                   lhs.handle if lhs is not None else None,
+                                                   x.handle, y.handle))
+
     WARM_MODES = {"previous": N.WARM_PREVIOUS, "zero": N.WARM_ZERO, "learned": N.WARM_LEARNED,
                   "learned_or_previous": N.WARM_LEARNED_OR_PREVIOUS,
                   "learned_correction": N.WARM_LEARNED_CORRECTION}
@@ -106,3 +121,27 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+
+def _handles(sims):
+    arr = (C.c_void_p * len(sims))(*[s.handle.value if hasattr(s.handle, "value") else s.handle
+                                     for s in sims])
+    return arr
+
+
+def step_batch(sims, steps: int = 1) -> list:
+    """Advance several simulations (one device, one precision) together:
+    each solve kind runs for all of them in one persistent launch.  Returns
+    one stats dict per simulation (seconds are the whole batch's)."""
+    stats = (N.StepStats * len(sims))()
+    N.check(N.lib().nsfem_simulation_step_batch(_handles(sims), len(sims), int(steps),
+                                                C.cast(stats, C.c_void_p)))
+    return [s.as_dict() for s in stats]
+
+
+def autotune_batch(sims) -> dict:
+    """Pick the wave-tile shape and blocks per instance of the batched solves
+    by timing the candidates; applied to every matrix of the batch."""
+    t = N.BatchTuning()
+    N.check(N.lib().nsfem_simulation_autotune_batch(_handles(sims), len(sims), C.byref(t)))
+    return t.as_dict()

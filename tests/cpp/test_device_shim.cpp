@@ -2,7 +2,9 @@
 // reference's solver drives GPUSimulation::GPUSimulationDevice
 // (include/assembly.h:1302-1557): triplet upload, products, BLAS-1, the
 // CG call with an absolute eps, and advection on a small P2 mesh.  Checks
-// are closed forms computed here on the host.  Exit status 0 = all passed.
+// are closed forms computed here on the host.  Built as a program (exit
+// status 0 = all passed) and as a shared library whose
+// nsfem_device_shim_selftest() the GPU test calls in its own process.
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -67,7 +69,8 @@ static std::vector<double> host_mult(const Triplet &t, const std::vector<double>
     return y;
 }
 
-int main() {
+static int run_all() {
+    failures = 0;
     Dev dev;
     ErrorCode ec = dev.init(0);
     if (ec.hasError()) {
@@ -178,5 +181,13 @@ int main() {
     CHECK(ec.hasError() && ec.getMessage()[0] != 0, "empty slot accepted");
 
     std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "PASSED", failures);
-    return failures ? 1 : 0;
+    std::fflush(stdout);
+    return failures;
 }
+
+#ifdef NSFEM_SHIM_LIBRARY
+// loaded into the test process (ctypes): returns the number of failures
+extern "C" int nsfem_device_shim_selftest() { return run_all(); }
+#else
+int main() { return run_all() ? 1 : 0; }
+#endif
