@@ -185,6 +185,188 @@ def cpu_baseline(m, nu, dt, gpu_iters_per_step, sample_iters):
     }
 
 
+def _line(metric, value, unit, steps, ms, config, **extra):
+    out = {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1, "steps": steps,
+           "ms_per_step": round(ms, 3), "higher_is_better": True, "vs_baseline": None,
+           "data": "synthetic", "config": config}
+    out.update(extra)
+    return out
+
+
+def bench_c1(dev, args):
+    """32x32 cavity (2048 P2-P1 elements), nu = 0.01, dt = 0.01, 100 steps of
+    the plain FP64 method on the GPU; the cpu_baseline leg runs the same 100
+    steps in the CPU oracle and reports its speed and the field difference."""
+    import numpy as np
+
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation
+    from oracle import oracle as O
+
+    steps = 100
+    m = mesh.cavity(32)
+    nu, dt = 0.01, 0.01
+    sim = Simulation(m, nu, dt, device=dev, rtol=1e-8, precision="fp64")
+    dev.synchronize()
+    t = time.perf_counter()
+    st = sim.step(steps)
+    wall = time.perf_counter() - t
+    u, v, p = sim.fields()
+    sim.close()
+    asm = O.Assembly(m, nu, dt)
+    tree = O.kd_build(m.xy(), m.elems())
+    ou, ov, op = O.initial_fields(m)
+    t = time.perf_counter()
+    for _ in range(steps):
+        ou, ov, op, _ = O.step(asm, tree, m, dt, ou, ov, op, rtol=1e-8)
+    cpu = time.perf_counter() - t
+    err = {k: float(np.abs(a - b).max()) for k, a, b in (("u", u, ou), ("v", v, ov), ("p", p, op))}
+    scale = {k: float(np.abs(b).max()) for k, b in (("u", ou), ("v", ov), ("p", op))}
+    log(f"c1: {steps / wall:.1f} steps/s on the GPU, {steps / cpu:.1f} on the CPU oracle; "
+        f"max |gpu - oracle| {err}")
+    cpu_baseline = {"value": round(steps / cpu, 3), "unit": "steps/s", "cores": O.get_threads(),
+                    "kind": "port",
+                    "sample": "the whole workload: 100 oracle steps (FP64 CG + KD advection) "
+                              "from the same initial state",
+                    "field_max_abs_difference_gpu_vs_oracle": err, "field_max_abs": scale}
+    return _line("steps/sec, 32x32 cavity, FP64 CG at 1e-8 rel. residual", steps / wall,
+                 "steps/s", steps, 1e3 * wall / steps,
+                 {"workload": "lid-driven cavity 32x32 squares (2048 P2-P1 triangles, 4225 velocity "
+                              "/ 1089 pressure nodes), nu=0.01, dt=0.01, 100 steps"},
+                 dtype="f64", cg_iterations_per_step=[x / steps for x in st["cg_iterations"]],
+                 cpu_baseline=cpu_baseline)
+
+
+def bench_c2(dev, args):
+    """512x128 channel past the disk (jitter seed 2), nu = 1e-3, dt = 5e-4,
+    200 steps in each form."""
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    steps = 200
+    m = mesh.channel(512, 128, seed=2)
+    sim = Simulation(m, 1e-3, 5e-4, device=dev, rtol=1e-8, precision="fp64")
+    res = {}
+    for form in ("fp64", "fp32_refined"):
+        sim.reset()
+        sim.configure(precision=form)
+        dev.synchronize()
+        t = time.perf_counter()
+        st = sim.step(steps)
+        wall = time.perf_counter() - t
+        res[form] = {"steps_per_s": round(steps / wall, 2),
+                     "cg_iterations_per_step": [x / steps for x in st["cg_iterations"]],
+                     "max_relative_residual": st["max_relative_residual"]}
+        log(f"c2 {form}: {steps / wall:.1f} steps/s, iterations/step {res[form]['cg_iterations_per_step']}")
+    sim.close()
+    best = max(res, key=lambda f: res[f]["steps_per_s"])
+    return _line("steps/sec, 512x128 channel, 1e-8 rel. residual", res[best]["steps_per_s"],
+                 "steps/s", steps, 1e3 / res[best]["steps_per_s"],
+                 {"workload": f"channel [0,2.2]x[0,0.41] 512x128 jittered (seed 2) minus the disk: "
+                              f"{m.element_count} P2-P1 triangles, {m.node_count} velocity / "
+                              f"{m.pressure_count} pressure nodes, nu=1e-3, dt=5e-4, 200 steps",
+                  "form": best},
+                 dtype="f32+f64refine" if best == "fp32_refined" else "f64", forms=res)
+
+
+def bench_c3(dev, args):
+    """64 cavities, side n ~ U{16..256}, lid ~ U(0.5, 2), nu ~ logU(1e-3,
+    1e-2) (seed 3), dt = 0.005, 50 steps in FP32-refined form: all 64 in one
+    batch (each solve kind one persistent launch for all), after the
+    autotuner, against the same 64 stepped one after another."""
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation, autotune_batch, step_batch
+
+    steps = 50
+    specs = mesh.cavity_batch(64, seed=3)
+    total_p = sum(m.pressure_count for m, _ in specs)
+    sims = [Simulation(m, nu, 0.005, device=dev, rtol=1e-8, precision="fp32_refined")
+            for m, nu in specs]
+    tune = autotune_batch(sims)
+    log(f"c3: autotune {tune}")
+    for s in sims:
+        s.reset()
+    dev.synchronize()
+    t = time.perf_counter()
+    stats = step_batch(sims, steps)
+    batched = time.perf_counter() - t
+    for s in sims:
+        s.reset()
+    dev.synchronize()
+    t = time.perf_counter()
+    for s in sims:
+        s.step(steps)
+    serial = time.perf_counter() - t
+    log(f"c3: batched {steps / batched:.2f} batch-steps/s, one by one {steps / serial:.2f}")
+    its = [sum(st["cg_iterations"][k] for st in stats) / steps for k in range(5)]
+    for s in sims:
+        s.close()
+    return _line("batch steps/sec (all 64 instances advanced one step), 1e-8 rel. residual",
+                 steps / batched, "steps/s", steps, 1e3 * batched / steps,
+                 {"workload": f"64 lid-driven cavities, n ~ U{{16..256}} (seed 3), "
+                              f"{total_p} pressure nodes in total, lid ~ U(0.5,2), "
+                              f"nu ~ logU(1e-3,1e-2), dt=0.005, 50 steps", "form": "fp32_refined"},
+                 dtype="f32+f64refine", autotune=tune,
+                 one_by_one_steps_per_s=round(steps / serial, 3),
+                 batching_speedup=round(serial / batched, 3),
+                 cg_iterations_per_batch_step=its)
+
+
+def bench_c5(dev, args):
+    """Learned warm start: trained on (rhs, pressure) pairs of 512x128
+    channels with inflow max ~ U(0.5, 2) and reseeded jitter; evaluated on
+    held-out seeds at 512x128 and 1024x256 (pressure CG iterations/step and
+    steps/s per initial guess)."""
+    import numpy as np
+
+    from navier_stokes_fem_amd import learned as L
+    from navier_stokes_fem_amd import mesh
+
+    rng = np.random.default_rng(2024)
+    n_train, per = 24, 40
+    train = [mesh.channel(512, 128, seed=1000 + k, umax=float(rng.uniform(0.5, 2.0)))
+             for k in range(n_train)]
+    t = time.perf_counter()
+    nets = {}
+    for target in ("solution", "correction"):
+        pairs = L.collect_pairs(dev, train, dt=5e-4, steps=per, warmup=10, every=2, target=target,
+                                log=None)
+        log(f"c5: {len(pairs)} {target} pairs in {time.perf_counter() - t:.1f} s")
+        net = L.WarmStartNet(dev, seed=1)
+        hist = L.train(net, pairs, epochs=6, lr=2e-3, lr_final=1e-4)
+        log(f"c5: {target} training loss per epoch {[round(h, 4) for h in hist]}")
+        nets[target] = (net, hist, len(pairs))
+        for g, b, p in pairs:
+            b.close()
+            p.close()
+    evals = {}
+    for nx, ny, seed in ((512, 128, 900001), (1024, 256, 900002)):
+        held = mesh.channel(nx, ny, seed=seed, umax=1.25)
+        r = L.evaluate_warm_start(dev, held, nets["solution"][0], steps=20, warmup=10, dt=5e-4,
+                                  modes=("previous", "zero", "learned", "learned_or_previous",
+                                         "learned_correction"),
+                                  correction_net=nets["correction"][0])
+        evals[f"{nx}x{ny}"] = r
+        for mode, v in r.items():
+            log(f"c5 {nx}x{ny} {mode}: {v['pressure_iterations_per_step']:.1f} its/step, "
+                f"{v['steps_per_s']:.1f} steps/s")
+    base = evals["1024x256"]["previous"]
+    best_mode = min(evals["1024x256"], key=lambda k: evals["1024x256"][k]["pressure_iterations_per_step"])
+    for net, _, _ in nets.values():
+        net.close()
+    return _line("pressure CG iterations/step with the learned warm start vs previous-step guess "
+                 "(1024x256 held-out channel)",
+                 evals["1024x256"]["learned_correction"]["pressure_iterations_per_step"],
+                 "iterations/step", 20, 1e3 / evals["1024x256"]["learned_correction"]["steps_per_s"],
+                 {"workload": "channel 512x128 training meshes (24 seeds, inflow max ~ U(0.5,2)), "
+                              "held-out seeds at 512x128 and 1024x256, nu=1e-3, dt=5e-4",
+                  "form": "fp64"},
+                 higher_is_better=False, dtype="bf16 net (fp32 accumulate) + f64 CG",
+                 training={k: {"pairs": v[2], "loss_per_epoch": v[1]} for k, v in nets.items()},
+                 evaluation=evals, previous_guess_iterations=base["pressure_iterations_per_step"],
+                 best_mode=best_mode)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +376,10 @@ def main():
     ap.add_argument("--form", choices=["both", "fp32_refined", "fp64"], default="both")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=10)
+    ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5"], default="c4",
+                    help="c4 (default): the 2048^2 cavity headline; c1: 32^2 cavity, fields vs "
+                         "the CPU oracle; c2: 512x128 channel; c3: 64-cavity batch; c5: learned "
+                         "warm start on channels")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only nsfem_saxpy on 2^26 doubles x 5 (known HBM bytes: "
                          "16 B read + 8 B written per element) to scale rocprofv3 "
@@ -210,6 +396,14 @@ def main():
 
     from navier_stokes_fem_amd import mesh
     from navier_stokes_fem_amd.device import GPUSimulationDevice
+
+    if args.config != "c4":
+        runner = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c5": bench_c5}[args.config]
+        dev = GPUSimulationDevice(0)
+        out = runner(dev, args)
+        dev.close()
+        print(json.dumps(out), flush=True)
+        return
 
     if args.calibrate:
         dev = GPUSimulationDevice(0)
