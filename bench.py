@@ -90,9 +90,10 @@ def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
             "achieved_GBs": round(byt / sec / 1e9, 1), "frac": round(byt / sec / 1e9 / PEAK_HBM_GBS, 4)}
 
 
-def run_form(sim, form, warmup, steps, n_p, nnz_p):
+def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous"):
     sim.reset()
     sim.configure(precision=form, rtol=1e-8, pair_velocity=True)
+    sim.set_warm_start(warm)
     for i in range(warmup):
         sim.step(1)
         log(f"{form}: warmup step {i + 1}/{warmup} done")
@@ -376,6 +377,9 @@ def main():
     ap.add_argument("--form", choices=["both", "fp32_refined", "fp64"], default="both")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=10)
+    ap.add_argument("--warm", choices=["previous", "extrapolate"], default="previous",
+                    help="initial guess of the chosen (fp32_refined) form's pressure solves; the "
+                         "fp64 form always starts from the previous pressure, as the reference")
     ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5"], default="c4",
                     help="c4 (default): the 2048^2 cavity headline; c1: 32^2 cavity, fields vs "
                          "the CPU oracle; c2: 512x128 channel; c3: 64-cavity batch; c5: learned "
@@ -438,7 +442,8 @@ def main():
     results = {}
     forms = ["fp64", "fp32_refined"] if args.form == "both" else [args.form]
     for form in forms:
-        wall, st = run_form(sim, form, args.warmup, args.steps, n_p, nnz_p)
+        wall, st = run_form(sim, form, args.warmup, args.steps, n_p, nnz_p,
+                            warm=args.warm if form == "fp32_refined" else "previous")
         results[form] = (wall, st)
         log(f"{form}: {args.steps / wall:.3f} steps/s")
 
@@ -476,7 +481,8 @@ def main():
                                f"({m.element_count} P2-P1 triangles, {m.node_count} velocity / "
                                f"{m.pressure_count} pressure nodes), nu=1e-3, dt=1e-3, "
                                f"rel tol 1e-8 per solve",
-                   "form": chosen, "pair_velocity": True, "parallelism": "single GPU"},
+                   "form": chosen, "pair_velocity": True, "parallelism": "single GPU",
+                   "pressure_initial_guess": args.warm if chosen == "fp32_refined" else "previous"},
         "roofline": roofline,
     }
     for form, (w, s) in results.items():

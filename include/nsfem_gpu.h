@@ -382,6 +382,8 @@ int nsfem_gnn_train_step(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
 #define NSFEM_WARM_LEARNED_OR_PREVIOUS 3 /* the one with the smaller residual */
 #define NSFEM_WARM_LEARNED_CORRECTION 4  /* p_prev + network(b - A p_prev) if its
                                             residual is smaller, else p_prev */
+#define NSFEM_WARM_EXTRAPOLATE 5         /* 2 p_n - p_(n-1) if its residual is
+                                            smaller, else p_n (no network) */
 int nsfem_simulation_set_warm_start(nsfem_simulation *sim, int mode, nsfem_gnn *gnn);
 /* Copies the last step's pressure right-hand side and pressure into device
  * buffers (either may be NULL): training pairs for the network. */

@@ -31,6 +31,7 @@ WARM_ZERO = 1
 WARM_LEARNED = 2
 WARM_LEARNED_OR_PREVIOUS = 3
 WARM_LEARNED_CORRECTION = 4
+WARM_EXTRAPOLATE = 5
 GNN_PARAM_COUNT = 4481
 
 

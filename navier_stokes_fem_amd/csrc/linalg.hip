@@ -449,6 +449,20 @@ __device__ unsigned long long g_cg_trace[kTraceIters][5];
         if (threadIdx.x == 0 && (it) < kTraceIters)                                 \
             atomicMax(&g_cg_trace[(it)][(slot)], (unsigned long long)wall_clock64()); \
     } while (0)
+}  // namespace
+// trace builds only: read / clear the phase timestamps
+extern "C" int nsfem_trace_read(unsigned long long *out, int iters) {
+    const int n = iters < kTraceIters ? iters : kTraceIters;
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_cg_trace), sizeof(unsigned long long) * 5 * n) !=
+        hipSuccess)
+        return -1;
+    return n;
+}
+extern "C" int nsfem_trace_clear() {
+    static unsigned long long zero[kTraceIters][5];
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_cg_trace), zero, sizeof(zero)) == hipSuccess ? 0 : -1;
+}
+namespace {
 #else
 #define CG_TRACE_STAMP(it, slot) \
     do {                         \
@@ -1023,9 +1037,14 @@ __device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const in
                 if (active[c]) {
                     iters[c] = k;
                     if (MIXED) {
+                        // a drop by rdelta triggers the update once
+                        // ru_min_gap iterations have passed, a drop by
+                        // rdelta^2 at once (FP32 rounding must not pile up
+                        // over a steep fall of the residual)
+                        const double d2 = A.rdelta * A.rdelta;
                         dropped[c] = g[c] <= tol2[c] ||
-                                     (g[c] < A.rdelta * A.rdelta * rmax2[c] &&
-                                      k - last_ru >= A.ru_min_gap);
+                                     (g[c] < d2 * rmax2[c] &&
+                                      (k - last_ru >= A.ru_min_gap || g[c] < d2 * d2 * rmax2[c]));
                         want_ru = want_ru || dropped[c];
                     } else if (g[c] <= tol2[c]) {
                         active[c] = false;  // its last update is still pending
@@ -1194,12 +1213,13 @@ void set_cg_params(const Device &d, const CgRequest &req, int n, CgArgs &a) {
     a.rtol = req.rtol;
     a.atol = req.atol;
     // reliable updates of the mixed solver: when the recursive residual has
-    // fallen by rdelta since the last one (but not sooner than ru_min_gap
-    // iterations after it, which matters on fast-converging systems), or
-    // after ru_gap iterations.  Forced updates every 1000 iterations cost the
-    // 2048^2 cavity pressure solve 26% more iterations, so ru_gap is only a
-    // backstop.  Tuning overrides: NSFEM_CG_RDELTA, NSFEM_CG_RU_GAP,
-    // NSFEM_CG_RU_MIN_GAP.
+    // fallen by rdelta since the last one, or after ru_gap iterations.
+    // Measured on the 2048^2 cavity pressure matrix from x0 = 0: forced
+    // updates every 1000 iterations cost 26% more iterations, and so does
+    // holding updates back for 8 iterations after the last one (the default
+    // minimum gap is therefore 0; it saves time only on systems that
+    // converge in a few dozen iterations).  Tuning overrides:
+    // NSFEM_CG_RDELTA, NSFEM_CG_RU_GAP, NSFEM_CG_RU_MIN_GAP.
     static const double rdelta_env = [] {
         const char *e = std::getenv("NSFEM_CG_RDELTA");
         return e ? std::atof(e) : 0.0;
@@ -1216,7 +1236,7 @@ void set_cg_params(const Device &d, const CgRequest &req, int n, CgArgs &a) {
         return e ? std::atoi(e) : -1;
     }();
     a.ru_gap = ru_gap_env > 0 ? ru_gap_env : 20000;
-    a.ru_min_gap = ru_min_gap_env >= 0 ? ru_min_gap_env : 8;
+    a.ru_min_gap = ru_min_gap_env >= 0 ? ru_min_gap_env : 0;
     a.timeout_ticks = (unsigned long long)d.wall_khz * 1000ull * 20ull;  // 20 s per barrier
 }
 

@@ -46,6 +46,8 @@ struct Simulation {
     std::vector<double> p_area;  // lumped P1 area of each pressure node
     std::vector<float> p_xy;     // coordinates of the pressure nodes
     DeviceArray<double> pguess, ptmp, pscal;
+    DeviceArray<double> p_older, p_keep;  // p_(n-1) and p_n for the extrapolation
+    bool have_older = false;
     int batch_block_scale = 1;  // blocks per instance in batched solves (autotuned)
 };
 
@@ -155,6 +157,7 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
 void simulation_reset(Simulation &s) {
     Device &d = *s.dev;
     d.set_current();
+    s.have_older = false;
     launch_fill(d, 2 * s.V, 0.0, s.uv.ptr);
     launch_fill(d, s.P, 0.0, s.p.ptr);
     launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, s.uv.ptr);
@@ -218,6 +221,16 @@ void pressure_guess(Simulation &s) {
         select_better_guess(d, s.mats[NSFEM_PRESSURE_STIFFNESS], s.prhs.ptr, s.pguess.ptr, s.p.ptr,
                             s.ptmp.ptr, s.pscal.ptr);
         return;
+    case NSFEM_WARM_EXTRAPOLATE:
+        // p0 = 2 p_n - p_(n-1) when it has the smaller residual; p_n is kept
+        // for the next step's extrapolation
+        launch_copy(d, s.P, s.p.ptr, s.p_keep.ptr);
+        if (s.have_older) {
+            launch_axpby(d, s.P, -1.0, 2.0, s.p_older.ptr, s.p.ptr, s.pguess.ptr);
+            select_better_guess(d, s.mats[NSFEM_PRESSURE_STIFFNESS], s.prhs.ptr, s.pguess.ptr,
+                                s.p.ptr, s.ptmp.ptr, s.pscal.ptr);
+        }
+        return;
     case NSFEM_WARM_LEARNED_CORRECTION: {
         // r = b - A p_prev; guess = p_prev + network(r), kept only if its
         // residual is smaller than the previous pressure's
@@ -231,10 +244,17 @@ void pressure_guess(Simulation &s) {
     }
 }
 
+// after the pressure solve: the extrapolating mode's p_n becomes p_(n-1)
+void pressure_after_solve(Simulation &s) {
+    if (s.warm_mode != NSFEM_WARM_EXTRAPOLATE) return;
+    std::swap(s.p_older, s.p_keep);
+    s.have_older = true;
+}
+
 }  // namespace
 
 void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
-    NSFEM_REQUIRE(mode >= NSFEM_WARM_PREVIOUS && mode <= NSFEM_WARM_LEARNED_CORRECTION,
+    NSFEM_REQUIRE(mode >= NSFEM_WARM_PREVIOUS && mode <= NSFEM_WARM_EXTRAPOLATE,
                   NSFEM_INVALID_ARGUMENT, "unknown warm-start mode");
     const bool learned = mode == NSFEM_WARM_LEARNED || mode == NSFEM_WARM_LEARNED_OR_PREVIOUS ||
                          mode == NSFEM_WARM_LEARNED_CORRECTION;
@@ -256,11 +276,17 @@ void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
         s.graph = gnn_graph_create(d, s.P, A.nnz, rs.data(), col.data(), val.data(),
                                    s.p_area.data(), s.p_xy.data());
     }
-    if (mode == NSFEM_WARM_LEARNED_OR_PREVIOUS || mode == NSFEM_WARM_LEARNED_CORRECTION) {
+    if (mode == NSFEM_WARM_LEARNED_OR_PREVIOUS || mode == NSFEM_WARM_LEARNED_CORRECTION ||
+        mode == NSFEM_WARM_EXTRAPOLATE) {
         s.pguess.resize(s.P);
         s.ptmp.resize(s.P);
         s.pscal.resize(2);
     }
+    if (mode == NSFEM_WARM_EXTRAPOLATE) {
+        s.p_older.resize(s.P);
+        s.p_keep.resize(s.P);
+    }
+    s.have_older = false;
     s.gnn = learned ? g : nullptr;
     s.warm_mode = mode;
 }
@@ -319,6 +345,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.prhs.ptr);
         pressure_guess(s);
         solve(s, NSFEM_PRESSURE_STIFFNESS, s.prhs.ptr, s.p.ptr, 1, o[2]);
+        pressure_after_solve(s);
         // 4. velocity correction
         launch_spmv(d, PD, s.p.ptr, nullptr, s.rhs.ptr, 0);
         launch_copy(d, 2 * V, s.uv.ptr, s.du.ptr);
@@ -431,6 +458,7 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
             rq[i] = request(*S[i], NSFEM_PRESSURE_STIFFNESS, S[i]->prhs.ptr, S[i]->p.ptr, 1);
         std::vector<nsfem_cg_stats> sp_(2 * I);
         check(run_cg_batch(d, rq, sp_.data(), scale, &sec[1]), "pressure");
+        for (Simulation *sp : S) pressure_after_solve(*sp);
         for (Simulation *sp : S) {
             Simulation &s = *sp;
             launch_spmv(d, s.mats[NSFEM_PRESSURE_DIVERGENCE], s.p.ptr, nullptr, s.rhs.ptr, 0);
@@ -575,6 +603,8 @@ void simulation_detach(Simulation *s) {
     s->gnn = nullptr;
     s->warm_mode = NSFEM_WARM_PREVIOUS;
     s->pguess.release();
+    s->p_older.release();
+    s->p_keep.release();
     s->ptmp.release();
     s->pscal.release();
     s->uv.release();

@@ -243,7 +243,7 @@ def evaluate_warm_start(device, mesh, net, modes=("previous", "zero", "learned",
         if warmup:
             sim.step(warmup)
         model = correction_net if mode == "learned_correction" else net
-        sim.set_warm_start(mode, model if mode.startswith("learned") else None)
+        sim.set_warm_start(mode, model if mode.startswith("learned") else None)  # noqa
         device.synchronize()
         t = time.perf_counter()
         st = sim.step(steps)
