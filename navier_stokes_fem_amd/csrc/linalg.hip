@@ -73,6 +73,21 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *lds) {
     __syncthreads();
 }
 
+// Matrix streams (column indices, values) are read once per product: load
+// them non-temporally so they do not evict the vector entries the gathers
+// reuse from L2.
+#ifndef NSFEM_STREAM_LOADS
+#define NSFEM_STREAM_LOADS 1
+#endif
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+#if NSFEM_STREAM_LOADS
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+
 template <typename T>
 __device__ __forceinline__ T fma_t(T a, T b, T c);
 template <>
@@ -213,8 +228,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, const int32_t *__r
         for (int i = 0; i < kItems; ++i) {
             const int k = tid + i * kBlock;
             if (k < nz) {
-                cj[i] = col[k0 + k];
-                vj[i] = val[k0 + k];
+                cj[i] = ld_stream(col + k0 + k);
+                vj[i] = ld_stream(val + k0 + k);
             }
         }
 #pragma unroll
@@ -793,7 +808,7 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
 #pragma unroll
     for (int i = 0; i < kWItems; ++i) {
         const int k = lane + 64 * i;
-        cn[i] = A.col[k < w.nz ? w.k0 + k : 0];
+        cn[i] = ld_stream(A.col + (k < w.nz ? w.k0 + k : 0));
     }
     for (; t < T1; t += NW) {
         const WTile cur = w;
@@ -807,7 +822,7 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
 #pragma unroll
             for (int i = 0; i < kWItems; ++i) {
                 const int k = lane + 64 * i;
-                cn[i] = A.col[k < w.nz ? w.k0 + k : 0];
+                cn[i] = ld_stream(A.col + (k < w.nz ? w.k0 + k : 0));
             }
         }
         // 2. owner rows (clamped so the loads stay unconditional)
@@ -836,7 +851,7 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
 #pragma unroll
                 for (int i = 0; i < IB; ++i) {
                     const int k = lane + 64 * (i0 + i);
-                    vj[i] = val[k < cur.nz ? cur.k0 + k : 0];
+                    vj[i] = ld_stream(val + (k < cur.nz ? cur.k0 + k : 0));
                     rj[i] = ldv<NC, VT>(r, cj[i0 + i]);
                     pj[i] = ldv<NC, VT>(pold, cj[i0 + i]);
                 }
