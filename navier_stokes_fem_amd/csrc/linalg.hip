@@ -128,6 +128,46 @@ std::vector<int32_t> cut_rows(const std::vector<int32_t> &rs, int rows, int max_
 }
 }  // namespace
 
+namespace {
+// dcol[k] = col[k] - first row of k's wave tile, one wave per tile; bad[0]
+// is raised when an offset does not fit 16 bits.
+__global__ void k_col_delta(int n_wtiles, const int4 *__restrict__ meta,
+                            const int32_t *__restrict__ col, int16_t *__restrict__ dcol,
+                            int *bad) {
+    const int t = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (t >= n_wtiles) return;
+    const int4 m = meta[t];
+    int out = 0;
+    for (int k = threadIdx.x & 63; k < m.w; k += 64) {
+        const int dv = col[m.z + k] - m.x;
+        out |= (dv < -32768 || dv > 32767) ? 1 : 0;
+        dcol[m.z + k] = (int16_t)dv;
+    }
+    if (out) __hip_atomic_fetch_or(bad, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The CG streams 16-bit column offsets instead of 32-bit columns when every
+// offset from its wave tile's first row fits (banded numberings: the
+// offsets span about twice the mesh width).
+void build_col_delta(Matrix &m) {
+    m.dcol_ok = false;
+    m.dcol.release();
+    if (m.nnz <= 0 || m.rows != m.cols || m.n_wtiles <= 0) return;
+    m.dcol.resize((size_t)m.nnz);
+    DeviceArray<int> bad;
+    bad.resize(1);
+    NSFEM_HIP(hipMemset(bad.ptr, 0, sizeof(int)));
+    constexpr int kWavesPerBlock = 4;
+    k_col_delta<<<(m.n_wtiles + kWavesPerBlock - 1) / kWavesPerBlock, 64 * kWavesPerBlock>>>(
+        m.n_wtiles, m.wmeta.ptr, m.col.ptr, m.dcol.ptr, bad.ptr);
+    NSFEM_HIP(hipGetLastError());
+    int h = 1;
+    NSFEM_HIP(hipMemcpy(&h, bad.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    m.dcol_ok = h == 0;
+    if (!m.dcol_ok) m.dcol.release();
+}
+}  // namespace
+
 void build_tiles(Matrix &m, const std::vector<int32_t> &rs) {
     std::vector<int32_t> t = cut_rows(rs, m.rows, kTileRows, kTileNnz);
     m.n_tiles = (int)t.size() - 1;
@@ -161,6 +201,7 @@ void build_wave_tiles(Matrix &m, int rows, int nnz) {
     NSFEM_HIP(hipMemcpy(m.wmeta.ptr, meta.data(), meta.size() * sizeof(int4),
                         hipMemcpyHostToDevice));
     m.block_tiles_grid = 0;
+    build_col_delta(m);
 }
 
 // Persistent blocks own contiguous wave-tile ranges balanced by (nnz + rows).
@@ -494,6 +535,7 @@ struct CgArgs {
     const int32_t *block_tiles;
     const int32_t *rs;
     const int32_t *col;
+    const int16_t *dcol;  // col - wave tile's first row (int16 kernels)
     const double *val64;
     const float *val32;
     const double *b;  // [NC][n]
@@ -685,11 +727,33 @@ __device__ __forceinline__ WTile wave_tile(const int4 *meta, int t) {
     return w;
 }
 
+// Column word of nonzero k: the column (IT = int32_t) or its offset from the
+// first row of the nonzero's wave tile (IT = int16_t).
+template <typename IT>
+__device__ __forceinline__ int col_word(const CgArgs &A, int k) {
+    if constexpr (std::is_same<IT, int16_t>::value)
+        return (int)ld_stream(A.dcol + k);
+    else
+        return ld_stream(A.col + k);
+}
+// Column from a column word of a wave tile whose first row is r0.
+template <typename IT>
+__device__ __forceinline__ int col_of(int word, int r0) {
+    if constexpr (std::is_same<IT, int16_t>::value)
+        return r0 + word;
+    else
+        return word;
+}
+template <typename IT>
+__device__ __forceinline__ int col_at(const CgArgs &A, int r0, int k) {
+    return col_of<IT>(col_word<IT>(A, k), r0);
+}
+
 // FP64 residual over the block's wave tiles: r = b - A x (x column-major).
 // SETUP: also p_old = 0 and b.b.
 // Accumulates per column: acc[c] = r.r (FP64), acc[NC+c] = b.b,
 // acc[2NC+c] = (FP32 r).(FP32 r).
-template <int NC, int BS, bool MIXED, bool SETUP>
+template <int NC, int BS, bool MIXED, bool SETUP, typename IT>
 __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave,
                                double (&acc)[3 * NC]) {
     using VT = typename std::conditional<MIXED, float, double>::type;
@@ -727,7 +791,7 @@ __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave
                     const int k = lane + 64 * (i0 + i);
                     const bool ok = k < w.nz;
                     const int idx = ok ? w.k0 + k : 0;
-                    const int cc = A.col[idx];
+                    const int cc = col_at<IT>(A, w.r0, idx);
                     const double vv = A.val64[idx];
                     cj[i] = ok ? cc : 0;
                     vj[i] = ok ? vv : 0.0;
@@ -753,7 +817,8 @@ __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave
             for (int c = 0; c < NC; ++c) {
                 double part = 0.0;
                 for (int k = lane; k < w.nz; k += 64)
-                    part += A.val64[w.k0 + k] * A.x[(size_t)c * A.n + A.col[w.k0 + k]];
+                    part += A.val64[w.k0 + k] *
+                            A.x[(size_t)c * A.n + col_at<IT>(A, w.r0, w.k0 + k)];
                 s[c] = wave_sum(part);
             }
         }
@@ -781,7 +846,7 @@ __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave
 //   deferred FP64 solution update x += pend * p_old, delta += (p_new, q).
 // The products and row sums accumulate in FP64 in both precisions (the
 // product of two FP32 numbers is exact in FP64); only q is stored as VT.
-template <int NC, int BS, bool MIXED>
+template <int NC, int BS, bool MIXED, typename IT>
 __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
                         const typename std::conditional<MIXED, float, double>::type *pold,
                         typename std::conditional<MIXED, float, double>::type *pnew,
@@ -803,26 +868,33 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
     // column: upload makes col[0] = 0 for an empty matrix); their products
     // land in LDS slots that no row sum reads, so no loaded value is masked
     // and no load has to be waited for before the products are staged.
+    // The prefetched words stay raw until the tile is consumed (no ALU use
+    // of a load in flight).  Lanes beyond the tile read its first nonzero
+    // (a valid column of the tile); a tile without nonzeros gathers nothing.
     WTile w = wave_tile(A.meta, t);
     int cn[kWItems];
+    {
+        const int kf = w.nz > 0 ? w.k0 : 0;
 #pragma unroll
-    for (int i = 0; i < kWItems; ++i) {
-        const int k = lane + 64 * i;
-        cn[i] = ld_stream(A.col + (k < w.nz ? w.k0 + k : 0));
+        for (int i = 0; i < kWItems; ++i) {
+            const int k = lane + 64 * i;
+            cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
+        }
     }
     for (; t < T1; t += NW) {
         const WTile cur = w;
         int cj[kWItems];
 #pragma unroll
-        for (int i = 0; i < kWItems; ++i) cj[i] = cn[i];
-        // 1. next tile: descriptor and columns
+        for (int i = 0; i < kWItems; ++i) cj[i] = col_of<IT>(cn[i], cur.r0);
+        // 1. next tile: descriptor and column words
         const int tn = t + NW;
         if (tn < T1) {
             w = wave_tile(A.meta, tn);
+            const int kf = w.nz > 0 ? w.k0 : 0;
 #pragma unroll
             for (int i = 0; i < kWItems; ++i) {
                 const int k = lane + 64 * i;
-                cn[i] = ld_stream(A.col + (k < w.nz ? w.k0 + k : 0));
+                cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
             }
         }
         // 2. owner rows (clamped so the loads stay unconditional)
@@ -841,7 +913,9 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
         double s[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) s[c] = 0.0;
-        if (cur.nz <= kWaveNnz) {
+        if (cur.nz == 0) {
+            // rows without nonzeros: q = 0
+        } else if (cur.nz <= kWaveNnz) {
             // 3. values and gathers, then one wait for the whole tile
             constexpr int IB = (NC == 2) ? 4 : kWItems;  // register budget
 #pragma unroll
@@ -878,7 +952,7 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
             for (int c = 0; c < NC; ++c) {
                 double part = 0.0;
                 for (int k = lane; k < cur.nz; k += 64) {
-                    const int j = A.col[cur.k0 + k];
+                    const int j = col_at<IT>(A, cur.r0, cur.k0 + k);
                     const VecN<NC, VT> rjj = ldv<NC, VT>(r, j);
                     const VecN<NC, VT> pjj = ldv<NC, VT>(pold, j);
                     part += (double)val[cur.k0 + k] *
@@ -952,7 +1026,7 @@ __device__ void flush_x(const CgArgs &A, int R0, int R1, const VT *p, const doub
 // The persistent CG run by one block of a solve that owns `nb` blocks: `bid`
 // is the block's index among them (reduction slots), `L` its logical index
 // (tile ownership).  prod_raw / lds are the block's LDS.
-template <int NC, int BS, bool MIXED>
+template <int NC, int BS, bool MIXED, typename IT>
 __device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const int nb, const int L,
                                         double *prod_raw, double *lds) {
     using VT = typename std::conditional<MIXED, float, double>::type;
@@ -968,7 +1042,7 @@ __device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const in
     double acc[3 * NC];
 #pragma unroll
     for (int k = 0; k < 3 * NC; ++k) acc[k] = 0.0;
-    residual_phase<NC, BS, MIXED, true>(A, T0, T1, lds64, acc);
+    residual_phase<NC, BS, MIXED, true, IT>(A, T0, T1, lds64, acc);
     if (!grid_sum<3 * NC, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
     {
         double tol2[NC], gamma[NC], rmax2[NC];
@@ -1017,7 +1091,7 @@ __device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const in
 #pragma unroll
             for (int c = 0; c < NC; ++c) delta[c] = 0.0;
             CG_TRACE_STAMP(k, 0);
-            phase_a<NC, BS, MIXED>(A, T0, T1, lds64, pold, pnew, beta, pend, has_pend, delta);
+            phase_a<NC, BS, MIXED, IT>(A, T0, T1, lds64, pold, pnew, beta, pend, has_pend, delta);
             CG_TRACE_MAX(k, 1);
             if (!grid_sum<NC, BS>(A.sync, delta, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
             CG_TRACE_STAMP(k, 2);
@@ -1083,7 +1157,7 @@ __device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const in
                 double ra[3 * NC];
 #pragma unroll
                 for (int kk = 0; kk < 3 * NC; ++kk) ra[kk] = 0.0;
-                residual_phase<NC, BS, MIXED, false>(A, T0, T1, lds64, ra);
+                residual_phase<NC, BS, MIXED, false, IT>(A, T0, T1, lds64, ra);
                 if (!grid_sum<3 * NC, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
                 ++refines;
 #pragma unroll
@@ -1160,20 +1234,20 @@ timeout:
 
 // One block per CU: BS / 256 waves per SIMD (4 for 1024 threads, 128 VGPRs
 // per wave; 2 for 512 threads, up to 256 VGPRs).
-template <int NC, int BS, bool MIXED>
+template <int NC, int BS, bool MIXED, typename IT>
 __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
     constexpr int NW = BS / 64;
     __shared__ double prod_raw[NW * kWaveNnz * NC];
     __shared__ double lds[kMaxRed * NW];
-    cg_body<NC, BS, MIXED>(A, blockIdx.x, gridDim.x, xcd_swizzle(blockIdx.x, gridDim.x), prod_raw,
-                           lds);
+    cg_body<NC, BS, MIXED, IT>(A, blockIdx.x, gridDim.x, xcd_swizzle(blockIdx.x, gridDim.x),
+                               prod_raw, lds);
 }
 
 // Many independent systems in one persistent launch: block i works for
 // instance map[i].x as its block map[i].y of args[instance].nblocks; each
 // instance reduces over its own blocks with its own SyncState, and its blocks
 // leave the kernel when it converges.
-template <int NC, int BS, bool MIXED>
+template <int NC, int BS, bool MIXED, typename IT>
 __global__ __launch_bounds__(BS, BS / 256) void k_cg_batch(const CgArgs *__restrict__ args,
                                                            const int2 *__restrict__ map) {
     constexpr int NW = BS / 64;
@@ -1183,21 +1257,59 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg_batch(const CgArgs *__restr
     const int inst = __builtin_amdgcn_readfirstlane(m.x);
     const int lb = __builtin_amdgcn_readfirstlane(m.y);
     const CgArgs A = args[inst];
-    cg_body<NC, BS, MIXED>(A, lb, A.nblocks, lb, prod_raw, lds);
+    cg_body<NC, BS, MIXED, IT>(A, lb, A.nblocks, lb, prod_raw, lds);
 }
 
-template <int NC, int BS, bool MIXED>
+constexpr int kCgBlock = 1024;
+
+template <int NC, bool MIXED, typename IT>
 void cg_launch(Device &d, const CgArgs &args, int grid) {
     void *kargs[] = {(void *)&args};
-    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg<NC, BS, MIXED>, dim3(grid),
-                                         dim3(BS), kargs, 0, d.stream));
+    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg<NC, kCgBlock, MIXED, IT>, dim3(grid),
+                                         dim3(kCgBlock), kargs, 0, d.stream));
 }
-
-template <int NC, int BS, bool MIXED>
+template <int NC, bool MIXED, typename IT>
 int cg_max_grid(Device &d) {
     int per_cu = 0;
-    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg<NC, BS, MIXED>, BS, 0));
+    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg<NC, kCgBlock, MIXED, IT>,
+                                                           kCgBlock, 0));
     return std::max(1, std::min(per_cu * d.cu_count, kMaxGridBlocks));
+}
+
+// Calls f.template run<NC, MIXED, IT>() for the solve's column count,
+// precision and column-index width.
+template <typename F>
+void cg_dispatch(int nc, bool mixed, bool idx16, F &&f) {
+    if (nc == 1) {
+        if (mixed) idx16 ? f.template run<1, true, int16_t>() : f.template run<1, true, int32_t>();
+        else idx16 ? f.template run<1, false, int16_t>() : f.template run<1, false, int32_t>();
+    } else {
+        if (mixed) idx16 ? f.template run<2, true, int16_t>() : f.template run<2, true, int32_t>();
+        else idx16 ? f.template run<2, false, int16_t>() : f.template run<2, false, int32_t>();
+    }
+}
+
+struct CgMaxGrid {
+    Device &d;
+    int out;
+    template <int NC, bool MX, typename IT>
+    void run() { out = cg_max_grid<NC, MX, IT>(d); }
+};
+struct CgLaunch {
+    Device &d;
+    const CgArgs &a;
+    int grid;
+    template <int NC, bool MX, typename IT>
+    void run() { cg_launch<NC, MX, IT>(d, a, grid); }
+};
+
+// 16-bit column offsets unless NSFEM_CG_IDX16=0 (comparison runs)
+bool want_idx16() {
+    static const bool on = [] {
+        const char *e = std::getenv("NSFEM_CG_IDX16");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
 }
 
 }  // namespace
@@ -1270,27 +1382,14 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
     Matrix &m = *req.m;
     const int n = m.rows, nc = req.ncol;
     if (mixed) ensure_val32(d, m);
-    // block size: 1024 threads (16 waves per CU, 128 VGPRs each) by default.
-    // The loop is bound by gather latency, so waves in flight matter more
-    // than registers: 1024 beats 512 threads (8 waves, up to 256 VGPRs) on
-    // every measured solve.  NSFEM_CG_BLOCK=512 selects the smaller block.
-    static const int bs_env = [] {
-        const char *e = std::getenv("NSFEM_CG_BLOCK");
-        return e ? std::atoi(e) : 0;
-    }();
-    const int BSz = bs_env == 512 ? 512 : 1024;
-    int maxg;
-    if (BSz == 1024) {
-        if (nc == 1)
-            maxg = mixed ? cg_max_grid<1, 1024, true>(d) : cg_max_grid<1, 1024, false>(d);
-        else
-            maxg = mixed ? cg_max_grid<2, 1024, true>(d) : cg_max_grid<2, 1024, false>(d);
-    } else {
-        if (nc == 1)
-            maxg = mixed ? cg_max_grid<1, 512, true>(d) : cg_max_grid<1, 512, false>(d);
-        else
-            maxg = mixed ? cg_max_grid<2, 512, true>(d) : cg_max_grid<2, 512, false>(d);
-    }
+    // 1024 threads per block (16 waves per CU, 128 VGPRs each): the loop is
+    // bound by gather latency, so waves in flight matter more than
+    // registers; 512 threads (8 waves, up to 256 VGPRs) was slower on every
+    // measured solve.
+    const bool idx16 = m.dcol_ok && want_idx16();
+    CgMaxGrid mg{d, 0};
+    cg_dispatch(nc, mixed, idx16, mg);
+    const int maxg = mg.out;
     int want = req.grid;
     if (want <= 0) {
         // tuning override: NSFEM_CG_GRID=<blocks> (capped by co-residency)
@@ -1298,11 +1397,11 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
         if (env) want = std::atoi(env);
     }
     int grid = want > 0 ? std::min(want, maxg)
-                        : std::min(maxg, std::max(1, m.n_wtiles / (2 * (BSz / 64))));
+                        : std::min(maxg, std::max(1, m.n_wtiles / (2 * (kCgBlock / 64))));
     grid = std::max(1, std::min(grid, std::max(1, m.n_wtiles)));
     if (std::getenv("NSFEM_CG_VERBOSE"))
-        std::fprintf(stderr, "[nsfem] cg rows=%d cols=%d wave tiles=%d block=%d grid=%d maxgrid=%d mixed=%d\n",
-                     n, nc, m.n_wtiles, BSz, grid, maxg, (int)mixed);
+        std::fprintf(stderr, "[nsfem] cg rows=%d cols=%d wave tiles=%d grid=%d maxgrid=%d mixed=%d idx16=%d\n",
+                     n, nc, m.n_wtiles, grid, maxg, (int)mixed, (int)idx16);
     ensure_block_tiles(d, m, grid);
 
     const size_t vt = mixed ? sizeof(float) : sizeof(double);
@@ -1318,6 +1417,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
     a.block_tiles = m.block_tiles.ptr;
     a.rs = m.row_start.ptr;
     a.col = m.col.ptr;
+    a.dcol = m.dcol.ptr;
     a.val64 = m.val.ptr;
     a.val32 = m.val32.ptr;
     a.b = req.b;
@@ -1337,23 +1437,8 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
         NSFEM_HIP(hipEventRecord(d.ev_start, d.stream));
         k_sync_reset<<<(ngroups + 255) / 256, 256, 0, d.stream>>>(a.sync, ngroups);
         NSFEM_HIP(hipGetLastError());
-        if (BSz == 1024) {
-            if (nc == 1) {
-                if (mixed) cg_launch<1, 1024, true>(d, a, grid);
-                else cg_launch<1, 1024, false>(d, a, grid);
-            } else {
-                if (mixed) cg_launch<2, 1024, true>(d, a, grid);
-                else cg_launch<2, 1024, false>(d, a, grid);
-            }
-        } else {
-            if (nc == 1) {
-                if (mixed) cg_launch<1, 512, true>(d, a, grid);
-                else cg_launch<1, 512, false>(d, a, grid);
-            } else {
-                if (mixed) cg_launch<2, 512, true>(d, a, grid);
-                else cg_launch<2, 512, false>(d, a, grid);
-            }
-        }
+        CgLaunch ln{d, a, grid};
+        cg_dispatch(nc, mixed, idx16, ln);
         NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
         NSFEM_HIP(hipMemcpyAsync(P.res, d.cg_res.ptr, sizeof(CgColumnResult) * nc,
                                  hipMemcpyDeviceToHost, d.stream));
@@ -1427,20 +1512,35 @@ int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
 // ------------------------------------------------------------ batched CG
 namespace {
 
-template <int NC, bool MIXED>
+template <int NC, bool MIXED, typename IT>
 void cg_batch_launch(Device &d, const CgArgs *args, const int2 *map, int grid) {
     void *kargs[] = {(void *)&args, (void *)&map};
-    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg_batch<NC, 1024, MIXED>, dim3(grid),
-                                         dim3(1024), kargs, 0, d.stream));
+    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg_batch<NC, kCgBlock, MIXED, IT>,
+                                         dim3(grid), dim3(kCgBlock), kargs, 0, d.stream));
 }
 
-template <int NC, bool MIXED>
+template <int NC, bool MIXED, typename IT>
 int cg_batch_max_grid(Device &d) {
     int per_cu = 0;
-    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_batch<NC, 1024, MIXED>,
-                                                           1024, 0));
+    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, k_cg_batch<NC, kCgBlock, MIXED, IT>, kCgBlock, 0));
     return std::max(1, std::min(per_cu * d.cu_count, kMaxGridBlocks));
 }
+
+struct CgBatchMaxGrid {
+    Device &d;
+    int out;
+    template <int NC, bool MX, typename IT>
+    void run() { out = cg_batch_max_grid<NC, MX, IT>(d); }
+};
+struct CgBatchLaunch {
+    Device &d;
+    const CgArgs *args;
+    const int2 *map;
+    int grid;
+    template <int NC, bool MX, typename IT>
+    void run() { cg_batch_launch<NC, MX, IT>(d, args, map, grid); }
+};
 
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
@@ -1485,8 +1585,11 @@ int run_cg_batch(Device &d, const std::vector<CgRequest> &reqs, nsfem_cg_stats *
                       "a batch needs one column count and one precision");
         NSFEM_REQUIRE(nc == 1 || nc == 2, NSFEM_INVALID_ARGUMENT, "columns must be 1 or 2");
     }
-    const int maxg = nc == 1 ? (mixed ? cg_batch_max_grid<1, true>(d) : cg_batch_max_grid<1, false>(d))
-                             : (mixed ? cg_batch_max_grid<2, true>(d) : cg_batch_max_grid<2, false>(d));
+    bool idx16 = want_idx16();
+    for (const CgRequest &r : reqs) idx16 = idx16 && r.m->dcol_ok;
+    CgBatchMaxGrid mg{d, 0};
+    cg_dispatch(nc, mixed, idx16, mg);
+    const int maxg = mg.out;
     int worst = NSFEM_OK;
     const size_t vt = mixed ? sizeof(float) : sizeof(double);
     // instances beyond the co-resident block count run in further launches
@@ -1525,6 +1628,7 @@ int run_cg_batch(Device &d, const std::vector<CgRequest> &reqs, nsfem_cg_stats *
             a.block_tiles = m.block_tiles.ptr;
             a.rs = m.row_start.ptr;
             a.col = m.col.ptr;
+            a.dcol = m.dcol.ptr;
             a.val64 = m.val.ptr;
             a.val32 = m.val32.ptr;
             a.b = r.b;
@@ -1554,13 +1658,8 @@ int run_cg_batch(Device &d, const std::vector<CgRequest> &reqs, nsfem_cg_stats *
         NSFEM_HIP(hipGetLastError());
         NSFEM_HIP(hipEventRecord(d.ev_start, d.stream));
         const CgArgs *dargs = reinterpret_cast<const CgArgs *>(d.batch_args.ptr);
-        if (nc == 1) {
-            if (mixed) cg_batch_launch<1, true>(d, dargs, d.batch_map.ptr, grid);
-            else cg_batch_launch<1, false>(d, dargs, d.batch_map.ptr, grid);
-        } else {
-            if (mixed) cg_batch_launch<2, true>(d, dargs, d.batch_map.ptr, grid);
-            else cg_batch_launch<2, false>(d, dargs, d.batch_map.ptr, grid);
-        }
+        CgBatchLaunch ln{d, dargs, d.batch_map.ptr, grid};
+        cg_dispatch(nc, mixed, idx16, ln);
         NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
         // FP64 true residuals of every instance, one host synchronisation
         if (!mixed) {

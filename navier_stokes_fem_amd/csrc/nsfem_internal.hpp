@@ -124,6 +124,10 @@ struct Matrix {
     // ownership of wave tiles by persistent blocks, cached per grid size
     DeviceArray<int32_t> block_tiles;
     int block_tiles_grid = 0;
+    // column minus the first row of the nonzero's wave tile, as int16; built
+    // with the wave tiles of a square matrix when every offset fits
+    DeviceArray<int16_t> dcol;
+    bool dcol_ok = false;
     bool ready = false;
     std::mutex mtx;
 };
@@ -131,6 +135,8 @@ struct Matrix {
 inline void matrix_release(Matrix &m) {
     m.row_start.release(); m.col.release(); m.val.release(); m.val32.release();
     m.tiles.release(); m.wtiles.release(); m.wmeta.release(); m.block_tiles.release();
+    m.dcol.release();
+    m.dcol_ok = false;
     m.block_tiles_grid = 0;
     m.ready = false;
 }
