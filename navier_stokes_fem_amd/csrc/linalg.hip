@@ -540,7 +540,8 @@ struct CgArgs {
     const float *val32;
     const double *b;  // [NC][n]
     double *x;        // [NC][n]
-    void *r, *p0, *p1, *q;  // interleaved [n][NC]
+    void *r, *p0, *p1, *q;  // interleaved [n][NC] (two-column solves)
+    void *st[2];            // packed {r, p, q} rows of one-column mixed solves
     double rtol, atol, rdelta;
     int max_iter, max_refine;
     int ru_gap;      // mixed: force a reliable update after this many iterations
@@ -727,6 +728,22 @@ __device__ __forceinline__ WTile wave_tile(const int4 *meta, int t) {
     return w;
 }
 
+// One-column mixed solves keep each row's FP32 CG state packed as {r, p, q}
+// in 12 bytes, so that one dwordx3 gather brings everything a neighbour
+// contributes.  A wave64 gather costs the same return-path time whatever its
+// width per lane, so one wide gather replaces the two narrow ones (r, p) of
+// the two-phase iteration.
+struct alignas(4) RowState {
+    float r, p, q;
+};
+
+__device__ __forceinline__ RowState ld_state(const void *st, int i) {
+    return reinterpret_cast<const RowState *>(st)[i];
+}
+__device__ __forceinline__ void st_state(void *st, int i, const RowState &v) {
+    reinterpret_cast<RowState *>(st)[i] = v;
+}
+
 // Column word of nonzero k: the column (IT = int32_t) or its offset from the
 // first row of the nonzero's wave tile (IT = int16_t).
 template <typename IT>
@@ -753,9 +770,12 @@ __device__ __forceinline__ int col_at(const CgArgs &A, int r0, int k) {
 // SETUP: also p_old = 0 and b.b.
 // Accumulates per column: acc[c] = r.r (FP64), acc[NC+c] = b.b,
 // acc[2NC+c] = (FP32 r).(FP32 r).
-template <int NC, int BS, bool MIXED, bool SETUP, typename IT>
+// PACKED (one-column mixed solves): r goes to the packed state `st`; SETUP
+// also clears its p and q.
+template <int NC, int BS, bool MIXED, bool SETUP, typename IT, bool PACKED = false>
 __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave,
-                               double (&acc)[3 * NC]) {
+                               double (&acc)[3 * NC], void *st = nullptr) {
+    static_assert(!PACKED || (NC == 1 && MIXED), "packed state: one FP32 column");
     using VT = typename std::conditional<MIXED, float, double>::type;
     constexpr int NW = BS / 64;
     const int lane = threadIdx.x & 63;
@@ -833,9 +853,16 @@ __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave
                 if (MIXED) acc[2 * NC + c] += (double)rv.v[c] * (double)rv.v[c];
                 zero.v[c] = (VT)0;
             }
-            stv<NC, VT>(r, row, rv);
-            if (SETUP) {
-                stv<NC, VT>(pold, row, zero);
+            if constexpr (PACKED) {
+                if constexpr (SETUP)
+                    st_state(st, row, RowState{(float)rv.v[0], 0.f, 0.f});
+                else
+                    reinterpret_cast<RowState *>(st)[row].r = (float)rv.v[0];
+            } else {
+                stv<NC, VT>(r, row, rv);
+                if (SETUP) {
+                    stv<NC, VT>(pold, row, zero);
+                }
             }
         }
     }
@@ -1232,6 +1259,310 @@ timeout:
     }
 }
 
+// ---------------------------------------------------------------------
+// One-column mixed solves: single-reduction iteration over the packed FP32
+// row state.
+//
+// With r_{k-1}, p_{k-1}, q_{k-1} = A p_{k-1} of the last phase, one fused
+// phase forms, over the block's wave tiles,
+//   r_k = r_{k-1} - alpha q_{k-1},  p_k = r_k + beta p_{k-1}  (owner rows),
+//   q_k = A p_k, each neighbour's p_k recomputed from its gathered state by
+//   the same two roundings as its owner's (products summed in FP64),
+//   the deferred FP64 solution update x += pend p_{k-1},
+// and the FP64 sums (p_k, q_k), (r_k, r_k), (r_k, q_k), (q_k, q_k), all in one
+// grid reduction: gamma_{k+1} = (r,r) - 2 alpha (r,q) + alpha^2 (q,q) gives
+// beta_{k+1} without a second reduction (Meurant's single-synchronisation
+// form of Hestenes-Stiefel CG, equal to it in exact arithmetic).  The state
+// of step k goes to the other buffer, so no block overwrites rows that
+// another still gathers.  (The FP64 solve keeps the two-phase iteration: its
+// packed state would need two gathers per nonzero, and measured slower.)
+__device__ __forceinline__ float next_r(float alpha, float r, float q) {
+    return __builtin_fmaf(-alpha, q, r);
+}
+
+template <int BS, typename IT>
+__device__ void phase_f1(const CgArgs &A, int T0, int T1, double *lds_wave, const void *st_old,
+                         void *st_new, const float alpha, const float beta, const double pend,
+                         const bool has_pend, double (&sums)[4]) {
+    constexpr int NW = BS / 64;
+    const float *val = A.val32;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int t = T0 + wave;
+    if (t >= T1) return;
+    // column words of the next tile stay raw until the tile is consumed;
+    // lanes beyond a tile read its first nonzero (a valid column)
+    WTile w = wave_tile(A.meta, t);
+    int cn[kWItems];
+    {
+        const int kf = w.nz > 0 ? w.k0 : 0;
+#pragma unroll
+        for (int i = 0; i < kWItems; ++i) {
+            const int k = lane + 64 * i;
+            cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
+        }
+    }
+    for (; t < T1; t += NW) {
+        const WTile cur = w;
+        int cj[kWItems];
+#pragma unroll
+        for (int i = 0; i < kWItems; ++i) cj[i] = col_of<IT>(cn[i], cur.r0);
+        const int tn = t + NW;
+        if (tn < T1) {
+            w = wave_tile(A.meta, tn);
+            const int kf = w.nz > 0 ? w.k0 : 0;
+#pragma unroll
+            for (int i = 0; i < kWItems; ++i) {
+                const int k = lane + 64 * i;
+                cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
+            }
+        }
+        const int row = cur.r0 + lane;
+        const bool has_row = row < cur.r1;
+        const int rowc = has_row ? row : cur.r0;
+        const int rsa = A.rs[rowc];
+        const int rsb = A.rs[rowc + 1];
+        const RowState so = ld_state(st_old, rowc);
+        double xo = 0.0;
+        if (has_pend) xo = A.x[rowc];
+        double s = 0.0;
+        if (cur.nz == 0) {
+            // rows without nonzeros: q = 0
+        } else if (cur.nz <= kWaveNnz) {
+            constexpr int IB = kWItems / 2;  // registers in flight
+#pragma unroll
+            for (int i0 = 0; i0 < kWItems; i0 += IB) {
+                float vj[IB];
+                RowState gj[IB];
+#pragma unroll
+                for (int i = 0; i < IB; ++i) {
+                    const int k = lane + 64 * (i0 + i);
+                    vj[i] = ld_stream(val + (k < cur.nz ? cur.k0 + k : 0));
+                    gj[i] = ld_state(st_old, cj[i0 + i]);
+                }
+#pragma unroll
+                for (int i = 0; i < IB; ++i) {
+                    const int k = lane + 64 * (i0 + i);
+                    lds_wave[k] = (double)vj[i] *
+                                  (double)__builtin_fmaf(beta, gj[i].p, next_r(alpha, gj[i].r, gj[i].q));
+                }
+            }
+            __syncwarp();
+            const int ra = rsa - cur.k0;
+            const int rb = has_row ? rsb - cur.k0 : ra;
+            for (int k = ra; k < rb; ++k) s += lds_wave[k];
+            __syncwarp();
+        } else {
+            // a single row longer than a tile: wave-wide sum
+            double part = 0.0;
+            for (int k = lane; k < cur.nz; k += 64) {
+                const RowState g = ld_state(st_old, col_at<IT>(A, cur.r0, cur.k0 + k));
+                part += (double)val[cur.k0 + k] *
+                        (double)__builtin_fmaf(beta, g.p, next_r(alpha, g.r, g.q));
+            }
+            s = wave_sum(part);
+        }
+        if (has_row) {
+            RowState sn;
+            sn.r = next_r(alpha, so.r, so.q);
+            sn.p = __builtin_fmaf(beta, so.p, sn.r);
+            sn.q = (float)s;
+            const double rd = (double)sn.r, qd = (double)sn.q;
+            sums[0] += (double)sn.p * s;
+            sums[1] += rd * rd;
+            sums[2] += rd * qd;
+            sums[3] += qd * qd;
+            st_state(st_new, row, sn);
+            if (has_pend) A.x[row] = __builtin_fma(pend, (double)so.p, xo);
+        }
+    }
+}
+
+// x += pend p over the block's rows (p from the packed state)
+template <int BS>
+__device__ void flush_x1(const CgArgs &A, int R0, int R1, const void *st, double pend) {
+    for (int row = R0 + (int)threadIdx.x; row < R1; row += BS)
+        A.x[row] = __builtin_fma(pend, (double)ld_state(st, row).p, A.x[row]);
+}
+
+// Control state of a one-column solve, in LDS: thread 0 updates it after
+// every grid reduction and the block reads the step coefficients from it,
+// so the bookkeeping holds no registers during the phases.
+struct CgCtl1 {
+    double tol2, gamma, rmax2, last_rr, ru_rr;
+    double pend, alpha, beta, g;
+    int iters, status, stall, active, dropped;
+    int k, last_ru, refines, ru, has_pend, out_of_iters;
+};
+
+// After the phase reduction: alpha_k, gamma_{k+1} and the reliable-update
+// trigger.  A drop of the recursive residual by rdelta triggers the update
+// once ru_min_gap iterations have passed, a drop by rdelta^2 at once (FP32
+// rounding must not pile up over a steep fall); ru_gap iterations and the
+// iteration limit force one.
+__device__ void ctl_after_phase(CgCtl1 &C, const CgArgs &A, const double (&sums)[4]) {
+    const int k = ++C.k;
+    double alpha = 0.0, g = 0.0;
+    bool dropped = false;
+    if (C.active) {
+        const double delta = sums[0], gk = sums[1];
+        if (!(delta > 0.0)) {
+            C.status = NSFEM_BREAKDOWN;
+            C.active = 0;
+            C.last_rr = gk;
+        } else {
+            alpha = gk / delta;
+            g = fmax(gk - 2.0 * alpha * sums[2] + alpha * alpha * sums[3], 0.0);
+            C.gamma = gk;
+            C.iters = k;
+            const double d2 = A.rdelta * A.rdelta;
+            dropped = g <= C.tol2 ||
+                      (g < d2 * C.rmax2 && (k - C.last_ru >= A.ru_min_gap || g < d2 * d2 * C.rmax2));
+        }
+    }
+    C.pend = alpha;
+    C.alpha = alpha;
+    C.g = g;
+    C.dropped = dropped;
+    C.has_pend = 1;
+    C.out_of_iters = k >= A.max_iter;
+    C.ru = C.active && (dropped || k - C.last_ru >= A.ru_gap || C.out_of_iters);
+    if (C.ru) C.last_ru = k;
+}
+
+// After a reliable update (x flushed, r := b - A x in FP64): the FP64
+// residual decides convergence; after a drop reported by the recursion the
+// FP64 residual must fall below (1 + rdelta)/2 of its best value so far, and
+// three misses in a row end the FP32 iteration (stagnation; an update
+// forced by ru_gap alone is not judged).
+__device__ void ctl_after_update(CgCtl1 &C, const CgArgs &A, const double (&ra)[3]) {
+    C.has_pend = 0;
+    C.pend = 0.0;
+    C.alpha = 0.0;
+    ++C.refines;
+    C.last_rr = ra[0];
+    if (ra[0] <= C.tol2) {
+        C.active = 0;
+    } else {
+        C.g = ra[2];
+        C.rmax2 = ra[0];
+        if (C.dropped) {
+            const double sr = 0.5 * (1.0 + A.rdelta);
+            C.stall = ra[0] > sr * sr * C.ru_rr ? C.stall + 1 : 0;
+        }
+        C.ru_rr = fmin(C.ru_rr, ra[0]);
+        if (C.stall >= 3) {
+            C.status = NSFEM_STAGNATION;
+            C.active = 0;
+        } else if (C.refines >= A.max_refine) {
+            C.status = NSFEM_MAX_ITERATIONS_REACHED;
+            C.active = 0;
+        }
+    }
+}
+
+// End of an iteration: iteration limit, beta_{k+1} = gamma_{k+1} / gamma_k.
+__device__ void ctl_end(CgCtl1 &C) {
+    if (C.active && C.out_of_iters) {
+        C.status = NSFEM_MAX_ITERATIONS_REACHED;
+        C.active = 0;
+        C.last_rr = C.g;
+    }
+    C.beta = C.active ? C.g / C.gamma : 0.0;
+    if (!C.active) C.alpha = 0.0;
+}
+
+// The one-column mixed persistent CG run by one block (arguments as
+// cg_body).  The recursive residual is replaced by the FP64 b - A x whenever
+// it has fallen by rdelta (reliable updates).
+template <int BS, typename IT>
+__device__ __forceinline__ void cg_body1(const CgArgs &A, const int bid, const int nb, const int L,
+                                         double *prod_raw, double *lds) {
+    constexpr bool MIXED = true;  // residual_phase precision
+    __shared__ CgCtl1 C;
+    const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
+    const int R0 = A.tiles[T0], R1 = A.tiles[T1];
+    const int wave = threadIdx.x >> 6;
+    double *lds64 = prod_raw + wave * kWaveNnz;
+    unsigned epoch = 0;
+    const bool t0 = threadIdx.x == 0;
+    const bool writer = (L == 0 && t0);
+    int cur = 0;  // st[cur]: state of step k - 1
+
+    {
+        double acc[3] = {0.0, 0.0, 0.0};
+        residual_phase<1, BS, MIXED, true, IT, true>(A, T0, T1, lds64, acc, A.st[0]);
+        if (!grid_sum<3, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+        if (t0) {
+            const double bn = sqrt(acc[1]), r0n = sqrt(acc[0]);
+            const double ref = bn > 0.0 ? bn : r0n;
+            const double tol = fmax(A.rtol * ref, A.atol);
+            C.tol2 = tol * tol;
+            C.active = acc[0] > C.tol2;
+            C.gamma = acc[2];
+            C.rmax2 = C.last_rr = C.ru_rr = acc[0];
+            C.g = 0.0;
+            C.iters = 0;
+            C.status = NSFEM_OK;
+            C.stall = C.dropped = 0;
+            C.pend = C.alpha = C.beta = 0.0;
+            C.k = C.last_ru = C.refines = C.ru = C.has_pend = C.out_of_iters = 0;
+            if (writer) {
+                A.res[0].rhs_norm = bn;
+                A.res[0].initial_residual = r0n;
+                A.res[0].tolerance = tol;
+            }
+        }
+        __syncthreads();
+    }
+    while (C.active) {
+        const int nx = cur ^ 1;
+        const float alpha = (float)C.alpha, beta = (float)C.beta;
+        const double pend = C.pend;
+        const bool has_pend = C.has_pend != 0;
+        double sums[4] = {0.0, 0.0, 0.0, 0.0};
+        CG_TRACE_STAMP(C.k, 0);
+        phase_f1<BS, IT>(A, T0, T1, lds64, A.st[cur], A.st[nx], alpha, beta, pend, has_pend, sums);
+        CG_TRACE_MAX(C.k, 1);
+        if (!grid_sum<4, BS>(A.sync, sums, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+        CG_TRACE_STAMP(C.k, 2);
+        if (t0) {
+            ctl_after_phase(C, A, sums);
+            if (!C.ru) ctl_end(C);
+        }
+        __syncthreads();
+        if (C.ru) {
+            // apply the pending x += alpha p_k (FP64), then replace the
+            // recursive residual: r_k := b - A x, alpha := 0
+            flush_x1<BS>(A, R0, R1, A.st[nx], C.pend);
+            double dummy[1] = {0.0};
+            if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+            double ra[3] = {0.0, 0.0, 0.0};
+            residual_phase<1, BS, MIXED, false, IT, true>(A, T0, T1, lds64, ra, A.st[nx]);
+            if (!grid_sum<3, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+            if (t0) {
+                ctl_after_update(C, A, ra);
+                ctl_end(C);
+            }
+            __syncthreads();
+        }
+        if (!C.active) {
+            if (C.has_pend) flush_x1<BS>(A, R0, R1, A.st[nx], C.pend);
+            break;
+        }
+        cur = nx;
+    }
+    if (writer) {
+        A.res[0].iterations = C.iters;
+        A.res[0].refinements = C.refines;
+        A.res[0].status = C.status;
+        A.res[0].last_residual = sqrt(C.last_rr);
+    }
+    return;
+timeout:
+    if (writer) A.res[0].status = NSFEM_BARRIER_TIMEOUT;
+}
+
 // One block per CU: BS / 256 waves per SIMD (4 for 1024 threads, 128 VGPRs
 // per wave; 2 for 512 threads, up to 256 VGPRs).
 template <int NC, int BS, bool MIXED, typename IT>
@@ -1239,8 +1570,11 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
     constexpr int NW = BS / 64;
     __shared__ double prod_raw[NW * kWaveNnz * NC];
     __shared__ double lds[kMaxRed * NW];
-    cg_body<NC, BS, MIXED, IT>(A, blockIdx.x, gridDim.x, xcd_swizzle(blockIdx.x, gridDim.x),
-                               prod_raw, lds);
+    const int L = xcd_swizzle(blockIdx.x, gridDim.x);
+    if constexpr (NC == 1 && MIXED)
+        cg_body1<BS, IT>(A, blockIdx.x, gridDim.x, L, prod_raw, lds);
+    else
+        cg_body<NC, BS, MIXED, IT>(A, blockIdx.x, gridDim.x, L, prod_raw, lds);
 }
 
 // Many independent systems in one persistent launch: block i works for
@@ -1257,7 +1591,10 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg_batch(const CgArgs *__restr
     const int inst = __builtin_amdgcn_readfirstlane(m.x);
     const int lb = __builtin_amdgcn_readfirstlane(m.y);
     const CgArgs A = args[inst];
-    cg_body<NC, BS, MIXED, IT>(A, lb, A.nblocks, lb, prod_raw, lds);
+    if constexpr (NC == 1 && MIXED)
+        cg_body1<BS, IT>(A, lb, A.nblocks, lb, prod_raw, lds);
+    else
+        cg_body<NC, BS, MIXED, IT>(A, lb, A.nblocks, lb, prod_raw, lds);
 }
 
 constexpr int kCgBlock = 1024;
@@ -1367,6 +1704,13 @@ void set_cg_params(const Device &d, const CgRequest &req, int n, CgArgs &a) {
     a.timeout_ticks = (unsigned long long)d.wall_khz * 1000ull * 20ull;  // 20 s per barrier
 }
 
+// Packed FP32 {r, p, q} rows of a one-column mixed solve: two copies of 3
+// vectors of `vb` bytes (a vector holds n values of the solve's precision).
+void set_state_buffers(CgArgs &a, unsigned char *w, size_t vb) {
+    a.st[0] = w;
+    a.st[1] = w + 3 * vb;
+}
+
 struct CgPass {
     CgColumnResult res[2];
     int iters[2] = {0, 0}, refines[2] = {0, 0};
@@ -1406,7 +1750,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
 
     const size_t vt = mixed ? sizeof(float) : sizeof(double);
     const size_t vec_bytes = ((size_t)n * nc * vt + 255) / 256 * 256;
-    d.work.ensure(vec_bytes * 4);
+    d.work.ensure(vec_bytes * 6);
     d.sync_mem.ensure(sizeof(SyncState) + 256);
     d.cg_res.ensure(2);
 
@@ -1427,6 +1771,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
     a.p0 = w + vec_bytes;
     a.p1 = w + 2 * vec_bytes;
     a.q = w + 3 * vec_bytes;
+    set_state_buffers(a, w, vec_bytes);
     set_cg_params(d, req, n, a);
     unsigned char *sm = d.sync_mem.ptr;
     a.sync = reinterpret_cast<SyncState *>(sm + (256 - ((uintptr_t)sm % 256)) % 256);
@@ -1606,7 +1951,7 @@ int run_cg_batch(Device &d, const std::vector<CgRequest> &reqs, nsfem_cg_stats *
             ensure_block_tiles(d, *rq[i]->m, nb[i]);
             vbytes[i] = round256((size_t)rq[i]->m->rows * nc * vt);
             voff[i] = work;
-            work += 4 * vbytes[i];
+            work += 6 * vbytes[i];
         }
         const size_t sync_bytes = round256(sizeof(SyncState));
         d.batch_work.ensure(std::max<size_t>(work, 256));
@@ -1638,6 +1983,7 @@ int run_cg_batch(Device &d, const std::vector<CgRequest> &reqs, nsfem_cg_stats *
             a.p0 = w + vbytes[i];
             a.p1 = w + 2 * vbytes[i];
             a.q = w + 3 * vbytes[i];
+            set_state_buffers(a, w, vbytes[i]);
             set_cg_params(d, r, m.rows, a);
             a.sync = reinterpret_cast<SyncState *>(sm + i * sync_bytes);
             a.res = d.batch_res.ptr + 2 * i;
