@@ -377,7 +377,7 @@ def main():
     ap.add_argument("--form", choices=["both", "fp32_refined", "fp64"], default="both")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-iters", type=int, default=10)
-    ap.add_argument("--warm", choices=["previous", "extrapolate"], default="previous",
+    ap.add_argument("--warm", choices=["previous", "extrapolate", "projection"], default="previous",
                     help="initial guess of the chosen (fp32_refined) form's pressure solves; the "
                          "fp64 form always starts from the previous pressure, as the reference")
     ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5"], default="c4",

@@ -384,6 +384,9 @@ int nsfem_gnn_train_step(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
                                             residual is smaller, else p_prev */
 #define NSFEM_WARM_EXTRAPOLATE 5         /* 2 p_n - p_(n-1) if its residual is
                                             smaller, else p_n (no network) */
+#define NSFEM_WARM_PROJECTION 6          /* A-norm projection of the solution onto
+                                            the span of past pressures (Fischer);
+                                            basis depth NSFEM_PROJECTION_DEPTH */
 int nsfem_simulation_set_warm_start(nsfem_simulation *sim, int mode, nsfem_gnn *gnn);
 /* Copies the last step's pressure right-hand side and pressure into device
  * buffers (either may be NULL): training pairs for the network. */

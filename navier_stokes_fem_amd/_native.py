@@ -32,6 +32,7 @@ WARM_LEARNED = 2
 WARM_LEARNED_OR_PREVIOUS = 3
 WARM_LEARNED_CORRECTION = 4
 WARM_EXTRAPOLATE = 5
+WARM_PROJECTION = 6
 GNN_PARAM_COUNT = 4481
 
 

@@ -15,6 +15,7 @@
 // the matrix is read once for both).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <unordered_map>
@@ -48,6 +49,10 @@ struct Simulation {
     DeviceArray<double> pguess, ptmp, pscal;
     DeviceArray<double> p_older, p_keep;  // p_(n-1) and p_n for the extrapolation
     bool have_older = false;
+    // projection warm start: A-orthonormal basis X of past pressure
+    // corrections and A X, proj_max vectors each (proj_dim in use)
+    DeviceArray<double> proj_x, proj_ax, proj_w, proj_s;
+    int proj_dim = 0, proj_max = 0;
     int batch_block_scale = 1;  // blocks per instance in batched solves (autotuned)
 };
 
@@ -158,6 +163,7 @@ void simulation_reset(Simulation &s) {
     Device &d = *s.dev;
     d.set_current();
     s.have_older = false;
+    s.proj_dim = 0;
     launch_fill(d, 2 * s.V, 0.0, s.uv.ptr);
     launch_fill(d, s.P, 0.0, s.p.ptr);
     launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, s.uv.ptr);
@@ -200,6 +206,81 @@ void solve(Simulation &s, int slot, const double *b, double *x, int ncol, SolveO
     o.seconds = st[0].seconds;
 }
 
+// out[i] = X_i . y for the first l basis vectors (one host synchronisation)
+void proj_dots(Simulation &s, int l, const double *y, double *out) {
+    Device &d = *s.dev;
+    launch_fill(d, l, 0.0, s.proj_s.ptr);
+    for (int i = 0; i < l; ++i)
+        launch_dot_accumulate(d, s.P, s.proj_x.ptr + (size_t)i * s.P, y, s.proj_s.ptr + i);
+    NSFEM_HIP(hipMemcpyAsync(out, s.proj_s.ptr, sizeof(double) * l, hipMemcpyDeviceToHost,
+                             d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+}
+
+double dev_dot(Simulation &s, const double *a, const double *b) {
+    Device &d = *s.dev;
+    double h = 0.0;
+    launch_fill(d, 1, 0.0, s.proj_s.ptr);
+    launch_dot_accumulate(d, s.P, a, b, s.proj_s.ptr);
+    NSFEM_HIP(hipMemcpyAsync(&h, s.proj_s.ptr, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    return h;
+}
+
+// Projection onto the span of past solutions (P. F. Fischer, "Projection
+// techniques for iterative solution of Ax = b with successive right-hand
+// sides", Comput. Methods Appl. Mech. Engrg. 163 (1998) 193-204).  With an
+// A-orthonormal basis X of the past corrections, x0 = sum_i (X_i . b) X_i is
+// the best approximation of A^-1 b in the A-norm within their span, which
+// holds the previous pressure: x0 is no worse than it in the A-norm.  The
+// guess is kept in pguess for the basis update.
+void projection_guess(Simulation &s) {
+    Device &d = *s.dev;
+    const int l = s.proj_dim;
+    if (l > 0) {
+        std::vector<double> a(l);
+        proj_dots(s, l, s.prhs.ptr, a.data());
+        launch_fill(d, s.P, 0.0, s.p.ptr);
+        for (int i = 0; i < l; ++i) launch_axpy(d, s.P, a[i], s.proj_x.ptr + (size_t)i * s.P, s.p.ptr);
+    }
+    launch_copy(d, s.P, s.p.ptr, s.pguess.ptr);
+}
+
+// After the solve: the correction dx = p - x0 (the whole solution after a
+// restart), A-orthogonalised against X by Gram-Schmidt applied twice, is
+// normalised in the A-norm and appended; a full basis restarts from the
+// latest solution.  A correction that the basis already spans (A-norm below
+// 1e-12 of its own before orthogonalisation) is dropped.
+void projection_update(Simulation &s) {
+    Device &d = *s.dev;
+    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+    const int P = s.P;
+    if (s.proj_dim == s.proj_max) s.proj_dim = 0;
+    const int l = s.proj_dim;
+    double *dx = s.ptmp.ptr, *w = s.proj_w.ptr;
+    if (l == 0)
+        launch_copy(d, P, s.p.ptr, dx);
+    else
+        launch_axpby(d, P, 1.0, -1.0, s.p.ptr, s.pguess.ptr, dx);
+    launch_spmv(d, A, dx, nullptr, w, 0);
+    const double n0 = dev_dot(s, dx, w);
+    if (!(n0 > 0.0)) return;
+    std::vector<double> c(std::max(l, 1));
+    for (int pass = 0; pass < 2 && l > 0; ++pass) {
+        proj_dots(s, l, w, c.data());  // X_i . A dx = (A X_i) . dx
+        for (int i = 0; i < l; ++i) {
+            launch_axpy(d, P, -c[i], s.proj_x.ptr + (size_t)i * P, dx);
+            launch_axpy(d, P, -c[i], s.proj_ax.ptr + (size_t)i * P, w);
+        }
+    }
+    const double n1 = dev_dot(s, dx, w);
+    if (!(n1 > 1e-12 * n0)) return;
+    const double inv = 1.0 / std::sqrt(n1);
+    launch_axpby(d, P, inv, 0.0, dx, dx, s.proj_x.ptr + (size_t)l * P);
+    launch_axpby(d, P, inv, 0.0, w, w, s.proj_ax.ptr + (size_t)l * P);
+    s.proj_dim = l + 1;
+}
+
 // Initial guess of the pressure solve.  The reference keeps the previous
 // step's pressure (include/assembly.h:608-615); the learned modes replace it
 // with the network's prediction from the right-hand side (or keep whichever
@@ -231,6 +312,9 @@ void pressure_guess(Simulation &s) {
                                 s.p.ptr, s.ptmp.ptr, s.pscal.ptr);
         }
         return;
+    case NSFEM_WARM_PROJECTION:
+        projection_guess(s);
+        return;
     case NSFEM_WARM_LEARNED_CORRECTION: {
         // r = b - A p_prev; guess = p_prev + network(r), kept only if its
         // residual is smaller than the previous pressure's
@@ -244,8 +328,13 @@ void pressure_guess(Simulation &s) {
     }
 }
 
-// after the pressure solve: the extrapolating mode's p_n becomes p_(n-1)
+// after the pressure solve: the extrapolating mode's p_n becomes p_(n-1);
+// the projecting mode extends its basis
 void pressure_after_solve(Simulation &s) {
+    if (s.warm_mode == NSFEM_WARM_PROJECTION) {
+        projection_update(s);
+        return;
+    }
     if (s.warm_mode != NSFEM_WARM_EXTRAPOLATE) return;
     std::swap(s.p_older, s.p_keep);
     s.have_older = true;
@@ -254,7 +343,7 @@ void pressure_after_solve(Simulation &s) {
 }  // namespace
 
 void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
-    NSFEM_REQUIRE(mode >= NSFEM_WARM_PREVIOUS && mode <= NSFEM_WARM_EXTRAPOLATE,
+    NSFEM_REQUIRE(mode >= NSFEM_WARM_PREVIOUS && mode <= NSFEM_WARM_PROJECTION,
                   NSFEM_INVALID_ARGUMENT, "unknown warm-start mode");
     const bool learned = mode == NSFEM_WARM_LEARNED || mode == NSFEM_WARM_LEARNED_OR_PREVIOUS ||
                          mode == NSFEM_WARM_LEARNED_CORRECTION;
@@ -277,7 +366,7 @@ void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
                                    s.p_area.data(), s.p_xy.data());
     }
     if (mode == NSFEM_WARM_LEARNED_OR_PREVIOUS || mode == NSFEM_WARM_LEARNED_CORRECTION ||
-        mode == NSFEM_WARM_EXTRAPOLATE) {
+        mode == NSFEM_WARM_EXTRAPOLATE || mode == NSFEM_WARM_PROJECTION) {
         s.pguess.resize(s.P);
         s.ptmp.resize(s.P);
         s.pscal.resize(2);
@@ -286,6 +375,20 @@ void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
         s.p_older.resize(s.P);
         s.p_keep.resize(s.P);
     }
+    if (mode == NSFEM_WARM_PROJECTION) {
+        // basis size: NSFEM_PROJECTION_DEPTH (default 16), 2 x depth pressure
+        // vectors of device memory
+        static const int depth_env = [] {
+            const char *e = std::getenv("NSFEM_PROJECTION_DEPTH");
+            return e ? std::atoi(e) : 0;
+        }();
+        s.proj_max = depth_env > 0 ? std::min(depth_env, 256) : 16;
+        s.proj_x.resize((size_t)s.proj_max * s.P);
+        s.proj_ax.resize((size_t)s.proj_max * s.P);
+        s.proj_w.resize(s.P);
+        s.proj_s.resize((size_t)s.proj_max + 1);
+    }
+    s.proj_dim = 0;
     s.have_older = false;
     s.gnn = learned ? g : nullptr;
     s.warm_mode = mode;
@@ -605,6 +708,8 @@ void simulation_detach(Simulation *s) {
     s->pguess.release();
     s->p_older.release();
     s->p_keep.release();
+    s->proj_x.release();
+    s->proj_ax.release();
     s->ptmp.release();
     s->pscal.release();
     s->uv.release();

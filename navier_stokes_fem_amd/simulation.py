@@ -94,15 +94,17 @@ class Simulation:
     WARM_MODES = {"previous": N.WARM_PREVIOUS, "zero": N.WARM_ZERO, "learned": N.WARM_LEARNED,
                   "learned_or_previous": N.WARM_LEARNED_OR_PREVIOUS,
                   "learned_correction": N.WARM_LEARNED_CORRECTION,
-                  "extrapolate": N.WARM_EXTRAPOLATE}
+                  "extrapolate": N.WARM_EXTRAPOLATE, "projection": N.WARM_PROJECTION}
 
     def set_warm_start(self, mode: str = "previous", net=None):
         """Initial guess of the pressure solve: ``previous`` (the reference's
         choice), ``zero``, ``learned`` (``net``'s prediction from the
         right-hand side), ``learned_or_previous`` (the smaller residual) or
         ``learned_correction`` (previous pressure + the network's correction
-        from the previous pressure's residual, if that lowers the residual) or
-        ``extrapolate`` (2 p_n - p_(n-1), if that lowers the residual)."""
+        from the previous pressure's residual, if that lowers the residual),
+        ``extrapolate`` (2 p_n - p_(n-1), if that lowers the residual) or
+        ``projection`` (the A-norm projection of the solution onto the span of
+        past pressures, Fischer 1998)."""
         N.check(N.lib().nsfem_simulation_set_warm_start(
             self.handle, self.WARM_MODES[mode], net.handle if net is not None else None))
 

@@ -110,10 +110,11 @@ def test_warm_start_keeps_the_solution(gpu):
     m = mesh.cavity(16)
     net = L.WarmStartNet(gpu, seed=2)
     fields = {}
-    for mode in ("previous", "zero", "learned", "learned_or_previous", "learned_correction"):
+    for mode in ("previous", "zero", "learned", "learned_or_previous", "learned_correction",
+                 "extrapolate", "projection"):
         sim = Simulation(m, 0.01, 0.01, device=gpu, rtol=1e-11)
         sim.set_warm_start(mode, net if mode.startswith("learned") else None)
-        st = sim.step(3)
+        st = sim.step(4)
         assert max(st["max_relative_residual"]) <= 1e-11 * 1.01
         fields[mode] = sim.fields()
         sim.close()
