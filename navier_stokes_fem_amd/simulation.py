@@ -108,6 +108,13 @@ class Simulation:
         N.check(N.lib().nsfem_simulation_set_warm_start(
             self.handle, self.WARM_MODES[mode], net.handle if net is not None else None))
 
+    def export(self, folder: str, time_step: int):
+        """Write the current fields as ``<folder>/out_<time_step>.json``, the
+        reference's result file (include/assembly.h:953-979)."""
+        from .results import export_simulation
+
+        return export_simulation(self, folder, time_step)
+
     def copy_pressure_system(self, rhs=None, p=None):
         """Copy the last step's pressure right-hand side and solution into
         device buffers (``DeviceBuffer`` of pressure-count doubles)."""

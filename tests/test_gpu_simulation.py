@@ -86,3 +86,21 @@ def test_projection_warm_start(gpu, precision):
     for a, b in zip(f1, f0):
         assert _rel(a, b) < 1e-6
     assert s1["cg_iterations"][2] < 0.9 * s0["cg_iterations"][2]
+
+
+def test_export_fields(gpu, tmp_path):
+    """The exported result file holds exactly the simulation's fields."""
+    from navier_stokes_fem_amd import mesh, results
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    m = mesh.cavity(16)
+    sim = Simulation(m, 0.01, 0.01, device=gpu, rtol=1e-8)
+    sim.step(3)
+    path = sim.export(str(tmp_path), 3)
+    u, v, p = sim.fields()
+    sim.close()
+    sol = results.read_solution(path)
+    assert sol["timeStep"] == 3
+    np.testing.assert_array_equal(sol["u"], u)
+    np.testing.assert_array_equal(sol["v"], v)
+    np.testing.assert_array_equal(sol["p"], p)
