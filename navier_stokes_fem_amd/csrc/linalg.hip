@@ -1699,7 +1699,7 @@ void set_cg_params(const Device &d, const CgRequest &req, int n, CgArgs &a) {
         const char *e = std::getenv("NSFEM_CG_RU_MIN_GAP");
         return e ? std::atoi(e) : -1;
     }();
-    a.ru_gap = ru_gap_env > 0 ? ru_gap_env : 20000;
+    a.ru_gap = ru_gap_env > 0 ? ru_gap_env : (req.ru_gap > 0 ? req.ru_gap : 20000);
     a.ru_min_gap = ru_min_gap_env >= 0 ? ru_min_gap_env : 0;
     a.timeout_ticks = (unsigned long long)d.wall_khz * 1000ull * 20ull;  // 20 s per barrier
 }

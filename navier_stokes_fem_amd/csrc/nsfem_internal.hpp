@@ -245,6 +245,7 @@ struct CgRequest {
     double rtol, atol;
     int precision;
     double reliable_delta = 0.5;
+    int ru_gap = 0;   // mixed: forced reliable update after this many iterations (0 = 20000)
     int grid = 0;     // 0 = automatic
 };
 // Runs the persistent CG; fills stats[ncol].  Returns the worst status.

@@ -180,6 +180,19 @@ struct SolveOut {
     double seconds;
 };
 
+// Reliable-update policy of the mixed solver per system.  The pressure
+// Poisson solve keeps an update on every halving of the residual: from a
+// cold start on the 2048^2 cavity a 10x or 100x threshold lets the FP32
+// recursion stall (no convergence in 40 000 iterations, against 18 831 with
+// halving).  The velocity systems (diffusion, mass) are well conditioned and
+// warm-started: a 100x threshold with a forced update every 1000 iterations
+// cuts their updates 4-5 fold and their time by 20-40% on that cavity.
+void set_update_policy(CgRequest &r, int slot) {
+    if (slot == NSFEM_PRESSURE_STIFFNESS) return;
+    r.reliable_delta = 0.01;
+    r.ru_gap = 1000;
+}
+
 void solve(Simulation &s, int slot, const double *b, double *x, int ncol, SolveOut &o) {
     Device &d = *s.dev;
     CgRequest rq{};
@@ -191,6 +204,7 @@ void solve(Simulation &s, int slot, const double *b, double *x, int ncol, SolveO
     rq.rtol = s.opt.rtol;
     rq.atol = 0.0;
     rq.precision = s.opt.precision;
+    set_update_policy(rq, slot);
     nsfem_cg_stats st[2]{};
     const int status = run_cg(d, rq, st);
     if (status != NSFEM_OK) {
@@ -523,6 +537,7 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
         r.rtol = s.opt.rtol;
         r.atol = 0.0;
         r.precision = s.opt.precision;
+        set_update_policy(r, slot);
         return r;
     };
     auto check = [&](int status, const char *what) {
