@@ -9,12 +9,16 @@ nu = 1e-3, dt = 1e-3, lid u = (1, 0).  A step = advect + two diffusion solves
 GPU (include/assembly.h:1516-1554 of the reference).
 
 Two forms run on the same assembled simulation from the same initial state:
-  * fp64          plain FP64 CG (the baseline form)
+  * fp64          plain FP64 CG with the reference's initial guesses (the
+                  previous pressure for the pressure solve)
   * fp32_refined  FP32 matrix values and r, p, q vectors, FP64 solution,
                   FP64 accumulation of products and dots, FP64 reliable
-                  updates of the residual
+                  updates of the residual; pressure solves start from the
+                  A-norm projection onto past pressures (--warm)
 Both stop every solve at ||b - Ax|| <= 1e-8 ||b|| on the FP64 residual; the
-faster of the two is the reported `value` (the chosen form).
+faster of the two is the reported `value` (the chosen form).  fp32_refined
+also runs once with the previous-pressure guess, reported beside them, to
+separate the precision's share of the speed-up from the initial guess's.
 
 Also reported: the roofline of the dominant kernel (the pressure CG kernel
 of the chosen form: algorithmic bytes / measured kernel time, HIP events on
@@ -22,6 +26,7 @@ the kernel's stream) and the CPU oracle timed on this host on a bounded sample
 of the same workload (cpu_baseline).
 
   python bench.py [--steps K] [--warmup W] [--n 2048] [--no-cpu] [--form both]
+                  [--warm projection|previous|extrapolate] [--config NAME]
 """
 import argparse
 import json
@@ -194,7 +199,7 @@ def _line(metric, value, unit, steps, ms, config, **extra):
     return out
 
 
-def bench_c1(dev, args):
+def bench_cavity32(dev, args):
     """32x32 cavity (2048 P2-P1 elements), nu = 0.01, dt = 0.01, 100 steps of
     the plain FP64 method on the GPU; the cpu_baseline leg runs the same 100
     steps in the CPU oracle and reports its speed and the field difference."""
@@ -223,7 +228,7 @@ def bench_c1(dev, args):
     cpu = time.perf_counter() - t
     err = {k: float(np.abs(a - b).max()) for k, a, b in (("u", u, ou), ("v", v, ov), ("p", p, op))}
     scale = {k: float(np.abs(b).max()) for k, b in (("u", ou), ("v", ov), ("p", op))}
-    log(f"c1: {steps / wall:.1f} steps/s on the GPU, {steps / cpu:.1f} on the CPU oracle; "
+    log(f"cavity32: {steps / wall:.1f} steps/s on the GPU, {steps / cpu:.1f} on the CPU oracle; "
         f"max |gpu - oracle| {err}")
     cpu_baseline = {"value": round(steps / cpu, 3), "unit": "steps/s", "cores": O.get_threads(),
                     "kind": "port",
@@ -238,9 +243,10 @@ def bench_c1(dev, args):
                  cpu_baseline=cpu_baseline)
 
 
-def bench_c2(dev, args):
+def bench_channel512(dev, args):
     """512x128 channel past the disk (jitter seed 2), nu = 1e-3, dt = 5e-4,
-    200 steps in each form."""
+    200 steps in each form: plain FP64, FP32 + FP64 refinement with the
+    previous-pressure guess, and with the projection guess."""
     from navier_stokes_fem_amd import mesh
     from navier_stokes_fem_amd.simulation import Simulation
 
@@ -248,9 +254,12 @@ def bench_c2(dev, args):
     m = mesh.channel(512, 128, seed=2)
     sim = Simulation(m, 1e-3, 5e-4, device=dev, rtol=1e-8, precision="fp64")
     res = {}
-    for form in ("fp64", "fp32_refined"):
+    for form, prec, warm in (("fp64", "fp64", "previous"),
+                             ("fp32_refined", "fp32_refined", "previous"),
+                             ("fp32_refined_projection", "fp32_refined", "projection")):
         sim.reset()
-        sim.configure(precision=form)
+        sim.configure(precision=prec)
+        sim.set_warm_start(warm)
         dev.synchronize()
         t = time.perf_counter()
         st = sim.step(steps)
@@ -258,7 +267,7 @@ def bench_c2(dev, args):
         res[form] = {"steps_per_s": round(steps / wall, 2),
                      "cg_iterations_per_step": [x / steps for x in st["cg_iterations"]],
                      "max_relative_residual": st["max_relative_residual"]}
-        log(f"c2 {form}: {steps / wall:.1f} steps/s, iterations/step {res[form]['cg_iterations_per_step']}")
+        log(f"channel512 {form}: {steps / wall:.1f} steps/s, iterations/step {res[form]['cg_iterations_per_step']}")
     sim.close()
     best = max(res, key=lambda f: res[f]["steps_per_s"])
     return _line("steps/sec, 512x128 channel, 1e-8 rel. residual", res[best]["steps_per_s"],
@@ -267,10 +276,10 @@ def bench_c2(dev, args):
                               f"{m.element_count} P2-P1 triangles, {m.node_count} velocity / "
                               f"{m.pressure_count} pressure nodes, nu=1e-3, dt=5e-4, 200 steps",
                   "form": best},
-                 dtype="f32+f64refine" if best == "fp32_refined" else "f64", forms=res)
+                 dtype="f32+f64refine" if best.startswith("fp32") else "f64", forms=res)
 
 
-def bench_c3(dev, args):
+def bench_cavity_batch64(dev, args):
     """64 cavities, side n ~ U{16..256}, lid ~ U(0.5, 2), nu ~ logU(1e-3,
     1e-2) (seed 3), dt = 0.005, 50 steps in FP32-refined form: all 64 in one
     batch (each solve kind one persistent launch for all), after the
@@ -284,7 +293,7 @@ def bench_c3(dev, args):
     sims = [Simulation(m, nu, 0.005, device=dev, rtol=1e-8, precision="fp32_refined")
             for m, nu in specs]
     tune = autotune_batch(sims)
-    log(f"c3: autotune {tune}")
+    log(f"cavity_batch64: autotune {tune}")
     for s in sims:
         s.reset()
     dev.synchronize()
@@ -298,7 +307,7 @@ def bench_c3(dev, args):
     for s in sims:
         s.step(steps)
     serial = time.perf_counter() - t
-    log(f"c3: batched {steps / batched:.2f} batch-steps/s, one by one {steps / serial:.2f}")
+    log(f"cavity_batch64: batched {steps / batched:.2f} batch-steps/s, one by one {steps / serial:.2f}")
     its = [sum(st["cg_iterations"][k] for st in stats) / steps for k in range(5)]
     for s in sims:
         s.close()
@@ -313,7 +322,7 @@ def bench_c3(dev, args):
                  cg_iterations_per_batch_step=its)
 
 
-def bench_c5(dev, args):
+def bench_learned_channels(dev, args):
     """Learned warm start: trained on (rhs, pressure) pairs of 512x128
     channels with inflow max ~ U(0.5, 2) and reseeded jitter; evaluated on
     held-out seeds at 512x128 and 1024x256 (pressure CG iterations/step and
@@ -332,10 +341,10 @@ def bench_c5(dev, args):
     for target in ("solution", "correction"):
         pairs = L.collect_pairs(dev, train, dt=5e-4, steps=per, warmup=10, every=2, target=target,
                                 log=None)
-        log(f"c5: {len(pairs)} {target} pairs in {time.perf_counter() - t:.1f} s")
+        log(f"learned_channels: {len(pairs)} {target} pairs in {time.perf_counter() - t:.1f} s")
         net = L.WarmStartNet(dev, seed=1)
         hist = L.train(net, pairs, epochs=6, lr=2e-3, lr_final=1e-4)
-        log(f"c5: {target} training loss per epoch {[round(h, 4) for h in hist]}")
+        log(f"learned_channels: {target} training loss per epoch {[round(h, 4) for h in hist]}")
         nets[target] = (net, hist, len(pairs))
         for g, b, p in pairs:
             b.close()
@@ -349,7 +358,7 @@ def bench_c5(dev, args):
                                   correction_net=nets["correction"][0])
         evals[f"{nx}x{ny}"] = r
         for mode, v in r.items():
-            log(f"c5 {nx}x{ny} {mode}: {v['pressure_iterations_per_step']:.1f} its/step, "
+            log(f"learned_channels {nx}x{ny} {mode}: {v['pressure_iterations_per_step']:.1f} its/step, "
                 f"{v['steps_per_s']:.1f} steps/s")
     base = evals["1024x256"]["previous"]
     best_mode = min(evals["1024x256"], key=lambda k: evals["1024x256"][k]["pressure_iterations_per_step"])
@@ -384,10 +393,13 @@ def main():
                          "from the previous pressure, as the reference.  With another guess than "
                          "previous, fp32_refined also runs with the previous pressure once, to "
                          "separate the precision's share of the speed-up from the guess's")
-    ap.add_argument("--config", choices=["c4", "c1", "c2", "c3", "c5"], default="c4",
-                    help="c4 (default): the 2048^2 cavity headline; c1: 32^2 cavity, fields vs "
-                         "the CPU oracle; c2: 512x128 channel; c3: 64-cavity batch; c5: learned "
-                         "warm start on channels")
+    ap.add_argument("--config", default="cavity2048",
+                    choices=["cavity2048", "cavity32", "channel512", "cavity_batch64",
+                             "learned_channels"],
+                    help="cavity2048 (default): the 2048^2 cavity headline; cavity32: 32^2 "
+                         "cavity, fields vs the CPU oracle; channel512: 512x128 channel past a "
+                         "disk; cavity_batch64: 64 cavities in one launch; learned_channels: "
+                         "learned warm start trained and evaluated on channels")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only nsfem_saxpy on 2^26 doubles x 5 (known HBM bytes: "
                          "16 B read + 8 B written per element) to scale rocprofv3 "
@@ -405,8 +417,10 @@ def main():
     from navier_stokes_fem_amd import mesh
     from navier_stokes_fem_amd.device import GPUSimulationDevice
 
-    if args.config != "c4":
-        runner = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c5": bench_c5}[args.config]
+    if args.config != "cavity2048":
+        runner = {"cavity32": bench_cavity32, "channel512": bench_channel512,
+                  "cavity_batch64": bench_cavity_batch64,
+                  "learned_channels": bench_learned_channels}[args.config]
         dev = GPUSimulationDevice(0)
         out = runner(dev, args)
         dev.close()

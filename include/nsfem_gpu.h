@@ -302,11 +302,12 @@ int nsfem_simulation_set_options(nsfem_simulation *sim, const nsfem_sim_options 
 /* Back to the initial state: zero fields with the velocity Dirichlet values
  * imposed (include/assembly.h:1503-1507). */
 int nsfem_simulation_reset(nsfem_simulation *sim);
-/* Batches (north_star C3): `count` simulations of one device and one
- * precision advance `steps` steps together; each solve kind (u/v diffusion
- * as 2 columns, pressure, u/v mass as 2 columns) runs for all of them in
- * one persistent launch.  stats: `count` entries (nullable), accumulated;
- * seconds and cg_seconds are the whole batch's. */
+/* Batches (many independent cavities or channels of any sizes): `count`
+ * simulations of one device and one precision advance `steps` steps
+ * together; each solve kind (u/v diffusion as 2 columns, pressure, u/v mass
+ * as 2 columns) runs for all of them in one persistent launch.  stats:
+ * `count` entries (nullable), accumulated; seconds and cg_seconds are the
+ * whole batch's. */
 int nsfem_simulation_step_batch(nsfem_simulation *const *sims, int count, int steps,
                                 nsfem_step_stats *stats);
 typedef struct {
