@@ -313,12 +313,14 @@ int nsfem_simulation_step_batch(nsfem_simulation *const *sims, int count, int st
 typedef struct {
     int32_t rows_per_wave;          /* wave-tile rows chosen (64, 32 or 16) */
     int32_t nnz_per_wave;           /* wave-tile nonzero cap chosen */
-    int32_t block_scale;            /* blocks per instance multiplier (1, 2, 4) */
-    int32_t pad;
+    int32_t block_scale;            /* blocks per instance multiplier (1, 2, 4), pressure */
+    int32_t velocity_block_scale;   /* the same for the two-column velocity solves */
     double seconds_per_iteration;   /* batched pressure iteration, best candidate */
 } nsfem_batch_tuning;
 /* Times the candidate wave-tile shapes and block splits on the batch's
- * pressure systems and applies the fastest to all of the batch's matrices. */
+ * pressure systems and applies the fastest to all of the batch's matrices;
+ * then times the block splits of the two-column diffusion systems with that
+ * shape for the velocity solves. */
 int nsfem_simulation_autotune_batch(nsfem_simulation *const *sims, int count,
                                     nsfem_batch_tuning *out);
 

@@ -78,12 +78,13 @@ class StepStats(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("rows_per_wave", C.c_int32), ("nnz_per_wave", C.c_int32),
-                ("block_scale", C.c_int32), ("pad", C.c_int32),
+                ("block_scale", C.c_int32), ("velocity_block_scale", C.c_int32),
                 ("seconds_per_iteration", C.c_double)]
 
     def as_dict(self):
         return {"rows_per_wave": self.rows_per_wave, "nnz_per_wave": self.nnz_per_wave,
                 "block_scale": self.block_scale,
+                "velocity_block_scale": self.velocity_block_scale,
                 "seconds_per_iteration": self.seconds_per_iteration}
 
 

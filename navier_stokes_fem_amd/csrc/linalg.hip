@@ -88,6 +88,15 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 #endif
 }
 
+// 16-bit column offsets unless NSFEM_CG_IDX16=0 (comparison runs)
+bool want_idx16() {
+    static const bool on = [] {
+        const char *e = std::getenv("NSFEM_CG_IDX16");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
+}
+
 template <typename T>
 __device__ __forceinline__ T fma_t(T a, T b, T c);
 template <>
@@ -146,26 +155,30 @@ __global__ void k_col_delta(int n_wtiles, const int4 *__restrict__ meta,
     if (out) __hip_atomic_fetch_or(bad, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The CG streams 16-bit column offsets instead of 32-bit columns when every
-// offset from its wave tile's first row fits (banded numberings: the
-// offsets span about twice the mesh width).
-void build_col_delta(Matrix &m) {
-    m.dcol_ok = false;
-    m.dcol.release();
-    if (m.nnz <= 0 || m.rows != m.cols || m.n_wtiles <= 0) return;
-    m.dcol.resize((size_t)m.nnz);
+// Kernels stream 16-bit column offsets instead of 32-bit columns when every
+// offset from its tile's first row fits (banded numberings: the offsets span
+// about twice the mesh width).  Returns whether they do; `out` is released
+// if not.
+bool col_delta(const Matrix &m, int n_tiles, const int4 *meta, DeviceArray<int16_t> &out) {
+    out.release();
+    if (m.nnz <= 0 || m.rows != m.cols || n_tiles <= 0) return false;
+    out.resize((size_t)m.nnz);
     DeviceArray<int> bad;
     bad.resize(1);
     NSFEM_HIP(hipMemset(bad.ptr, 0, sizeof(int)));
     constexpr int kWavesPerBlock = 4;
-    k_col_delta<<<(m.n_wtiles + kWavesPerBlock - 1) / kWavesPerBlock, 64 * kWavesPerBlock>>>(
-        m.n_wtiles, m.wmeta.ptr, m.col.ptr, m.dcol.ptr, bad.ptr);
+    k_col_delta<<<(n_tiles + kWavesPerBlock - 1) / kWavesPerBlock, 64 * kWavesPerBlock>>>(
+        n_tiles, meta, m.col.ptr, out.ptr, bad.ptr);
     NSFEM_HIP(hipGetLastError());
     int h = 1;
     NSFEM_HIP(hipMemcpy(&h, bad.ptr, sizeof(int), hipMemcpyDeviceToHost));
-    m.dcol_ok = h == 0;
-    if (!m.dcol_ok) m.dcol.release();
+    if (h != 0) out.release();
+    return h == 0;
 }
+
+// wave tiles: the persistent CG's offsets
+void build_col_delta(Matrix &m) { m.dcol_ok = col_delta(m, m.n_wtiles, m.wmeta.ptr, m.dcol); }
+
 }  // namespace
 
 void build_tiles(Matrix &m, const std::vector<int32_t> &rs) {
@@ -732,7 +745,11 @@ __device__ __forceinline__ WTile wave_tile(const int4 *meta, int t) {
 // in 12 bytes, so that one dwordx3 gather brings everything a neighbour
 // contributes.  A wave64 gather costs the same return-path time whatever its
 // width per lane, so one wide gather replaces the two narrow ones (r, p) of
-// the two-phase iteration.
+// the two-phase iteration.  (The solution stays FP64 and is updated every
+// iteration: an FP32 accumulator between reliable updates, multiplied by an
+// ill-conditioned A, pollutes the true residual at ~eps32 * cond(A) of its
+// last level and cost the 2048^2 cavity's pressure solve 50% more
+// iterations.)
 struct alignas(4) RowState {
     float r, p, q;
 };
@@ -898,6 +915,8 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
     // The prefetched words stay raw until the tile is consumed (no ALU use
     // of a load in flight).  Lanes beyond the tile read its first nonzero
     // (a valid column of the tile); a tile without nonzeros gathers nothing.
+    // Item slices wholly beyond a tile's nonzeros issue no loads (a 64-row
+    // tile of a 7-point matrix fills 7 of its 8 slices): 5% faster here.
     WTile w = wave_tile(A.meta, t);
     int cn[kWItems];
     {
@@ -905,7 +924,7 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
 #pragma unroll
         for (int i = 0; i < kWItems; ++i) {
             const int k = lane + 64 * i;
-            cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
+            cn[i] = 64 * i < w.nz ? col_word<IT>(A, k < w.nz ? w.k0 + k : kf) : 0;
         }
     }
     for (; t < T1; t += NW) {
@@ -921,7 +940,7 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
 #pragma unroll
             for (int i = 0; i < kWItems; ++i) {
                 const int k = lane + 64 * i;
-                cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
+                cn[i] = 64 * i < w.nz ? col_word<IT>(A, k < w.nz ? w.k0 + k : kf) : 0;
             }
         }
         // 2. owner rows (clamped so the loads stay unconditional)
@@ -947,22 +966,31 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
             constexpr int IB = (NC == 2) ? 4 : kWItems;  // register budget
 #pragma unroll
             for (int i0 = 0; i0 < kWItems; i0 += IB) {
+                if (64 * i0 >= cur.nz) break;
                 MT vj[IB];
                 VecN<NC, VT> rj[IB], pj[IB];
 #pragma unroll
                 for (int i = 0; i < IB; ++i) {
                     const int k = lane + 64 * (i0 + i);
-                    vj[i] = ld_stream(val + (k < cur.nz ? cur.k0 + k : 0));
-                    rj[i] = ldv<NC, VT>(r, cj[i0 + i]);
-                    pj[i] = ldv<NC, VT>(pold, cj[i0 + i]);
+                    if (64 * (i0 + i) < cur.nz) {
+                        vj[i] = ld_stream(val + (k < cur.nz ? cur.k0 + k : 0));
+                        rj[i] = ldv<NC, VT>(r, cj[i0 + i]);
+                        pj[i] = ldv<NC, VT>(pold, cj[i0 + i]);
+                    } else {
+                        vj[i] = (MT)0;
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) rj[i].v[c] = pj[i].v[c] = (VT)0;
+                    }
                 }
 #pragma unroll
                 for (int i = 0; i < IB; ++i) {
                     const int k = lane + 64 * (i0 + i);
+                    if (64 * (i0 + i) < cur.nz) {
 #pragma unroll
-                    for (int c = 0; c < NC; ++c)
-                        lds_wave[k * NC + c] =
-                            (double)vj[i] * (double)fma_t<VT>(beta[c], pj[i].v[c], rj[i].v[c]);
+                        for (int c = 0; c < NC; ++c)
+                            lds_wave[k * NC + c] =
+                                (double)vj[i] * (double)fma_t<VT>(beta[c], pj[i].v[c], rj[i].v[c]);
+                    }
                 }
             }
             __syncwarp();
@@ -1331,6 +1359,8 @@ __device__ void phase_f1(const CgArgs &A, int T0, int T1, double *lds_wave, cons
         } else if (cur.nz <= kWaveNnz) {
             constexpr int IB = kWItems / 2;  // registers in flight
 #pragma unroll
+            // (loads stay unconditional: guarding the item slices beyond a
+            // tile's nonzeros serialised this loop's gathers, 2x slower)
             for (int i0 = 0; i0 < kWItems; i0 += IB) {
                 float vj[IB];
                 RowState gj[IB];
@@ -1640,14 +1670,6 @@ struct CgLaunch {
     void run() { cg_launch<NC, MX, IT>(d, a, grid); }
 };
 
-// 16-bit column offsets unless NSFEM_CG_IDX16=0 (comparison runs)
-bool want_idx16() {
-    static const bool on = [] {
-        const char *e = std::getenv("NSFEM_CG_IDX16");
-        return !(e && std::atoi(e) == 0);
-    }();
-    return on;
-}
 
 }  // namespace
 
@@ -1889,26 +1911,41 @@ struct CgBatchLaunch {
 
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
-// Blocks per instance: proportional to the instance's wave tiles (one block
-// per 32 tiles, as a single solve takes), at least one each, at most `maxg`
-// in total.
+// Blocks per instance, at most `maxg` in total.  Alone, an instance would get
+// one block per 32 wave tiles (`scale` times that, its cap here).  When the
+// batch wants more blocks than the device holds, every instance starts with
+// one and the rest go, one at a time, to the instance with the largest
+// expected time: wave tiles x iterations / blocks, with the iterations of a
+// Poisson-type solve growing like the mesh width (tiles^0.5).  So the
+// largest system, which needs the most iterations, does not become the
+// batch's critical path on a fair share.
 std::vector<int> split_blocks(const std::vector<const CgRequest *> &rq, int maxg, int scale) {
     const size_t I = rq.size();
-    std::vector<int> nb(I);
+    std::vector<int> want(I), nb(I, 1);
+    std::vector<double> work(I);
     int64_t total = 0;
     for (size_t i = 0; i < I; ++i) {
-        const int want = std::max(1, rq[i]->m->n_wtiles * scale / 32);
-        nb[i] = std::min(want, std::max(1, rq[i]->m->n_wtiles));
-        total += nb[i];
+        const int tiles = std::max(1, rq[i]->m->n_wtiles);
+        want[i] = std::min(tiles, std::max(1, tiles * scale / 32));
+        work[i] = std::pow((double)tiles, 1.5);
+        total += want[i];
     }
-    while (total > maxg) {  // trim the largest share until the batch fits
-        size_t big = 0;
-        for (size_t i = 1; i < I; ++i)
-            if (nb[i] > nb[big]) big = i;
-        if (nb[big] <= 1) break;
-        const int cut = std::max<int64_t>(1, std::min<int64_t>(nb[big] - 1, total - maxg));
-        nb[big] -= cut;
-        total -= cut;
+    if (total <= maxg) return want;
+    int64_t left = maxg - (int64_t)I;
+    while (left > 0) {
+        size_t big = I;
+        double worst = -1.0;
+        for (size_t i = 0; i < I; ++i) {
+            if (nb[i] >= want[i]) continue;
+            const double t = work[i] / nb[i];
+            if (t > worst) {
+                worst = t;
+                big = i;
+            }
+        }
+        if (big == I) break;
+        ++nb[big];
+        --left;
     }
     return nb;
 }

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <unordered_map>
@@ -53,7 +54,10 @@ struct Simulation {
     // corrections and A X, proj_max vectors each (proj_dim in use)
     DeviceArray<double> proj_x, proj_ax, proj_w, proj_s;
     int proj_dim = 0, proj_max = 0;
-    int batch_block_scale = 1;  // blocks per instance in batched solves (autotuned)
+    // blocks per instance in batched solves (autotuned): pressure, and the
+    // two-column velocity solves (their P2 matrices have ~4x the rows)
+    int batch_block_scale = 1;
+    int batch_block_scale_v = 1;
 };
 
 namespace {
@@ -525,6 +529,7 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
     d.set_current();
     const size_t I = S.size();
     const int scale = S[0]->batch_block_scale;
+    const int scale_v = S[0]->batch_block_scale_v;
     std::vector<nsfem_cg_stats> st(2 * I);
     std::vector<CgRequest> rq(I);
     auto request = [&](Simulation &s, int slot, const double *b, double *x, int ncol) {
@@ -565,7 +570,7 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
         for (size_t i = 0; i < I; ++i)
             rq[i] = request(*S[i], NSFEM_DIFFUSION, S[i]->rhs.ptr, S[i]->uv.ptr, 2);
         std::vector<nsfem_cg_stats> sd(2 * I);
-        check(run_cg_batch(d, rq, sd.data(), scale, &sec[0]), "diffusion");
+        check(run_cg_batch(d, rq, sd.data(), scale_v, &sec[0]), "diffusion");
         for (Simulation *sp : S) {
             Simulation &s = *sp;
             launch_spmv(d, s.mats[NSFEM_VELOCITY_DIVERGENCE], s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
@@ -585,7 +590,7 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
         for (size_t i = 0; i < I; ++i)
             rq[i] = request(*S[i], NSFEM_VELOCITY_MASS, S[i]->rhs.ptr, S[i]->du.ptr, 2);
         std::vector<nsfem_cg_stats> sm(2 * I);
-        check(run_cg_batch(d, rq, sm.data(), scale, &sec[2]), "velocity mass");
+        check(run_cg_batch(d, rq, sm.data(), scale_v, &sec[2]), "velocity mass");
         for (Simulation *sp : S) {
             Simulation &s = *sp;
             launch_add(d, 2 * s.V, s.du.ptr, s.uv.ptr);
@@ -638,10 +643,16 @@ void simulation_autotune_batch(const std::vector<Simulation *> &S, nsfem_batch_t
         off[i] = total;
         total += (size_t)S[i]->P;
     }
+    size_t total_v = 0;
+    std::vector<size_t> off_v(I);
+    for (size_t i = 0; i < I; ++i) {
+        off_v[i] = total_v;
+        total_v += 2 * (size_t)S[i]->V;
+    }
     DeviceArray<double> b, x;
-    b.resize(total);
-    x.resize(total);
-    launch_fill(d, (int)total, 1.0, b.ptr);
+    b.resize(std::max(total, total_v));
+    x.resize(std::max(total, total_v));
+    launch_fill(d, (int)std::max(total, total_v), 1.0, b.ptr);
     const int iters = 40;
     struct Shape { int rows, nnz; };
     const Shape shapes[3] = {{64, 512}, {32, 256}, {16, 128}};
@@ -671,6 +682,9 @@ void simulation_autotune_batch(const std::vector<Simulation *> &S, nsfem_batch_t
             launch_fill(d, (int)total, 0.0, x.ptr);
             double sec = 0.0;
             run_cg_batch(d, rq, st.data(), sc, &sec, false);
+            if (std::getenv("NSFEM_CG_VERBOSE"))
+                std::fprintf(stderr, "[nsfem] batch autotune rows=%d nnz=%d scale=%d: %.3f ms\n",
+                             sh.rows, sh.nnz, sc, sec * 1e3);
             if (sec < best) {
                 best = sec;
                 best_shape = sh;
@@ -683,11 +697,43 @@ void simulation_autotune_batch(const std::vector<Simulation *> &S, nsfem_batch_t
             if (m.ready && m.rows == m.cols) build_wave_tiles(m, best_shape.rows, best_shape.nnz);
         s->batch_block_scale = best_scale;
     }
+    // the two-column velocity solves, with the chosen tile shape: their
+    // matrices have ~4x the rows of the pressure matrix, so the share of
+    // blocks that suits the pressure can overfill the device for them
+    double best_v = 1e300;
+    int best_scale_v = 1;
+    for (int sc : scales) {
+        std::vector<CgRequest> rq(I);
+        for (size_t i = 0; i < I; ++i) {
+            CgRequest r{};
+            r.m = &S[i]->mats[NSFEM_DIFFUSION];
+            r.b = b.ptr + off_v[i];
+            r.x = x.ptr + off_v[i];
+            r.ncol = 2;
+            r.max_iter = iters;
+            r.rtol = 0.0;
+            r.atol = 0.0;
+            r.precision = S[0]->opt.precision;
+            set_update_policy(r, NSFEM_DIFFUSION);
+            rq[i] = r;
+        }
+        launch_fill(d, (int)total_v, 0.0, x.ptr);
+        double sec = 0.0;
+        run_cg_batch(d, rq, st.data(), sc, &sec, false);
+        if (std::getenv("NSFEM_CG_VERBOSE"))
+            std::fprintf(stderr, "[nsfem] batch autotune velocity scale=%d: %.3f ms\n", sc, sec * 1e3);
+        if (sec < best_v) {
+            best_v = sec;
+            best_scale_v = sc;
+        }
+    }
+    for (Simulation *s : S) s->batch_block_scale_v = best_scale_v;
     NSFEM_HIP(hipStreamSynchronize(d.stream));
     if (out) {
         out->rows_per_wave = best_shape.rows;
         out->nnz_per_wave = best_shape.nnz;
         out->block_scale = best_scale;
+        out->velocity_block_scale = best_scale_v;
         out->seconds_per_iteration = best / iters;
     }
 }

@@ -52,6 +52,7 @@ def test_batch_autotune_keeps_results(gpu):
     t = autotune_batch(tuned)
     print("autotune:", t)
     assert t["rows_per_wave"] in (64, 32, 16) and t["block_scale"] in (1, 2, 4)
+    assert t["velocity_block_scale"] in (1, 2, 4)
     assert t["seconds_per_iteration"] > 0
     step_batch(ref, 3)
     step_batch(tuned, 3)
