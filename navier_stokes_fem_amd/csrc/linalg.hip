@@ -208,8 +208,11 @@ void build_wave_tiles(Matrix &m, int rows, int nnz) {
     NSFEM_HIP(hipMemcpy(m.wtiles.ptr, w.data(), w.size() * sizeof(int32_t),
                         hipMemcpyHostToDevice));
     std::vector<int4> meta((size_t)std::max(m.n_wtiles, 1));
-    for (int i = 0; i < m.n_wtiles; ++i)
+    m.max_multirow_nz = 0;
+    for (int i = 0; i < m.n_wtiles; ++i) {
         meta[i] = make_int4(w[i], w[i + 1], rs[w[i]], rs[w[i + 1]] - rs[w[i]]);
+        if (w[i + 1] - w[i] > 1) m.max_multirow_nz = std::max(m.max_multirow_nz, meta[i].w);
+    }
     m.wmeta.resize(meta.size());
     NSFEM_HIP(hipMemcpy(m.wmeta.ptr, meta.data(), meta.size() * sizeof(int4),
                         hipMemcpyHostToDevice));
@@ -1308,7 +1311,9 @@ __device__ __forceinline__ float next_r(float alpha, float r, float q) {
     return __builtin_fmaf(-alpha, q, r);
 }
 
-template <int BS, typename IT>
+// WI: nonzeros per lane per tile (the host's bound on every multi-row tile's
+// nonzeros over 64); a single longer row is summed wave-wide.
+template <int BS, typename IT, int WI>
 __device__ void phase_f1(const CgArgs &A, int T0, int T1, double *lds_wave, const void *st_old,
                          void *st_new, const float alpha, const float beta, const double pend,
                          const bool has_pend, double (&sums)[4]) {
@@ -1321,26 +1326,26 @@ __device__ void phase_f1(const CgArgs &A, int T0, int T1, double *lds_wave, cons
     // column words of the next tile stay raw until the tile is consumed;
     // lanes beyond a tile read its first nonzero (a valid column)
     WTile w = wave_tile(A.meta, t);
-    int cn[kWItems];
+    int cn[WI];
     {
         const int kf = w.nz > 0 ? w.k0 : 0;
 #pragma unroll
-        for (int i = 0; i < kWItems; ++i) {
+        for (int i = 0; i < WI; ++i) {
             const int k = lane + 64 * i;
             cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
         }
     }
     for (; t < T1; t += NW) {
         const WTile cur = w;
-        int cj[kWItems];
+        int cj[WI];
 #pragma unroll
-        for (int i = 0; i < kWItems; ++i) cj[i] = col_of<IT>(cn[i], cur.r0);
+        for (int i = 0; i < WI; ++i) cj[i] = col_of<IT>(cn[i], cur.r0);
         const int tn = t + NW;
         if (tn < T1) {
             w = wave_tile(A.meta, tn);
             const int kf = w.nz > 0 ? w.k0 : 0;
 #pragma unroll
-            for (int i = 0; i < kWItems; ++i) {
+            for (int i = 0; i < WI; ++i) {
                 const int k = lane + 64 * i;
                 cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
             }
@@ -1356,22 +1361,22 @@ __device__ void phase_f1(const CgArgs &A, int T0, int T1, double *lds_wave, cons
         double s = 0.0;
         if (cur.nz == 0) {
             // rows without nonzeros: q = 0
-        } else if (cur.nz <= kWaveNnz) {
-            constexpr int IB = kWItems / 2;  // registers in flight
+        } else if (cur.nz <= 64 * WI) {
+            constexpr int IB = 4;  // registers in flight
 #pragma unroll
             // (loads stay unconditional: guarding the item slices beyond a
             // tile's nonzeros serialised this loop's gathers, 2x slower)
-            for (int i0 = 0; i0 < kWItems; i0 += IB) {
+            for (int i0 = 0; i0 < WI; i0 += IB) {
                 float vj[IB];
                 RowState gj[IB];
 #pragma unroll
-                for (int i = 0; i < IB; ++i) {
+                for (int i = 0; i < IB && i0 + i < WI; ++i) {
                     const int k = lane + 64 * (i0 + i);
                     vj[i] = ld_stream(val + (k < cur.nz ? cur.k0 + k : 0));
                     gj[i] = ld_state(st_old, cj[i0 + i]);
                 }
 #pragma unroll
-                for (int i = 0; i < IB; ++i) {
+                for (int i = 0; i < IB && i0 + i < WI; ++i) {
                     const int k = lane + 64 * (i0 + i);
                     lds_wave[k] = (double)vj[i] *
                                   (double)__builtin_fmaf(beta, gj[i].p, next_r(alpha, gj[i].r, gj[i].q));
@@ -1505,7 +1510,7 @@ __device__ void ctl_end(CgCtl1 &C) {
 // The one-column mixed persistent CG run by one block (arguments as
 // cg_body).  The recursive residual is replaced by the FP64 b - A x whenever
 // it has fallen by rdelta (reliable updates).
-template <int BS, typename IT>
+template <int BS, typename IT, int WI>
 __device__ __forceinline__ void cg_body1(const CgArgs &A, const int bid, const int nb, const int L,
                                          double *prod_raw, double *lds) {
     constexpr bool MIXED = true;  // residual_phase precision
@@ -1552,7 +1557,8 @@ __device__ __forceinline__ void cg_body1(const CgArgs &A, const int bid, const i
         const bool has_pend = C.has_pend != 0;
         double sums[4] = {0.0, 0.0, 0.0, 0.0};
         CG_TRACE_STAMP(C.k, 0);
-        phase_f1<BS, IT>(A, T0, T1, lds64, A.st[cur], A.st[nx], alpha, beta, pend, has_pend, sums);
+        phase_f1<BS, IT, WI>(A, T0, T1, lds64, A.st[cur], A.st[nx], alpha, beta, pend, has_pend,
+                             sums);
         CG_TRACE_MAX(C.k, 1);
         if (!grid_sum<4, BS>(A.sync, sums, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
         CG_TRACE_STAMP(C.k, 2);
@@ -1595,14 +1601,15 @@ timeout:
 
 // One block per CU: BS / 256 waves per SIMD (4 for 1024 threads, 128 VGPRs
 // per wave; 2 for 512 threads, up to 256 VGPRs).
-template <int NC, int BS, bool MIXED, typename IT>
+// WI (items per lane of a tile) < 8 is used by the one-column mixed solve only.
+template <int NC, int BS, bool MIXED, typename IT, int WI>
 __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
     constexpr int NW = BS / 64;
     __shared__ double prod_raw[NW * kWaveNnz * NC];
     __shared__ double lds[kMaxRed * NW];
     const int L = xcd_swizzle(blockIdx.x, gridDim.x);
     if constexpr (NC == 1 && MIXED)
-        cg_body1<BS, IT>(A, blockIdx.x, gridDim.x, L, prod_raw, lds);
+        cg_body1<BS, IT, WI>(A, blockIdx.x, gridDim.x, L, prod_raw, lds);
     else
         cg_body<NC, BS, MIXED, IT>(A, blockIdx.x, gridDim.x, L, prod_raw, lds);
 }
@@ -1611,7 +1618,7 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
 // instance map[i].x as its block map[i].y of args[instance].nblocks; each
 // instance reduces over its own blocks with its own SyncState, and its blocks
 // leave the kernel when it converges.
-template <int NC, int BS, bool MIXED, typename IT>
+template <int NC, int BS, bool MIXED, typename IT, int WI>
 __global__ __launch_bounds__(BS, BS / 256) void k_cg_batch(const CgArgs *__restrict__ args,
                                                            const int2 *__restrict__ map) {
     constexpr int NW = BS / 64;
@@ -1622,52 +1629,75 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg_batch(const CgArgs *__restr
     const int lb = __builtin_amdgcn_readfirstlane(m.y);
     const CgArgs A = args[inst];
     if constexpr (NC == 1 && MIXED)
-        cg_body1<BS, IT>(A, lb, A.nblocks, lb, prod_raw, lds);
+        cg_body1<BS, IT, WI>(A, lb, A.nblocks, lb, prod_raw, lds);
     else
         cg_body<NC, BS, MIXED, IT>(A, lb, A.nblocks, lb, prod_raw, lds);
 }
 
 constexpr int kCgBlock = 1024;
 
-template <int NC, bool MIXED, typename IT>
+template <int NC, bool MIXED, typename IT, int WI>
 void cg_launch(Device &d, const CgArgs &args, int grid) {
     void *kargs[] = {(void *)&args};
-    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg<NC, kCgBlock, MIXED, IT>, dim3(grid),
-                                         dim3(kCgBlock), kargs, 0, d.stream));
+    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg<NC, kCgBlock, MIXED, IT, WI>,
+                                         dim3(grid), dim3(kCgBlock), kargs, 0, d.stream));
 }
-template <int NC, bool MIXED, typename IT>
+template <int NC, bool MIXED, typename IT, int WI>
 int cg_max_grid(Device &d) {
     int per_cu = 0;
-    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg<NC, kCgBlock, MIXED, IT>,
-                                                           kCgBlock, 0));
+    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, k_cg<NC, kCgBlock, MIXED, IT, WI>, kCgBlock, 0));
     return std::max(1, std::min(per_cu * d.cu_count, kMaxGridBlocks));
 }
 
-// Calls f.template run<NC, MIXED, IT>() for the solve's column count,
-// precision and column-index width.
-template <typename F>
-void cg_dispatch(int nc, bool mixed, bool idx16, F &&f) {
+// Calls f.template run<NC, MIXED, IT, WI>() for the solve's column count,
+// precision, column-index width and (one-column mixed solves) items per lane.
+template <int WI, typename F>
+void cg_dispatch_wi(int nc, bool mixed, bool idx16, F &f) {
     if (nc == 1) {
-        if (mixed) idx16 ? f.template run<1, true, int16_t>() : f.template run<1, true, int32_t>();
-        else idx16 ? f.template run<1, false, int16_t>() : f.template run<1, false, int32_t>();
+        if (mixed) {
+            if (idx16) f.template run<1, true, int16_t, WI>();
+            else f.template run<1, true, int32_t, WI>();
+        } else {
+            if (idx16) f.template run<1, false, int16_t, kWItems>();
+            else f.template run<1, false, int32_t, kWItems>();
+        }
     } else {
-        if (mixed) idx16 ? f.template run<2, true, int16_t>() : f.template run<2, true, int32_t>();
-        else idx16 ? f.template run<2, false, int16_t>() : f.template run<2, false, int32_t>();
+        if (mixed) {
+            if (idx16) f.template run<2, true, int16_t, kWItems>();
+            else f.template run<2, true, int32_t, kWItems>();
+        } else {
+            if (idx16) f.template run<2, false, int16_t, kWItems>();
+            else f.template run<2, false, int32_t, kWItems>();
+        }
     }
+}
+template <typename F>
+void cg_dispatch(int nc, bool mixed, bool idx16, int wi, F &&f) {
+    if (wi <= 4) cg_dispatch_wi<4>(nc, mixed, idx16, f);
+    else if (wi <= 7) cg_dispatch_wi<7>(nc, mixed, idx16, f);
+    else cg_dispatch_wi<kWItems>(nc, mixed, idx16, f);
+}
+
+// Items per lane the one-column mixed iteration needs for a matrix: every
+// multi-row wave tile's nonzeros over 64, rounded up to 4, 7 or 8.
+int items_per_lane(const Matrix &m) {
+    const int need = (m.max_multirow_nz + 63) / 64;
+    return need <= 4 ? 4 : need <= 7 ? 7 : kWItems;
 }
 
 struct CgMaxGrid {
     Device &d;
     int out;
-    template <int NC, bool MX, typename IT>
-    void run() { out = cg_max_grid<NC, MX, IT>(d); }
+    template <int NC, bool MX, typename IT, int WI>
+    void run() { out = cg_max_grid<NC, MX, IT, WI>(d); }
 };
 struct CgLaunch {
     Device &d;
     const CgArgs &a;
     int grid;
-    template <int NC, bool MX, typename IT>
-    void run() { cg_launch<NC, MX, IT>(d, a, grid); }
+    template <int NC, bool MX, typename IT, int WI>
+    void run() { cg_launch<NC, MX, IT, WI>(d, a, grid); }
 };
 
 
@@ -1753,8 +1783,9 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
     // registers; 512 threads (8 waves, up to 256 VGPRs) was slower on every
     // measured solve.
     const bool idx16 = m.dcol_ok && want_idx16();
+    const int wi = items_per_lane(m);
     CgMaxGrid mg{d, 0};
-    cg_dispatch(nc, mixed, idx16, mg);
+    cg_dispatch(nc, mixed, idx16, wi, mg);
     const int maxg = mg.out;
     int want = req.grid;
     if (want <= 0) {
@@ -1805,7 +1836,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
         k_sync_reset<<<(ngroups + 255) / 256, 256, 0, d.stream>>>(a.sync, ngroups);
         NSFEM_HIP(hipGetLastError());
         CgLaunch ln{d, a, grid};
-        cg_dispatch(nc, mixed, idx16, ln);
+        cg_dispatch(nc, mixed, idx16, wi, ln);
         NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
         NSFEM_HIP(hipMemcpyAsync(P.res, d.cg_res.ptr, sizeof(CgColumnResult) * nc,
                                  hipMemcpyDeviceToHost, d.stream));
@@ -1879,34 +1910,34 @@ int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
 // ------------------------------------------------------------ batched CG
 namespace {
 
-template <int NC, bool MIXED, typename IT>
+template <int NC, bool MIXED, typename IT, int WI>
 void cg_batch_launch(Device &d, const CgArgs *args, const int2 *map, int grid) {
     void *kargs[] = {(void *)&args, (void *)&map};
-    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg_batch<NC, kCgBlock, MIXED, IT>,
+    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg_batch<NC, kCgBlock, MIXED, IT, WI>,
                                          dim3(grid), dim3(kCgBlock), kargs, 0, d.stream));
 }
 
-template <int NC, bool MIXED, typename IT>
+template <int NC, bool MIXED, typename IT, int WI>
 int cg_batch_max_grid(Device &d) {
     int per_cu = 0;
     NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, k_cg_batch<NC, kCgBlock, MIXED, IT>, kCgBlock, 0));
+        &per_cu, k_cg_batch<NC, kCgBlock, MIXED, IT, WI>, kCgBlock, 0));
     return std::max(1, std::min(per_cu * d.cu_count, kMaxGridBlocks));
 }
 
 struct CgBatchMaxGrid {
     Device &d;
     int out;
-    template <int NC, bool MX, typename IT>
-    void run() { out = cg_batch_max_grid<NC, MX, IT>(d); }
+    template <int NC, bool MX, typename IT, int WI>
+    void run() { out = cg_batch_max_grid<NC, MX, IT, WI>(d); }
 };
 struct CgBatchLaunch {
     Device &d;
     const CgArgs *args;
     const int2 *map;
     int grid;
-    template <int NC, bool MX, typename IT>
-    void run() { cg_batch_launch<NC, MX, IT>(d, args, map, grid); }
+    template <int NC, bool MX, typename IT, int WI>
+    void run() { cg_batch_launch<NC, MX, IT, WI>(d, args, map, grid); }
 };
 
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -1968,9 +1999,13 @@ int run_cg_batch(Device &d, const std::vector<CgRequest> &reqs, nsfem_cg_stats *
         NSFEM_REQUIRE(nc == 1 || nc == 2, NSFEM_INVALID_ARGUMENT, "columns must be 1 or 2");
     }
     bool idx16 = want_idx16();
-    for (const CgRequest &r : reqs) idx16 = idx16 && r.m->dcol_ok;
+    int wi = 4;
+    for (const CgRequest &r : reqs) {
+        idx16 = idx16 && r.m->dcol_ok;
+        wi = std::max(wi, items_per_lane(*r.m));
+    }
     CgBatchMaxGrid mg{d, 0};
-    cg_dispatch(nc, mixed, idx16, mg);
+    cg_dispatch(nc, mixed, idx16, wi, mg);
     const int maxg = mg.out;
     int worst = NSFEM_OK;
     const size_t vt = mixed ? sizeof(float) : sizeof(double);
@@ -2042,7 +2077,7 @@ int run_cg_batch(Device &d, const std::vector<CgRequest> &reqs, nsfem_cg_stats *
         NSFEM_HIP(hipEventRecord(d.ev_start, d.stream));
         const CgArgs *dargs = reinterpret_cast<const CgArgs *>(d.batch_args.ptr);
         CgBatchLaunch ln{d, dargs, d.batch_map.ptr, grid};
-        cg_dispatch(nc, mixed, idx16, ln);
+        cg_dispatch(nc, mixed, idx16, wi, ln);
         NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
         // FP64 true residuals of every instance, one host synchronisation
         if (!mixed) {

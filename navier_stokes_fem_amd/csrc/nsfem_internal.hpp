@@ -121,6 +121,7 @@ struct Matrix {
     std::vector<int64_t> host_wtile_prefix;  // [n_wtiles + 1] cumulative nnz + rows
     std::vector<int32_t> host_rs;            // row_start (wave tiles are rebuilt from it)
     int wave_rows = kWaveRows, wave_nnz = kWaveNnz;  // current wave-tile shape
+    int max_multirow_nz = 0;  // most nonzeros of a wave tile with more than one row
     // ownership of wave tiles by persistent blocks, cached per grid size
     DeviceArray<int32_t> block_tiles;
     int block_tiles_grid = 0;
