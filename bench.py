@@ -68,7 +68,7 @@ def cg_bytes(nnz, n, iters, refinements, mixed, columns=1):
 # WRITE_SIZE (scaled by a k_axpy calibration) on this workload; source:
 # profiles/k_cg_cavity2048_hbm_traffic.md.  roofline.traffic = this figure
 # x the iterations of the launch.
-MEASURED_BYTES_PER_ITERATION = {False: 731.6e6, True: 460.5e6}
+MEASURED_BYTES_PER_ITERATION = {False: 721.9e6, True: 457.7e6}
 
 
 def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
