@@ -56,6 +56,38 @@ def test_spmv_long_row(gpu, oracle):
     assert abs(y[1] - ref[1]) <= 1e-12 * np.abs(val[3:5003] * x[col[3:5003]]).sum()
 
 
+def test_spmv_square_long_row(gpu, oracle):
+    """A square banded matrix with one row longer than a tile: the other rows
+    bit-exact, the long row (block-reduction path) within rounding."""
+    rng = np.random.default_rng(4)
+    n = 3000
+    rows = []
+    for r in range(n):
+        if r == 1500:
+            c = np.sort(rng.choice(np.arange(max(0, r - 1400), min(n, r + 1400)), 900, replace=False))
+        else:
+            c = np.unique(np.clip(r + np.array([-40, -1, 0, 1, 40]), 0, n - 1))
+        rows.append(c)
+    rs = np.concatenate([[0], np.cumsum([len(c) for c in rows])]).astype(np.int32)
+    col = np.concatenate(rows).astype(np.int32)
+    val = rng.standard_normal(len(col))
+    gpu.upload_csr(8, rs, col, val)
+    x = rng.standard_normal(n)
+    lhs = rng.standard_normal(n)
+    by, bl = _buf(gpu, np.zeros(n)), _buf(gpu, lhs)
+    gpu.sp_r_mult(8, _buf(gpu, x), by)
+    y = by.download()
+    ref = oracle.spmv((rs, col, val), x)
+    others = np.arange(n) != 1500
+    assert np.array_equal(y[others], ref[others])
+    k0, k1 = rs[1500], rs[1501]
+    assert abs(y[1500] - ref[1500]) <= 1e-12 * np.abs(val[k0:k1] * x[col[k0:k1]]).sum()
+    gpu.sp_r_mult_sub(8, bl, _buf(gpu, x), bl)
+    got = bl.download()
+    want = oracle.spmv_sub((rs, col, val), lhs, x)
+    assert np.array_equal(got[others], want[others])
+
+
 def test_triplet_upload_sums_duplicates(gpu, oracle):
     rng = np.random.default_rng(5)
     n = 50
