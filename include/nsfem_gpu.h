@@ -371,9 +371,12 @@ int nsfem_gnn_predict(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
 int nsfem_gnn_gradients(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
                         const nsfem_buffer *b, const nsfem_buffer *p,
                         float *grads, double *loss);
-/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on one (b, p) pair; the loss
- * before the step is written to *loss when loss is not NULL (that
- * synchronises; NULL keeps the step asynchronous). */
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on one (b, p) pair, the
+ * gradient first scaled down to a norm of at most 4 times the running
+ * average (decay 0.95) of earlier steps' norms; the loss before the step is
+ * written to *loss when loss is not NULL (that synchronises; NULL keeps the
+ * step asynchronous).  nsfem_gnn_set_params resets the Adam moments and
+ * the running norm. */
 int nsfem_gnn_train_step(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
                          const nsfem_buffer *b, const nsfem_buffer *p,
                          float learning_rate, double *loss);

@@ -1,7 +1,8 @@
 // Learned warm start for the pressure CG: a 3-layer message-passing network
 // over the pressure-matrix graph that maps the divergence right-hand side to
 // an initial guess.  Forward and backward passes, the loss and an Adam step
-// run here, on the GPU; CG still solves to the same tolerance afterwards, so
+// run here, on the GPU (the Adam step clips the gradient norm at 4 times its
+// running average); CG still solves to the same tolerance afterwards, so
 // a poor guess costs iterations, never accuracy.  (The reference has no
 // learned component: the pressure solve is warm-started from the previous
 // pressure, include/assembly.h:608-615.)
@@ -402,12 +403,41 @@ __global__ void k_reduce_layer(int nb, int kp, int kin, const double *__restrict
     else gc[e - kp * H] = (float)s;
 }
 
+// Gradient-norm clipping, one block: |g| against a running average of the
+// norms of earlier steps (clip[0]); the gradient is scaled by
+// clip[1] = min(1, kClipRatio * average / |g|) before the Adam update, so a
+// pair whose relative loss is far above the rest (a target that is nearly
+// zero against its input scale) moves the weights no more than an ordinary
+// pair.  The average starts at the first step's norm and follows the clipped
+// norms with decay kClipDecay.
+constexpr float kClipRatio = 4.f;
+constexpr float kClipDecay = 0.95f;
+
+__global__ __launch_bounds__(kRB) void k_clip(int count, const float *__restrict__ g,
+                                              float *__restrict__ clip) {
+    double v[1] = {0.0};
+    for (int k = threadIdx.x; k < count; k += kRB) v[0] += (double)g[k] * (double)g[k];
+    __shared__ double total;
+    double *part = &total;
+    block_partial<1>(v, part);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const float norm = (float)sqrt(total);
+    float avg = clip[0];
+    if (!(avg > 0.f)) avg = norm;
+    const float limit = kClipRatio * avg;
+    const float f = norm > limit ? limit / norm : 1.f;
+    clip[0] = kClipDecay * avg + (1.f - kClipDecay) * fminf(norm, limit);
+    clip[1] = f;
+}
+
 __global__ void k_adam(int count, float *__restrict__ w, const float *__restrict__ g,
-                       float *__restrict__ m, float *__restrict__ v, float lr, float b1, float b2,
-                       float eps, float bc1, float bc2) {
+                       const float *__restrict__ clip, float *__restrict__ m,
+                       float *__restrict__ v, float lr, float b1, float b2, float eps, float bc1,
+                       float bc2) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
-    const float gk = g[k];
+    const float gk = g[k] * clip[1];
     const float mk = b1 * m[k] + (1.f - b1) * gk;
     const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
     m[k] = mk;
@@ -576,6 +606,7 @@ GnnNet *gnn_create(Device &d, uint64_t seed) {
     g->grads.resize(NPARAM);
     g->m.resize(NPARAM);
     g->v.resize(NPARAM);
+    g->clip.resize(2);
     g->wt1.resize(H * K1);
     g->wt2.resize(H * K2);
     g->wr2.resize(H * K2);
@@ -585,6 +616,7 @@ GnnNet *gnn_create(Device &d, uint64_t seed) {
     NSFEM_HIP(hipMemset(g->grads.ptr, 0, NPARAM * sizeof(float)));
     NSFEM_HIP(hipMemset(g->m.ptr, 0, NPARAM * sizeof(float)));
     NSFEM_HIP(hipMemset(g->v.ptr, 0, NPARAM * sizeof(float)));
+    NSFEM_HIP(hipMemset(g->clip.ptr, 0, 2 * sizeof(float)));
     gnn_pack(*g);
     NSFEM_HIP(hipStreamSynchronize(d.stream));
     return g;
@@ -598,6 +630,7 @@ void gnn_set_params(GnnNet &g, const float *h) {
                              d.stream));
     NSFEM_HIP(hipMemsetAsync(g.m.ptr, 0, NPARAM * sizeof(float), d.stream));
     NSFEM_HIP(hipMemsetAsync(g.v.ptr, 0, NPARAM * sizeof(float), d.stream));
+    NSFEM_HIP(hipMemsetAsync(g.clip.ptr, 0, 2 * sizeof(float), d.stream));
     g.adam_t = 0;
     gnn_pack(g);
     NSFEM_HIP(hipStreamSynchronize(d.stream));
@@ -700,8 +733,10 @@ void gnn_train_step(GnnNet &g, GnnGraph &gr, const double *b, const double *p, f
     ++g.adam_t;
     const float b1 = 0.9f, b2 = 0.999f;
     const float bc1 = 1.f - std::pow(b1, (float)g.adam_t), bc2 = 1.f - std::pow(b2, (float)g.adam_t);
+    gnn::k_clip<<<1, gnn::kRB, 0, d.stream>>>(gnn::NPARAM, g.grads.ptr, g.clip.ptr);
     gnn::k_adam<<<gnn::blocks_for(gnn::NPARAM), gnn::kRB, 0, d.stream>>>(
-        gnn::NPARAM, g.params.ptr, g.grads.ptr, g.m.ptr, g.v.ptr, lr, b1, b2, 1e-8f, bc1, bc2);
+        gnn::NPARAM, g.params.ptr, g.grads.ptr, g.clip.ptr, g.m.ptr, g.v.ptr, lr, b1, b2, 1e-8f,
+        bc1, bc2);
     NSFEM_HIP(hipGetLastError());
     gnn_pack(g);
     if (loss) {

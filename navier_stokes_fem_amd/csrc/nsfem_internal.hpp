@@ -329,6 +329,7 @@ struct GnnGraph {
 struct GnnNet {
     Device *dev = nullptr;
     DeviceArray<float> params, grads, m, v;
+    DeviceArray<float> clip;  // running gradient norm, this step's gradient scale
     DeviceArray<uint16_t> wt1, wt2, wr2, wt3, wr3;
     int adam_t = 0;
     int cap = 0;  // workspace rows

@@ -25,8 +25,8 @@ import numpy as np
 from . import _native as N
 from .device import DeviceBuffer, assemble
 
-__all__ = ["PressureGraph", "WarmStartNet", "node_areas", "collect_pairs", "mean_loss", "train",
-           "evaluate_warm_start"]
+__all__ = ["PressureGraph", "WarmStartNet", "node_areas", "collect_pair_sets", "collect_pairs",
+           "mean_loss", "train", "evaluate_warm_start"]
 
 
 def node_areas(mesh) -> np.ndarray:
@@ -152,47 +152,86 @@ class WarmStartNet:
             pass
 
 
-def collect_pairs(device, meshes, viscosity=1e-3, dt=5e-4, steps=40, warmup=5, every=1,
-                  precision="fp64", target="solution", log=None):
-    """Run the plain solver on each mesh and keep one training pair per
-    pressure solve of steps warmup+1 .. warmup+steps (every `every`-th):
+def collect_pair_sets(device, meshes, viscosity=1e-3, dt=5e-4, steps=40, warmup=5, every=1,
+                      precision="fp64", targets=("solution",), log=None, log_every=50):
+    """Run the plain solver on every mesh and keep, per mesh, one training
+    pair per pressure solve of steps warmup+1 .. warmup+steps (every
+    `every`-th), for each requested target:
 
-    * target="solution":   (graph, b, p): the right-hand side and the
-      pressure CG's solution (for the ``learned`` modes);
-    * target="correction": (graph, r, delta) with r = b - A p_prev and
+    * "solution":   (graph, b, p): the right-hand side and the pressure CG's
+      solution (for the ``learned`` modes);
+    * "correction": (graph, r, delta) with r = b - A p_prev and
       delta = p - p_prev, p_prev the previous step's pressure (for the
-      ``learned_correction`` mode)."""
-    from .simulation import Simulation
+      ``learned_correction`` mode).
 
-    if target not in ("solution", "correction"):
-        raise ValueError("target must be 'solution' or 'correction'")
-    pairs = []
-    for mi, m in enumerate(meshes):
-        P = m.pressure_count
-        sim = Simulation(m, viscosity, dt, device=device, precision=precision)
-        graph = PressureGraph.from_mesh(device, m, viscosity, dt)
-        if warmup:
-            sim.step(warmup)
-        prev = DeviceBuffer(device, 8 * P) if target == "correction" else None
+    All meshes step together (``step_batch``: each solve kind of every mesh
+    in one persistent launch), so the pairs of many meshes cost little more
+    than those of one.  Returns {target: [pairs]}; pairs of one mesh are
+    contiguous, meshes in the given order."""
+    from .simulation import Simulation, step_batch
+
+    targets = tuple(targets)
+    if not targets or any(t not in ("solution", "correction") for t in targets):
+        raise ValueError("targets must be a non-empty subset of ('solution', 'correction')")
+    sims, graphs, prevs = [], [], []
+    per_mesh = {t: [[] for _ in meshes] for t in targets}
+    try:
+        for m in meshes:
+            sims.append(Simulation(m, viscosity, dt, device=device, precision=precision))
+            graphs.append(PressureGraph.from_mesh(device, m, viscosity, dt))
+            if "correction" in targets:
+                prevs.append(DeviceBuffer(device, 8 * m.pressure_count))
+
+        def advance():
+            if len(sims) > 1:
+                step_batch(sims, 1)
+            else:
+                sims[0].step(1)
+
+        for _ in range(warmup):
+            advance()
+        t0 = time.perf_counter()
         for k in range(steps):
-            if prev is not None:
-                sim.copy_pressure_system(None, prev)
-            sim.step(1)
+            for s, prev in zip(sims, prevs):
+                s.copy_pressure_system(None, prev)
+            advance()
             if k % every:
                 continue
-            b = DeviceBuffer(device, 8 * P)
-            p = DeviceBuffer(device, 8 * P)
-            sim.copy_pressure_system(b, p)
-            if prev is not None:
-                sim.sp_r_mult(0, prev, b, lhs=b)         # b <- b - A p_prev
-                device.saxpby(P, 1.0, -1.0, p, prev, p)  # p <- p - p_prev
-            pairs.append((graph, b, p))
-        if prev is not None:
+            for i, (m, s, g) in enumerate(zip(meshes, sims, graphs)):
+                P = m.pressure_count
+                b = DeviceBuffer(device, 8 * P)
+                p = DeviceBuffer(device, 8 * P)
+                s.copy_pressure_system(b, p)
+                if "correction" in targets:
+                    r = DeviceBuffer(device, 8 * P)
+                    d = DeviceBuffer(device, 8 * P)
+                    s.sp_r_mult(0, prevs[i], r, lhs=b)               # r <- b - A p_prev
+                    device.saxpby(P, 1.0, -1.0, p, prevs[i], d)      # d <- p - p_prev
+                    per_mesh["correction"][i].append((g, r, d))
+                if "solution" in targets:
+                    per_mesh["solution"][i].append((g, b, p))
+                else:
+                    b.close()
+                    p.close()
+            if log and (k + 1) % log_every == 0:
+                log(f"collected step {k + 1}/{steps} of {len(sims)} meshes "
+                    f"({time.perf_counter() - t0:.1f} s)")
+        device.synchronize()
+    finally:
+        for prev in prevs:
             prev.close()
-        sim.close()
-        if log:
-            log(f"collected {len(pairs)} pairs ({mi + 1}/{len(meshes)} meshes)")
-    return pairs
+        for s in sims:
+            s.close()
+    return {t: [pair for mesh_pairs in per_mesh[t] for pair in mesh_pairs] for t in targets}
+
+
+def collect_pairs(device, meshes, viscosity=1e-3, dt=5e-4, steps=40, warmup=5, every=1,
+                  precision="fp64", target="solution", log=None):
+    """``collect_pair_sets`` for one target: the list of its pairs."""
+    if target not in ("solution", "correction"):
+        raise ValueError("target must be 'solution' or 'correction'")
+    return collect_pair_sets(device, meshes, viscosity, dt, steps, warmup, every, precision,
+                             (target,), log)[target]
 
 
 def mean_loss(net, pairs):

@@ -85,6 +85,52 @@ def test_gradients_match_oracle(gpu, kind):
     pg.close()
 
 
+def test_adam_step_clips_outlier_gradients(gpu):
+    """Two Adam steps against the update restated on the host: an ordinary
+    pair (its norm starts the running average, no clipping), then the same
+    right-hand side with the target scaled by 1e-3, whose relative loss has
+    a gradient ~1e5 times larger and is scaled down to 4 times the running
+    norm before the moments see it."""
+    from navier_stokes_fem_amd import learned as L
+    from navier_stokes_fem_amd.device import DeviceBuffer
+    from oracle import gnn
+
+    og, pg, b, p, bb, pb = _setup(gpu, _mesh("channel"), seed=2)
+    params = gnn.init_params(7)
+    rng = np.random.default_rng(8)
+    for k in params:
+        params[k] = (params[k] + 0.05 * rng.standard_normal(params[k].shape)).astype(np.float32)
+    flat0 = L.WarmStartNet.flatten(params)
+    net = L.WarmStartNet(gpu)
+    net.set_params(flat0)
+    small = DeviceBuffer.from_array(gpu, 1e-3 * p)
+    lr = 1e-3
+    w = flat0.astype(np.float64)
+    m = np.zeros_like(w)
+    v = np.zeros_like(w)
+    avg = 0.0
+    for t, target in ((1, pb), (2, small)):
+        _, g = net.gradients(pg, bb, target)  # bit-identical to the step's own
+        g = g.astype(np.float64)
+        norm = float(np.linalg.norm(g))
+        avg = avg if avg > 0 else norm
+        f = min(1.0, 4.0 * avg / norm)
+        print(f"step {t}: gradient norm {norm:.3e}, running {avg:.3e}, scale {f:.3e}")
+        assert (f == 1.0) if t == 1 else (f < 1e-3)
+        avg = 0.95 * avg + 0.05 * min(norm, 4.0 * avg)
+        m = 0.9 * m + 0.1 * f * g
+        v = 0.999 * v + 0.001 * (f * g) ** 2
+        w = w - lr * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+        net.train_step(pg, bb, target, lr)
+        got = net.get_params()
+        err = np.abs(got - w).max()
+        print(f"step {t}: max |device - host| weight {err:.2e}")
+        assert err < 2e-6
+    small.close()
+    net.close()
+    pg.close()
+
+
 def test_training_reduces_loss(gpu):
     from navier_stokes_fem_amd import learned as L
     from navier_stokes_fem_amd import mesh
