@@ -36,6 +36,7 @@ constexpr int kWaves = kBlock / 64;
 // per lane); tiles hold at most kTileNnz - 3 nonzeros so that the aligned
 // span of any tile fits kTileNnz products, kGroups groups per thread.
 constexpr int kSpmvTileNnz = kTileNnz - 3;
+typedef short i16x4 __attribute__((ext_vector_type(4)));
 constexpr int kGroups = kTileNnz / (4 * kBlock);
 static_assert(kGroups * 4 * kBlock == kTileNnz, "tile must be whole groups per thread");
 typedef int i32x4 __attribute__((ext_vector_type(4)));
@@ -197,6 +198,16 @@ void build_tiles(Matrix &m, const std::vector<int32_t> &rs) {
     NSFEM_HIP(hipMemcpy(m.tiles.ptr, t.data(), t.size() * sizeof(int32_t),
                         hipMemcpyHostToDevice));
     m.host_rs = rs;
+    {
+        std::vector<int4> meta((size_t)std::max(m.n_tiles, 1));
+        for (int i = 0; i < m.n_tiles; ++i)
+            meta[i] = make_int4(t[i], t[i + 1], rs[t[i]], rs[t[i + 1]] - rs[t[i]]);
+        DeviceArray<int4> dmeta;
+        dmeta.resize(meta.size());
+        NSFEM_HIP(hipMemcpy(dmeta.ptr, meta.data(), meta.size() * sizeof(int4),
+                            hipMemcpyHostToDevice));
+        m.bcol_ok = col_delta(m, m.n_tiles, dmeta.ptr, m.bcol);
+    }
     build_wave_tiles(m, kWaveRows, kWaveNnz);
 }
 
@@ -271,11 +282,14 @@ namespace {
 
 // MODE 0: y = A x.  MODE 1: y = lhs - A x (y may alias lhs).
 // MODE 2: y = A x - lhs.
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int64_t nnz,
+// IT = int16_t reads the columns as 16-bit offsets from the tile's first row
+// (bcol, square matrices whose offsets fit): 2 bytes less per nonzero.
+template <int MODE, typename IT>
+__global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_t nnz,
                                                  const int32_t *__restrict__ tiles,
                                                  const int32_t *__restrict__ rs,
                                                  const int32_t *__restrict__ col,
+                                                 const int16_t *__restrict__ bcol,
                                                  const double *__restrict__ val,
                                                  const double *__restrict__ x,
                                                  const double *lhs, double *y) {
@@ -305,7 +319,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int64_t nnz,
             if (grp < ng) {
                 const int64_t base = ka + 4 * (int64_t)grp;
                 if (base + 4 <= nnz) {
-                    c[g] = ld_stream(reinterpret_cast<const i32x4 *>(col + base));
+                    if constexpr (std::is_same<IT, int16_t>::value) {
+                        // entries of a neighbouring tile decode relative to
+                        // the wrong row: clamped into the vector, never summed
+                        const i16x4 d = ld_stream(reinterpret_cast<const i16x4 *>(bcol + base));
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            c[g][e] = min(max(r0 + (int)d[e], 0), n_rows - 1);
+                    } else {
+                        c[g] = ld_stream(reinterpret_cast<const i32x4 *>(col + base));
+                    }
                     v[g][0] = ld_stream(reinterpret_cast<const f64x2 *>(val + base));
                     v[g][1] = ld_stream(reinterpret_cast<const f64x2 *>(val + base + 2));
                 } else {
@@ -446,22 +469,32 @@ inline int ew_grid(int n) { return std::max(1, std::min((n + kBlock - 1) / kBloc
 
 }  // namespace
 
+namespace {
+template <int MODE, typename IT>
+void spmv_launch(Device &d, Matrix &m, const double *x, const double *lhs, double *y) {
+    k_spmv<MODE, IT><<<m.n_tiles, kBlock, 0, d.stream>>>(m.n_tiles, m.rows, m.nnz, m.tiles.ptr,
+                                                          m.row_start.ptr, m.col.ptr,
+                                                          m.bcol.ptr, m.val.ptr, x, lhs, y);
+}
+
+template <int MODE>
+void spmv_launch(Device &d, Matrix &m, const double *x, const double *lhs, double *y) {
+    if (m.bcol_ok && want_idx16())
+        spmv_launch<MODE, int16_t>(d, m, x, lhs, y);
+    else
+        spmv_launch<MODE, int32_t>(d, m, x, lhs, y);
+}
+}  // namespace
+
 void launch_spmv(Device &d, Matrix &m, const double *x, const double *lhs, double *y,
                  int mode) {
     if (m.rows == 0) return;
-    const int grid = m.n_tiles;
     if (mode == 0)
-        k_spmv<0><<<grid, kBlock, 0, d.stream>>>(m.n_tiles, m.nnz, m.tiles.ptr,
-                                                 m.row_start.ptr, m.col.ptr, m.val.ptr, x, lhs,
-                                                 y);
+        spmv_launch<0>(d, m, x, lhs, y);
     else if (mode == 1)
-        k_spmv<1><<<grid, kBlock, 0, d.stream>>>(m.n_tiles, m.nnz, m.tiles.ptr,
-                                                 m.row_start.ptr, m.col.ptr, m.val.ptr, x, lhs,
-                                                 y);
+        spmv_launch<1>(d, m, x, lhs, y);
     else
-        k_spmv<2><<<grid, kBlock, 0, d.stream>>>(m.n_tiles, m.nnz, m.tiles.ptr,
-                                                 m.row_start.ptr, m.col.ptr, m.val.ptr, x, lhs,
-                                                 y);
+        spmv_launch<2>(d, m, x, lhs, y);
     NSFEM_HIP(hipGetLastError());
 }
 

@@ -129,6 +129,9 @@ struct Matrix {
     // with the wave tiles of a square matrix when every offset fits
     DeviceArray<int16_t> dcol;
     bool dcol_ok = false;
+    // the same for the product's tiles (offsets from the tile's first row)
+    DeviceArray<int16_t> bcol;
+    bool bcol_ok = false;
     bool ready = false;
     std::mutex mtx;
 };
@@ -138,6 +141,8 @@ inline void matrix_release(Matrix &m) {
     m.tiles.release(); m.wtiles.release(); m.wmeta.release(); m.block_tiles.release();
     m.dcol.release();
     m.dcol_ok = false;
+    m.bcol.release();
+    m.bcol_ok = false;
     m.block_tiles_grid = 0;
     m.ready = false;
 }
