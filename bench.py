@@ -199,6 +199,102 @@ def _line(metric, value, unit, steps, ms, config, **extra):
     return out
 
 
+# HBM bytes per launch of the product measured with rocprofv3 (FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0, scaled by the saxpy calibration) on the 2048x2048
+# cavity's matrices; keyed by matrix.
+MEASURED_SPMV_BYTES = {"pressure_stiffness": 387_601_329, "diffusion": 2_880_960_586}
+
+
+def bench_spmv(dev, args):
+    """CSR product on the n x n cavity's pressure matrix (P1, 7 nonzeros per
+    row, 16-bit column offsets) and diffusion matrix (P2, 32-bit columns):
+    mean device time per launch over 100 back-to-back launches, three rounds
+    after 3 warm-up launches, timed with HIP events on the library's stream.
+    Achieved = reference-CSR bytes per launch, (4+8) nnz + 4 (rows+1) +
+    8 cols + 8 rows, over that time.  cpu_baseline: the oracle's C row loop
+    (OpenMP, rows split statically over the threads) on the pressure matrix,
+    whose result the GPU product must equal bit for bit."""
+    import numpy as np
+
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.device import assemble
+    from oracle import oracle as O
+
+    m = mesh.cavity(args.n)
+    log(f"spmv: assembling the {args.n}x{args.n} cavity")
+    a = assemble(m, 1e-3, 1e-3)
+    rng = np.random.default_rng(7)
+    res = {}
+    check = None
+    for slot, name in ((0, "pressure_stiffness"), (2, "diffusion")):
+        rs, col, val, _ = a[name]
+        dev.upload_csr(slot, rs, col, val)
+        n_rows, nnz = len(rs) - 1, len(col)
+        n_cols = n_rows  # square, as uploaded
+        xh = rng.uniform(-1, 1, n_cols)
+        x = dev.buffer(xh)
+        y = dev.buffer(np.zeros(n_rows))
+        for _ in range(3):
+            dev.sp_r_mult(slot, x, y)
+        rounds = [dev.time_sp_r_mult(slot, x, y, 100) for _ in range(3)]
+        sec = float(np.mean(rounds))
+        byt = 12 * nnz + 4 * (n_rows + 1) + 8 * n_cols + 8 * n_rows
+        res[name] = {"rows": n_rows, "nnz": nnz, "us_per_launch": round(sec * 1e6, 2),
+                     "us_per_round": [round(r * 1e6, 2) for r in rounds],
+                     "algorithmic_bytes": byt, "achieved_GBs": round(byt / sec / 1e9, 1),
+                     "frac": round(byt / sec / 1e9 / PEAK_HBM_GBS, 4),
+                     "measured_hbm_bytes": MEASURED_SPMV_BYTES.get(name)}
+        log(f"spmv {name}: {res[name]}")
+        if slot == 0:
+            yd = y.download()
+            t = time.perf_counter()
+            reps = 0
+            while reps < 20 or time.perf_counter() - t < 10.0:
+                yo = O.spmv((rs, col, val), xh)
+                reps += 1
+            cpu_sec = (time.perf_counter() - t) / reps
+            check = {"value": round(byt / cpu_sec / 1e9, 2), "unit": "GB/s",
+                     "cores": O.get_threads(),
+                     "kind": "port", "sample": f"{reps} products of the pressure matrix by "
+                                               f"the oracle's C row loop",
+                     "seconds_per_product": round(cpu_sec, 5),
+                     "gpu_equals_oracle_bitwise": bool(np.array_equal(yd, yo))}
+        x.close()
+        y.close()
+    # a stream on the same chip for scale: nsfem_saxpy (16 B read + 8 B
+    # written per element) on 2^25 doubles, 100 launches between two
+    # synchronisations (launch gaps included)
+    n = 1 << 25
+    xs = dev.buffer(rng.uniform(-1, 1, n))
+    ys = dev.buffer(np.zeros(n))
+    for _ in range(3):
+        dev.saxpy(n, 0.5, xs, ys)
+    dev.synchronize()
+    t = time.perf_counter()
+    for _ in range(100):
+        dev.saxpy(n, 0.5, xs, ys)
+    dev.synchronize()
+    sec = (time.perf_counter() - t) / 100
+    stream = {"kernel": "k_axpy (nsfem_saxpy), 2^25 doubles", "us_per_launch": round(sec * 1e6, 1),
+              "GBs": round(24 * n / sec / 1e9, 1)}
+    log(f"spmv: stream reference {stream}")
+    xs.close()
+    ys.close()
+    p = res["pressure_stiffness"]
+    traffic = MEASURED_SPMV_BYTES.get("pressure_stiffness")
+    roofline = {"bound": "hbm", "achieved": p["achieved_GBs"], "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": p["frac"], "traffic": traffic,
+                "kernel": "k_spmv<0, short> (pressure matrix)"}
+    return _line("CSR product achieved HBM GB/s (reference CSR bytes), n x n cavity pressure matrix",
+                 p["achieved_GBs"], "GB/s", 300, p["us_per_launch"] * 1e-3,
+                 {"workload": f"lid-driven cavity {args.n}x{args.n}: pressure matrix "
+                              f"{p['rows']} rows / {p['nnz']} nonzeros, diffusion matrix "
+                              f"{res['diffusion']['rows']} rows / {res['diffusion']['nnz']} "
+                              f"nonzeros, x ~ U(-1, 1) seed 7"},
+                 dtype="f64", roofline=roofline, matrices=res, stream_reference=stream,
+                 cpu_baseline=check)
+
+
 def bench_cavity32(dev, args):
     """32x32 cavity (2048 P2-P1 elements), nu = 0.01, dt = 0.01, 100 steps of
     the plain FP64 method on the GPU; the cpu_baseline leg runs the same 100
@@ -411,11 +507,12 @@ def main():
                          "separate the precision's share of the speed-up from the guess's")
     ap.add_argument("--config", default="cavity2048",
                     choices=["cavity2048", "cavity32", "channel512", "cavity_batch64",
-                             "learned_channels"],
+                             "learned_channels", "spmv"],
                     help="cavity2048 (default): the 2048^2 cavity headline; cavity32: 32^2 "
                          "cavity, fields vs the CPU oracle; channel512: 512x128 channel past a "
                          "disk; cavity_batch64: 64 cavities in one launch; learned_channels: "
-                         "learned warm start trained and evaluated on channels")
+                         "learned warm start trained and evaluated on channels; spmv: the "
+                         "CSR product on the --n cavity's pressure and diffusion matrices")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only nsfem_saxpy on 2^26 doubles x 5 (known HBM bytes: "
                          "16 B read + 8 B written per element) to scale rocprofv3 "
@@ -436,7 +533,7 @@ def main():
     if args.config != "cavity2048":
         runner = {"cavity32": bench_cavity32, "channel512": bench_channel512,
                   "cavity_batch64": bench_cavity_batch64,
-                  "learned_channels": bench_learned_channels}[args.config]
+                  "learned_channels": bench_learned_channels, "spmv": bench_spmv}[args.config]
         dev = GPUSimulationDevice(0)
         out = runner(dev, args)
         dev.close()

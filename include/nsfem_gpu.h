@@ -115,6 +115,12 @@ int nsfem_sp_r_mult(nsfem_device *dev, int slot, const nsfem_buffer *mult,
  * gpu/matrix_math.cu:66-99). */
 int nsfem_sp_r_mult_sub(nsfem_device *dev, int slot, const nsfem_buffer *lhs,
                         const nsfem_buffer *mult, nsfem_buffer *res);
+/* Device time of `repeats` back-to-back spRMult launches (res = A*mult) on
+ * the handle's stream, from HIP events around them: *seconds = the mean per
+ * launch.  Synchronises.  No counterpart in the reference: it measures the
+ * product as the solves run it, without host launch gaps in the figure. */
+int nsfem_time_sp_r_mult(nsfem_device *dev, int slot, const nsfem_buffer *mult,
+                         nsfem_buffer *res, int repeats, double *seconds);
 /* saxpy: y += a*x (:201-222, gpu/matrix_math.cu:106-122). */
 int nsfem_saxpy(nsfem_device *dev, int n, double a, const nsfem_buffer *x,
                 nsfem_buffer *y);

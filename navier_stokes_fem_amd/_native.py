@@ -117,6 +117,7 @@ _SIGNATURES = {
     "nsfem_buffer_device_pointer": ([vp], vp),
     "nsfem_sp_r_mult": ([vp, C.c_int, vp, vp], C.c_int),
     "nsfem_sp_r_mult_sub": ([vp, C.c_int, vp, vp, vp], C.c_int),
+    "nsfem_time_sp_r_mult": ([vp, C.c_int, vp, vp, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "nsfem_saxpy": ([vp, C.c_int, dbl, vp, vp], C.c_int),
     "nsfem_saxpby": ([vp, C.c_int, dbl, dbl, vp, vp, vp], C.c_int),
     "nsfem_dot_product": ([vp, C.c_int, vp, vp, vp], C.c_int),

@@ -415,6 +415,29 @@ int nsfem_sp_r_mult(nsfem_device *h, int s, const nsfem_buffer *x, nsfem_buffer 
     });
 }
 
+int nsfem_time_sp_r_mult(nsfem_device *h, int s, const nsfem_buffer *x, nsfem_buffer *y,
+                         int repeats, double *seconds) {
+    return guard([&] {
+        Device &d = dev(h);
+        Matrix &m = slot(d, s);
+        const double *xp = dptr(x, m.cols, "spRMult mult");
+        double *yp = dptr(y, m.rows, "spRMult res");
+        NSFEM_REQUIRE((const void *)xp != (const void *)yp, NSFEM_INVALID_ARGUMENT,
+                      "spRMult: mult must not alias res");
+        NSFEM_REQUIRE(repeats > 0 && seconds, NSFEM_INVALID_ARGUMENT,
+                      "time_sp_r_mult: repeats must be positive and seconds non-null");
+        std::lock_guard<std::mutex> lk(d.mtx);
+        d.set_current();
+        NSFEM_HIP(hipEventRecord(d.ev_start, d.stream));
+        for (int i = 0; i < repeats; ++i) launch_spmv(d, m, xp, nullptr, yp, 0);
+        NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
+        NSFEM_HIP(hipEventSynchronize(d.ev_stop));
+        float ms = 0.f;
+        NSFEM_HIP(hipEventElapsedTime(&ms, d.ev_start, d.ev_stop));
+        *seconds = (double)ms * 1e-3 / repeats;
+    });
+}
+
 int nsfem_sp_r_mult_sub(nsfem_device *h, int s, const nsfem_buffer *lhs, const nsfem_buffer *x,
                         nsfem_buffer *y) {
     return guard([&] {

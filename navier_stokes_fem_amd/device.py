@@ -225,6 +225,14 @@ class GPUSimulationDevice:
     def sp_r_mult(self, slot, mult: DeviceBuffer, res: DeviceBuffer):
         N.check(N.lib().nsfem_sp_r_mult(self.handle, int(slot), mult.handle, res.handle))
 
+    def time_sp_r_mult(self, slot, mult: DeviceBuffer, res: DeviceBuffer, repeats: int = 100):
+        """Mean device seconds of one ``sp_r_mult`` launch over `repeats`
+        back-to-back launches (HIP events on the handle's stream)."""
+        sec = C.c_double(0.0)
+        N.check(N.lib().nsfem_time_sp_r_mult(self.handle, int(slot), mult.handle, res.handle,
+                                             int(repeats), C.byref(sec)))
+        return sec.value
+
     def sp_r_mult_sub(self, slot, lhs: DeviceBuffer, mult: DeviceBuffer, res: DeviceBuffer):
         N.check(N.lib().nsfem_sp_r_mult_sub(self.handle, int(slot), lhs.handle, mult.handle,
                                             res.handle))

@@ -56,6 +56,35 @@ def test_spmv_long_row(gpu, oracle):
     assert abs(y[1] - ref[1]) <= 1e-12 * np.abs(val[3:5003] * x[col[3:5003]]).sum()
 
 
+def test_spmv_offsets_at_the_ends_and_timing(gpu, oracle):
+    """A square matrix with columns r-300, r-7, r, r+7, r+300 (16-bit offsets
+    from each tile's first row): the first and last groups of a tile hold
+    entries of the neighbouring tiles, which decode below 0 in the first
+    tile and past n-1 in the last and must be clamped, never summed.  The
+    product is bit-exact; the event-timed entry point gives the same y and a
+    positive time, and rejects a non-positive repeat count."""
+    from navier_stokes_fem_amd import _native as N
+
+    rng = np.random.default_rng(12)
+    n = 1999
+    rows = [np.unique(np.clip(r + np.array([-300, -7, 0, 7, 300]), 0, n - 1)) for r in range(n)]
+    rs = np.concatenate([[0], np.cumsum([len(c) for c in rows])]).astype(np.int32)
+    col = np.concatenate(rows).astype(np.int32)
+    val = rng.standard_normal(len(col))
+    gpu.upload_csr(8, rs, col, val)
+    x = rng.standard_normal(n)
+    ref = oracle.spmv((rs, col, val), x)
+    bx, by = _buf(gpu, x), _buf(gpu, np.zeros(n))
+    gpu.sp_r_mult(8, bx, by)
+    assert np.array_equal(by.download(), ref)
+    by2 = _buf(gpu, np.zeros(n))
+    sec = gpu.time_sp_r_mult(8, bx, by2, repeats=5)
+    assert 0.0 < sec < 1.0
+    assert np.array_equal(by2.download(), ref)
+    with pytest.raises(N.NsfemError):
+        gpu.time_sp_r_mult(8, bx, by2, repeats=0)
+
+
 def test_spmv_square_long_row(gpu, oracle):
     """A square banded matrix with one row longer than a tile: the other rows
     bit-exact, the long row (block-reduction path) within rounding."""
