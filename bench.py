@@ -95,10 +95,12 @@ def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
             "achieved_GBs": round(byt / sec / 1e9, 1), "frac": round(byt / sec / 1e9 / PEAK_HBM_GBS, 4)}
 
 
-def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous"):
+def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
+             coarse_rtol=1e-6):
     sim.reset()
     sim.configure(precision=form, rtol=1e-8, pair_velocity=True)
     sim.set_warm_start(warm)
+    sim.set_coarse_space(coarse, rtol=coarse_rtol)
     for i in range(warmup):
         sim.step(1)
         log(f"{form}: warmup step {i + 1}/{warmup} done")
@@ -116,6 +118,8 @@ def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous"):
                                                                        st["max_relative_residual"])]
             agg["seconds"] += st["seconds"]
             agg["advect_seconds"] += st["advect_seconds"]
+            agg["coarse_iterations"] += st["coarse_iterations"]
+            agg["coarse_seconds"] += st["coarse_seconds"]
             agg["steps"] += 1
         log(f"{form}: step {i + 1}/{steps}: {st['seconds'] * 1e3:.1f} ms, "
             f"iterations {st['cg_iterations']}")
@@ -505,6 +509,11 @@ def main():
                          "from the previous pressure, as the reference.  With another guess than "
                          "previous, fp32_refined also runs with the previous pressure once, to "
                          "separate the precision's share of the speed-up from the guess's")
+    ap.add_argument("--coarse-ratio", type=int, default=4,
+                    help="coarse-grid correction of the fp32_refined form's pressure guesses: "
+                         "P1 interpolation from the (n/ratio)^2 cavity (0 = off)")
+    ap.add_argument("--coarse-rtol", type=float, default=1e-6,
+                    help="relative tolerance of the coarse CG solves")
     ap.add_argument("--config", default="cavity2048",
                     choices=["cavity2048", "cavity32", "channel512", "cavity_batch64",
                              "learned_channels", "spmv"],
@@ -570,15 +579,31 @@ def main():
     n_p = m.pressure_count
     nnz_p = sim.matrix_info(0)[2]
 
+    coarse = None
+    if args.coarse_ratio:
+        t = time.perf_counter()
+        coarse = mesh.cavity_prolongation(args.n, args.coarse_ratio)
+        log(f"prolongation from the {args.n // args.coarse_ratio}^2 cavity "
+            f"({time.perf_counter() - t:.1f} s)")
     results = {}
     forms = ["fp64", "fp32_refined"] if args.form == "both" else [args.form]
     for form in forms:
+        mixed_form = form == "fp32_refined"
         wall, st = run_form(sim, form, args.warmup, args.steps, n_p, nnz_p,
-                            warm=args.warm if form == "fp32_refined" else "previous")
+                            warm=args.warm if mixed_form else "previous",
+                            coarse=coarse if mixed_form else None, coarse_rtol=args.coarse_rtol)
         results[form] = (wall, st)
         log(f"{form}: {args.steps / wall:.3f} steps/s")
+    # attribution: the same mixed solver without the coarse correction, and
+    # with the reference's previous-pressure guess alone
     attribution = {}
-    if "fp32_refined" in forms and args.warm != "previous":
+    if "fp32_refined" in forms and coarse is not None and args.warm != "previous":
+        wall_a, st_a = run_form(sim, "fp32_refined", args.warmup, args.steps, n_p, nnz_p,
+                                warm=args.warm)
+        attribution[f"fp32_refined_{args.warm}_guess"] = (wall_a, st_a)
+        log(f"fp32_refined ({args.warm} guess, no coarse correction): "
+            f"{args.steps / wall_a:.3f} steps/s")
+    if "fp32_refined" in forms and (args.warm != "previous" or coarse is not None):
         wall_a, st_a = run_form(sim, "fp32_refined", args.warmup, args.steps, n_p, nnz_p,
                                 warm="previous")
         attribution["fp32_refined_previous_guess"] = (wall_a, st_a)
@@ -619,7 +644,11 @@ def main():
                                f"{m.pressure_count} pressure nodes), nu=1e-3, dt=1e-3, "
                                f"rel tol 1e-8 per solve",
                    "form": chosen, "pair_velocity": True, "parallelism": "single GPU",
-                   "pressure_initial_guess": args.warm if chosen == "fp32_refined" else "previous"},
+                   "pressure_initial_guess": (
+                       (args.warm + (f" + coarse-grid correction (P1 from the "
+                                     f"{args.n // args.coarse_ratio}^2 cavity, coarse CG to "
+                                     f"{args.coarse_rtol:g})" if coarse is not None else ""))
+                       if chosen == "fp32_refined" else "previous")},
         "roofline": roofline,
     }
     for form, (w, s) in list(results.items()) + list(attribution.items()):
@@ -627,6 +656,8 @@ def main():
                      "cg_iterations": s["cg_iterations"], "max_relative_residual": s["max_relative_residual"],
                      "cg_seconds": [round(x, 4) for x in s["cg_seconds"]],
                      "cg_refinements": s["cg_refinements"],
+                     "coarse_iterations": s["coarse_iterations"],
+                     "coarse_seconds": round(s["coarse_seconds"], 4),
                      "advect_seconds": round(s["advect_seconds"], 4)}
     # SpMV micro-benchmark on the assembled pressure and diffusion matrices
     out["spmv"] = {}

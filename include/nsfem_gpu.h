@@ -284,6 +284,8 @@ typedef struct {
     double advect_seconds;
     double cg_seconds[5];
     int64_t cg_refinements[5];  /* FP64 reliable updates (refined precision) */
+    int64_t coarse_iterations;  /* CG iterations of the coarse-grid corrections */
+    double coarse_seconds;      /* their device time */
 } nsfem_step_stats;
 
 int nsfem_simulation_create(nsfem_device *dev, int node_count,
@@ -400,6 +402,19 @@ int nsfem_gnn_train_step(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
                                             the span of past pressures (Fischer);
                                             basis depth NSFEM_PROJECTION_DEPTH */
 int nsfem_simulation_set_warm_start(nsfem_simulation *sim, int mode, nsfem_gnn *gnn);
+/* Coarse-grid correction of the pressure solve's initial guess, applied
+ * after the warm-start mode's guess x0: with the prolongation P (one row per
+ * pressure node, coarse_count columns, CSR with ascending columns) and
+ * A_c = P^T A P (formed here, on the host), A_c e = P^T (b - A x0) is solved
+ * by CG from zero to relative residual coarse_rtol, and x0 += P e (pressure
+ * Dirichlet values imposed again; a coarse solve that does not converge
+ * leaves x0 unchanged).  The pressure CG then runs from it to the same
+ * tolerance as from any guess.  coarse_count = 0 removes the coarse space.
+ * No counterpart in the reference, whose pressure CG starts from the
+ * previous pressure (include/assembly.h:608-615). */
+int nsfem_simulation_set_coarse_space(nsfem_simulation *sim, int coarse_count, int64_t nnz,
+                                      const int32_t *row_start, const int32_t *col,
+                                      const double *val, double coarse_rtol);
 /* Copies the last step's pressure right-hand side and pressure into device
  * buffers (either may be NULL): training pairs for the network. */
 int nsfem_simulation_copy_pressure_system(nsfem_simulation *sim, nsfem_buffer *rhs,

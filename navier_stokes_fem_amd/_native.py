@@ -66,14 +66,17 @@ class StepStats(C.Structure):
     _fields_ = [("steps", C.c_int32), ("cg_iterations", C.c_int64 * 5),
                 ("max_relative_residual", C.c_double * 5), ("seconds", C.c_double),
                 ("advect_seconds", C.c_double), ("cg_seconds", C.c_double * 5),
-                ("cg_refinements", C.c_int64 * 5)]
+                ("cg_refinements", C.c_int64 * 5), ("coarse_iterations", C.c_int64),
+                ("coarse_seconds", C.c_double)]
 
     def as_dict(self):
         return {"steps": self.steps, "cg_iterations": list(self.cg_iterations),
                 "max_relative_residual": list(self.max_relative_residual),
                 "seconds": self.seconds, "advect_seconds": self.advect_seconds,
                 "cg_seconds": list(self.cg_seconds),
-                "cg_refinements": list(self.cg_refinements)}
+                "cg_refinements": list(self.cg_refinements),
+                "coarse_iterations": self.coarse_iterations,
+                "coarse_seconds": self.coarse_seconds}
 
 
 class BatchTuning(C.Structure):
@@ -155,6 +158,8 @@ _SIGNATURES = {
                                       C.POINTER(i64)], C.c_int),
     "nsfem_simulation_sp_r_mult": ([vp, C.c_int, vp, vp, vp], C.c_int),
     "nsfem_simulation_set_warm_start": ([vp, C.c_int, vp], C.c_int),
+    "nsfem_simulation_set_coarse_space": ([vp, C.c_int, C.c_int64, vp, vp, vp, C.c_double],
+                                          C.c_int),
     "nsfem_simulation_copy_pressure_system": ([vp, vp, vp], C.c_int),
     "nsfem_gnn_create": ([vp, C.c_uint64, PP], C.c_int),
     "nsfem_gnn_destroy": ([vp], C.c_int),

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -58,6 +59,16 @@ struct Simulation {
     // two-column velocity solves (their P2 matrices have ~4x the rows)
     int batch_block_scale = 1;
     int batch_block_scale_v = 1;
+    // coarse-grid correction of the pressure guess: prolongation P (P x nc),
+    // its transpose and A_c = P^T A P; residual, coarse right-hand side and
+    // solution vectors; the last correction's CG iterations and seconds
+    bool has_coarse = false;
+    int nc = 0;
+    double coarse_rtol = 1e-6;
+    Matrix cP, cPT, cA;
+    DeviceArray<double> cy, cr, ce;
+    int64_t coarse_iters = 0;
+    double coarse_seconds = 0.0;
 };
 
 namespace {
@@ -304,7 +315,190 @@ void projection_update(Simulation &s) {
 // with the network's prediction from the right-hand side (or keep whichever
 // of the two has the smaller residual), or add the network's correction,
 // predicted from the previous pressure's residual, to the previous pressure.
+// ------------------------------------------------------------ coarse space
+namespace {
+
+// CSR transpose of an n x nc matrix (rows of the result list the original
+// rows in ascending order).
+void csr_transpose(int n, int nc, const int32_t *rs, const int32_t *col, const double *val,
+                   std::vector<int32_t> &trs, std::vector<int32_t> &tcol,
+                   std::vector<double> &tval) {
+    trs.assign((size_t)nc + 1, 0);
+    for (int64_t k = 0; k < rs[n]; ++k) ++trs[(size_t)col[k] + 1];
+    for (int c = 0; c < nc; ++c) trs[c + 1] += trs[c];
+    tcol.resize((size_t)rs[n]);
+    tval.resize((size_t)rs[n]);
+    std::vector<int32_t> fill(trs.begin(), trs.end() - 1);
+    for (int r = 0; r < n; ++r)
+        for (int32_t k = rs[r]; k < rs[r + 1]; ++k) {
+            const int32_t at = fill[col[k]]++;
+            tcol[at] = r;
+            tval[at] = val[k];
+        }
+}
+
+// A_c = P^T A P row by row: coarse row c sums P^T[c, f] A[f, g] P[g, c'] over
+// the rows f of P^T's row c, the entries g of A's row f and the entries c' of
+// P's row g, in that (CSR) order, so the result does not depend on the
+// thread count.  Rows are split over `threads` workers.
+void galerkin_product(int n, const std::vector<int32_t> &ars, const std::vector<int32_t> &acol,
+                      const std::vector<double> &aval, int nc, const int32_t *prs,
+                      const int32_t *pcol, const double *pval, const std::vector<int32_t> &trs,
+                      const std::vector<int32_t> &tcol, const std::vector<double> &tval,
+                      int threads, std::vector<int32_t> &crs, std::vector<int32_t> &ccol,
+                      std::vector<double> &cval) {
+    (void)n;
+    const int T = std::max(1, std::min(threads, nc));
+    std::vector<std::vector<int32_t>> pc(T), pr(T);
+    std::vector<std::vector<double>> pv(T);
+    auto work = [&](int t) {
+        const int c0 = (int)((int64_t)nc * t / T), c1 = (int)((int64_t)nc * (t + 1) / T);
+        std::vector<double> acc((size_t)nc, 0.0);
+        std::vector<char> seen((size_t)nc, 0);
+        std::vector<int32_t> touched;
+        std::vector<int32_t> &rcnt = pr[t];
+        rcnt.reserve((size_t)(c1 - c0));
+        for (int c = c0; c < c1; ++c) {
+            touched.clear();
+            for (int32_t a = trs[c]; a < trs[c + 1]; ++a) {
+                const int f = tcol[a];
+                const double wf = tval[a];
+                for (int32_t b = ars[f]; b < ars[f + 1]; ++b) {
+                    const int g = acol[b];
+                    const double wg = wf * aval[b];
+                    for (int32_t e = prs[g]; e < prs[g + 1]; ++e) {
+                        const int cc = pcol[e];
+                        if (!seen[cc]) {
+                            seen[cc] = 1;
+                            touched.push_back(cc);
+                        }
+                        acc[cc] += wg * pval[e];
+                    }
+                }
+            }
+            std::sort(touched.begin(), touched.end());
+            for (int cc : touched) {
+                pc[t].push_back(cc);
+                pv[t].push_back(acc[cc]);
+                acc[cc] = 0.0;
+                seen[cc] = 0;
+            }
+            rcnt.push_back((int32_t)touched.size());
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    crs.assign((size_t)nc + 1, 0);
+    int c = 0;
+    for (int t = 0; t < T; ++t)
+        for (int32_t cnt : pr[t]) {
+            crs[c + 1] = crs[c] + cnt;
+            ++c;
+        }
+    NSFEM_REQUIRE((int64_t)crs[nc] < (int64_t)INT32_MAX, NSFEM_INVALID_ARGUMENT,
+                  "coarse space: the coarse matrix is too large");
+    ccol.clear();
+    cval.clear();
+    ccol.reserve((size_t)crs[nc]);
+    cval.reserve((size_t)crs[nc]);
+    for (int t = 0; t < T; ++t) {
+        ccol.insert(ccol.end(), pc[t].begin(), pc[t].end());
+        cval.insert(cval.end(), pv[t].begin(), pv[t].end());
+    }
+}
+
+// x0 += P e, A_c e = P^T (b - A x0) solved from zero to coarse_rtol; the
+// pressure Dirichlet values are imposed again.  A coarse solve that does
+// not converge leaves the guess as it was.
+void coarse_correct(Simulation &s) {
+    Device &d = *s.dev;
+    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+    launch_spmv(d, A, s.p.ptr, s.prhs.ptr, s.cy.ptr, 1);
+    launch_spmv(d, s.cPT, s.cy.ptr, nullptr, s.cr.ptr, 0);
+    launch_fill(d, s.nc, 0.0, s.ce.ptr);
+    CgRequest rq{};
+    rq.m = &s.cA;
+    rq.b = s.cr.ptr;
+    rq.x = s.ce.ptr;
+    rq.ncol = 1;
+    rq.max_iter = -1;
+    rq.rtol = s.coarse_rtol;
+    rq.atol = 0.0;
+    rq.precision = s.opt.precision;
+    nsfem_cg_stats st{};
+    const int status = run_cg(d, rq, &st);
+    s.coarse_iters = st.iterations;
+    s.coarse_seconds = st.seconds;
+    if (status != NSFEM_OK) return;
+    launch_spmv(d, s.cP, s.ce.ptr, nullptr, s.cy.ptr, 0);
+    launch_add(d, s.P, s.cy.ptr, s.p.ptr);
+    launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.p.ptr);
+}
+
+}  // namespace
+
+void simulation_set_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
+                                 const int32_t *col, const double *val, double rtol) {
+    Device &d = *s.dev;
+    d.set_current();
+    if (nc <= 0) {
+        matrix_release(s.cP);
+        matrix_release(s.cPT);
+        matrix_release(s.cA);
+        s.cy.release();
+        s.cr.release();
+        s.ce.release();
+        s.has_coarse = false;
+        s.nc = 0;
+        return;
+    }
+    NSFEM_REQUIRE(nc <= s.P && nnz > 0 && rs && col && val, NSFEM_INVALID_ARGUMENT,
+                  "coarse space: need 0 < coarse count <= pressure count and a non-empty P");
+    NSFEM_REQUIRE(rtol > 0.0 && rtol < 1.0, NSFEM_INVALID_ARGUMENT,
+                  "coarse space: the coarse tolerance must lie in (0, 1)");
+    s.has_coarse = false;
+    upload_csr(d, s.cP, s.P, nc, nnz, rs, col, val);  // validates P
+    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+    std::vector<int32_t> acol((size_t)A.nnz);
+    std::vector<double> aval((size_t)A.nnz);
+    if (A.nnz) {
+        NSFEM_HIP(hipMemcpy(acol.data(), A.col.ptr, acol.size() * sizeof(int32_t),
+                            hipMemcpyDeviceToHost));
+        NSFEM_HIP(hipMemcpy(aval.data(), A.val.ptr, aval.size() * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+    std::vector<int32_t> trs, tcol, crs, ccol;
+    std::vector<double> tval, cval;
+    csr_transpose(s.P, nc, rs, col, val, trs, tcol, tval);
+    const int threads = s.opt.threads > 0 ? s.opt.threads
+                                          : (int)std::max(1u, std::thread::hardware_concurrency());
+    galerkin_product(s.P, A.host_rs, acol, aval, nc, rs, col, val, trs, tcol, tval, threads, crs,
+                     ccol, cval);
+    upload_csr(d, s.cPT, nc, s.P, nnz, trs.data(), tcol.data(), tval.data());
+    upload_csr(d, s.cA, nc, nc, (int64_t)ccol.size(), crs.data(), ccol.data(), cval.data());
+    s.cy.resize(s.P);
+    s.cr.resize(nc);
+    s.ce.resize(nc);
+    s.nc = nc;
+    s.coarse_rtol = rtol;
+    s.coarse_iters = 0;
+    s.coarse_seconds = 0.0;
+    s.has_coarse = true;
+}
+
+void pressure_guess_base(Simulation &s);
+
+// the mode's guess, then the coarse-grid correction when a coarse space is set
 void pressure_guess(Simulation &s) {
+    pressure_guess_base(s);
+    s.coarse_iters = 0;
+    s.coarse_seconds = 0.0;
+    if (s.has_coarse) coarse_correct(s);
+}
+
+void pressure_guess_base(Simulation &s) {
     Device &d = *s.dev;
     switch (s.warm_mode) {
     case NSFEM_WARM_PREVIOUS:
@@ -465,6 +659,8 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         launch_spmv(d, BD, s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
         launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.prhs.ptr);
         pressure_guess(s);
+        const int64_t coarse_iters = s.coarse_iters;
+        const double coarse_seconds = s.coarse_seconds;
         solve(s, NSFEM_PRESSURE_STIFFNESS, s.prhs.ptr, s.p.ptr, 1, o[2]);
         pressure_after_solve(s);
         // 4. velocity correction
@@ -502,6 +698,8 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
                                  o[2].seconds, o[3].seconds,
                                  s.opt.pair_velocity ? 0.0 : o[4].seconds};
             for (int k = 0; k < 5; ++k) stats->cg_seconds[k] += t[k];
+            stats->coarse_iterations += coarse_iters;
+            stats->coarse_seconds += coarse_seconds;
         }
     }
     (void)P;
@@ -571,11 +769,15 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
             rq[i] = request(*S[i], NSFEM_DIFFUSION, S[i]->rhs.ptr, S[i]->uv.ptr, 2);
         std::vector<nsfem_cg_stats> sd(2 * I);
         check(run_cg_batch(d, rq, sd.data(), scale_v, &sec[0]), "diffusion");
-        for (Simulation *sp : S) {
-            Simulation &s = *sp;
+        for (size_t i = 0; i < I; ++i) {
+            Simulation &s = *S[i];
             launch_spmv(d, s.mats[NSFEM_VELOCITY_DIVERGENCE], s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
             launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.prhs.ptr);
             pressure_guess(s);
+            if (stats) {
+                stats[i].coarse_iterations += s.coarse_iters;
+                stats[i].coarse_seconds += s.coarse_seconds;
+            }
         }
         for (size_t i = 0; i < I; ++i)
             rq[i] = request(*S[i], NSFEM_PRESSURE_STIFFNESS, S[i]->prhs.ptr, S[i]->p.ptr, 1);
@@ -888,6 +1090,18 @@ int nsfem_simulation_set_warm_start(nsfem_simulation *s, int mode, nsfem_gnn *g)
         NSFEM_REQUIRE(!g || g->impl, NSFEM_INVALID_ARGUMENT, "network of a destroyed device");
         std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
         simulation_set_warm_start(*s->impl, mode, g ? g->impl : nullptr);
+    });
+}
+
+int nsfem_simulation_set_coarse_space(nsfem_simulation *s, int coarse_count, int64_t nnz,
+                                      const int32_t *row_start, const int32_t *col,
+                                      const double *val, double coarse_rtol) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_set_coarse_space(*s->impl, coarse_count, nnz, row_start, col, val,
+                                    coarse_rtol);
     });
 }
 

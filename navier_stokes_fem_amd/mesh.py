@@ -140,6 +140,43 @@ def cavity(n: int, lid_speed: float = 1.0) -> Mesh:
     return m
 
 
+def cavity_prolongation(n: int, ratio: int):
+    """P1 interpolation from the pressure nodes of the (n/ratio)^2 cavity to
+    those of the n^2 cavity (both numbered row-major, squares split along the
+    same diagonal, so every coarse triangle is a union of fine ones): a fine
+    node in the coarse square (I, J) at local position (a, b) takes the
+    barycentric weights of its coarse triangle, (1-a, a-b, b) on the corners
+    (0,0), (1,0), (1,1) when a >= b, else (1-b, a, b-a) on (0,0), (1,1),
+    (0,1).  Linear functions are reproduced exactly.  Returns (row_start,
+    col, val, coarse_count) for ``Simulation.set_coarse_space``."""
+    if ratio < 2 or n % ratio:
+        raise ValueError("ratio must be >= 2 and divide n")
+    nc = n // ratio
+    f = np.arange((n + 1) ** 2)
+    j, i = np.divmod(f, n + 1)
+    I = np.minimum(i // ratio, nc - 1)
+    J = np.minimum(j // ratio, nc - 1)
+    a = (i - I * ratio) / ratio
+    b = (j - J * ratio) / ratio
+    c00 = J * (nc + 1) + I
+    c10, c01 = c00 + 1, c00 + nc + 1
+    c11 = c01 + 1
+    lower = a >= b
+    cols = np.stack([c00, np.where(lower, c10, c11), np.where(lower, c11, c01)], axis=1)
+    w = np.stack([np.where(lower, 1 - a, 1 - b), np.where(lower, a - b, a),
+                  np.where(lower, b, b - a)], axis=1)
+    rows = np.repeat(f, 3)
+    cols = cols.reshape(-1)
+    w = w.reshape(-1)
+    keep = w > 0
+    rows, cols, w = rows[keep], cols[keep], w[keep]
+    order = np.lexsort((cols, rows))
+    rows, cols, w = rows[order], cols[order], w[order]
+    rs = np.zeros(len(f) + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=len(f)), out=rs[1:])
+    return rs.astype(np.int32), cols.astype(np.int32), w.astype(np.float64), (nc + 1) ** 2
+
+
 def cavity_batch(count: int = 64, seed: int = 3, n_min: int = 16, n_max: int = 256):
     """A batch of independent lid-driven cavities of uneven size: side
     ``n ~ U{n_min..n_max}``, lid speed ``U ~ U(0.5, 2)``, viscosity

@@ -115,6 +115,26 @@ class Simulation:
 
         return export_simulation(self, folder, time_step)
 
+    def set_coarse_space(self, prolongation=None, rtol: float = 1e-6):
+        """Coarse-grid correction of every pressure solve's initial guess:
+        ``prolongation`` = (row_start, col, val, coarse_count), one row per
+        pressure node (e.g. ``mesh.cavity_prolongation``); the library forms
+        P^T A P and solves the restricted residual equation to ``rtol`` before
+        each pressure CG.  None removes it."""
+        if prolongation is None:
+            N.check(N.lib().nsfem_simulation_set_coarse_space(self.handle, 0, 0, None, None, None,
+                                                              float(rtol)))
+            return
+        rs, col, val, nc = prolongation
+        rs = np.ascontiguousarray(rs, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        if len(rs) != self.mesh.pressure_count + 1:
+            raise ValueError("the prolongation needs one row per pressure node")
+        N.check(N.lib().nsfem_simulation_set_coarse_space(self.handle, int(nc), len(col),
+                                                          N.ptr(rs), N.ptr(col), N.ptr(val),
+                                                          float(rtol)))
+
     def copy_pressure_system(self, rhs=None, p=None):
         """Copy the last step's pressure right-hand side and solution into
         device buffers (``DeviceBuffer`` of pressure-count doubles)."""

@@ -1,0 +1,68 @@
+"""CPU tests of the coarse-grid correction's pieces: the cavity P1
+prolongation (exact on linear functions, a partition of unity with
+non-negative weights, one row per pressure node) and, with the oracle, the
+premise of the correction: the previous pressure plus P (P^T A P)^-1 P^T r
+starts the plain CG closer, in iterations, than the previous pressure."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+from navier_stokes_fem_amd import mesh
+
+
+@pytest.mark.parametrize("n,ratio", [(4, 2), (12, 4), (24, 8), (96, 2)])
+def test_cavity_prolongation_is_p1_interpolation(n, ratio):
+    rs, col, val, nc = mesh.cavity_prolongation(n, ratio)
+    m = n // ratio + 1
+    assert nc == m * m and len(rs) == (n + 1) ** 2 + 1
+    P = sp.csr_matrix((val, col, rs), shape=((n + 1) ** 2, nc))
+    assert np.all(val > 0) and np.all(val <= 1)
+    np.testing.assert_array_equal(P.sum(axis=1).A1, 1.0)
+    for r in range(P.shape[0]):  # ascending columns, at most 3 per row
+        c = col[rs[r]:rs[r + 1]]
+        assert len(c) <= 3 and np.all(np.diff(c) > 0)
+    jc, ic = np.divmod(np.arange(nc), m)
+    jf, i_f = np.divmod(np.arange((n + 1) ** 2), n + 1)
+    for a, b, c in ((0.3, 1.7, -2.1), (1.0, 0.0, 0.0), (0.0, -0.5, 4.0)):
+        coarse = a + b * ic / (m - 1) + c * jc / (m - 1)
+        fine = a + b * i_f / n + c * jf / n
+        np.testing.assert_allclose(P @ coarse, fine, rtol=0, atol=1e-14)
+    # coarse nodes coincide with every ratio-th fine node
+    corner = (jc * ratio) * (n + 1) + ic * ratio
+    np.testing.assert_array_equal(P[corner].toarray(), np.eye(nc))
+
+
+def test_cavity_prolongation_rejects_bad_ratio():
+    with pytest.raises(ValueError):
+        mesh.cavity_prolongation(10, 4)
+    with pytest.raises(ValueError):
+        mesh.cavity_prolongation(8, 1)
+
+
+def test_coarse_correction_premise_with_the_oracle():
+    from oracle import oracle as O
+
+    n = 128
+    m = mesh.cavity(n)
+    asm = O.Assembly(m, 1e-3, 1e-3)
+    tree = O.kd_build(m.xy(), m.elems())
+    rs, col, val = asm.csr["pressure_stiffness"]
+    A = sp.csr_matrix((val, col, rs), shape=(len(rs) - 1,) * 2)
+    prs, pcol, pval, nc = mesh.cavity_prolongation(n, 2)
+    P = sp.csr_matrix((pval, pcol, prs), shape=(A.shape[0], nc))
+    solve = spl.factorized((P.T @ A @ P).tocsc())
+    u, v, p = O.initial_fields(m)
+    ps = [p]
+    for _ in range(8):
+        u, v, p, _ = O.step(asm, tree, m, 1e-3, u, v, p)
+        ps.append(p)
+    plain = corrected = 0
+    for k in range(5, 9):
+        b = A @ ps[k]
+        x0 = ps[k - 1]
+        xc = x0 + P @ solve(P.T @ (b - A @ x0))
+        plain += O.cg((rs, col, val), b, x0, rtol=1e-8)[2].iterations
+        corrected += O.cg((rs, col, val), b, xc, rtol=1e-8)[2].iterations
+    print(f"oracle CG iterations: previous {plain}, corrected {corrected}")
+    assert corrected < 0.75 * plain
