@@ -272,7 +272,9 @@ typedef struct {
     double rtol;              /* per-solve relative tolerance (1e-8) */
     int32_t max_iterations;   /* per solve, -1 = rows */
     int32_t precision;        /* NSFEM_PRECISION_* */
-    int32_t pair_velocity;    /* solve u and v together (2-column CG) */
+    int32_t pair_velocity;    /* solve u and v together (2-column CG) while the
+                                 last step's column iteration counts stay within
+                                 2x (+8) of each other, else one after the other */
     int32_t threads;          /* host threads for assembly / KD build, 0 = all */
 } nsfem_sim_options;
 

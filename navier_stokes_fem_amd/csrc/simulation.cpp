@@ -12,7 +12,8 @@
 //   5. u += du, v += dv; u[b] = g_u, v[b] = g_v                  (:687-708)
 // The reference runs steps 2 and 4 as two separate solves; with
 // pair_velocity they run as one 2-column CG (same iterates per column,
-// the matrix is read once for both).
+// the matrix is read once for both) while the two columns need similar
+// iteration counts (pair_columns).
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -29,6 +30,34 @@ namespace nsfem {
 
 struct Simulation;
 void simulation_reset(Simulation &s);
+
+// A-orthonormal basis X of past solutions of one matrix, with A X, for the
+// projection guess (a pressure solve, and the coarse solves of the coarse
+// correction); w, dx, guess: vectors of the matrix's size; s: dot results.
+struct ProjBasis {
+    int n = 0, dim = 0, max = 0;
+    DeviceArray<double> x, ax, w, dx, guess, s;
+    void reserve(int rows, int depth) {
+        n = rows;
+        max = depth;
+        dim = 0;
+        x.resize((size_t)depth * rows);
+        ax.resize((size_t)depth * rows);
+        w.resize(rows);
+        dx.resize(rows);
+        guess.resize(rows);
+        s.resize((size_t)depth + 1);
+    }
+    void release() {
+        x.release();
+        ax.release();
+        w.release();
+        dx.release();
+        guess.release();
+        s.release();
+        n = dim = max = 0;
+    }
+};
 
 struct Simulation {
     Device *dev = nullptr;
@@ -51,10 +80,8 @@ struct Simulation {
     DeviceArray<double> pguess, ptmp, pscal;
     DeviceArray<double> p_older, p_keep;  // p_(n-1) and p_n for the extrapolation
     bool have_older = false;
-    // projection warm start: A-orthonormal basis X of past pressure
-    // corrections and A X, proj_max vectors each (proj_dim in use)
-    DeviceArray<double> proj_x, proj_ax, proj_w, proj_s;
-    int proj_dim = 0, proj_max = 0;
+    // projection warm start of the pressure solve
+    ProjBasis proj;
     // blocks per instance in batched solves (autotuned): pressure, and the
     // two-column velocity solves (their P2 matrices have ~4x the rows)
     int batch_block_scale = 1;
@@ -67,8 +94,12 @@ struct Simulation {
     double coarse_rtol = 1e-6;
     Matrix cP, cPT, cA;
     DeviceArray<double> cy, cr, ce;
+    ProjBasis cproj;  // projection guess of the coarse solves
     int64_t coarse_iters = 0;
     double coarse_seconds = 0.0;
+    // last step's iteration counts of the u and v columns of the diffusion
+    // (0) and mass (1) solves: they decide whether the next step pairs them
+    int64_t vel_iters[2][2] = {{0, 0}, {0, 0}};
 };
 
 namespace {
@@ -178,7 +209,9 @@ void simulation_reset(Simulation &s) {
     Device &d = *s.dev;
     d.set_current();
     s.have_older = false;
-    s.proj_dim = 0;
+    s.proj.dim = 0;
+    s.cproj.dim = 0;
+    for (auto &row : s.vel_iters) row[0] = row[1] = 0;
     launch_fill(d, 2 * s.V, 0.0, s.uv.ptr);
     launch_fill(d, s.P, 0.0, s.p.ptr);
     launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, s.uv.ptr);
@@ -236,22 +269,19 @@ void solve(Simulation &s, int slot, const double *b, double *x, int ncol, SolveO
 }
 
 // out[i] = X_i . y for the first l basis vectors (one host synchronisation)
-void proj_dots(Simulation &s, int l, const double *y, double *out) {
-    Device &d = *s.dev;
-    launch_fill(d, l, 0.0, s.proj_s.ptr);
+void proj_dots(Device &d, ProjBasis &B, int l, const double *y, double *out) {
+    launch_fill(d, l, 0.0, B.s.ptr);
     for (int i = 0; i < l; ++i)
-        launch_dot_accumulate(d, s.P, s.proj_x.ptr + (size_t)i * s.P, y, s.proj_s.ptr + i);
-    NSFEM_HIP(hipMemcpyAsync(out, s.proj_s.ptr, sizeof(double) * l, hipMemcpyDeviceToHost,
-                             d.stream));
+        launch_dot_accumulate(d, B.n, B.x.ptr + (size_t)i * B.n, y, B.s.ptr + i);
+    NSFEM_HIP(hipMemcpyAsync(out, B.s.ptr, sizeof(double) * l, hipMemcpyDeviceToHost, d.stream));
     NSFEM_HIP(hipStreamSynchronize(d.stream));
 }
 
-double dev_dot(Simulation &s, const double *a, const double *b) {
-    Device &d = *s.dev;
+double dev_dot(Device &d, ProjBasis &B, const double *a, const double *b) {
     double h = 0.0;
-    launch_fill(d, 1, 0.0, s.proj_s.ptr);
-    launch_dot_accumulate(d, s.P, a, b, s.proj_s.ptr);
-    NSFEM_HIP(hipMemcpyAsync(&h, s.proj_s.ptr, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    launch_fill(d, 1, 0.0, B.s.ptr);
+    launch_dot_accumulate(d, B.n, a, b, B.s.ptr);
+    NSFEM_HIP(hipMemcpyAsync(&h, B.s.ptr, sizeof(double), hipMemcpyDeviceToHost, d.stream));
     NSFEM_HIP(hipStreamSynchronize(d.stream));
     return h;
 }
@@ -259,55 +289,52 @@ double dev_dot(Simulation &s, const double *a, const double *b) {
 // Projection onto the span of past solutions (P. F. Fischer, "Projection
 // techniques for iterative solution of Ax = b with successive right-hand
 // sides", Comput. Methods Appl. Mech. Engrg. 163 (1998) 193-204).  With an
-// A-orthonormal basis X of the past corrections, x0 = sum_i (X_i . b) X_i is
+// A-orthonormal basis X of the past corrections, x = sum_i (X_i . b) X_i is
 // the best approximation of A^-1 b in the A-norm within their span, which
-// holds the previous pressure: x0 is no worse than it in the A-norm.  The
-// guess is kept in pguess for the basis update.
-void projection_guess(Simulation &s) {
-    Device &d = *s.dev;
-    const int l = s.proj_dim;
+// holds the previous solution: x is no worse than it in the A-norm.  An
+// empty basis leaves x as it is.  The guess is kept for the basis update.
+void projection_guess(Device &d, ProjBasis &B, const double *b, double *x) {
+    const int l = B.dim;
     if (l > 0) {
         std::vector<double> a(l);
-        proj_dots(s, l, s.prhs.ptr, a.data());
-        launch_fill(d, s.P, 0.0, s.p.ptr);
-        for (int i = 0; i < l; ++i) launch_axpy(d, s.P, a[i], s.proj_x.ptr + (size_t)i * s.P, s.p.ptr);
+        proj_dots(d, B, l, b, a.data());
+        launch_fill(d, B.n, 0.0, x);
+        for (int i = 0; i < l; ++i) launch_axpy(d, B.n, a[i], B.x.ptr + (size_t)i * B.n, x);
     }
-    launch_copy(d, s.P, s.p.ptr, s.pguess.ptr);
+    launch_copy(d, B.n, x, B.guess.ptr);
 }
 
-// After the solve: the correction dx = p - x0 (the whole solution after a
-// restart), A-orthogonalised against X by Gram-Schmidt applied twice, is
+// After the solve x: the correction dx = x - guess (the whole solution after
+// a restart), A-orthogonalised against X by Gram-Schmidt applied twice, is
 // normalised in the A-norm and appended; a full basis restarts from the
 // latest solution.  A correction that the basis already spans (A-norm below
 // 1e-12 of its own before orthogonalisation) is dropped.
-void projection_update(Simulation &s) {
-    Device &d = *s.dev;
-    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
-    const int P = s.P;
-    if (s.proj_dim == s.proj_max) s.proj_dim = 0;
-    const int l = s.proj_dim;
-    double *dx = s.ptmp.ptr, *w = s.proj_w.ptr;
+void projection_update(Device &d, ProjBasis &B, Matrix &A, const double *x) {
+    const int n = B.n;
+    if (B.dim == B.max) B.dim = 0;
+    const int l = B.dim;
+    double *dx = B.dx.ptr, *w = B.w.ptr;
     if (l == 0)
-        launch_copy(d, P, s.p.ptr, dx);
+        launch_copy(d, n, x, dx);
     else
-        launch_axpby(d, P, 1.0, -1.0, s.p.ptr, s.pguess.ptr, dx);
+        launch_axpby(d, n, 1.0, -1.0, x, B.guess.ptr, dx);
     launch_spmv(d, A, dx, nullptr, w, 0);
-    const double n0 = dev_dot(s, dx, w);
+    const double n0 = dev_dot(d, B, dx, w);
     if (!(n0 > 0.0)) return;
     std::vector<double> c(std::max(l, 1));
     for (int pass = 0; pass < 2 && l > 0; ++pass) {
-        proj_dots(s, l, w, c.data());  // X_i . A dx = (A X_i) . dx
+        proj_dots(d, B, l, w, c.data());  // X_i . A dx = (A X_i) . dx
         for (int i = 0; i < l; ++i) {
-            launch_axpy(d, P, -c[i], s.proj_x.ptr + (size_t)i * P, dx);
-            launch_axpy(d, P, -c[i], s.proj_ax.ptr + (size_t)i * P, w);
+            launch_axpy(d, n, -c[i], B.x.ptr + (size_t)i * n, dx);
+            launch_axpy(d, n, -c[i], B.ax.ptr + (size_t)i * n, w);
         }
     }
-    const double n1 = dev_dot(s, dx, w);
+    const double n1 = dev_dot(d, B, dx, w);
     if (!(n1 > 1e-12 * n0)) return;
     const double inv = 1.0 / std::sqrt(n1);
-    launch_axpby(d, P, inv, 0.0, dx, dx, s.proj_x.ptr + (size_t)l * P);
-    launch_axpby(d, P, inv, 0.0, w, w, s.proj_ax.ptr + (size_t)l * P);
-    s.proj_dim = l + 1;
+    launch_axpby(d, n, inv, 0.0, dx, dx, B.x.ptr + (size_t)l * n);
+    launch_axpby(d, n, inv, 0.0, w, w, B.ax.ptr + (size_t)l * n);
+    B.dim = l + 1;
 }
 
 // Initial guess of the pressure solve.  The reference keeps the previous
@@ -418,6 +445,7 @@ void coarse_correct(Simulation &s) {
     launch_spmv(d, A, s.p.ptr, s.prhs.ptr, s.cy.ptr, 1);
     launch_spmv(d, s.cPT, s.cy.ptr, nullptr, s.cr.ptr, 0);
     launch_fill(d, s.nc, 0.0, s.ce.ptr);
+    projection_guess(d, s.cproj, s.cr.ptr, s.ce.ptr);
     CgRequest rq{};
     rq.m = &s.cA;
     rq.b = s.cr.ptr;
@@ -432,6 +460,7 @@ void coarse_correct(Simulation &s) {
     s.coarse_iters = st.iterations;
     s.coarse_seconds = st.seconds;
     if (status != NSFEM_OK) return;
+    projection_update(d, s.cproj, s.cA, s.ce.ptr);
     launch_spmv(d, s.cP, s.ce.ptr, nullptr, s.cy.ptr, 0);
     launch_add(d, s.P, s.cy.ptr, s.p.ptr);
     launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.p.ptr);
@@ -450,6 +479,7 @@ void simulation_set_coarse_space(Simulation &s, int nc, int64_t nnz, const int32
         s.cy.release();
         s.cr.release();
         s.ce.release();
+        s.cproj.release();
         s.has_coarse = false;
         s.nc = 0;
         return;
@@ -481,6 +511,7 @@ void simulation_set_coarse_space(Simulation &s, int nc, int64_t nnz, const int32
     s.cy.resize(s.P);
     s.cr.resize(nc);
     s.ce.resize(nc);
+    s.cproj.reserve(nc, 16);
     s.nc = nc;
     s.coarse_rtol = rtol;
     s.coarse_iters = 0;
@@ -525,7 +556,7 @@ void pressure_guess_base(Simulation &s) {
         }
         return;
     case NSFEM_WARM_PROJECTION:
-        projection_guess(s);
+        projection_guess(*s.dev, s.proj, s.prhs.ptr, s.p.ptr);
         return;
     case NSFEM_WARM_LEARNED_CORRECTION: {
         // r = b - A p_prev; guess = p_prev + network(r), kept only if its
@@ -544,7 +575,7 @@ void pressure_guess_base(Simulation &s) {
 // the projecting mode extends its basis
 void pressure_after_solve(Simulation &s) {
     if (s.warm_mode == NSFEM_WARM_PROJECTION) {
-        projection_update(s);
+        projection_update(*s.dev, s.proj, s.mats[NSFEM_PRESSURE_STIFFNESS], s.p.ptr);
         return;
     }
     if (s.warm_mode != NSFEM_WARM_EXTRAPOLATE) return;
@@ -578,7 +609,7 @@ void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
                                    s.p_area.data(), s.p_xy.data());
     }
     if (mode == NSFEM_WARM_LEARNED_OR_PREVIOUS || mode == NSFEM_WARM_LEARNED_CORRECTION ||
-        mode == NSFEM_WARM_EXTRAPOLATE || mode == NSFEM_WARM_PROJECTION) {
+        mode == NSFEM_WARM_EXTRAPOLATE) {
         s.pguess.resize(s.P);
         s.ptmp.resize(s.P);
         s.pscal.resize(2);
@@ -588,19 +619,15 @@ void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
         s.p_keep.resize(s.P);
     }
     if (mode == NSFEM_WARM_PROJECTION) {
-        // basis size: NSFEM_PROJECTION_DEPTH (default 16), 2 x depth pressure
-        // vectors of device memory
+        // basis size: NSFEM_PROJECTION_DEPTH (default 16), 2 x depth + 3
+        // pressure vectors of device memory
         static const int depth_env = [] {
             const char *e = std::getenv("NSFEM_PROJECTION_DEPTH");
             return e ? std::atoi(e) : 0;
         }();
-        s.proj_max = depth_env > 0 ? std::min(depth_env, 256) : 16;
-        s.proj_x.resize((size_t)s.proj_max * s.P);
-        s.proj_ax.resize((size_t)s.proj_max * s.P);
-        s.proj_w.resize(s.P);
-        s.proj_s.resize((size_t)s.proj_max + 1);
+        s.proj.reserve(s.P, depth_env > 0 ? std::min(depth_env, 256) : 16);
     }
-    s.proj_dim = 0;
+    s.proj.dim = 0;
     s.have_older = false;
     s.gnn = learned ? g : nullptr;
     s.warm_mode = mode;
@@ -619,6 +646,20 @@ void simulation_copy_pressure(Simulation &s, double *rhs, double *p) {
     if (rhs) launch_copy(d, s.P, s.prhs.ptr, rhs);
     if (p) launch_copy(d, s.P, s.p.ptr, p);
     NSFEM_HIP(hipStreamSynchronize(d.stream));
+}
+
+// Pairs the u and v columns of a velocity system (one 2-column solve: the
+// matrix is streamed once for both) unless the last step's column counts
+// were unbalanced: the 2-column CG runs both columns to the slower one's
+// count, so when one needs more than twice the other's iterations (+8) the
+// columns are solved one after the other.  On the 2048^2 cavity the
+// diffusion columns take ~28 and ~124 iterations (separately 107 ms a step
+// against 120 paired), the mass columns ~28 and ~24 (paired 34 ms against 45).
+bool pair_columns(const Simulation &s, int sys) {
+    if (!s.opt.pair_velocity) return false;
+    const int64_t a = s.vel_iters[sys][0], b = s.vel_iters[sys][1];
+    if (a <= 0 || b <= 0) return true;
+    return std::max(a, b) <= 2 * std::min(a, b) + 8;
 }
 
 void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
@@ -646,7 +687,8 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_v.ptr, s.rhs.ptr + V);
         launch_copy(d, 2 * V, s.utv.ptr, s.uv.ptr);
         SolveOut o[5]{};
-        if (s.opt.pair_velocity) {
+        const bool pair_d = pair_columns(s, 0);
+        if (pair_d) {
             solve(s, NSFEM_DIFFUSION, s.rhs.ptr, s.uv.ptr, 2, o[0]);
             o[1].iters[0] = o[0].iters[1];
             o[1].refines[0] = o[0].refines[1];
@@ -655,6 +697,8 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
             solve(s, NSFEM_DIFFUSION, s.rhs.ptr, u, 1, o[0]);
             solve(s, NSFEM_DIFFUSION, s.rhs.ptr + V, v, 1, o[1]);
         }
+        s.vel_iters[0][0] = o[0].iters[0];
+        s.vel_iters[0][1] = o[1].iters[0];
         // 3. pressure
         launch_spmv(d, BD, s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
         launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.prhs.ptr);
@@ -666,7 +710,8 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         // 4. velocity correction
         launch_spmv(d, PD, s.p.ptr, nullptr, s.rhs.ptr, 0);
         launch_copy(d, 2 * V, s.uv.ptr, s.du.ptr);
-        if (s.opt.pair_velocity) {
+        const bool pair_m = pair_columns(s, 1);
+        if (pair_m) {
             solve(s, NSFEM_VELOCITY_MASS, s.rhs.ptr, s.du.ptr, 2, o[3]);
             o[4].iters[0] = o[3].iters[1];
             o[4].refines[0] = o[3].refines[1];
@@ -675,6 +720,8 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
             solve(s, NSFEM_VELOCITY_MASS, s.rhs.ptr, s.du.ptr, 1, o[3]);
             solve(s, NSFEM_VELOCITY_MASS, s.rhs.ptr + V, s.du.ptr + V, 1, o[4]);
         }
+        s.vel_iters[1][0] = o[3].iters[0];
+        s.vel_iters[1][1] = o[4].iters[0];
         // 5. update and Dirichlet values
         launch_add(d, 2 * V, s.du.ptr, s.uv.ptr);
         launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, u);
@@ -694,9 +741,8 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
                 stats->max_relative_residual[k] =
                     std::max(stats->max_relative_residual[k], o[k].rel[0]);
             }
-            const double t[5] = {o[0].seconds, s.opt.pair_velocity ? 0.0 : o[1].seconds,
-                                 o[2].seconds, o[3].seconds,
-                                 s.opt.pair_velocity ? 0.0 : o[4].seconds};
+            const double t[5] = {o[0].seconds, pair_d ? 0.0 : o[1].seconds, o[2].seconds,
+                                 o[3].seconds, pair_m ? 0.0 : o[4].seconds};
             for (int k = 0; k < 5; ++k) stats->cg_seconds[k] += t[k];
             stats->coarse_iterations += coarse_iters;
             stats->coarse_seconds += coarse_seconds;
@@ -971,8 +1017,8 @@ void simulation_detach(Simulation *s) {
     s->pguess.release();
     s->p_older.release();
     s->p_keep.release();
-    s->proj_x.release();
-    s->proj_ax.release();
+    s->proj.release();
+    s->cproj.release();
     s->ptmp.release();
     s->pscal.release();
     s->uv.release();

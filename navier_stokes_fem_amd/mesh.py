@@ -40,6 +40,9 @@ class Mesh:
     p_bc_values: np.ndarray        # float64 [Bp]
     name: str = "mesh"
     info: dict = field(default_factory=dict)
+    # int32 [P, 2]: logical (i, j) of each pressure node on the generator's
+    # structured grid (cavity, channel), None for other meshes
+    grid: np.ndarray = None
 
     @property
     def node_count(self) -> int:
@@ -136,7 +139,8 @@ def cavity(n: int, lid_speed: float = 1.0) -> Mesh:
     m = Mesh(nodes=nodes, elements=elements, pressure_count=P,
              vel_bc_nodes=vb, vel_bc_u=gu, vel_bc_v=gv,
              p_bc_nodes=np.array([0], dtype=np.int32), p_bc_values=np.zeros(1),
-             name=f"cavity{n}", info={"n": n, "lid_speed": float(lid_speed)})
+             name=f"cavity{n}", info={"n": n, "lid_speed": float(lid_speed)},
+             grid=np.stack([vi, vj], axis=1).astype(np.int32))
     return m
 
 
@@ -175,6 +179,73 @@ def cavity_prolongation(n: int, ratio: int):
     rs = np.zeros(len(f) + 1, np.int64)
     np.cumsum(np.bincount(rows, minlength=len(f)), out=rs[1:])
     return rs.astype(np.int32), cols.astype(np.int32), w.astype(np.float64), (nc + 1) ** 2
+
+
+def grid_prolongation(mesh: Mesh, ratio: int):
+    """Interpolation from a coarse grid onto the pressure nodes of a mesh
+    built on a structured grid (``mesh.grid``: cavity, channel), for
+    ``Simulation.set_coarse_space``.  Coarse nodes are the grid positions
+    (I*ratio, J*ratio) that carry a pressure node without a pressure
+    Dirichlet value.  A fine node at logical position (i, j) takes the
+    barycentric weights, in logical coordinates, of the coarse triangle
+    around it (squares split along the same diagonal as the mesh's):
+
+    * corners whose grid position has no node (a hole: the channel's disk)
+      are dropped and the other weights rescaled to sum 1;
+    * corners on the pressure Dirichlet boundary are dropped without
+      rescaling, so every coarse function vanishes there;
+    * pressure Dirichlet nodes get an empty row.
+
+    Returns (row_start, col, val, coarse_count)."""
+    g = mesh.grid
+    if g is None:
+        raise ValueError("the mesh carries no structured grid")
+    if ratio < 2:
+        raise ValueError("ratio must be >= 2")
+    P = mesh.pressure_count
+    gi = g[:, 0].astype(np.int64)
+    gj = g[:, 1].astype(np.int64)
+    nx, ny = int(gi.max()), int(gj.max())
+    if nx % ratio or ny % ratio:
+        raise ValueError("ratio must divide the grid's extents")
+    lut = -np.ones((ny + 1) * (nx + 1), np.int64)
+    lut[gj * (nx + 1) + gi] = np.arange(P)
+    dirichlet = np.zeros(P, bool)
+    dirichlet[np.asarray(mesh.p_bc_nodes, np.int64)] = True
+    mx, my = nx // ratio, ny // ratio
+    cI, cJ = np.meshgrid(np.arange(mx + 1), np.arange(my + 1), indexing="xy")
+    cf = lut[(cJ.reshape(-1) * ratio) * (nx + 1) + cI.reshape(-1) * ratio]
+    exists = cf >= 0
+    valid = exists.copy()
+    valid[exists] = ~dirichlet[cf[exists]]
+    cid = -np.ones(len(cf), np.int64)
+    cid[valid] = np.arange(int(valid.sum()))
+    I = np.minimum(gi // ratio, mx - 1)
+    J = np.minimum(gj // ratio, my - 1)
+    a = (gi - I * ratio) / ratio
+    b = (gj - J * ratio) / ratio
+    c00 = J * (mx + 1) + I
+    c10, c01 = c00 + 1, c00 + mx + 1
+    c11 = c01 + 1
+    lower = a >= b
+    corners = np.stack([c00, np.where(lower, c10, c11), np.where(lower, c11, c01)], axis=1)
+    w = np.stack([np.where(lower, 1 - a, 1 - b), np.where(lower, a - b, a),
+                  np.where(lower, b, b - a)], axis=1)
+    w = w * exists[corners]
+    tot = w.sum(axis=1, keepdims=True)
+    w = np.divide(w, tot, out=np.zeros_like(w), where=tot > 0)
+    w = w * valid[corners]
+    w[dirichlet] = 0.0
+    rows = np.repeat(np.arange(P), 3)
+    cols = cid[corners].reshape(-1)
+    w = w.reshape(-1)
+    keep = w > 0
+    rows, cols, w = rows[keep], cols[keep], w[keep]
+    order = np.lexsort((cols, rows))
+    rows, cols, w = rows[order], cols[order], w[order]
+    rs = np.zeros(P + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=P), out=rs[1:])
+    return rs.astype(np.int32), cols.astype(np.int32), w, int(valid.sum())
 
 
 def cavity_batch(count: int = 64, seed: int = 3, n_min: int = 16, n_max: int = 256):
@@ -269,7 +340,8 @@ def channel(nx: int = 512, ny: int = 128, seed: int = 2, jitter: float = 0.2,
              p_bc_nodes=pb, p_bc_values=np.zeros(len(pb)),
              name=f"channel{nx}x{ny}s{seed}",
              info={"nx": nx, "ny": ny, "seed": seed, "umax": umax,
-                   "removed_triangles": int((~keep).sum())})
+                   "removed_triangles": int((~keep).sum())},
+             grid=np.stack([gi, gj], axis=1).astype(np.int32))
     return m
 
 

@@ -66,3 +66,43 @@ def test_coarse_correction_premise_with_the_oracle():
         corrected += O.cg((rs, col, val), b, xc, rtol=1e-8)[2].iterations
     print(f"oracle CG iterations: previous {plain}, corrected {corrected}")
     assert corrected < 0.75 * plain
+
+
+def test_grid_prolongation_on_the_channel():
+    """The channel's grid minus the disk: Dirichlet (outlet) rows are empty,
+    every coarse column is used, and rows whose coarse triangle has all its
+    corners (no hole, no outlet) reproduce linear functions of the logical
+    coordinates exactly."""
+    m = mesh.channel(64, 16, seed=3)
+    rs, col, val, nc = mesh.grid_prolongation(m, 4)
+    P = sp.csr_matrix((val, col, rs), shape=(m.pressure_count, nc))
+    cnt = np.diff(rs)
+    assert np.all(cnt[m.p_bc_nodes] == 0)
+    assert np.all(np.bincount(col, minlength=nc) > 0)
+    gi, gj = m.grid[:, 0], m.grid[:, 1]
+    # coarse node coordinates: the fine node at each coarse position
+    coarse_ij = np.zeros((nc, 2))
+    ident = (P.toarray() == 1.0)
+    r, c = np.nonzero(ident)
+    coarse_ij[c] = m.grid[r]
+    full = np.isclose(P.sum(axis=1).A1, 1.0)
+    far = (gi < 64 - 8) & full
+    for a, b, cc in ((0.5, 2.0, -1.0), (1.0, 0.0, 0.3)):
+        fine = a + b * gi + cc * gj
+        coarse = a + b * coarse_ij[:, 0] + cc * coarse_ij[:, 1]
+        np.testing.assert_allclose((P @ coarse)[far], fine[far], rtol=0, atol=1e-12)
+    assert far.sum() > 0.8 * m.pressure_count
+
+
+def test_grid_prolongation_matches_the_cavity_operator_away_from_the_pin():
+    m = mesh.cavity(16)
+    rs, col, val, nc = mesh.grid_prolongation(m, 4)
+    crs, ccol, cval, cnc = mesh.cavity_prolongation(16, 4)
+    assert nc == cnc - 1  # the pinned pressure node's coarse node is dropped
+    A = sp.csr_matrix((val, col, rs), shape=(289, nc)).toarray()
+    B = sp.csr_matrix((cval, ccol, crs), shape=(289, cnc)).toarray()[:, 1:]
+    untouched = np.abs(sp.csr_matrix((cval, ccol, crs), shape=(289, cnc)).toarray()[:, 0]) == 0
+    assert np.array_equal(A[untouched], B[untouched])
+    assert np.diff(rs)[0] == 0
+    with pytest.raises(ValueError):
+        mesh.grid_prolongation(m, 3)
