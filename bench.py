@@ -96,11 +96,12 @@ def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
 
 
 def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
-             coarse_rtol=1e-6):
+             coarse_rtol=1e-6, velocity_projection=0):
     sim.reset()
     sim.configure(precision=form, rtol=1e-8, pair_velocity=True)
     sim.set_warm_start(warm)
     sim.set_coarse_space(coarse, rtol=coarse_rtol)
+    sim.set_velocity_projection(velocity_projection)
     for i in range(warmup):
         sim.step(1)
         log(f"{form}: warmup step {i + 1}/{warmup} done")
@@ -125,7 +126,28 @@ def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
             f"iterations {st['cg_iterations']}")
     sim.device.synchronize()
     wall = time.perf_counter() - t0
+    agg["fields"] = sim.fields()
     return wall, agg
+
+
+def field_difference(fields, ref):
+    """Per field (u, v, p) after the same steps: max |a - b|, ||a - b|| / ||b||
+    and max |b|; for p also the relative difference once each field's mean
+    is removed (a 1e-8 relative residual fixes the pressure only up to its
+    smoothest, near-constant mode, whose eigenvalue is tiny)."""
+    import numpy as np
+
+    out = {}
+    for name, a, b in zip(("u", "v", "p"), fields, ref):
+        d = a - b
+        nb = float(np.linalg.norm(b))
+        out[name] = {"max_abs": float(np.abs(d).max()), "rel_l2": float(np.linalg.norm(d) / nb)
+                     if nb > 0 else 0.0, "max_abs_reference": float(np.abs(b).max())}
+        if name == "p":
+            bc = b - b.mean()
+            out[name]["rel_l2_mean_removed"] = float(np.linalg.norm(d - d.mean()) /
+                                                     max(float(np.linalg.norm(bc)), 1e-300))
+    return out
 
 
 def cpu_baseline(m, nu, dt, gpu_iters_per_step, sample_iters):
@@ -514,6 +536,11 @@ def main():
                          "P1 interpolation from the (n/ratio)^2 cavity (0 = off)")
     ap.add_argument("--coarse-rtol", type=float, default=1e-6,
                     help="relative tolerance of the coarse CG solves")
+    ap.add_argument("--velocity-projection", type=int, default=0,
+                    help="fp32_refined form: diffusion guesses from the advected velocity plus the "
+                         "projection of the correction onto this many past ones (0 = off, the "
+                         "default: closer to a tightly solved run, but the pressure solves that "
+                         "follow take more iterations)")
     ap.add_argument("--config", default="cavity2048",
                     choices=["cavity2048", "cavity32", "channel512", "cavity_batch64",
                              "learned_channels", "spmv"],
@@ -591,7 +618,8 @@ def main():
         mixed_form = form == "fp32_refined"
         wall, st = run_form(sim, form, args.warmup, args.steps, n_p, nnz_p,
                             warm=args.warm if mixed_form else "previous",
-                            coarse=coarse if mixed_form else None, coarse_rtol=args.coarse_rtol)
+                            coarse=coarse if mixed_form else None, coarse_rtol=args.coarse_rtol,
+                            velocity_projection=args.velocity_projection if mixed_form else 0)
         results[form] = (wall, st)
         log(f"{form}: {args.steps / wall:.3f} steps/s")
     # attribution: the same mixed solver without the coarse correction, and
@@ -644,6 +672,10 @@ def main():
                                f"{m.pressure_count} pressure nodes), nu=1e-3, dt=1e-3, "
                                f"rel tol 1e-8 per solve",
                    "form": chosen, "pair_velocity": True, "parallelism": "single GPU",
+                   "diffusion_initial_guess": (
+                       f"advected velocity + projection of the correction (depth "
+                       f"{args.velocity_projection})" if chosen == "fp32_refined" and
+                       args.velocity_projection else "advected velocity"),
                    "pressure_initial_guess": (
                        (args.warm + (f" + coarse-grid correction (P1 from the "
                                      f"{args.n // args.coarse_ratio}^2 cavity, coarse CG to "
@@ -668,6 +700,11 @@ def main():
     if "fp64" in results and "fp32_refined" in results:
         out["speedup_vs_fp64"] = round((args.steps / results["fp32_refined"][0]) /
                                        (args.steps / results["fp64"][0]), 3)
+        out["field_difference_vs_fp64"] = field_difference(results["fp32_refined"][1]["fields"],
+                                                           results["fp64"][1]["fields"])
+        for name, (w, st_a) in attribution.items():
+            out[name]["field_difference_vs_fp64"] = field_difference(st_a["fields"],
+                                                                      results["fp64"][1]["fields"])
     sim.close()
     dev.close()  # release the device before interpreter teardown
     if not args.no_cpu:

@@ -414,6 +414,13 @@ int nsfem_simulation_set_warm_start(nsfem_simulation *sim, int mode, nsfem_gnn *
  * tolerance as from any guess.  coarse_count = 0 removes the coarse space.
  * No counterpart in the reference, whose pressure CG starts from the
  * previous pressure (include/assembly.h:608-615). */
+/* Initial guess of the diffusion solves: the advected velocity (the
+ * reference's, include/assembly.h:1521-1531) plus the A-norm projection of
+ * the correction onto the span of the last `depth` corrections of each
+ * velocity column (Fischer's projection on the diffusion matrix); 0 = off.
+ * Costs 2 x (2 depth + 3) velocity vectors of device memory.  No
+ * counterpart in the reference. */
+int nsfem_simulation_set_velocity_projection(nsfem_simulation *sim, int depth);
 int nsfem_simulation_set_coarse_space(nsfem_simulation *sim, int coarse_count, int64_t nnz,
                                       const int32_t *row_start, const int32_t *col,
                                       const double *val, double coarse_rtol);

@@ -95,6 +95,11 @@ struct Simulation {
     Matrix cP, cPT, cA;
     DeviceArray<double> cy, cr, ce;
     ProjBasis cproj;  // projection guess of the coarse solves
+    // projection guess of the diffusion solves' corrections u - u~ (u~ = the
+    // advected velocity), one basis per column; vdel: the guessed, then the
+    // true, corrections
+    ProjBasis vproj[2];
+    DeviceArray<double> vdel;
     int64_t coarse_iters = 0;
     double coarse_seconds = 0.0;
     // last step's iteration counts of the u and v columns of the diffusion
@@ -211,6 +216,7 @@ void simulation_reset(Simulation &s) {
     s.have_older = false;
     s.proj.dim = 0;
     s.cproj.dim = 0;
+    s.vproj[0].dim = s.vproj[1].dim = 0;
     for (auto &row : s.vel_iters) row[0] = row[1] = 0;
     launch_fill(d, 2 * s.V, 0.0, s.uv.ptr);
     launch_fill(d, s.P, 0.0, s.p.ptr);
@@ -468,6 +474,21 @@ void coarse_correct(Simulation &s) {
 
 }  // namespace
 
+void simulation_set_velocity_projection(Simulation &s, int depth) {
+    NSFEM_REQUIRE(depth >= 0 && depth <= 256, NSFEM_INVALID_ARGUMENT,
+                  "velocity projection depth must lie in [0, 256]");
+    s.dev->set_current();
+    if (depth == 0) {
+        s.vproj[0].release();
+        s.vproj[1].release();
+        s.vdel.release();
+        return;
+    }
+    s.vproj[0].reserve(s.V, depth);
+    s.vproj[1].reserve(s.V, depth);
+    s.vdel.resize(2 * (size_t)s.V);
+}
+
 void simulation_set_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
                                  const int32_t *col, const double *val, double rtol) {
     Device &d = *s.dev;
@@ -648,6 +669,37 @@ void simulation_copy_pressure(Simulation &s, double *rhs, double *p) {
     NSFEM_HIP(hipStreamSynchronize(d.stream));
 }
 
+// Initial guess of the diffusion solves: the advected velocity u~ (the
+// reference's) plus the A-norm projection of the correction u - u~ onto the
+// span of past corrections, with A the diffusion matrix and the residual
+// b - A u~ as the projected right-hand side.  u~ is kept in du (free until
+// the velocity-correction step) for the basis update.
+void diffusion_guess(Simulation &s) {
+    Device &d = *s.dev;
+    const int V = s.V;
+    Matrix &A = s.mats[NSFEM_DIFFUSION];
+    for (int c = 0; c < 2; ++c) {
+        double *x = s.uv.ptr + (size_t)c * V, *del = s.vdel.ptr + (size_t)c * V;
+        ProjBasis &B = s.vproj[c];
+        launch_spmv(d, A, x, s.rhs.ptr + (size_t)c * V, B.w.ptr, 1);  // r0 = b - A u~
+        launch_fill(d, V, 0.0, del);
+        projection_guess(d, B, B.w.ptr, del);
+        launch_copy(d, V, x, s.du.ptr + (size_t)c * V);
+        launch_add(d, V, del, x);
+    }
+}
+
+// After the diffusion solves: the true corrections u - u~ extend the bases.
+void diffusion_after_solve(Simulation &s) {
+    Device &d = *s.dev;
+    const int V = s.V;
+    for (int c = 0; c < 2; ++c) {
+        double *del = s.vdel.ptr + (size_t)c * V;
+        launch_axpby(d, V, 1.0, -1.0, s.uv.ptr + (size_t)c * V, s.du.ptr + (size_t)c * V, del);
+        projection_update(d, s.vproj[c], s.mats[NSFEM_DIFFUSION], del);
+    }
+}
+
 // Pairs the u and v columns of a velocity system (one 2-column solve: the
 // matrix is streamed once for both) unless the last step's column counts
 // were unbalanced: the 2-column CG runs both columns to the slower one's
@@ -687,6 +739,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_v.ptr, s.rhs.ptr + V);
         launch_copy(d, 2 * V, s.utv.ptr, s.uv.ptr);
         SolveOut o[5]{};
+        if (s.vproj[0].max > 0) diffusion_guess(s);
         const bool pair_d = pair_columns(s, 0);
         if (pair_d) {
             solve(s, NSFEM_DIFFUSION, s.rhs.ptr, s.uv.ptr, 2, o[0]);
@@ -699,6 +752,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         }
         s.vel_iters[0][0] = o[0].iters[0];
         s.vel_iters[0][1] = o[1].iters[0];
+        if (s.vproj[0].max > 0) diffusion_after_solve(s);
         // 3. pressure
         launch_spmv(d, BD, s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
         launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.prhs.ptr);
@@ -1019,6 +1073,9 @@ void simulation_detach(Simulation *s) {
     s->p_keep.release();
     s->proj.release();
     s->cproj.release();
+    s->vproj[0].release();
+    s->vproj[1].release();
+    s->vdel.release();
     s->ptmp.release();
     s->pscal.release();
     s->uv.release();
@@ -1148,6 +1205,15 @@ int nsfem_simulation_set_coarse_space(nsfem_simulation *s, int coarse_count, int
         std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
         simulation_set_coarse_space(*s->impl, coarse_count, nnz, row_start, col, val,
                                     coarse_rtol);
+    });
+}
+
+int nsfem_simulation_set_velocity_projection(nsfem_simulation *s, int depth) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_set_velocity_projection(*s->impl, depth);
     });
 }
 

@@ -115,6 +115,12 @@ class Simulation:
 
         return export_simulation(self, folder, time_step)
 
+    def set_velocity_projection(self, depth: int = 16):
+        """Start each diffusion solve from the advected velocity plus the
+        A-norm projection of its correction onto the last ``depth``
+        corrections (0 = the advected velocity alone, the reference's)."""
+        N.check(N.lib().nsfem_simulation_set_velocity_projection(self.handle, int(depth)))
+
     def set_coarse_space(self, prolongation=None, rtol: float = 1e-6):
         """Coarse-grid correction of every pressure solve's initial guess:
         ``prolongation`` = (row_start, col, val, coarse_count), one row per

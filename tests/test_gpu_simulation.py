@@ -125,6 +125,79 @@ def test_coarse_grid_correction(gpu, precision, mode):
     assert s1["cg_iterations"][2] < 0.7 * s0["cg_iterations"][2]
 
 
+def _errors_vs(fields, truth):
+    return [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fields, truth)]
+
+
+def _cavity_run(gpu, m, steps, precision, rtol, warm="previous", coarse=None, velocity=0):
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    sim = Simulation(m, 1e-3, 1e-3, device=gpu, rtol=rtol, precision=precision)
+    sim.set_warm_start(warm)
+    if coarse is not None:
+        sim.set_coarse_space(coarse)
+    sim.set_velocity_projection(velocity)
+    st = sim.step(steps)
+    f = sim.fields()
+    sim.close()
+    return f, st
+
+
+def test_velocity_projection(gpu):
+    """The diffusion guess u~ + projection of the correction: every solve
+    meets the tolerance, the diffusion solves take far fewer iterations, and
+    the fields are at least as close to a tightly solved run (FP64, 1e-13)
+    as without it.  (Two guesses need not give the same fields to 1e-6:
+    with the Dirichlet rows' O(1) values in ||b||, a 1e-8 relative residual
+    fixes the interior velocity only to ~1e-4, the same for the reference's
+    method.)  Depth 0 switches it off; a negative depth is rejected."""
+    from navier_stokes_fem_amd import _native as N
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    m = mesh.cavity(64)
+    truth, _ = _cavity_run(gpu, m, 10, "fp64", 1e-13)
+    f0, s0 = _cavity_run(gpu, m, 10, "fp32_refined", 1e-8)
+    f1, s1 = _cavity_run(gpu, m, 10, "fp32_refined", 1e-8, velocity=16)
+    e0, e1 = _errors_vs(f0, truth), _errors_vs(f1, truth)
+    d0 = s0["cg_iterations"][0] + s0["cg_iterations"][1]
+    d1 = s1["cg_iterations"][0] + s1["cg_iterations"][1]
+    print(f"diffusion iterations {d0} -> {d1}; errors vs 1e-13 run {e0} -> {e1}")
+    assert max(s1["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+    assert d1 < 0.7 * d0
+    for a, b in zip(e1, e0):
+        assert a <= 1.2 * b
+    sim = Simulation(m, 1e-3, 1e-3, device=gpu)
+    sim.set_velocity_projection(16)
+    sim.step(2)
+    sim.set_velocity_projection(0)
+    sim.step(1)
+    with pytest.raises(N.NsfemError):
+        sim.set_velocity_projection(-1)
+    sim.close()
+
+
+@pytest.mark.parametrize("velocity", [0, 16])
+def test_chosen_form_is_as_accurate_as_the_plain_method(gpu, velocity):
+    """FP32 + FP64 refinement, projection guess and coarse correction (with
+    and without the velocity projection) against the plain FP64 method with
+    the previous guesses, both at 1e-8: per field, the chosen form's distance
+    to a tightly solved run (FP64, 1e-13) is at most 1.2x the plain
+    method's."""
+    from navier_stokes_fem_amd import mesh
+
+    m = mesh.cavity(128)
+    truth, _ = _cavity_run(gpu, m, 10, "fp64", 1e-13)
+    fp, _ = _cavity_run(gpu, m, 10, "fp64", 1e-8)
+    fc, sc = _cavity_run(gpu, m, 10, "fp32_refined", 1e-8, warm="projection",
+                         coarse=mesh.cavity_prolongation(128, 4), velocity=velocity)
+    ep, ec = _errors_vs(fp, truth), _errors_vs(fc, truth)
+    print(f"rel L2 error vs 1e-13 run (u, v, p): plain {ep}, chosen {ec}")
+    assert max(sc["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+    for a, b in zip(ec, ep):
+        assert a <= 1.2 * b
+
+
 def test_coarse_space_rejects_bad_input(gpu):
     from navier_stokes_fem_amd import _native as N
     from navier_stokes_fem_amd import mesh
