@@ -429,8 +429,24 @@ def bench_cavity_batch64(dev, args):
     for s in sims:
         s.step(steps)
     serial = time.perf_counter() - t
-    log(f"cavity_batch64: batched {steps / batched:.2f} batch-steps/s, one by one {steps / serial:.2f}")
+    # the same batch with a coarse-grid correction (P1 from the n/2 cavity)
+    # for every even n, the instances' coarse solves in one batched launch
+    coarse_count = 0
+    for (m, _), s in zip(specs, sims):
+        n = m.info["n"]
+        if n % 2 == 0:
+            s.set_coarse_space(mesh.cavity_prolongation(n, 2))
+            coarse_count += 1
+        s.reset()
+    dev.synchronize()
+    t = time.perf_counter()
+    stats_c = step_batch(sims, steps)
+    batched_c = time.perf_counter() - t
+    log(f"cavity_batch64: batched {steps / batched:.2f} batch-steps/s, one by one "
+        f"{steps / serial:.2f}; batched with coarse corrections ({coarse_count} instances) "
+        f"{steps / batched_c:.2f}")
     its = [sum(st["cg_iterations"][k] for st in stats) / steps for k in range(5)]
+    its_c = [sum(st["cg_iterations"][k] for st in stats_c) / steps for k in range(5)]
     for s in sims:
         s.close()
     return _line("batch steps/sec (all 64 instances advanced one step), 1e-8 rel. residual",
@@ -441,7 +457,13 @@ def bench_cavity_batch64(dev, args):
                  dtype="f32+f64refine", autotune=tune,
                  one_by_one_steps_per_s=round(steps / serial, 3),
                  batching_speedup=round(serial / batched, 3),
-                 cg_iterations_per_batch_step=its)
+                 cg_iterations_per_batch_step=its,
+                 with_coarse_correction={
+                     "steps_per_s": round(steps / batched_c, 3),
+                     "instances": coarse_count,
+                     "cg_iterations_per_batch_step": its_c,
+                     "coarse_iterations_per_batch_step":
+                         sum(st["coarse_iterations"] for st in stats_c) / steps})
 
 
 def bench_learned_channels(dev, args):

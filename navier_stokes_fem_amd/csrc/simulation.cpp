@@ -442,16 +442,22 @@ void galerkin_product(int n, const std::vector<int32_t> &ars, const std::vector<
     }
 }
 
-// x0 += P e, A_c e = P^T (b - A x0) solved from zero to coarse_rtol; the
-// pressure Dirichlet values are imposed again.  A coarse solve that does
-// not converge leaves the guess as it was.
-void coarse_correct(Simulation &s) {
+// The coarse correction in three parts, so that a batch can solve every
+// instance's coarse system in one launch: prepare restricts r0 = b - A x0
+// into cr and starts ce from the projection onto past coarse solutions;
+// coarse_request describes the solve A_c e = P^T r0 (to coarse_rtol);
+// finish adds P e to x0 and imposes the pressure Dirichlet values again,
+// unless the coarse solve did not converge (then the guess stays as it was).
+void coarse_prepare(Simulation &s) {
     Device &d = *s.dev;
     Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
     launch_spmv(d, A, s.p.ptr, s.prhs.ptr, s.cy.ptr, 1);
     launch_spmv(d, s.cPT, s.cy.ptr, nullptr, s.cr.ptr, 0);
     launch_fill(d, s.nc, 0.0, s.ce.ptr);
     projection_guess(d, s.cproj, s.cr.ptr, s.ce.ptr);
+}
+
+CgRequest coarse_request(Simulation &s) {
     CgRequest rq{};
     rq.m = &s.cA;
     rq.b = s.cr.ptr;
@@ -461,15 +467,26 @@ void coarse_correct(Simulation &s) {
     rq.rtol = s.coarse_rtol;
     rq.atol = 0.0;
     rq.precision = s.opt.precision;
-    nsfem_cg_stats st{};
-    const int status = run_cg(d, rq, &st);
-    s.coarse_iters = st.iterations;
-    s.coarse_seconds = st.seconds;
+    return rq;
+}
+
+void coarse_finish(Simulation &s, int status, int64_t iterations, double seconds) {
+    Device &d = *s.dev;
+    s.coarse_iters = iterations;
+    s.coarse_seconds = seconds;
     if (status != NSFEM_OK) return;
     projection_update(d, s.cproj, s.cA, s.ce.ptr);
     launch_spmv(d, s.cP, s.ce.ptr, nullptr, s.cy.ptr, 0);
     launch_add(d, s.P, s.cy.ptr, s.p.ptr);
     launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.p.ptr);
+}
+
+void coarse_correct(Simulation &s) {
+    coarse_prepare(s);
+    const CgRequest rq = coarse_request(s);
+    nsfem_cg_stats st{};
+    const int status = run_cg(*s.dev, rq, &st);
+    coarse_finish(s, status, st.iterations, st.seconds);
 }
 
 }  // namespace
@@ -869,16 +886,35 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
             rq[i] = request(*S[i], NSFEM_DIFFUSION, S[i]->rhs.ptr, S[i]->uv.ptr, 2);
         std::vector<nsfem_cg_stats> sd(2 * I);
         check(run_cg_batch(d, rq, sd.data(), scale_v, &sec[0]), "diffusion");
+        // pressure guesses; the instances' coarse corrections share one
+        // batched launch
+        std::vector<size_t> with_coarse;
+        std::vector<CgRequest> crq;
         for (size_t i = 0; i < I; ++i) {
             Simulation &s = *S[i];
             launch_spmv(d, s.mats[NSFEM_VELOCITY_DIVERGENCE], s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
             launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.prhs.ptr);
-            pressure_guess(s);
-            if (stats) {
-                stats[i].coarse_iterations += s.coarse_iters;
-                stats[i].coarse_seconds += s.coarse_seconds;
+            pressure_guess_base(s);
+            s.coarse_iters = 0;
+            s.coarse_seconds = 0.0;
+            if (s.has_coarse) {
+                coarse_prepare(s);
+                with_coarse.push_back(i);
+                crq.push_back(coarse_request(s));
             }
         }
+        if (!crq.empty()) {
+            std::vector<nsfem_cg_stats> cst(2 * crq.size());
+            double csec = 0.0;
+            run_cg_batch(d, crq, cst.data(), scale, &csec);
+            for (size_t k = 0; k < with_coarse.size(); ++k)
+                coarse_finish(*S[with_coarse[k]], cst[2 * k].status, cst[2 * k].iterations, csec);
+        }
+        if (stats)
+            for (size_t i = 0; i < I; ++i) {
+                stats[i].coarse_iterations += S[i]->coarse_iters;
+                stats[i].coarse_seconds += S[i]->coarse_seconds;
+            }
         for (size_t i = 0; i < I; ++i)
             rq[i] = request(*S[i], NSFEM_PRESSURE_STIFFNESS, S[i]->prhs.ptr, S[i]->p.ptr, 1);
         std::vector<nsfem_cg_stats> sp_(2 * I);

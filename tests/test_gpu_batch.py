@@ -85,3 +85,45 @@ def test_batch_of_many_small_instances(gpu):
     alone.close()
     for s in sims:
         s.close()
+
+
+def test_batch_coarse_correction(gpu):
+    """Coarse corrections of several instances share one batched launch:
+    every solve meets the tolerance, the instance without a coarse space (odd
+    n) runs no coarse solve, the pressure solves of the others take fewer
+    iterations than the same batch without coarse spaces (six steps from
+    rest on small meshes: a few per cent), and the fields
+    match the same simulations with the same coarse spaces stepped one by one
+    (FP64 at 1e-11, to 1e-7 of the field scale)."""
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import step_batch
+
+    specs = [(64, 1.0, 1e-3), (48, 0.7, 2e-3), (33, 1.5, 1e-3), (96, 1.2, 1e-3)]
+    rtol = 1e-11
+
+    def with_coarse(sims):
+        for (n, _, _), s in zip(specs, sims):
+            if n % 2 == 0:
+                s.set_coarse_space(mesh.cavity_prolongation(n, 2))
+        return sims
+
+    plain = _batch(gpu, specs, "fp64", rtol)
+    batched = with_coarse(_batch(gpu, specs, "fp64", rtol))
+    one = with_coarse(_batch(gpu, specs, "fp64", rtol))
+    st_p = step_batch(plain, 6)
+    st_c = step_batch(batched, 6)
+    for s in one:
+        s.step(6)
+    its_p = sum(st["cg_iterations"][2] for (n, _, _), st in zip(specs, st_p) if n % 2 == 0)
+    its_c = sum(st["cg_iterations"][2] for (n, _, _), st in zip(specs, st_c) if n % 2 == 0)
+    print(f"pressure iterations of the instances with a coarse space: {its_p} -> {its_c}; "
+          f"coarse {[st['coarse_iterations'] for st in st_c]}")
+    for (n, _, _), st in zip(specs, st_c):
+        assert max(st["max_relative_residual"]) <= rtol * 1.01
+        assert (st["coarse_iterations"] > 0) == (n % 2 == 0)
+    assert its_c < its_p  # a few % this early; the single-mesh tests measure the gain
+    for a, b in zip(one, batched):
+        for fa, fb in zip(a.fields(), b.fields()):
+            np.testing.assert_allclose(fb, fa, rtol=0, atol=1e-7 * max(1.0, np.abs(fa).max()))
+    for s in plain + batched + one:
+        s.close()
