@@ -1112,6 +1112,13 @@ void simulation_detach(Simulation *s) {
     s->vproj[0].release();
     s->vproj[1].release();
     s->vdel.release();
+    matrix_release(s->cP);
+    matrix_release(s->cPT);
+    matrix_release(s->cA);
+    s->cy.release();
+    s->cr.release();
+    s->ce.release();
+    s->has_coarse = false;
     s->ptmp.release();
     s->pscal.release();
     s->uv.release();

@@ -218,6 +218,29 @@ def test_coarse_space_rejects_bad_input(gpu):
     sim.close()
 
 
+def test_device_closed_under_a_simulation():
+    """Closing the device releases a simulation's memory, its coarse space
+    and projection bases included; every later call on the simulation
+    raises instead of touching freed memory, and closing it still works."""
+    from navier_stokes_fem_amd import _native as N
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.device import GPUSimulationDevice
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    dev = GPUSimulationDevice(0)
+    sim = Simulation(mesh.cavity(16), 1e-3, 1e-3, device=dev, precision="fp32_refined")
+    sim.set_warm_start("projection")
+    sim.set_coarse_space(mesh.cavity_prolongation(16, 2))
+    sim.set_velocity_projection(4)
+    assert sim.step(2)["coarse_iterations"] > 0
+    dev.close()
+    for call in (lambda: sim.step(1), lambda: sim.set_coarse_space(None),
+                 lambda: sim.set_velocity_projection(0), lambda: sim.fields()):
+        with pytest.raises(N.NsfemError):
+            call()
+    sim.close()
+
+
 def test_export_fields(gpu, tmp_path):
     """The exported result file holds exactly the simulation's fields."""
     from navier_stokes_fem_amd import mesh, results
