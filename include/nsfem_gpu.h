@@ -336,7 +336,11 @@ int nsfem_simulation_autotune_batch(nsfem_simulation *const *sims, int count,
 
 /* The simulation's own matrices (ids as the device slots:
  * NSFEM_PRESSURE_STIFFNESS ... NSFEM_PRESSURE_DIVERGENCE): shape, and
- * y = A x (lhs == NULL) or y = lhs - A x on device buffers. */
+ * y = A x (lhs == NULL) or y = lhs - A x on device buffers.  Inside a
+ * simulation the velocity nodes are renumbered by position (sorted by y,
+ * then x) for locality; pressure vectors keep the caller's numbering, while
+ * velocity-side vectors of these products are in that internal order
+ * (nsfem_simulation_get_fields / set_fields map the fields back). */
 int nsfem_simulation_matrix_info(nsfem_simulation *sim, int matrix, int *rows, int *cols,
                                  int64_t *nnz);
 int nsfem_simulation_sp_r_mult(nsfem_simulation *sim, int matrix, const nsfem_buffer *lhs,

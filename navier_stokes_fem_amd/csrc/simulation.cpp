@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -105,6 +106,9 @@ struct Simulation {
     // last step's iteration counts of the u and v columns of the diffusion
     // (0) and mass (1) solves: they decide whether the next step pairs them
     int64_t vel_iters[2][2] = {{0, 0}, {0, 0}};
+    // internal order of the velocity nodes (velocity_order): vorder[new] =
+    // caller's index, vpos[caller's index] = new
+    std::vector<int32_t> vorder, vpos;
 };
 
 namespace {
@@ -136,6 +140,74 @@ void put(Device &d, DeviceArray<T> &dst, const T *src, size_t n) {
 
 }  // namespace
 
+namespace {
+
+// Velocity nodes in lattice order, sorted by (y, x).  The reference numbers
+// the P2 nodes vertices first and edge midpoints after them, so a row of a
+// velocity matrix reaches two distant index blocks; sorted by position,
+// every row's columns lie within about two lattice rows of it (+-8,196 at the
+// 2048^2 cavity): the gathers stay local and the 16-bit column offsets fit
+// (the 2048^2 diffusion CG 765 -> 669 us per mixed iteration, its product
+// 561 -> 497 us).  The pressure nodes keep the caller's numbering; the
+// simulation maps the velocity fields at its interface.  Returns new -> old.
+std::vector<int32_t> velocity_order(int V, const float *xy) {
+    std::vector<int32_t> order((size_t)V);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        const float ya = xy[2 * (size_t)a + 1], yb = xy[2 * (size_t)b + 1];
+        if (ya != yb) return ya < yb;
+        const float xa = xy[2 * (size_t)a], xb = xy[2 * (size_t)b];
+        if (xa != xb) return xa < xb;
+        return a < b;
+    });
+    return order;
+}
+
+// Renumbers a CSR: new row r is old row row_old[r] (rows kept when empty),
+// column c becomes col_new[c] (kept when empty); columns ascend again within
+// each row.  Rows are split over `threads` workers.
+void permute_csr(HostCsr &c, const std::vector<int32_t> &row_old,
+                 const std::vector<int32_t> &col_new, int threads) {
+    HostCsr o;
+    o.rows = c.rows;
+    o.cols = c.cols;
+    o.rs.assign((size_t)c.rows + 1, 0);
+    for (int r = 0; r < c.rows; ++r) {
+        const int src = row_old.empty() ? r : row_old[r];
+        o.rs[r + 1] = o.rs[r] + (c.rs[src + 1] - c.rs[src]);
+    }
+    o.col.resize(c.col.size());
+    o.val.resize(c.val.size());
+    const int T = std::max(1, std::min(threads, std::max(1, c.rows / 4096)));
+    auto work = [&](int t) {
+        std::vector<std::pair<int32_t, double>> row;
+        const int r0 = (int)((int64_t)c.rows * t / T), r1 = (int)((int64_t)c.rows * (t + 1) / T);
+        for (int r = r0; r < r1; ++r) {
+            const int src = row_old.empty() ? r : row_old[r];
+            row.clear();
+            for (int32_t e = c.rs[src]; e < c.rs[src + 1]; ++e)
+                row.emplace_back(col_new.empty() ? c.col[e] : col_new[c.col[e]], c.val[e]);
+            std::sort(row.begin(), row.end(),
+                      [](const std::pair<int32_t, double> &a, const std::pair<int32_t, double> &b) {
+                          return a.first < b.first;
+                      });
+            int32_t k = o.rs[r];
+            for (const auto &e : row) {
+                o.col[k] = e.first;
+                o.val[k] = e.second;
+                ++k;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    c = std::move(o);
+}
+
+}  // namespace
+
 Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
                               const int32_t *el, int nvb, const int32_t *vb, const double *gu,
                               const double *gv, int npb, const int32_t *pb, const double *gp,
@@ -150,6 +222,44 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
                   opt.threads, A);
     HostKdTree tree;
     kd_build_host(V, xy, E, el, -1, -1, tree);
+    // renumber the velocity nodes (velocity_order): matrices, right-hand-side
+    // corrections, Dirichlet nodes, coordinates and elements
+    const int threads = opt.threads > 0 ? opt.threads
+                                        : (int)std::max(1u, std::thread::hardware_concurrency());
+    std::vector<int32_t> vorder = velocity_order(V, xy), vpos((size_t)V);
+    for (int i = 0; i < V; ++i) vpos[vorder[i]] = i;
+    {
+        NSFEM_REQUIRE(A.m[0].rows == V && A.m[0].cols == V && A.m[1].rows == V &&
+                          A.m[1].cols == V && A.m[3].rows == P && A.m[3].cols == 2 * V &&
+                          A.m[4].rows == 2 * V && A.m[4].cols == P,
+                      NSFEM_ERROR, "simulation: unexpected matrix shapes");
+        std::vector<int32_t> order2(2 * (size_t)V), pos2(2 * (size_t)V);
+        for (int i = 0; i < V; ++i) {
+            order2[i] = vorder[i];
+            order2[V + i] = V + vorder[i];
+            pos2[i] = vpos[i];
+            pos2[V + i] = V + vpos[i];
+        }
+        const std::vector<int32_t> none;
+        permute_csr(A.m[0], vorder, vpos, threads);
+        permute_csr(A.m[1], vorder, vpos, threads);
+        permute_csr(A.m[3], none, pos2, threads);
+        permute_csr(A.m[4], order2, none, threads);
+        std::vector<double> cu((size_t)V), cv((size_t)V);
+        for (int i = 0; i < V; ++i) {
+            cu[i] = A.c_u[vorder[i]];
+            cv[i] = A.c_v[vorder[i]];
+        }
+        A.c_u.swap(cu);
+        A.c_v.swap(cv);
+    }
+    std::vector<float> xy_in((size_t)2 * V);
+    for (int i = 0; i < V; ++i) {
+        xy_in[2 * (size_t)i] = xy[2 * (size_t)vorder[i]];
+        xy_in[2 * (size_t)i + 1] = xy[2 * (size_t)vorder[i] + 1];
+    }
+    std::vector<int32_t> el_in((size_t)6 * E);
+    for (size_t k = 0; k < el_in.size(); ++k) el_in[k] = vpos[el[k]];
     d.set_current();
     std::unique_ptr<Simulation> owner(new Simulation());  // freed if an upload throws
     Simulation *s = owner.get();
@@ -165,7 +275,7 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
         std::vector<double>().swap(c.val);
     }
     upload_tree(d, s->tree, (int32_t)(tree.nodes.size() / 2), tree.nodes.data(), (int64_t)tree.leaf.size(),
-                tree.leaf.data(), V, xy, E, el);
+                tree.leaf.data(), V, xy_in.data(), E, el_in.data());
 
     s->V = V;
     s->P = P;
@@ -184,6 +294,7 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
     std::vector<int32_t> vi, pi;
     std::vector<double> vu, vv, pv, dummy;
     last_wins(nvb, vb, gu, gv, vi, vu, vv);
+    for (int32_t &node : vi) node = vpos[node];
     last_wins(npb, pb, gp, nullptr, pi, pv, dummy);
     s->nvb = (int)vi.size();
     s->npb = (int)pi.size();
@@ -201,6 +312,8 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
         for (int k = 0; k < 3; ++k) s->p_area[t[k]] += area / 3.0;
     }
     s->p_xy.assign(xy, xy + 2 * (size_t)P);
+    s->vorder.swap(vorder);
+    s->vpos.swap(vpos);
     simulation_reset(*s);
     {
         std::lock_guard<std::mutex> lk(d.buf_mtx);
@@ -1076,24 +1189,43 @@ void simulation_autotune_batch(const std::vector<Simulation *> &S, nsfem_batch_t
     }
 }
 
+// velocity fields in the caller's node order (the device holds them in the
+// internal order of velocity_order)
 void simulation_get(Simulation &s, double *u, double *v, double *p) {
     Device &d = *s.dev;
     d.set_current();
-    if (u) NSFEM_HIP(hipMemcpyAsync(u, s.uv.ptr, sizeof(double) * s.V, hipMemcpyDeviceToHost, d.stream));
-    if (v)
-        NSFEM_HIP(hipMemcpyAsync(v, s.uv.ptr + s.V, sizeof(double) * s.V, hipMemcpyDeviceToHost,
-                                 d.stream));
+    const size_t V = (size_t)s.V;
+    std::vector<double> tmp;
+    if (u || v) {
+        tmp.resize(2 * V);
+        NSFEM_HIP(hipMemcpyAsync(tmp.data(), s.uv.ptr, sizeof(double) * 2 * V,
+                                 hipMemcpyDeviceToHost, d.stream));
+    }
     if (p) NSFEM_HIP(hipMemcpyAsync(p, s.p.ptr, sizeof(double) * s.P, hipMemcpyDeviceToHost, d.stream));
     NSFEM_HIP(hipStreamSynchronize(d.stream));
+    for (size_t i = 0; i < V && (u || v); ++i) {
+        if (u) u[i] = tmp[s.vpos[i]];
+        if (v) v[i] = tmp[V + s.vpos[i]];
+    }
 }
 
 void simulation_set(Simulation &s, const double *u, const double *v, const double *p) {
     Device &d = *s.dev;
     d.set_current();
-    if (u) NSFEM_HIP(hipMemcpyAsync(s.uv.ptr, u, sizeof(double) * s.V, hipMemcpyHostToDevice, d.stream));
-    if (v)
-        NSFEM_HIP(hipMemcpyAsync(s.uv.ptr + s.V, v, sizeof(double) * s.V, hipMemcpyHostToDevice,
-                                 d.stream));
+    const size_t V = (size_t)s.V;
+    std::vector<double> tmp;
+    if (u || v) {
+        tmp.resize(2 * V);
+        NSFEM_HIP(hipMemcpyAsync(tmp.data(), s.uv.ptr, sizeof(double) * 2 * V,
+                                 hipMemcpyDeviceToHost, d.stream));
+        NSFEM_HIP(hipStreamSynchronize(d.stream));
+        for (size_t i = 0; i < V; ++i) {
+            if (u) tmp[s.vpos[i]] = u[i];
+            if (v) tmp[V + s.vpos[i]] = v[i];
+        }
+        NSFEM_HIP(hipMemcpyAsync(s.uv.ptr, tmp.data(), sizeof(double) * 2 * V,
+                                 hipMemcpyHostToDevice, d.stream));
+    }
     if (p) NSFEM_HIP(hipMemcpyAsync(s.p.ptr, p, sizeof(double) * s.P, hipMemcpyHostToDevice, d.stream));
     NSFEM_HIP(hipStreamSynchronize(d.stream));
 }
