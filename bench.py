@@ -96,11 +96,14 @@ def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
 
 
 def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
-             coarse_rtol=1e-6, velocity_projection=0):
+             coarse_rtol=1e-6, velocity_projection=0, coarse_levels=()):
     sim.reset()
     sim.configure(precision=form, rtol=1e-8, pair_velocity=True)
     sim.set_warm_start(warm)
     sim.set_coarse_space(coarse, rtol=coarse_rtol)
+    if coarse is not None:
+        for level in coarse_levels:
+            sim.add_coarse_level(level)
     sim.set_velocity_projection(velocity_projection)
     for i in range(warmup):
         sim.step(1)
@@ -128,6 +131,27 @@ def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
     wall = time.perf_counter() - t0
     agg["fields"] = sim.fields()
     return wall, agg
+
+
+def coarse_levels(n, count):
+    """Prolongations of `count` multigrid levels below the n x n cavity's P1
+    coarse space, each halving the grid (n must stay even); count < 0 halves
+    down to the 16 x 16 grid."""
+    from navier_stokes_fem_amd import mesh
+
+    out = []
+    if count < 0:
+        count = 0
+        k = n
+        while k > 16 and k % 2 == 0:
+            k //= 2
+            count += 1
+    for _ in range(count):
+        if n % 2 or n < 4:
+            raise ValueError(f"cannot halve the {n}^2 coarse grid again")
+        out.append(mesh.cavity_prolongation(n, 2))
+        n //= 2
+    return out
 
 
 def field_difference(fields, ref):
@@ -567,11 +591,15 @@ def main():
                          "from the previous pressure, as the reference.  With another guess than "
                          "previous, fp32_refined also runs with the previous pressure once, to "
                          "separate the precision's share of the speed-up from the guess's")
-    ap.add_argument("--coarse-ratio", type=int, default=4,
+    ap.add_argument("--coarse-ratio", type=int, default=2,
                     help="coarse-grid correction of the fp32_refined form's pressure guesses: "
                          "P1 interpolation from the (n/ratio)^2 cavity (0 = off)")
     ap.add_argument("--coarse-rtol", type=float, default=1e-6,
-                    help="relative tolerance of the coarse CG solves")
+                    help="relative tolerance of the coarse solves")
+    ap.add_argument("--coarse-levels", type=int, default=-1,
+                    help="multigrid levels below the coarse space (each halves the grid; -1 = "
+                         "halve down to the 16x16 grid); with levels the coarse system is solved "
+                         "by V-cycles instead of CG")
     ap.add_argument("--velocity-projection", type=int, default=0,
                     help="fp32_refined form: diffusion guesses from the advected velocity plus the "
                          "projection of the correction onto this many past ones (0 = off, the "
@@ -648,6 +676,7 @@ def main():
         coarse = mesh.cavity_prolongation(args.n, args.coarse_ratio)
         log(f"prolongation from the {args.n // args.coarse_ratio}^2 cavity "
             f"({time.perf_counter() - t:.1f} s)")
+    levels = coarse_levels(args.n // args.coarse_ratio, args.coarse_levels) if coarse else []
     results = {}
     forms = ["fp64", "fp32_refined"] if args.form == "both" else [args.form]
     for form in forms:
@@ -655,7 +684,8 @@ def main():
         wall, st = run_form(sim, form, args.warmup, args.steps, n_p, nnz_p,
                             warm=args.warm if mixed_form else "previous",
                             coarse=coarse if mixed_form else None, coarse_rtol=args.coarse_rtol,
-                            velocity_projection=args.velocity_projection if mixed_form else 0)
+                            velocity_projection=args.velocity_projection if mixed_form else 0,
+                            coarse_levels=levels)
         results[form] = (wall, st)
         log(f"{form}: {args.steps / wall:.3f} steps/s")
     # attribution: the same mixed solver without the coarse correction, and
@@ -714,8 +744,10 @@ def main():
                        args.velocity_projection else "advected velocity"),
                    "pressure_initial_guess": (
                        (args.warm + (f" + coarse-grid correction (P1 from the "
-                                     f"{args.n // args.coarse_ratio}^2 cavity, coarse CG to "
-                                     f"{args.coarse_rtol:g})" if coarse is not None else ""))
+                                     f"{args.n // args.coarse_ratio}^2 cavity, coarse system "
+                                     + (f"by V-cycles over {len(levels)} more levels"
+                                        if levels else "by CG")
+                                     + f" to {args.coarse_rtol:g})" if coarse is not None else ""))
                        if chosen == "fp32_refined" else "previous")},
         "roofline": roofline,
     }

@@ -428,6 +428,16 @@ int nsfem_simulation_set_velocity_projection(nsfem_simulation *sim, int depth);
 int nsfem_simulation_set_coarse_space(nsfem_simulation *sim, int coarse_count, int64_t nnz,
                                       const int32_t *row_start, const int32_t *col,
                                       const double *val, double coarse_rtol);
+/* Adds a multigrid level below the deepest coarse level so far (the coarse
+ * space's, then each added one): P has one row per node of the level above,
+ * `count` columns (CSR, ascending columns), and the new level's matrix is
+ * P^T A_above P.  With levels present the coarse system is solved, to the
+ * same coarse_rtol, by V-cycles (damped Jacobi, 2 sweeps before and after
+ * on each level, CG to 1e-10 on the deepest) instead of CG.  Setting the
+ * coarse space again removes the levels.  No counterpart in the reference. */
+int nsfem_simulation_add_coarse_level(nsfem_simulation *sim, int count, int64_t nnz,
+                                      const int32_t *row_start, const int32_t *col,
+                                      const double *val);
 /* Copies the last step's pressure right-hand side and pressure into device
  * buffers (either may be NULL): training pairs for the network. */
 int nsfem_simulation_copy_pressure_system(nsfem_simulation *sim, nsfem_buffer *rhs,

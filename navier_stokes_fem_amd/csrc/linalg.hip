@@ -450,6 +450,14 @@ __global__ void k_add(int n, const double *__restrict__ a, double *__restrict__ 
         y[i] = a[i] + y[i];
 }
 
+// x += w * dinv .* r (a damped Jacobi update from the residual r)
+__global__ void k_jacobi(int n, double w, const double *__restrict__ dinv,
+                         const double *__restrict__ r, double *__restrict__ x) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = __builtin_fma(w * dinv[i], r[i], x[i]);
+}
+
 __global__ void k_fill(int n, double v, double *__restrict__ y) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -534,6 +542,12 @@ void launch_scatter(Device &d, int count, const int32_t *idx, const double *val,
 void launch_add(Device &d, int n, const double *a, double *y) {
     if (n <= 0) return;
     k_add<<<ew_grid(n), kBlock, 0, d.stream>>>(n, a, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_jacobi(Device &d, int n, double w, const double *dinv, const double *r, double *x) {
+    if (n <= 0) return;
+    k_jacobi<<<ew_grid(n), kBlock, 0, d.stream>>>(n, w, dinv, r, x);
     NSFEM_HIP(hipGetLastError());
 }
 

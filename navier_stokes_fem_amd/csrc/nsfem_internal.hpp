@@ -277,6 +277,8 @@ void build_tri_xy(Device &d, KdTreeGpu &t);
 void launch_scatter(Device &d, int count, const int32_t *idx, const double *val,
                     double *dst);
 void launch_add(Device &d, int n, const double *a, double *y);  // y += a
+// x += w * dinv .* r
+void launch_jacobi(Device &d, int n, double w, const double *dinv, const double *r, double *x);
 void launch_copy(Device &d, int n, const double *src, double *dst);
 void launch_spmv_affine(Device &d, Matrix &m, const double *x, const double *c,
                         double *y);  // y = A x - c

@@ -127,6 +127,7 @@ class Simulation:
         pressure node (e.g. ``mesh.cavity_prolongation``); the library forms
         P^T A P and solves the restricted residual equation to ``rtol`` before
         each pressure CG.  None removes it."""
+        self._coarse_sizes = []
         if prolongation is None:
             N.check(N.lib().nsfem_simulation_set_coarse_space(self.handle, 0, 0, None, None, None,
                                                               float(rtol)))
@@ -140,6 +141,25 @@ class Simulation:
         N.check(N.lib().nsfem_simulation_set_coarse_space(self.handle, int(nc), len(col),
                                                           N.ptr(rs), N.ptr(col), N.ptr(val),
                                                           float(rtol)))
+        self._coarse_sizes = [int(nc)]
+
+    def add_coarse_level(self, prolongation):
+        """Add a multigrid level below the deepest coarse level:
+        ``prolongation`` = (row_start, col, val, count) with one row per node
+        of the level above (e.g. ``mesh.cavity_prolongation(n_above, 2)``).
+        With levels the coarse system is solved by V-cycles instead of CG."""
+        rs, col, val, nc = prolongation
+        rs = np.ascontiguousarray(rs, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        sizes = getattr(self, "_coarse_sizes", [])
+        if not sizes:
+            raise ValueError("set a coarse space before adding levels")
+        if len(rs) != sizes[-1] + 1:
+            raise ValueError("the level's prolongation needs one row per node of the level above")
+        N.check(N.lib().nsfem_simulation_add_coarse_level(self.handle, int(nc), len(col),
+                                                          N.ptr(rs), N.ptr(col), N.ptr(val)))
+        sizes.append(int(nc))
 
     def copy_pressure_system(self, rhs=None, p=None):
         """Copy the last step's pressure right-hand side and solution into
