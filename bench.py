@@ -594,17 +594,17 @@ def main():
     ap.add_argument("--coarse-ratio", type=int, default=2,
                     help="coarse-grid correction of the fp32_refined form's pressure guesses: "
                          "P1 interpolation from the (n/ratio)^2 cavity (0 = off)")
-    ap.add_argument("--coarse-rtol", type=float, default=1e-6,
+    ap.add_argument("--coarse-rtol", type=float, default=1e-5,
                     help="relative tolerance of the coarse solves")
     ap.add_argument("--coarse-levels", type=int, default=-1,
                     help="multigrid levels below the coarse space (each halves the grid; -1 = "
                          "halve down to the 16x16 grid); with levels the coarse system is solved "
                          "by V-cycles instead of CG")
-    ap.add_argument("--velocity-projection", type=int, default=0,
+    ap.add_argument("--velocity-projection", type=int, default=8,
                     help="fp32_refined form: diffusion guesses from the advected velocity plus the "
-                         "projection of the correction onto this many past ones (0 = off, the "
-                         "default: closer to a tightly solved run, but the pressure solves that "
-                         "follow take more iterations)")
+                         "projection of the correction onto this many past ones (0 = off: the "
+                         "reference's guess; on, the fields are closer to a tightly solved run and "
+                         "the diffusion solves shorter, the pressure solves that follow longer)")
     ap.add_argument("--config", default="cavity2048",
                     choices=["cavity2048", "cavity32", "channel512", "cavity_batch64",
                              "learned_channels", "spmv"],
