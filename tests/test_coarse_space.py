@@ -106,3 +106,32 @@ def test_grid_prolongation_matches_the_cavity_operator_away_from_the_pin():
     assert np.diff(rs)[0] == 0
     with pytest.raises(ValueError):
         mesh.grid_prolongation(m, 3)
+
+
+def test_coarse_levels_chain():
+    """The multigrid levels below a coarse space: each prolongation maps the
+    next coarser cavity's nodes onto the level above (rows = the size of the
+    level above, columns = its own), halving down to the 16 x 16 grid, and
+    P1 interpolation keeps each level exact on linear functions."""
+    import bench
+
+    levels = bench.coarse_levels(128, -1)
+    assert len(levels) == 3  # 128 -> 64 -> 32 -> 16
+    above = 129 ** 2
+    k = 128
+    for rs, col, val, count in levels:
+        k //= 2
+        assert len(rs) == above + 1 and count == (k + 1) ** 2
+        assert col.min() >= 0 and col.max() < count
+        P = sp.csr_matrix((val, col, rs), shape=(above, count))
+        g = np.linspace(0.0, 1.0, k + 1)
+        cx, cy = np.meshgrid(g, g)
+        gf = np.linspace(0.0, 1.0, 2 * k + 1)
+        fx, fy = np.meshgrid(gf, gf)
+        lin_c = (2.0 * cx - 3.0 * cy + 0.5).ravel()
+        lin_f = (2.0 * fx - 3.0 * fy + 0.5).ravel()
+        np.testing.assert_allclose(P @ lin_c, lin_f, atol=1e-12)
+        above = count
+    assert len(bench.coarse_levels(128, 2)) == 2
+    with pytest.raises(ValueError):
+        bench.coarse_levels(12, 3)  # 12 -> 6 -> 3 cannot halve again

@@ -87,14 +87,18 @@ def test_batch_of_many_small_instances(gpu):
         s.close()
 
 
-def test_batch_coarse_correction(gpu):
+@pytest.mark.parametrize("levels", [False, True])
+def test_batch_coarse_correction(gpu, levels):
     """Coarse corrections of several instances share one batched launch:
     every solve meets the tolerance, the instance without a coarse space (odd
     n) runs no coarse solve, the pressure solves of the others take fewer
     iterations than the same batch without coarse spaces (six steps from
     rest on small meshes: a few per cent), and the fields
     match the same simulations with the same coarse spaces stepped one by one
-    (FP64 at 1e-11, to 1e-7 of the field scale)."""
+    (FP64 at 1e-11, to 1e-7 of the field scale).  With `levels` the 64^2
+    and 96^2 instances also get a multigrid level below their coarse space
+    and solve their coarse systems by V-cycles, while the 48^2 one stays in
+    the batched coarse CG."""
     from navier_stokes_fem_amd import mesh
     from navier_stokes_fem_amd.simulation import step_batch
 
@@ -105,6 +109,8 @@ def test_batch_coarse_correction(gpu):
         for (n, _, _), s in zip(specs, sims):
             if n % 2 == 0:
                 s.set_coarse_space(mesh.cavity_prolongation(n, 2))
+                if levels and n != 48:
+                    s.add_coarse_level(mesh.cavity_prolongation(n // 2, 2))
         return sims
 
     plain = _batch(gpu, specs, "fp64", rtol)
