@@ -286,8 +286,10 @@ typedef struct {
     double advect_seconds;
     double cg_seconds[5];
     int64_t cg_refinements[5];  /* FP64 reliable updates (refined precision) */
-    int64_t coarse_iterations;  /* CG iterations of the coarse-grid corrections */
-    double coarse_seconds;      /* their device time */
+    int64_t coarse_iterations;  /* CG iterations of the coarse-grid corrections
+                                   (V-cycles when multigrid levels are added) */
+    double coarse_seconds;      /* their time (device time of the CG launches;
+                                   wall time of the V-cycles) */
 } nsfem_step_stats;
 
 int nsfem_simulation_create(nsfem_device *dev, int node_count,
