@@ -135,3 +135,62 @@ def test_coarse_levels_chain():
     assert len(bench.coarse_levels(128, 2)) == 2
     with pytest.raises(ValueError):
         bench.coarse_levels(12, 3)  # 12 -> 6 -> 3 cannot halve again
+
+
+def test_vcycle_settings_on_the_oracle_pressure_matrix():
+    """The V-cycle the library runs on the coarse system (damped Jacobi,
+    omega 0.8, 2 sweeps before and after on each level, Galerkin levels
+    P^T A P of the P1 prolongations, the deepest level solved exactly),
+    restated in numpy on the oracle's 128^2 pressure matrix: from a random
+    right-hand side (every mode, harder than the smooth residuals the
+    correction restricts) it reaches a 1e-6 relative residual in at most 16
+    cycles (measured 13: about 0.4 per cycle) and then agrees with the direct
+    solve to the accuracy that residual allows.  Without the post-smoothing
+    it needs 20 cycles; a transposed transfer does not fit the level sizes."""
+    from oracle import oracle as O
+
+    n = 128
+    m = mesh.cavity(n)
+    asm = O.Assembly(m, 1e-3, 1e-3)
+    rs, col, val = asm.csr["pressure_stiffness"]
+    A = sp.csr_matrix((val, col, rs), shape=(len(rs) - 1,) * 2)
+
+    def prolong(k):
+        r, c, v, cnt = mesh.cavity_prolongation(k, 2)
+        return sp.csr_matrix((v, c, r), shape=((k + 1) ** 2, cnt))
+
+    P0 = prolong(n)
+    mats = [(P0.T @ A @ P0).tocsr()]
+    trans = []
+    k = n // 2
+    while k > 16:
+        P = prolong(k)
+        trans.append(P)
+        mats.append((P.T @ mats[-1] @ P).tocsr())
+        k //= 2
+    dinv = [1.0 / M.diagonal() for M in mats]
+    deepest = spl.factorized(mats[-1].tocsc())
+
+    def vcycle(level, b, x):
+        M = mats[level]
+        if level == len(mats) - 1:
+            return deepest(b)
+        for _ in range(2):
+            x = x + 0.8 * dinv[level] * (b - M @ x)
+        e = vcycle(level + 1, trans[level].T @ (b - M @ x), np.zeros(trans[level].shape[1]))
+        x = x + trans[level] @ e
+        for _ in range(2):
+            x = x + 0.8 * dinv[level] * (b - M @ x)
+        return x
+
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(mats[0].shape[0])
+    x = np.zeros_like(b)
+    cycles = 0
+    while np.linalg.norm(b - mats[0] @ x) > 1e-6 * np.linalg.norm(b):
+        x = vcycle(0, b, x)
+        cycles += 1
+        assert cycles <= 16
+    exact = spl.spsolve(mats[0].tocsc(), b)
+    assert np.linalg.norm(x - exact) <= 1e-3 * np.linalg.norm(exact)
+    assert [M.shape[0] for M in mats] == [65 ** 2, 33 ** 2, 17 ** 2]
