@@ -440,6 +440,17 @@ int nsfem_simulation_set_coarse_space(nsfem_simulation *sim, int coarse_count, i
 int nsfem_simulation_add_coarse_level(nsfem_simulation *sim, int count, int64_t nnz,
                                       const int32_t *row_start, const int32_t *col,
                                       const double *val);
+/* enable != 0: the pressure solve becomes CG preconditioned by one V-cycle
+ * over the fine pressure grid, the coarse space and its levels (damped
+ * Jacobi, 2 sweeps before and after on every level), in FP64 whatever the
+ * precision option, to the same relative residual, from the warm-start
+ * mode's guess (the coarse correction of the guess is left out: the
+ * preconditioner holds it).  Needs a coarse space; removing the coarse
+ * space switches it off; single-simulation stepping only (a batch holding
+ * such a simulation is rejected).  enable = 0: back to the plain CG.  No
+ * counterpart in the reference, whose pressure CG is unpreconditioned
+ * (include/assembly.h:608-615). */
+int nsfem_simulation_set_pressure_multigrid(nsfem_simulation *sim, int enable);
 /* Copies the last step's pressure right-hand side and pressure into device
  * buffers (either may be NULL): training pairs for the network. */
 int nsfem_simulation_copy_pressure_system(nsfem_simulation *sim, nsfem_buffer *rhs,

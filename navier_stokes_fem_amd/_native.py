@@ -161,6 +161,7 @@ _SIGNATURES = {
     "nsfem_simulation_set_coarse_space": ([vp, C.c_int, C.c_int64, vp, vp, vp, C.c_double],
                                           C.c_int),
     "nsfem_simulation_add_coarse_level": ([vp, C.c_int, C.c_int64, vp, vp, vp], C.c_int),
+    "nsfem_simulation_set_pressure_multigrid": ([vp, C.c_int], C.c_int),
     "nsfem_simulation_set_velocity_projection": ([vp, C.c_int], C.c_int),
     "nsfem_simulation_copy_pressure_system": ([vp, vp, vp], C.c_int),
     "nsfem_gnn_create": ([vp, C.c_uint64, PP], C.c_int),

@@ -113,6 +113,11 @@ struct Simulation {
     std::vector<std::unique_ptr<MgLevel>> mg;
     Matrix deep_inv;  // dense inverse of the deepest level's matrix (as CSR)
     bool deep_inv_ok = false;
+    // pressure CG preconditioned by one V-cycle over the fine grid, the
+    // coarse space and its levels: 1/diag of the pressure matrix, the CG
+    // vectors r, z, p, q, a smoothing residual t and a dot scratch
+    bool pmg = false;
+    DeviceArray<double> pdinv, pr, pz, pp, pq, pt, pdot;
     DeviceArray<double> cdinv, cres;
     std::vector<int32_t> deep_rs, deep_col;
     std::vector<double> deep_val;
@@ -580,6 +585,17 @@ void galerkin_product(int n, const std::vector<int32_t> &ars, const std::vector<
     }
 }
 
+void release_pressure_multigrid(Simulation &s) {
+    s.pmg = false;
+    s.pdinv.release();
+    s.pr.release();
+    s.pz.release();
+    s.pp.release();
+    s.pq.release();
+    s.pt.release();
+    s.pdot.release();
+}
+
 // 1/diag of a square host CSR, uploaded (a zero diagonal is rejected)
 void upload_inverse_diagonal(Device &d, int n, const std::vector<int32_t> &rs,
                              const std::vector<int32_t> &col, const std::vector<double> &val,
@@ -735,6 +751,102 @@ void coarse_correct(Simulation &s) {
     coarse_finish(s, status, st.iterations, st.seconds);
 }
 
+// z = M r: one V-cycle from zero over the fine pressure grid (damped Jacobi
+// as on the coarse levels), the coarse space and its levels.  As many sweeps
+// after as before, restriction = transposed prolongation and the deepest
+// level solved exactly (or by CG to 1e-10) keep M symmetric, as
+// preconditioned CG needs.
+void fine_vcycle(Simulation &s, const double *r, double *z) {
+    Device &d = *s.dev;
+    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+    const int n = s.P;
+    launch_fill(d, n, 0.0, z);
+    for (int k = 0; k < kMgSweeps; ++k) {
+        launch_spmv(d, A, z, r, s.pt.ptr, 1);
+        launch_jacobi(d, n, kMgOmega, s.pdinv.ptr, s.pt.ptr, z);
+    }
+    launch_spmv(d, A, z, r, s.pt.ptr, 1);
+    launch_spmv(d, s.cPT, s.pt.ptr, nullptr, s.cr.ptr, 0);
+    launch_fill(d, s.nc, 0.0, s.ce.ptr);
+    vcycle(s, 0, s.cr.ptr, s.ce.ptr);
+    launch_spmv(d, s.cP, s.ce.ptr, nullptr, s.pt.ptr, 0);
+    launch_add(d, n, s.pt.ptr, z);
+    for (int k = 0; k < kMgSweeps; ++k) {
+        launch_spmv(d, A, z, r, s.pt.ptr, 1);
+        launch_jacobi(d, n, kMgOmega, s.pdinv.ptr, s.pt.ptr, z);
+    }
+}
+
+// a . b over n entries (one host synchronisation)
+double vec_dot(Simulation &s, int n, const double *a, const double *b) {
+    Device &d = *s.dev;
+    double h = 0.0;
+    launch_fill(d, 1, 0.0, s.pdot.ptr);
+    launch_dot_accumulate(d, n, a, b, s.pdot.ptr);
+    NSFEM_HIP(hipMemcpyAsync(&h, s.pdot.ptr, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    return h;
+}
+
+// The pressure solve as CG preconditioned by fine_vcycle, in FP64, from the
+// guess in p to ||b - A x|| <= rtol ||b||; convergence is decided on a
+// recomputed b - A x, which then replaces the recursive residual.
+void pcg_pressure(Simulation &s, SolveOut &o) {
+    Device &d = *s.dev;
+    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+    const int n = s.P;
+    const auto t0 = std::chrono::steady_clock::now();
+    double *x = s.p.ptr, *r = s.pr.ptr, *z = s.pz.ptr, *p = s.pp.ptr, *q = s.pq.ptr;
+    const double *b = s.prhs.ptr;
+    const double bb = vec_dot(s, n, b, b);
+    launch_spmv(d, A, x, b, r, 1);
+    double rr = vec_dot(s, n, r, r);
+    const double ref = bb > 0.0 ? bb : rr;
+    const double tol2 = s.opt.rtol * s.opt.rtol * ref;
+    const int64_t max_iter = s.opt.max_iterations > 0 ? s.opt.max_iterations : n;
+    int64_t k = 0;
+    int status = NSFEM_OK;
+    if (rr > tol2) {
+        status = NSFEM_MAX_ITERATIONS_REACHED;
+        fine_vcycle(s, r, z);
+        launch_copy(d, n, z, p);
+        double rz = vec_dot(s, n, r, z);
+        while (k < max_iter) {
+            launch_spmv(d, A, p, nullptr, q, 0);
+            const double pq = vec_dot(s, n, p, q);
+            if (!(pq > 0.0) || !(rz > 0.0)) {
+                status = NSFEM_BREAKDOWN;
+                break;
+            }
+            const double alpha = rz / pq;
+            launch_axpy(d, n, alpha, p, x);
+            launch_axpy(d, n, -alpha, q, r);
+            ++k;
+            rr = vec_dot(s, n, r, r);
+            if (rr <= tol2) {
+                launch_spmv(d, A, x, b, r, 1);
+                rr = vec_dot(s, n, r, r);
+                if (rr <= tol2) {
+                    status = NSFEM_OK;
+                    break;
+                }
+            }
+            fine_vcycle(s, r, z);
+            const double rz_new = vec_dot(s, n, r, z);
+            const double beta = rz_new / rz;
+            rz = rz_new;
+            launch_axpby(d, n, 1.0, beta, z, p, p);
+        }
+    }
+    if (status != NSFEM_OK)
+        fail(status, "V-cycle preconditioned CG on the pressure stiffness matrix did not "
+                     "converge (status " + std::to_string(status) + ")");
+    o.iters[0] = k;
+    o.refines[0] = 0;
+    o.rel[0] = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    o.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 }  // namespace
 
 void simulation_set_velocity_projection(Simulation &s, int depth) {
@@ -769,6 +881,7 @@ void simulation_set_coarse_space(Simulation &s, int nc, int64_t nnz, const int32
         s.deep_inv_ok = false;
         s.cdinv.release();
         s.cres.release();
+        release_pressure_multigrid(s);
         s.deep_rs.clear();
         s.deep_col.clear();
         s.deep_val.clear();
@@ -902,6 +1015,37 @@ void simulation_add_coarse_level(Simulation &s, int count, int64_t nnz, const in
     deep_dense_inverse(s);
 }
 
+// Switches the pressure solve to CG preconditioned by one V-cycle over the
+// fine grid, the coarse space and its levels (FP64; the coarse correction of
+// the guess is then left out: the preconditioner holds it).
+void simulation_set_pressure_multigrid(Simulation &s, int enable) {
+    Device &d = *s.dev;
+    d.set_current();
+    if (!enable) {
+        release_pressure_multigrid(s);
+        return;
+    }
+    NSFEM_REQUIRE(s.has_coarse, NSFEM_INVALID_ARGUMENT,
+                  "pressure multigrid: set a coarse space (and its levels) first");
+    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+    std::vector<int32_t> acol((size_t)A.nnz);
+    std::vector<double> aval((size_t)A.nnz);
+    if (A.nnz) {
+        NSFEM_HIP(hipMemcpy(acol.data(), A.col.ptr, acol.size() * sizeof(int32_t),
+                            hipMemcpyDeviceToHost));
+        NSFEM_HIP(hipMemcpy(aval.data(), A.val.ptr, aval.size() * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+    upload_inverse_diagonal(d, s.P, A.host_rs, acol, aval, s.pdinv);
+    s.pr.resize(s.P);
+    s.pz.resize(s.P);
+    s.pp.resize(s.P);
+    s.pq.resize(s.P);
+    s.pt.resize(s.P);
+    s.pdot.resize(1);
+    s.pmg = true;
+}
+
 void pressure_guess_base(Simulation &s);
 
 // the mode's guess, then the coarse-grid correction when a coarse space is set
@@ -909,7 +1053,7 @@ void pressure_guess(Simulation &s) {
     pressure_guess_base(s);
     s.coarse_iters = 0;
     s.coarse_seconds = 0.0;
-    if (s.has_coarse) coarse_correct(s);
+    if (s.has_coarse && !s.pmg) coarse_correct(s);  // the preconditioner holds it
 }
 
 void pressure_guess_base(Simulation &s) {
@@ -1121,7 +1265,10 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         pressure_guess(s);
         const int64_t coarse_iters = s.coarse_iters;
         const double coarse_seconds = s.coarse_seconds;
-        solve(s, NSFEM_PRESSURE_STIFFNESS, s.prhs.ptr, s.p.ptr, 1, o[2]);
+        if (s.pmg)
+            pcg_pressure(s, o[2]);
+        else
+            solve(s, NSFEM_PRESSURE_STIFFNESS, s.prhs.ptr, s.p.ptr, 1, o[2]);
         pressure_after_solve(s);
         // 4. velocity correction
         launch_spmv(d, PD, s.p.ptr, nullptr, s.rhs.ptr, 0);
@@ -1185,6 +1332,8 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
                       "a batch needs simulations of one device");
         NSFEM_REQUIRE(s->opt.precision == S[0]->opt.precision, NSFEM_INVALID_ARGUMENT,
                       "a batch needs one precision");
+        NSFEM_REQUIRE(!s->pmg, NSFEM_INVALID_ARGUMENT,
+                      "the V-cycle preconditioned pressure solve steps one simulation at a time");
     }
     d.set_current();
     const size_t I = S.size();
@@ -1484,6 +1633,7 @@ void simulation_detach(Simulation *s) {
     s->mg.clear();
     matrix_release(s->deep_inv);
     s->deep_inv_ok = false;
+    release_pressure_multigrid(*s);
     s->cdinv.release();
     s->cres.release();
     s->cy.release();
@@ -1630,6 +1780,15 @@ int nsfem_simulation_add_coarse_level(nsfem_simulation *s, int count, int64_t nn
                       "null simulation or destroyed device");
         std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
         simulation_add_coarse_level(*s->impl, count, nnz, row_start, col, val);
+    });
+}
+
+int nsfem_simulation_set_pressure_multigrid(nsfem_simulation *s, int enable) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_set_pressure_multigrid(*s->impl, enable);
     });
 }
 

@@ -161,6 +161,12 @@ class Simulation:
                                                           N.ptr(rs), N.ptr(col), N.ptr(val)))
         sizes.append(int(nc))
 
+    def set_pressure_multigrid(self, enable: bool = True):
+        """Solve the pressure by CG preconditioned with one V-cycle over the
+        fine grid, the coarse space and its levels (FP64; needs a coarse
+        space; single-simulation stepping only)."""
+        N.check(N.lib().nsfem_simulation_set_pressure_multigrid(self.handle, int(bool(enable))))
+
     def copy_pressure_system(self, rhs=None, p=None):
         """Copy the last step's pressure right-hand side and solution into
         device buffers (``DeviceBuffer`` of pressure-count doubles)."""

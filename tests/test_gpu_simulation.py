@@ -178,6 +178,42 @@ def test_multigrid_coarse_levels(gpu, grids):
     assert 0 < s1["coarse_iterations"] < 0.5 * s0["coarse_iterations"]
 
 
+@pytest.mark.skip(reason="V-cycle preconditioned pressure CG: built, not yet run on a GPU")
+def test_vcycle_preconditioned_pressure_cg(gpu):
+    """The pressure solve as CG preconditioned by one V-cycle (128^2 cavity,
+    coarse space from 64^2, levels 32^2 and 16^2): every solve meets the
+    tolerance, the fields match the plain CG run to 1e-6, the pressure
+    solves take at most 15 iterations a step (the CPU premise: 7-9), no
+    coarse correction runs, and a batch holding the simulation is rejected;
+    switching it off restores the plain solve."""
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation, step_batch
+
+    m = mesh.cavity(128)
+    out = {}
+    for pmg in (False, True):
+        sim = Simulation(m, 1e-3, 1e-3, device=gpu, rtol=1e-8, precision="fp64")
+        if pmg:
+            sim.set_coarse_space(mesh.cavity_prolongation(128, 2), rtol=1e-6)
+            for n in (64, 32):
+                sim.add_coarse_level(mesh.cavity_prolongation(n, 2))
+            sim.set_pressure_multigrid(True)
+        st = sim.step(8)
+        out[pmg] = (st, sim.fields())
+        if pmg:
+            with pytest.raises(Exception):
+                step_batch([sim], 1)
+            sim.set_pressure_multigrid(False)
+            assert sim.step(1)["cg_iterations"][2] > 15
+        sim.close()
+    (s0, f0), (s1, f1) = out[False], out[True]
+    print(f"pressure iterations plain {s0['cg_iterations'][2]}, V-cycle PCG {s1['cg_iterations'][2]}")
+    assert max(s1["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+    for a, b in zip(f1, f0):
+        assert _rel(a, b) < 1e-6
+    assert s1["cg_iterations"][2] <= 15 * 8 and s1["coarse_iterations"] == 0
+
+
 def _errors_vs(fields, truth):
     return [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fields, truth)]
 
