@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "nsfem_internal.hpp"
+#include "spd_inverse.hpp"
 
 namespace nsfem {
 
@@ -941,36 +942,9 @@ void deep_dense_inverse(Simulation &s) {
     s.deep_inv_ok = false;
     const int n = s.mg.back()->n;
     if (n > kMgDenseMax) return;
-    std::vector<double> L((size_t)n * n, 0.0);  // row-major lower triangle
-    for (int r = 0; r < n; ++r)
-        for (int32_t k = s.deep_rs[r]; k < s.deep_rs[r + 1]; ++k)
-            if (s.deep_col[k] <= r) L[(size_t)r * n + s.deep_col[k]] = s.deep_val[k];
-    for (int j = 0; j < n; ++j) {
-        double dj = L[(size_t)j * n + j];
-        for (int k = 0; k < j; ++k) dj -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
-        if (!(dj > 0.0)) return;  // not positive definite: the deepest level keeps CG
-        dj = std::sqrt(dj);
-        L[(size_t)j * n + j] = dj;
-        for (int i = j + 1; i < n; ++i) {
-            double v = L[(size_t)i * n + j];
-            for (int k = 0; k < j; ++k) v -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
-            L[(size_t)i * n + j] = v / dj;
-        }
-    }
-    // column c of the inverse: L y = e_c, then L^T x = y
-    std::vector<double> inv((size_t)n * n), y((size_t)n);
-    for (int c = 0; c < n; ++c) {
-        for (int i = 0; i < n; ++i) {
-            double v = i == c ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) v -= L[(size_t)i * n + k] * y[k];
-            y[i] = v / L[(size_t)i * n + i];
-        }
-        for (int i = n - 1; i >= 0; --i) {
-            double v = y[i];
-            for (int k = i + 1; k < n; ++k) v -= L[(size_t)k * n + i] * inv[(size_t)k * n + c];
-            inv[(size_t)i * n + c] = v / L[(size_t)i * n + i];
-        }
-    }
+    std::vector<double> inv;
+    if (!spd_inverse(n, s.deep_rs.data(), s.deep_col.data(), s.deep_val.data(), inv))
+        return;  // not positive definite: the deepest level keeps CG
     std::vector<int32_t> rs((size_t)n + 1), col((size_t)n * n);
     for (int r = 0; r <= n; ++r) rs[r] = r * n;
     for (int r = 0; r < n; ++r)
