@@ -858,13 +858,26 @@ void or_impose_velocity_bc(int nvb, const int32_t *vb, const double *gu,
 /* Dirichlet right-hand-side patch: rhs[b] = g; rhs[c] -= W[b][c]*g for the
  * weights row of b (include/assembly.h:503-524, :583-603). */
 static void bc_patch(const or_csr *W, int nb, const int32_t *bn,
-                     const double *g, double *rhs) {
+                     const double *g, double *rhs, double sign) {
     for (int k = 0; k < nb; ++k) {
         const int b = bn[k];
         rhs[b] = g[k];
         for (int32_t i = W->row_start[b]; i < W->row_start[b + 1]; ++i)
-            rhs[W->col[i]] -= W->val[i] * g[k];
+            rhs[W->col[i]] -= sign * (W->val[i] * g[k]);
     }
+}
+
+/* planted mistake of the composition tests (OR_FAULT_*), 0 = none */
+static int g_step_fault = OR_FAULT_NONE;
+
+void or_set_step_fault(int fault) {
+    g_step_fault = (fault > OR_FAULT_NONE && fault < OR_FAULT_COUNT) ? fault : OR_FAULT_NONE;
+}
+
+int or_get_step_fault(void) { return g_step_fault; }
+
+static void scale_vec(int64_t n, double a, double *x) {
+    for (int64_t i = 0; i < n; ++i) x[i] *= a;
 }
 
 static double rel_res(const or_cg_result *r) {
@@ -879,6 +892,7 @@ int or_step(const or_matrices *m, const uint32_t *kd_nodes,
             const double *gv, int npb, const int32_t *pb, const double *gp,
             double dt, double rtol, int max_iter, double *u, double *v,
             double *p, or_step_stats *st) {
+    const int fault = g_step_fault;
     double *ut = (double *)malloc(sizeof(double) * 2 * (size_t)V);
     double *vt = ut + V;
     double *rhs = (double *)malloc(sizeof(double) * 2 * (size_t)V);
@@ -887,32 +901,42 @@ int or_step(const or_matrices *m, const uint32_t *kd_nodes,
     double *uv = (double *)malloc(sizeof(double) * 2 * (size_t)V);
     or_cg_result res;
     int status = 0, rc;
+    const double vel_bc_sign = fault == OR_FAULT_VEL_BC_SIGN ? -1.0 : 1.0;
+    const double p_bc_sign = fault == OR_FAULT_P_BC_SIGN ? -1.0 : 1.0;
     memset(st, 0, sizeof(*st));
 
-    or_advect_f64(kd_nodes, leaf, xy, el, V, u, v, dt, ut, vt, 0);
+    if (fault == OR_FAULT_SKIP_ADVECTION) {
+        memcpy(ut, u, sizeof(double) * (size_t)V);
+        memcpy(vt, v, sizeof(double) * (size_t)V);
+    } else {
+        or_advect_f64(kd_nodes, leaf, xy, el, V, u, v, dt, ut, vt, 0);
+    }
 
     /* diffusion, u then v; x0 = advected velocity (:530-537) */
     or_spmv(V, m->velocity_mass.row_start, m->velocity_mass.col,
             m->velocity_mass.val, ut, rhs);
-    bc_patch(&m->velocity_weights, nvb, vb, gu, rhs);
+    bc_patch(&m->velocity_weights, nvb, vb, gu, rhs, vel_bc_sign);
     rc = or_cg(V, m->diffusion.row_start, m->diffusion.col, m->diffusion.val,
                rhs, ut, u, max_iter, rtol, 0.0, &res);
     st->cg_iterations[0] = res.iterations; st->cg_residual[0] = rel_res(&res);
     if (rc && !status) status = rc;
     or_spmv(V, m->velocity_mass.row_start, m->velocity_mass.col,
             m->velocity_mass.val, vt, rhs + V);
-    bc_patch(&m->velocity_weights, nvb, vb, gv, rhs + V);
+    bc_patch(&m->velocity_weights, nvb, vb, gv, rhs + V, vel_bc_sign);
     rc = or_cg(V, m->diffusion.row_start, m->diffusion.col, m->diffusion.val,
                rhs + V, vt, v, max_iter, rtol, 0.0, &res);
     st->cg_iterations[1] = res.iterations; st->cg_residual[1] = rel_res(&res);
     if (rc && !status) status = rc;
 
     /* pressure: rhs = velocityDivergence * [u; v] + BC; x0 = previous p */
-    memcpy(uv, u, sizeof(double) * (size_t)V);
-    memcpy(uv + V, v, sizeof(double) * (size_t)V);
+    const int swap = fault == OR_FAULT_SWAP_COMPONENTS;
+    memcpy(uv, swap ? v : u, sizeof(double) * (size_t)V);
+    memcpy(uv + V, swap ? u : v, sizeof(double) * (size_t)V);
     or_spmv(P, m->velocity_divergence.row_start, m->velocity_divergence.col,
             m->velocity_divergence.val, uv, prhs);
-    bc_patch(&m->pressure_weights, npb, pb, gp, prhs);
+    if (fault == OR_FAULT_DIV_SIGN) scale_vec(P, -1.0, prhs);
+    if (fault == OR_FAULT_DIV_SCALE) scale_vec(P, dt * dt, prhs);
+    bc_patch(&m->pressure_weights, npb, pb, gp, prhs, p_bc_sign);
     rc = or_cg(P, m->pressure_stiffness.row_start, m->pressure_stiffness.col,
                m->pressure_stiffness.val, prhs, p, p, max_iter, rtol, 0.0, &res);
     st->cg_iterations[2] = res.iterations; st->cg_residual[2] = rel_res(&res);
@@ -922,6 +946,8 @@ int or_step(const or_matrices *m, const uint32_t *kd_nodes,
      * x0 = velocity input; then velocityOut += velocityIn; BC (:631-708) */
     or_spmv(2 * V, m->pressure_divergence.row_start, m->pressure_divergence.col,
             m->pressure_divergence.val, p, rhs);
+    if (fault == OR_FAULT_GRAD_SIGN) scale_vec(2 * (int64_t)V, -1.0, rhs);
+    if (fault == OR_FAULT_GRAD_SCALE) scale_vec(2 * (int64_t)V, 1.0 / (dt * dt), rhs);
     rc = or_cg(V, m->velocity_mass.row_start, m->velocity_mass.col,
                m->velocity_mass.val, rhs, u, du, max_iter, rtol, 0.0, &res);
     st->cg_iterations[3] = res.iterations; st->cg_residual[3] = rel_res(&res);
@@ -930,9 +956,10 @@ int or_step(const or_matrices *m, const uint32_t *kd_nodes,
                m->velocity_mass.val, rhs + V, v, du + V, max_iter, rtol, 0.0, &res);
     st->cg_iterations[4] = res.iterations; st->cg_residual[4] = rel_res(&res);
     if (rc && !status) status = rc;
+    const double keep = fault == OR_FAULT_NO_UPDATE_ADD ? 0.0 : 1.0;
     for (int i = 0; i < V; ++i) {
-        u[i] = du[i] + u[i];
-        v[i] = du[V + i] + v[i];
+        u[i] = du[i] + keep * u[i];
+        v[i] = du[V + i] + keep * v[i];
     }
     or_impose_velocity_bc(nvb, vb, gu, gv, u, v);
     st->status = status;

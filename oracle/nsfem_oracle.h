@@ -37,8 +37,13 @@
  *   or_assemble : mass matrix row sums = element areas (closed form),
  *       stiffness annihilates constants and reproduces the discrete Laplacian
  *       of linear functions, divergence of a linear field, symmetry.
- *   or_step : whole-step invariants (Dirichlet values imposed, discrete
- *       divergence reduced by the projection) and GPU-vs-oracle fields.
+ *   or_step : the composition, by flows whose one-step outcome the
+ *       mathematics fixes (tests/step_cases.py): a uniform flow is a fixed
+ *       point; a potential flow is projected away with its pressure p dt =
+ *       phi - dt|u|^2/2, errors falling under refinement; a solid-body
+ *       rotation stays put with the centripetal pressure; first order in dt
+ *       (Runge estimate); mirror symmetry; exact Dirichlet values.  Every
+ *       planted mistake of or_set_step_fault fails at least one of them.
  */
 #ifndef NSFEM_ORACLE_H
 #define NSFEM_ORACLE_H
@@ -207,6 +212,26 @@ int or_step(const or_matrices *m, const uint32_t *kd_nodes,
             const int32_t *p_bc_nodes, const double *p_bc_values, double dt,
             double rtol, int max_iter, double *u, double *v, double *p,
             or_step_stats *stats);
+
+/* Planted mistakes for the tests of or_step's composition: each value makes
+ * or_step commit one plausible error, so a test can show that its checks
+ * fail on it (tests only; 0 restores the faithful step).  The setting is
+ * process-wide and not thread-safe. */
+enum {
+    OR_FAULT_NONE = 0,
+    OR_FAULT_DIV_SIGN,        /* pressure rhs from +(1/dt) B u instead of -(1/dt) B u */
+    OR_FAULT_GRAD_SIGN,       /* velocity correction from +dt G p instead of -dt G p */
+    OR_FAULT_VEL_BC_SIGN,     /* velocity Dirichlet patch adds W g instead of subtracting */
+    OR_FAULT_P_BC_SIGN,       /* pressure Dirichlet patch adds W g instead of subtracting */
+    OR_FAULT_DIV_SCALE,       /* divergence scaled by -dt instead of -1/dt */
+    OR_FAULT_GRAD_SCALE,      /* gradient scaled by -1/dt instead of -dt */
+    OR_FAULT_NO_UPDATE_ADD,   /* velocityOut += velocityIn left out */
+    OR_FAULT_SKIP_ADVECTION,  /* the diffusion starts from the unadvected velocity */
+    OR_FAULT_SWAP_COMPONENTS, /* the divergence reads [v; u] instead of [u; v] */
+    OR_FAULT_COUNT
+};
+void or_set_step_fault(int fault);
+int or_get_step_fault(void);
 
 /* imposeVelocityDirichlet (include/assembly.h:1559-1579). */
 void or_impose_velocity_bc(int vel_bc_count, const int32_t *vel_bc_nodes,

@@ -97,6 +97,8 @@ def lib():
                               C.c_int, _i32, _f64, _f64, C.c_int, _i32, _f64, C.c_double,
                               C.c_double, C.c_int, _f64, _f64, _f64, C.POINTER(StepStats)]
         L.or_impose_velocity_bc.argtypes = [C.c_int, _i32, _f64, _f64, _f64, _f64]
+        L.or_set_step_fault.argtypes = [C.c_int]
+        L.or_get_step_fault.restype = C.c_int
         _lib = L
     return _lib
 
@@ -321,6 +323,27 @@ def step(asm: Assembly, tree: Tree, mesh, dt, u, v, p, rtol=1e-8, max_iter=-1):
     stats = {"iterations": list(st.cg_iterations), "residual": list(st.cg_residual),
              "status": st.status}
     return u, v, p, stats
+
+
+# Planted mistakes of or_step (OR_FAULT_* of nsfem_oracle.h), for the tests
+# that show the composition checks catch them.
+STEP_FAULTS = ("div_sign", "grad_sign", "vel_bc_sign", "p_bc_sign", "div_scale",
+               "grad_scale", "no_update_add", "skip_advection", "swap_components")
+
+
+class step_fault:
+    """Context manager: or_step commits the named mistake inside the block."""
+
+    def __init__(self, name):
+        self.code = 0 if name in (None, "none") else STEP_FAULTS.index(name) + 1
+
+    def __enter__(self):
+        lib().or_set_step_fault(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_step_fault(0)
+        return False
 
 
 def initial_fields(mesh):
