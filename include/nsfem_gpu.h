@@ -319,7 +319,10 @@ int nsfem_simulation_reset(nsfem_simulation *sim);
  * together; each solve kind (u/v diffusion as 2 columns, pressure, u/v mass
  * as 2 columns) runs for all of them in one persistent launch.  stats:
  * `count` entries (nullable), accumulated; seconds and cg_seconds are the
- * whole batch's. */
+ * whole batch's.  A batch holding a simulation with the V-cycle
+ * preconditioned pressure solve or the velocity projection of the diffusion
+ * guesses is rejected (NSFEM_INVALID_ARGUMENT): both are single-simulation
+ * stepping only. */
 int nsfem_simulation_step_batch(nsfem_simulation *const *sims, int count, int steps,
                                 nsfem_step_stats *stats);
 typedef struct {
@@ -410,6 +413,13 @@ int nsfem_gnn_train_step(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
                                             the span of past pressures (Fischer);
                                             basis depth NSFEM_PROJECTION_DEPTH */
 int nsfem_simulation_set_warm_start(nsfem_simulation *sim, int mode, nsfem_gnn *gnn);
+/* Initial guess of the diffusion solves: the advected velocity (the
+ * reference's, include/assembly.h:1521-1531) plus the A-norm projection of
+ * the correction onto the span of the last `depth` corrections of each
+ * velocity column (Fischer's projection on the diffusion matrix); 0 = off.
+ * Costs 2 x (2 depth + 3) velocity vectors of device memory.  No
+ * counterpart in the reference. */
+int nsfem_simulation_set_velocity_projection(nsfem_simulation *sim, int depth);
 /* Coarse-grid correction of the pressure solve's initial guess, applied
  * after the warm-start mode's guess x0: with the prolongation P (one row per
  * pressure node, coarse_count columns, CSR with ascending columns) and
@@ -419,14 +429,10 @@ int nsfem_simulation_set_warm_start(nsfem_simulation *sim, int mode, nsfem_gnn *
  * leaves x0 unchanged).  The pressure CG then runs from it to the same
  * tolerance as from any guess.  coarse_count = 0 removes the coarse space.
  * No counterpart in the reference, whose pressure CG starts from the
- * previous pressure (include/assembly.h:608-615). */
-/* Initial guess of the diffusion solves: the advected velocity (the
- * reference's, include/assembly.h:1521-1531) plus the A-norm projection of
- * the correction onto the span of the last `depth` corrections of each
- * velocity column (Fischer's projection on the diffusion matrix); 0 = off.
- * Costs 2 x (2 depth + 3) velocity vectors of device memory.  No
- * counterpart in the reference. */
-int nsfem_simulation_set_velocity_projection(nsfem_simulation *sim, int depth);
+ * previous pressure (include/assembly.h:608-615).
+ * Setting or removing the coarse space drops its levels and switches the
+ * V-cycle preconditioner off; a call that fails part way (an invalid P, a
+ * zero coarse diagonal, device memory) leaves no coarse space. */
 int nsfem_simulation_set_coarse_space(nsfem_simulation *sim, int coarse_count, int64_t nnz,
                                       const int32_t *row_start, const int32_t *col,
                                       const double *val, double coarse_rtol);

@@ -444,6 +444,12 @@ __global__ void k_scatter(int count, const int32_t *__restrict__ idx,
     if (i < count) dst[idx[i]] = val[i];
 }
 
+__global__ void k_scatter_value(int count, const int32_t *__restrict__ idx, double v,
+                                double *__restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[idx[i]] = v;
+}
+
 __global__ void k_add(int n, const double *__restrict__ a, double *__restrict__ y) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -536,6 +542,12 @@ void launch_dot_accumulate(Device &d, int n, const double *a, const double *b,
 void launch_scatter(Device &d, int count, const int32_t *idx, const double *val, double *dst) {
     if (count <= 0) return;
     k_scatter<<<(count + kBlock - 1) / kBlock, kBlock, 0, d.stream>>>(count, idx, val, dst);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_scatter_value(Device &d, int count, const int32_t *idx, double v, double *dst) {
+    if (count <= 0) return;
+    k_scatter_value<<<(count + kBlock - 1) / kBlock, kBlock, 0, d.stream>>>(count, idx, v, dst);
     NSFEM_HIP(hipGetLastError());
 }
 
@@ -714,7 +726,9 @@ __device__ __forceinline__ double ld_relaxed(const double *p) {
 // threads poll with relaxed agent-scope loads, which read the coherence
 // point without touching the caches, and perform one acquire fence after the
 // wait (a single invalidate per block), before the block reads data other
-// blocks wrote.
+// blocks wrote.  The last arrival at a counter (the group's, then the top
+// one) reads the counter's release sequence and performs an acquire fence
+// before it adds the other blocks' partials.
 template <int K, int BS>
 __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *lds,
                          unsigned long long timeout, const int b, const int nb) {
@@ -734,7 +748,13 @@ __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *
         for (int k = 0; k < K; ++k) S->partial[k][b] = v[k];
         const unsigned prev = __hip_atomic_fetch_add(&S->arrive_group[g][0], 1u, __ATOMIC_RELEASE,
                                                      __HIP_MEMORY_SCOPE_AGENT);
-        s_group_last = (prev == epoch * (unsigned)gsize - 1u) ? 1 : 0;
+        const bool last = prev == epoch * (unsigned)gsize - 1u;
+        // the last arrival read the counter's release sequence (every
+        // member's release RMW): an acquire fence after it synchronises with
+        // all of them before the block reads their partials (the block
+        // barrier below carries the order to the reading threads)
+        if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_group_last = last ? 1 : 0;
     }
     __syncthreads();
     if (s_group_last) {
@@ -754,7 +774,9 @@ __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *
                                    __HIP_MEMORY_SCOPE_AGENT);
             const unsigned prev = __hip_atomic_fetch_add(&S->arrive_top, 1u, __ATOMIC_RELEASE,
                                                          __HIP_MEMORY_SCOPE_AGENT);
-            s_top_last = (prev == epoch * (unsigned)ng - 1u) ? 1 : 0;
+            const bool last = prev == epoch * (unsigned)ng - 1u;
+            if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // as above
+            s_top_last = last ? 1 : 0;
         }
         __syncthreads();
         if (s_top_last) {

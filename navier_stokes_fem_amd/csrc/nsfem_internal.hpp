@@ -276,6 +276,8 @@ void build_tri_xy(Device &d, KdTreeGpu &t);
 // elementwise helpers for the step driver
 void launch_scatter(Device &d, int count, const int32_t *idx, const double *val,
                     double *dst);
+void launch_scatter_value(Device &d, int count, const int32_t *idx, double v,
+                          double *dst);  // dst[idx[i]] = v
 void launch_add(Device &d, int n, const double *a, double *y);  // y += a
 // x += w * dinv .* r
 void launch_jacobi(Device &d, int n, double w, const double *dinv, const double *r, double *x);

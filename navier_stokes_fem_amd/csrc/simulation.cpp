@@ -756,7 +756,9 @@ void coarse_correct(Simulation &s) {
 // as on the coarse levels), the coarse space and its levels.  As many sweeps
 // after as before, restriction = transposed prolongation and the deepest
 // level solved exactly (or by CG to 1e-10) keep M symmetric, as
-// preconditioned CG needs.
+// preconditioned CG needs.  z is zeroed on the pressure Dirichlet rows
+// (D M D with D the projection off them stays symmetric; the solve holds
+// those rows at their values).
 void fine_vcycle(Simulation &s, const double *r, double *z) {
     Device &d = *s.dev;
     Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
@@ -776,6 +778,7 @@ void fine_vcycle(Simulation &s, const double *r, double *z) {
         launch_spmv(d, A, z, r, s.pt.ptr, 1);
         launch_jacobi(d, n, kMgOmega, s.pdinv.ptr, s.pt.ptr, z);
     }
+    launch_scatter_value(d, s.npb, s.pb_idx.ptr, 0.0, z);
 }
 
 // a . b over n entries (one host synchronisation)
@@ -799,6 +802,9 @@ void pcg_pressure(Simulation &s, SolveOut &o) {
     const auto t0 = std::chrono::steady_clock::now();
     double *x = s.p.ptr, *r = s.pr.ptr, *z = s.pz.ptr, *p = s.pp.ptr, *q = s.pq.ptr;
     const double *b = s.prhs.ptr;
+    // the Dirichlet rows (identity rows) start at their values, so their
+    // residual is zero and z, p stay zero there
+    launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, x);
     const double bb = vec_dot(s, n, b, b);
     launch_spmv(d, A, x, b, r, 1);
     double rr = vec_dot(s, n, r, r);
@@ -865,39 +871,60 @@ void simulation_set_velocity_projection(Simulation &s, int depth) {
     s.vdel.resize(2 * (size_t)s.V);
 }
 
+namespace {
+// Drops the coarse space, its levels and the V-cycle preconditioner.
+void release_coarse_space(Simulation &s) {
+    matrix_release(s.cP);
+    matrix_release(s.cPT);
+    matrix_release(s.cA);
+    s.cy.release();
+    s.cr.release();
+    s.ce.release();
+    s.cproj.release();
+    s.mg.clear();
+    matrix_release(s.deep_inv);
+    s.deep_inv_ok = false;
+    s.cdinv.release();
+    s.cres.release();
+    release_pressure_multigrid(s);
+    s.deep_rs.clear();
+    s.deep_col.clear();
+    s.deep_val.clear();
+    s.has_coarse = false;
+    s.nc = 0;
+}
+
+void build_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
+                        const int32_t *col, const double *val, double rtol);
+}  // namespace
+
+// Sets (nc > 0) or removes (nc <= 0) the coarse space.  A failure part way
+// leaves no coarse space at all (and no V-cycle preconditioner), never a
+// half-built one.
 void simulation_set_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
                                  const int32_t *col, const double *val, double rtol) {
     Device &d = *s.dev;
     d.set_current();
-    if (nc <= 0) {
-        matrix_release(s.cP);
-        matrix_release(s.cPT);
-        matrix_release(s.cA);
-        s.cy.release();
-        s.cr.release();
-        s.ce.release();
-        s.cproj.release();
-        s.mg.clear();
-        matrix_release(s.deep_inv);
-        s.deep_inv_ok = false;
-        s.cdinv.release();
-        s.cres.release();
-        release_pressure_multigrid(s);
-        s.deep_rs.clear();
-        s.deep_col.clear();
-        s.deep_val.clear();
-        s.has_coarse = false;
-        s.nc = 0;
-        return;
+    if (nc > 0) {
+        NSFEM_REQUIRE(nc <= s.P && nnz > 0 && rs && col && val, NSFEM_INVALID_ARGUMENT,
+                      "coarse space: need 0 < coarse count <= pressure count and a non-empty P");
+        NSFEM_REQUIRE(rtol > 0.0 && rtol < 1.0, NSFEM_INVALID_ARGUMENT,
+                      "coarse space: the coarse tolerance must lie in (0, 1)");
     }
-    NSFEM_REQUIRE(nc <= s.P && nnz > 0 && rs && col && val, NSFEM_INVALID_ARGUMENT,
-                  "coarse space: need 0 < coarse count <= pressure count and a non-empty P");
-    NSFEM_REQUIRE(rtol > 0.0 && rtol < 1.0, NSFEM_INVALID_ARGUMENT,
-                  "coarse space: the coarse tolerance must lie in (0, 1)");
-    s.has_coarse = false;
-    s.mg.clear();
-    matrix_release(s.deep_inv);
-    s.deep_inv_ok = false;
+    release_coarse_space(s);
+    if (nc <= 0) return;
+    try {
+        build_coarse_space(s, nc, nnz, rs, col, val, rtol);
+    } catch (...) {
+        release_coarse_space(s);
+        throw;
+    }
+}
+
+namespace {
+void build_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
+                        const int32_t *col, const double *val, double rtol) {
+    Device &d = *s.dev;
     upload_csr(d, s.cP, s.P, nc, nnz, rs, col, val);  // validates P
     Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
     std::vector<int32_t> acol((size_t)A.nnz);
@@ -932,6 +959,7 @@ void simulation_set_coarse_space(Simulation &s, int nc, int64_t nnz, const int32
     s.coarse_seconds = 0.0;
     s.has_coarse = true;
 }
+}  // namespace
 
 // Levels of at most this many nodes are solved by a dense inverse (formed
 // on the host by Cholesky) applied as a dense CSR product; larger ones by CG.
@@ -1239,7 +1267,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         pressure_guess(s);
         const int64_t coarse_iters = s.coarse_iters;
         const double coarse_seconds = s.coarse_seconds;
-        if (s.pmg)
+        if (s.pmg && s.has_coarse)
             pcg_pressure(s, o[2]);
         else
             solve(s, NSFEM_PRESSURE_STIFFNESS, s.prhs.ptr, s.p.ptr, 1, o[2]);
@@ -1308,6 +1336,9 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
                       "a batch needs one precision");
         NSFEM_REQUIRE(!s->pmg, NSFEM_INVALID_ARGUMENT,
                       "the V-cycle preconditioned pressure solve steps one simulation at a time");
+        NSFEM_REQUIRE(s->vproj[0].max == 0, NSFEM_INVALID_ARGUMENT,
+                      "the velocity projection of the diffusion guesses steps one simulation "
+                      "at a time");
     }
     d.set_current();
     const size_t I = S.size();

@@ -1,0 +1,265 @@
+"""The benchmark configurations at their configured sizes, on the GPU.
+
+* C1 -- 32^2 lid-driven cavity, nu = 0.01, dt = 0.01, 100 steps: every step
+  against the oracle's (or_step, include/assembly.h:1516-1554).
+* C2 -- 512x128 channel past the disk (seed 2), nu = 1e-3, dt = 5e-4: 5 steps
+  against the oracle, then 200 steps in FP64 and in FP32 + FP64 refinement,
+  checked by residuals (every solve, and the last pressure system
+  recomputed on the host) and by the per-field difference of the two forms.
+* C3 -- 64 cavities, n ~ U{16..256} (seed 3), 50 steps in one batch per solve
+  kind: the configured run (FP32 + refinement, autotuned, 1e-8) checked by
+  residuals, and a tight FP64 run of the same batch whose smallest instance
+  matches the oracle and two others match one-by-one stepping.
+* C4 -- 2048^2 cavity (8.4M triangles): 3 steps of the plain FP64 method and
+  of the chosen form, every solve's residual and the last pressure system
+  recomputed on the host with the oracle's product; departure points of
+  10^5 velocity nodes located on the 2048^2 tree (a leaf index array past
+  2^24 entries) checked for containment by brute force and against the
+  oracle's traversal, and the count of points the reference's float leaf
+  offset (gpu_cpu_shared/kd_tree_common.cuh:148-150) would place elsewhere
+  reported.
+
+Field tolerances: two solves stopped at the same relative residual differ
+by up to cond(A) times it, so field comparisons use tight solves (1e-10 and
+tighter); the configured 1e-8 runs are checked through their residuals.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel_max(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _oracle_steps(O, m, nu, dt, steps, rtol, u=None, v=None, p=None):
+    asm = O.Assembly(m, nu, dt)
+    tree = O.kd_build(m.xy(), m.elems())
+    if u is None:
+        u, v, p = O.initial_fields(m)
+    for _ in range(steps):
+        u, v, p, st = O.step(asm, tree, m, dt, u, v, p, rtol=rtol)
+        assert st["status"] == 0
+    return u, v, p, asm
+
+
+def _pressure_residual(gpu, sim, asm, O):
+    """||b - A p|| / ||b|| of the simulation's last pressure system, with the
+    system downloaded and the product taken by the oracle on its own
+    assembly of the matrix."""
+    P = sim.mesh.pressure_count
+    rb = gpu.buffer(nbytes=8 * P)
+    pb = gpu.buffer(nbytes=8 * P)
+    sim.copy_pressure_system(rb, pb)
+    b = rb.download()
+    p = pb.download()
+    rb.close()
+    pb.close()
+    K = asm.csr["pressure_stiffness"]
+    r = O.spmv_sub(K, b, p)
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+def test_c1_cavity32_100_steps(gpu, oracle, precision):
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    m = mesh.cavity(32)
+    print(f"C1 cavity32 100 steps {precision}", flush=True)
+    # the configured tolerance: every solve's residual
+    sim = Simulation(m, 0.01, 0.01, device=gpu, rtol=1e-8, precision=precision)
+    st = sim.step(100)
+    f8 = sim.fields()
+    sim.close()
+    assert st["steps"] == 100
+    assert max(st["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+    # parity with the oracle's steps, both solved tightly
+    sim = Simulation(m, 0.01, 0.01, device=gpu, rtol=1e-11, precision=precision)
+    sim.step(100)
+    u, v, p = sim.fields()
+    sim.close()
+    ou, ov, op, _ = _oracle_steps(oracle, m, 0.01, 0.01, 100, 1e-11)
+    err = [_rel_max(u, ou), _rel_max(v, ov), _rel_max(p, op)]
+    err8 = [_rel_l2(a, b) for a, b in zip(f8, (ou, ov, op))]
+    print(f"C1 {precision}: 100 steps, max-norm error vs oracle (1e-11 solves) {err}; "
+          f"rel L2 of the 1e-8 run vs the oracle's 1e-11 fields {err8}")
+    assert max(err) < 1e-7
+    assert max(err8) < 1e-4
+
+
+def test_c2_channel512_against_the_oracle(gpu, oracle):
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    m = mesh.channel(512, 128, seed=2)
+    print("C2 channel512x128 5 steps against the oracle", flush=True)
+    sim = Simulation(m, 1e-3, 5e-4, device=gpu, rtol=1e-10, precision="fp64")
+    st = sim.step(5)
+    u, v, p = sim.fields()
+    sim.close()
+    ou, ov, op, _ = _oracle_steps(oracle, m, 1e-3, 5e-4, 5, 1e-10)
+    err = [_rel_max(u, ou), _rel_max(v, ov), _rel_max(p, op)]
+    print(f"C2: 5 steps, iterations {st['cg_iterations']}, max-norm error vs oracle {err}")
+    assert max(st["max_relative_residual"]) <= 1e-10 * (1 + 1e-6)
+    assert max(err) < 1e-6
+
+
+def test_c2_channel512_200_steps(gpu, oracle):
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    m = mesh.channel(512, 128, seed=2)
+    asm = oracle.Assembly(m, 1e-3, 5e-4)
+    out = {}
+    for precision in ("fp64", "fp32_refined"):
+        print(f"C2 channel512x128 200 steps {precision}", flush=True)
+        sim = Simulation(m, 1e-3, 5e-4, device=gpu, rtol=1e-8, precision=precision)
+        t0 = time.time()
+        st = sim.step(200)
+        dt_wall = time.time() - t0
+        res_p = _pressure_residual(gpu, sim, asm, oracle)
+        out[precision] = sim.fields()
+        sim.close()
+        print(f"C2 {precision}: {200 / dt_wall:.1f} steps/s, iterations {st['cg_iterations']}, "
+              f"max relative residuals {st['max_relative_residual']}, last pressure system "
+              f"recomputed on the host {res_p:.3e}")
+        assert st["steps"] == 200
+        assert max(st["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+        assert res_p <= 1e-8 * (1 + 1e-6)
+        assert all(np.isfinite(a).all() for a in out[precision])
+    diff = {k: _rel_l2(a, b) for k, a, b in zip("uvp", out["fp32_refined"], out["fp64"])}
+    print(f"C2: FP32+refinement vs FP64 after 200 steps, rel L2 per field {diff}")
+    assert max(diff.values()) < 1e-2
+
+
+def test_c3_cavity_batch64_50_steps(gpu, oracle):
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation, autotune_batch, step_batch
+
+    specs = mesh.cavity_batch(64, seed=3)
+    print("C3 64 cavities n in 16..256, 50 steps: configured run", flush=True)
+    sims = [Simulation(m, nu, 0.005, device=gpu, rtol=1e-8, precision="fp32_refined")
+            for m, nu in specs]
+    tune = autotune_batch(sims)
+    t0 = time.time()
+    stats = step_batch(sims, 50)
+    wall = time.time() - t0
+    worst = max(max(s["max_relative_residual"]) for s in stats)
+    print(f"C3 configured: tuning {tune}, {50 / wall:.2f} batch-steps/s, worst residual {worst:.3e}")
+    assert all(s["steps"] == 50 for s in stats)
+    assert worst <= 1e-8 * (1 + 1e-6)
+    for s in sims:
+        u, v, p = s.fields()
+        assert np.isfinite(u).all() and np.isfinite(v).all() and np.isfinite(p).all()
+        s.close()
+
+    print("C3 64 cavities, 50 steps: tight FP64 batch against the oracle and one-by-one",
+          flush=True)
+    rtol = 1e-10
+    sims = [Simulation(m, nu, 0.005, device=gpu, rtol=rtol, precision="fp64") for m, nu in specs]
+    stats = step_batch(sims, 50)
+    assert max(max(s["max_relative_residual"]) for s in stats) <= rtol * (1 + 1e-6)
+    sizes = [m.info["n"] for m, _ in specs]
+    order = np.argsort(sizes, kind="stable")
+    small = int(order[0])
+    m, nu = specs[small]
+    ou, ov, op, _ = _oracle_steps(oracle, m, nu, 0.005, 50, rtol)
+    u, v, p = sims[small].fields()
+    err = [_rel_max(u, ou), _rel_max(v, ov), _rel_max(p, op)]
+    print(f"C3: instance {small} (n={sizes[small]}) vs the oracle: {err}")
+    assert max(err) < 1e-6
+    for i in (int(order[1]), int(order[len(order) // 2])):
+        m, nu = specs[i]
+        alone = Simulation(m, nu, 0.005, device=gpu, rtol=rtol, precision="fp64")
+        alone.step(50)
+        e = [_rel_max(a, b) for a, b in zip(sims[i].fields(), alone.fields())]
+        alone.close()
+        print(f"C3: instance {i} (n={sizes[i]}) batched vs one-by-one: {e}")
+        assert max(e) < 1e-6
+    for s in sims:
+        s.close()
+
+
+@pytest.mark.timeout(900)
+def test_c4_cavity2048_properties(gpu, oracle):
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.device import build_kdtree
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    n = 2048
+    t0 = time.time()
+    m = mesh.cavity(n)
+    print(f"C4 cavity{n}: mesh {time.time() - t0:.1f}s", flush=True)
+    asm = oracle.Assembly(m, 1e-3, 1e-3)
+    print(f"C4: oracle assembly {time.time() - t0:.1f}s", flush=True)
+    fields = {}
+    for form in ("fp64", "chosen"):
+        sim = Simulation(m, 1e-3, 1e-3, device=gpu, rtol=1e-8,
+                         precision="fp64" if form == "fp64" else "fp32_refined")
+        if form == "chosen":
+            sim.set_warm_start("projection")
+            sim.set_coarse_space(mesh.cavity_prolongation(n, 2), rtol=1e-5)
+            k = n // 2
+            while k >= 32:
+                sim.add_coarse_level(mesh.cavity_prolongation(k, 2))
+                k //= 2
+            sim.set_velocity_projection(8)
+        print(f"C4 {form}: set up {time.time() - t0:.1f}s", flush=True)
+        st = sim.step(3)
+        res_p = _pressure_residual(gpu, sim, asm, oracle)
+        fields[form] = sim.fields()
+        sim.close()
+        print(f"C4 {form}: 3 steps {st['seconds']:.2f}s, iterations {st['cg_iterations']}, "
+              f"max relative residuals {st['max_relative_residual']}, last pressure system "
+              f"recomputed on the host {res_p:.3e}", flush=True)
+        assert max(st["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+        assert res_p <= 1e-8 * (1 + 1e-6)
+        assert all(np.isfinite(a).all() for a in fields[form])
+
+    # departure points of the FP64 run's velocity, located on the 2048^2 tree
+    u, v, _ = fields["fp64"]
+    tree = build_kdtree(m)
+    leaf_count = len(tree.leaf_indices)
+    print(f"C4: tree {tree.node_count} nodes, {leaf_count} leaf entries "
+          f"({leaf_count / 2 ** 24:.2f} x 2^24), {time.time() - t0:.1f}s", flush=True)
+    assert leaf_count > 2 ** 24
+    rng = np.random.default_rng(11)
+    idx = rng.choice(m.node_count, size=100_000, replace=False)
+    xy = m.nodes[idx]
+    dt32 = np.float32(1e-3)
+    pts = np.stack([xy[:, 0] - u[idx].astype(np.float32) * dt32,
+                    xy[:, 1] - v[idx].astype(np.float32) * dt32], axis=1).astype(np.float32)
+    gpu.upload_kd_tree(tree, m)
+    e, xi, eta, nn = gpu.locate(pts)
+    # brute force: the containing cell of the structured cavity holds two
+    # triangles; the located one must contain the point (FP64 barycentrics)
+    el = m.elements
+    X = m.nodes.astype(np.float64)
+    assert (e >= 0).all(), "every departure point of the cavity lies inside it"
+    a, b, c = X[el[e, 0]], X[el[e, 1]], X[el[e, 2]]
+    p = pts.astype(np.float64)
+    det = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])
+    l1 = ((p[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (p[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])) / det
+    l2 = ((b[:, 0] - a[:, 0]) * (p[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (p[:, 0] - a[:, 0])) / det
+    tol = 1e-5
+    inside = (l1 >= -tol) & (l2 >= -tol) & (l1 + l2 <= 1 + tol)
+    print(f"C4: {len(pts)} departure points, all located, contained: {int(inside.sum())}")
+    assert inside.all()
+    # the oracle's traversal of the same tree, with exact integer offsets
+    ot = oracle.Tree(tree.nodes, tree.leaf_indices, tree.bbox, tree.max_depth)
+    oe, oxi, oeta, onn = oracle.kd_find(ot, m.xy(), m.elems(), pts)
+    assert np.array_equal(oe, e) and np.array_equal(oxi, xi) and np.array_equal(oeta, eta)
+    # the reference's float leaf offset
+    fe, _, _, _ = oracle.kd_find(ot, m.xy(), m.elems(), pts, float_offset=True)
+    moved = int((fe != e).sum())
+    lost = int((fe < 0).sum())
+    print(f"C4: with the reference's float leaf offsets {moved} of {len(pts)} departure points "
+          f"get another element ({lost} none at all)")

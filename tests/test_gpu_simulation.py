@@ -218,13 +218,16 @@ def _errors_vs(fields, truth):
     return [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fields, truth)]
 
 
-def _cavity_run(gpu, m, steps, precision, rtol, warm="previous", coarse=None, velocity=0):
+def _cavity_run(gpu, m, steps, precision, rtol, warm="previous", coarse=None, velocity=0,
+                coarse_rtol=1e-6, levels=()):
     from navier_stokes_fem_amd.simulation import Simulation
 
     sim = Simulation(m, 1e-3, 1e-3, device=gpu, rtol=rtol, precision=precision)
     sim.set_warm_start(warm)
     if coarse is not None:
-        sim.set_coarse_space(coarse)
+        sim.set_coarse_space(coarse, rtol=coarse_rtol)
+        for lv in levels:
+            sim.add_coarse_level(lv)
     sim.set_velocity_projection(velocity)
     st = sim.step(steps)
     f = sim.fields()
@@ -266,20 +269,31 @@ def test_velocity_projection(gpu):
     sim.close()
 
 
-@pytest.mark.parametrize("velocity", [0, 16])
-def test_chosen_form_is_as_accurate_as_the_plain_method(gpu, velocity):
+@pytest.mark.parametrize("chain", ["coarse_cg", "coarse_cg_velocity16", "bench_chain"])
+def test_chosen_form_is_as_accurate_as_the_plain_method(gpu, chain):
     """FP32 + FP64 refinement, projection guess and coarse correction (with
     and without the velocity projection) against the plain FP64 method with
     the previous guesses, both at 1e-8: per field, the chosen form's distance
     to a tightly solved run (FP64, 1e-13) is at most 1.2x the plain
-    method's."""
+    method's.  ``bench_chain`` is the headline's configuration on the 128^2
+    cavity: coarse space from the n/2 cavity, its system by V-cycles over
+    levels down to 16^2 to a coarse tolerance of 1e-5, velocity projection
+    depth 8."""
     from navier_stokes_fem_amd import mesh
 
     m = mesh.cavity(128)
     truth, _ = _cavity_run(gpu, m, 10, "fp64", 1e-13)
     fp, _ = _cavity_run(gpu, m, 10, "fp64", 1e-8)
-    fc, sc = _cavity_run(gpu, m, 10, "fp32_refined", 1e-8, warm="projection",
-                         coarse=mesh.cavity_prolongation(128, 4), velocity=velocity)
+    if chain == "bench_chain":
+        fc, sc = _cavity_run(gpu, m, 10, "fp32_refined", 1e-8, warm="projection",
+                             coarse=mesh.cavity_prolongation(128, 2), velocity=8,
+                             coarse_rtol=1e-5,
+                             levels=[mesh.cavity_prolongation(n, 2) for n in (64, 32)])
+        assert sc["coarse_iterations"] > 0
+    else:
+        fc, sc = _cavity_run(gpu, m, 10, "fp32_refined", 1e-8, warm="projection",
+                             coarse=mesh.cavity_prolongation(128, 4),
+                             velocity=16 if chain.endswith("16") else 0)
     ep, ec = _errors_vs(fp, truth), _errors_vs(fc, truth)
     print(f"rel L2 error vs 1e-13 run (u, v, p): plain {ep}, chosen {ec}")
     assert max(sc["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
@@ -305,6 +319,35 @@ def test_coarse_space_rejects_bad_input(gpu):
         sim.set_coarse_space((rs, bad, val, nc))
     assert sim.step(1)["coarse_iterations"] == 0  # nothing was set
     sim.close()
+
+
+def test_failed_coarse_space_leaves_the_plain_solve(gpu):
+    """A coarse space whose set-up fails part way (here a P column with no
+    entries: a zero coarse diagonal) leaves no coarse space and no V-cycle
+    preconditioner behind: the next steps are the plain solve's, bit for
+    bit."""
+    from navier_stokes_fem_amd import _native as N
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation
+
+    m = mesh.cavity(16)
+    sim = Simulation(m, 1e-3, 1e-3, device=gpu, rtol=1e-8)
+    rs, col, val, nc = mesh.cavity_prolongation(16, 2)
+    sim.set_coarse_space((rs, col, val, nc))
+    sim.set_pressure_multigrid(True)
+    with pytest.raises(N.NsfemError):
+        sim.set_coarse_space((rs, col, val, nc + 1))  # the last column is empty
+    st = sim.step(2)
+    f = sim.fields()
+    sim.close()
+    ref = Simulation(m, 1e-3, 1e-3, device=gpu, rtol=1e-8)
+    st0 = ref.step(2)
+    f0 = ref.fields()
+    ref.close()
+    assert st["coarse_iterations"] == 0
+    assert st["cg_iterations"] == st0["cg_iterations"]
+    for a, b in zip(f, f0):
+        np.testing.assert_array_equal(a, b)
 
 
 def test_device_closed_under_a_simulation():
