@@ -176,6 +176,23 @@ int nsfem_cg_solve(nsfem_device *dev, int slot, const nsfem_buffer *b,
                    nsfem_buffer *x, int columns, int max_iterations, double rtol,
                    double atol, int precision, nsfem_cg_stats *stats);
 
+/* nsfem_cg_solve without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, stopping rule, results and kernels as nsfem_cg_solve;
+ * every work vector, the grid-reduction state and the results live in one
+ * static device array of the library (about 48.4 MiB, held by each GPU the
+ * library runs on until the process ends).  Limit: rows * columns <=
+ * 1,048,576 (1 << 20) in either precision (an FP32 + refinement solve keeps
+ * room for its FP64 continuation); larger systems return
+ * NSFEM_INVALID_ARGUMENT.  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (from several threads or several device handles
+ * of the same GPU) race and must be serialised by the caller. */
+int nsfem_cg_solve_noalloc(nsfem_device *dev, int slot, const nsfem_buffer *b,
+                           nsfem_buffer *x, int columns, int max_iterations,
+                           double rtol, double atol, int precision,
+                           nsfem_cg_stats *stats);
+
 /* ------------------------------------------------------------ KD tree */
 typedef struct nsfem_kdtree nsfem_kdtree;
 

@@ -127,6 +127,8 @@ _SIGNATURES = {
     "nsfem_conjugate_gradient": ([vp, C.c_int, vp, vp, vp, C.c_int, dbl, C.c_int], C.c_int),
     "nsfem_cg_solve": ([vp, C.c_int, vp, vp, C.c_int, C.c_int, dbl, dbl, C.c_int,
                         C.POINTER(CgStats)], C.c_int),
+    "nsfem_cg_solve_noalloc": ([vp, C.c_int, vp, vp, C.c_int, C.c_int, dbl, dbl, C.c_int,
+                                C.POINTER(CgStats)], C.c_int),
     "nsfem_kdtree_build": ([C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, PP], C.c_int),
     "nsfem_kdtree_destroy": ([vp], C.c_int),
     "nsfem_kdtree_info": ([vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i32), vp], C.c_int),

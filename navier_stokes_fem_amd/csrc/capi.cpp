@@ -58,6 +58,9 @@ void upload_csr(Device &d, Matrix &m, int rows, int cols, int64_t nnz, const int
     }
     std::vector<int32_t> hrs(rs, rs + rows + 1);
     build_tiles(m, hrs);
+    // the FP32 copy the refined solver streams, made here so that no solve
+    // has to allocate it (the no-allocation entry point relies on it)
+    if (rows == cols && nnz > 0) ensure_val32(d, m);
     m.ready = true;
 }
 
@@ -187,6 +190,7 @@ int nsfem_device_create(int index, nsfem_device **out) {
             NSFEM_HIP(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
             NSFEM_HIP(hipEventCreate(&d.ev_start));
             NSFEM_HIP(hipEventCreate(&d.ev_stop));
+            scratch_init(d);  // this device's address of the static scratch array
         } catch (...) {
             delete h;
             throw;
@@ -513,6 +517,35 @@ int nsfem_cg_solve(nsfem_device *h, int s, const nsfem_buffer *b, nsfem_buffer *
         rq.atol = atol;
         rq.precision = precision;
         const int st = run_cg(d, rq, stats);
+        if (st != NSFEM_OK)
+            fail(st, st == NSFEM_MAX_ITERATIONS_REACHED ? "Max iterations reached!"
+                                                        : "CG breakdown: (Ap, p) <= 0");
+    });
+}
+
+int nsfem_cg_solve_noalloc(nsfem_device *h, int s, const nsfem_buffer *b, nsfem_buffer *x,
+                           int columns, int max_iterations, double rtol, double atol,
+                           int precision, nsfem_cg_stats *stats) {
+    return guard([&] {
+        Device &d = dev(h);
+        Matrix &m = slot(d, s);
+        NSFEM_REQUIRE(columns == 1 || columns == 2, NSFEM_INVALID_ARGUMENT,
+                      "columns must be 1 or 2");
+        NSFEM_REQUIRE(rtol >= 0 && atol >= 0, NSFEM_INVALID_ARGUMENT, "negative tolerance");
+        const double *bp = dptr(b, (int64_t)m.rows * columns, "cg b");
+        double *xp = dptr(x, (int64_t)m.rows * columns, "cg x");
+        std::lock_guard<std::mutex> lk(d.mtx);
+        d.set_current();
+        CgRequest rq{};
+        rq.m = &m;
+        rq.b = bp;
+        rq.x = xp;
+        rq.ncol = columns;
+        rq.max_iter = max_iterations;
+        rq.rtol = rtol;
+        rq.atol = atol;
+        rq.precision = precision;
+        const int st = run_cg_noalloc(d, rq, stats);
         if (st != NSFEM_OK)
             fail(st, st == NSFEM_MAX_ITERATIONS_REACHED ? "Max iterations reached!"
                                                         : "CG breakdown: (Ap, p) <= 0");

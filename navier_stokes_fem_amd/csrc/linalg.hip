@@ -235,14 +235,20 @@ void build_wave_tiles(Matrix &m, int rows, int nnz) {
     m.wmeta.resize(meta.size());
     NSFEM_HIP(hipMemcpy(m.wmeta.ptr, meta.data(), meta.size() * sizeof(int4),
                         hipMemcpyHostToDevice));
+    m.block_tiles.resize(kMaxGridBlocks + 1);
+    m.host_block_tiles.reserve(kMaxGridBlocks + 1);
     m.block_tiles_grid = 0;
     build_col_delta(m);
 }
 
 // Persistent blocks own contiguous wave-tile ranges balanced by (nnz + rows).
+// The table lives in arrays sized for the largest grid at tiling time, so
+// this only computes and copies.
 void ensure_block_tiles(Device &d, Matrix &m, int grid) {
-    if (m.block_tiles_grid == grid && m.block_tiles.ptr) return;
-    std::vector<int32_t> bt(grid + 1);
+    NSFEM_REQUIRE(grid >= 1 && grid <= kMaxGridBlocks, NSFEM_ERROR, "grid size out of range");
+    if (m.block_tiles_grid == grid) return;
+    std::vector<int32_t> &bt = m.host_block_tiles;
+    bt.resize((size_t)grid + 1);  // within the reserved capacity
     const int64_t total = m.host_wtile_prefix[m.n_wtiles];
     int t = 0;
     for (int b = 0; b <= grid; ++b) {
@@ -252,7 +258,6 @@ void ensure_block_tiles(Device &d, Matrix &m, int grid) {
     }
     bt[0] = 0;
     bt[grid] = m.n_wtiles;
-    m.block_tiles.resize(grid + 1);
     NSFEM_HIP(hipMemcpyAsync(m.block_tiles.ptr, bt.data(), bt.size() * sizeof(int32_t),
                              hipMemcpyHostToDevice, d.stream));
     NSFEM_HIP(hipStreamSynchronize(d.stream));
@@ -529,14 +534,24 @@ void launch_axpby(Device &d, int n, double a, double b, const double *x, const d
     NSFEM_HIP(hipGetLastError());
 }
 
+namespace {
+constexpr int kDotBlocks = 1024;  // per-block partials of the dot product
+
+// partial: kDotBlocks doubles of scratch
+void dot_accumulate(Device &d, int n, const double *a, const double *b, double *result,
+                    double *partial) {
+    const int nb = std::max(1, std::min((n + 4095) / 4096, kDotBlocks));
+    const int chunk = std::max(1, (n + nb - 1) / nb);
+    k_dot_partial<<<nb, kBlock, 0, d.stream>>>(n, chunk, a, b, partial);
+    k_dot_final<<<1, kBlock, 0, d.stream>>>(nb, partial, result);
+    NSFEM_HIP(hipGetLastError());
+}
+}  // namespace
+
 void launch_dot_accumulate(Device &d, int n, const double *a, const double *b,
                            double *result) {
-    const int nb = std::max(1, std::min((n + 4095) / 4096, 1024));
-    const int chunk = std::max(1, (n + nb - 1) / nb);
-    d.dot_partial.ensure(1024);
-    k_dot_partial<<<nb, kBlock, 0, d.stream>>>(n, chunk, a, b, d.dot_partial.ptr);
-    k_dot_final<<<1, kBlock, 0, d.stream>>>(nb, d.dot_partial.ptr, result);
-    NSFEM_HIP(hipGetLastError());
+    d.dot_partial.ensure(kDotBlocks);
+    dot_accumulate(d, n, a, b, result, d.dot_partial.ptr);
 }
 
 void launch_scatter(Device &d, int count, const int32_t *idx, const double *val, double *dst) {
@@ -1843,20 +1858,55 @@ struct CgLaunch {
 
 
 namespace {
+// Device memory one persistent solve works in: six vectors of `vec_bytes`
+// (r, p_old, p_new, q or the two packed {r, p, q} states), the grid
+// reduction's state, the per-column results and the dot product's scratch.
+struct CgWorkspace {
+    unsigned char *vectors = nullptr;
+    size_t vec_bytes = 0;
+    SyncState *sync = nullptr;
+    CgColumnResult *res = nullptr;  // [2]
+    double *dot_partial = nullptr;  // [kDotBlocks]
+    double *scalars = nullptr;      // [2]
+};
+
 // true residual ||b_c - A x_c|| for each column (FP64), into host res[c]
 void true_residuals(Device &d, Matrix &m, const double *b, const double *x, int nc,
-                    double *tmp, double *out) {
-    d.scalars.ensure(4);
-    launch_fill(d, nc, 0.0, d.scalars.ptr);
+                    double *tmp, const CgWorkspace &ws, double *out) {
+    launch_fill(d, nc, 0.0, ws.scalars);
     for (int c = 0; c < nc; ++c) {
         launch_spmv(d, m, x + (size_t)c * m.rows, b + (size_t)c * m.rows, tmp, 1);
-        launch_dot_accumulate(d, m.rows, tmp, tmp, d.scalars.ptr + c);
+        dot_accumulate(d, m.rows, tmp, tmp, ws.scalars + c, ws.dot_partial);
     }
     double h[2] = {0, 0};
-    NSFEM_HIP(hipMemcpyAsync(h, d.scalars.ptr, sizeof(double) * nc, hipMemcpyDeviceToHost,
+    NSFEM_HIP(hipMemcpyAsync(h, ws.scalars, sizeof(double) * nc, hipMemcpyDeviceToHost,
                              d.stream));
     NSFEM_HIP(hipStreamSynchronize(d.stream));
     for (int c = 0; c < nc; ++c) out[c] = std::sqrt(h[c]);
+}
+
+// Bytes of one CG vector of a solve (rounded to 256).
+size_t cg_vector_bytes(int n, int nc, bool mixed) {
+    const size_t vt = mixed ? sizeof(float) : sizeof(double);
+    return ((size_t)n * nc * vt + 255) / 256 * 256;
+}
+
+// The workspace in the device's own arrays, grown as needed.
+CgWorkspace device_workspace(Device &d, int n, int nc, bool mixed) {
+    CgWorkspace ws;
+    ws.vec_bytes = cg_vector_bytes(n, nc, mixed);
+    d.work.ensure(ws.vec_bytes * 6);
+    d.sync_mem.ensure(sizeof(SyncState) + 256);
+    d.cg_res.ensure(2);
+    d.dot_partial.ensure(kDotBlocks);
+    d.scalars.ensure(4);
+    ws.vectors = d.work.ptr;
+    unsigned char *sm = d.sync_mem.ptr;
+    ws.sync = reinterpret_cast<SyncState *>(sm + (256 - ((uintptr_t)sm % 256)) % 256);
+    ws.res = d.cg_res.ptr;
+    ws.dot_partial = d.dot_partial.ptr;
+    ws.scalars = d.scalars.ptr;
+    return ws;
 }
 }  // namespace
 
@@ -1911,8 +1961,10 @@ struct CgPass {
 // One persistent-CG solve in the given precision.  The FP64 path ends on the
 // recursive residual; the true residual is recomputed and, should rounding
 // drift have left it above the tolerance, the solve restarts from the
-// current x (at most twice).
-void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
+// current x (at most twice).  `fixed`: a workspace to work in (the
+// no-allocation solve), else the device's own arrays.
+void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
+             const CgWorkspace *fixed = nullptr) {
     Matrix &m = *req.m;
     const int n = m.rows, nc = req.ncol;
     if (mixed) ensure_val32(d, m);
@@ -1939,11 +1991,9 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
                      n, nc, m.n_wtiles, grid, maxg, (int)mixed, (int)idx16);
     ensure_block_tiles(d, m, grid);
 
-    const size_t vt = mixed ? sizeof(float) : sizeof(double);
-    const size_t vec_bytes = ((size_t)n * nc * vt + 255) / 256 * 256;
-    d.work.ensure(vec_bytes * 6);
-    d.sync_mem.ensure(sizeof(SyncState) + 256);
-    d.cg_res.ensure(2);
+    const CgWorkspace ws = fixed ? *fixed : device_workspace(d, n, nc, mixed);
+    const size_t vec_bytes = cg_vector_bytes(n, nc, mixed);
+    NSFEM_REQUIRE(vec_bytes <= ws.vec_bytes, NSFEM_ERROR, "CG workspace too small");
 
     CgArgs a{};
     a.n = n;
@@ -1957,16 +2007,15 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
     a.val32 = m.val32.ptr;
     a.b = req.b;
     a.x = req.x;
-    unsigned char *w = d.work.ptr;
+    unsigned char *w = ws.vectors;
     a.r = w;
     a.p0 = w + vec_bytes;
     a.p1 = w + 2 * vec_bytes;
     a.q = w + 3 * vec_bytes;
     set_state_buffers(a, w, vec_bytes);
     set_cg_params(d, req, n, a);
-    unsigned char *sm = d.sync_mem.ptr;
-    a.sync = reinterpret_cast<SyncState *>(sm + (256 - ((uintptr_t)sm % 256)) % 256);
-    a.res = d.cg_res.ptr;
+    a.sync = ws.sync;
+    a.res = ws.res;
     const int ngroups = (grid + kGroup - 1) / kGroup;
 
     for (int attempt = 0; attempt < 3; ++attempt) {
@@ -1976,14 +2025,14 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
         CgLaunch ln{d, a, grid};
         cg_dispatch(nc, mixed, idx16, wi, ln);
         NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
-        NSFEM_HIP(hipMemcpyAsync(P.res, d.cg_res.ptr, sizeof(CgColumnResult) * nc,
+        NSFEM_HIP(hipMemcpyAsync(P.res, ws.res, sizeof(CgColumnResult) * nc,
                                  hipMemcpyDeviceToHost, d.stream));
         NSFEM_HIP(hipStreamSynchronize(d.stream));
         float ms = 0.f;
         NSFEM_HIP(hipEventElapsedTime(&ms, d.ev_start, d.ev_stop));
         P.ms += ms;
         bool drift = false;
-        if (!mixed) true_residuals(d, m, req.b, req.x, nc, (double *)a.q, P.final_res);
+        if (!mixed) true_residuals(d, m, req.b, req.x, nc, (double *)a.q, ws, P.final_res);
         for (int c = 0; c < nc; ++c) {
             P.iters[c] += P.res[c].iterations;
             P.refines[c] += P.res[c].refinements;
@@ -1999,7 +2048,9 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P) {
 
 }  // namespace
 
-int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
+namespace {
+int run_cg_in(Device &d, const CgRequest &req, nsfem_cg_stats *stats,
+              const CgWorkspace *fixed) {
     Matrix &m = *req.m;
     NSFEM_REQUIRE(m.ready, NSFEM_NOT_READY, "matrix slot is empty");
     NSFEM_REQUIRE(m.rows == m.cols, NSFEM_INVALID_ARGUMENT, "CG needs a square matrix");
@@ -2010,7 +2061,7 @@ int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
                   "unknown precision");
     const int nc = req.ncol;
     CgPass P;
-    cg_pass(d, req, mixed, P);
+    cg_pass(d, req, mixed, P, fixed);
     CgPass F;  // FP64 continuation after an FP32 stagnation
     bool fallback = false;
     if (mixed) {
@@ -2022,7 +2073,7 @@ int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
             r2.rtol = 0.0;
             r2.atol = std::max(P.res[0].tolerance, nc == 2 ? P.res[1].tolerance : 0.0);
             if (nc == 2) r2.atol = std::min(P.res[0].tolerance, P.res[1].tolerance);
-            cg_pass(d, r2, false, F);
+            cg_pass(d, r2, false, F, fixed);
         }
     }
     int worst = NSFEM_OK;
@@ -2043,6 +2094,63 @@ int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
         }
     }
     return worst;
+}
+}  // namespace
+
+int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
+    return run_cg_in(d, req, stats, nullptr);
+}
+
+// ------------------------------------------------------ no-allocation CG
+// The device memory of the no-allocation solve: one static array per GPU
+// (each device the library runs on holds its own copy until the process
+// ends).  Host code never uses this name as an address: scratch_init looks
+// up the device address once per device handle, and every kernel argument
+// and copy uses that address.  Layout: the reduction state, the per-column
+// results, the dot product's partials and scalars, then six CG vectors of
+// kNoAllocVecBytes (an FP64 system of rows x columns <= kNoAllocMaxValues;
+// an FP32 + refinement solve needs the same room for its FP64 fallback).
+namespace {
+constexpr size_t kNoAllocVecBytes = (size_t)kNoAllocMaxValues * sizeof(double);
+constexpr size_t kScratchSync = 0;
+constexpr size_t kScratchRes = kScratchSync + (sizeof(SyncState) + 255) / 256 * 256;
+constexpr size_t kScratchDot = kScratchRes + 256;
+constexpr size_t kScratchScal = kScratchDot + kDotBlocks * sizeof(double);
+constexpr size_t kScratchVec = kScratchScal + 256;
+constexpr size_t kScratchBytes = kScratchVec + 6 * kNoAllocVecBytes;
+static_assert(2 * sizeof(CgColumnResult) <= 256, "results slot");
+static_assert(sizeof(SyncState) % 256 == 0, "reduction state alignment");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+}  // namespace
+
+void scratch_init(Device &d) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        fail(NSFEM_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") +
+                                  hipGetErrorString(e));
+    d.scratch = static_cast<unsigned char *>(ptr);
+}
+
+int run_cg_noalloc(Device &d, const CgRequest &req, nsfem_cg_stats *stats) {
+    NSFEM_REQUIRE(d.scratch, NSFEM_NOT_READY, "device scratch not set up");
+    NSFEM_REQUIRE(req.m && req.m->ready, NSFEM_NOT_READY, "matrix slot is empty");
+    const int64_t values = (int64_t)req.m->rows * req.ncol;
+    NSFEM_REQUIRE(values <= kNoAllocMaxValues, NSFEM_INVALID_ARGUMENT,
+                  "no-allocation CG: rows x columns = " + std::to_string(values) +
+                      " exceeds the scratch capacity of " + std::to_string(kNoAllocMaxValues) +
+                      " values; use nsfem_cg_solve");
+    NSFEM_REQUIRE(req.precision != NSFEM_PRECISION_FP32_REFINED || req.m->val32.ptr,
+                  NSFEM_NOT_READY, "the matrix has no FP32 copy");
+    CgWorkspace ws;
+    ws.sync = reinterpret_cast<SyncState *>(d.scratch + kScratchSync);
+    ws.res = reinterpret_cast<CgColumnResult *>(d.scratch + kScratchRes);
+    ws.dot_partial = reinterpret_cast<double *>(d.scratch + kScratchDot);
+    ws.scalars = reinterpret_cast<double *>(d.scratch + kScratchScal);
+    ws.vectors = d.scratch + kScratchVec;
+    ws.vec_bytes = kNoAllocVecBytes;
+    return run_cg_in(d, req, stats, &ws);
 }
 
 // ------------------------------------------------------------ batched CG

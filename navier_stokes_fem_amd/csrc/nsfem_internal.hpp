@@ -122,8 +122,11 @@ struct Matrix {
     std::vector<int32_t> host_rs;            // row_start (wave tiles are rebuilt from it)
     int wave_rows = kWaveRows, wave_nnz = kWaveNnz;  // current wave-tile shape
     int max_multirow_nz = 0;  // most nonzeros of a wave tile with more than one row
-    // ownership of wave tiles by persistent blocks, cached per grid size
+    // ownership of wave tiles by persistent blocks, cached per grid size:
+    // both arrays hold kMaxGridBlocks + 1 entries from the tiling on, so a
+    // change of grid size copies and never allocates
     DeviceArray<int32_t> block_tiles;
+    std::vector<int32_t> host_block_tiles;
     int block_tiles_grid = 0;
     // column minus the first row of the nonzero's wave tile, as int16; built
     // with the wave tiles of a square matrix when every offset fits
@@ -200,6 +203,9 @@ struct Device {
     DeviceArray<CgColumnResult> cg_res;
     DeviceArray<double> dot_partial;
     DeviceArray<double> scalars;
+    // this device's copy of the library's static scratch array (the
+    // no-allocation solve), looked up once at device creation
+    unsigned char *scratch = nullptr;
     int wall_khz = 100000;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::mutex mtx;  // serialises solves on this handle
@@ -256,6 +262,13 @@ struct CgRequest {
 };
 // Runs the persistent CG; fills stats[ncol].  Returns the worst status.
 int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats);
+// The same solve with every work vector and the reduction state in the
+// device's static scratch array: no device memory is allocated.  Systems of
+// rows x columns <= kNoAllocMaxValues only (NSFEM_INVALID_ARGUMENT beyond).
+constexpr int64_t kNoAllocMaxValues = int64_t(1) << 20;
+int run_cg_noalloc(Device &d, const CgRequest &req, nsfem_cg_stats *stats);
+// Looks up this device's address of the scratch array (device current).
+void scratch_init(Device &d);
 // Many independent solves (same column count and precision) in one
 // persistent launch; stats[2 * i + c] for request i, column c.  block_scale
 // multiplies the blocks each instance gets (autotuned).  With `finish`, an
