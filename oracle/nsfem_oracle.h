@@ -24,7 +24,10 @@
  *       on random small CSR matrices (closed form).
  *   or_triplet_to_csr : scipy.sparse coo->csr on random triplets.
  *   or_cg : dense Cholesky solve (numpy) on assembled cavity matrices;
- *       true residual <= rtol*||b||; A-orthogonality of directions.
+ *       true residual <= rtol*||b||; A-orthogonality of directions.  Parity
+ *       with the reference's CPU solver itself (SMM::ConjugateGradient of
+ *       github.com/vasil-pashov/sparse_matrix_math@master, not vendored) is
+ *       unpinned: its stopping rule and FP32 iterate sequence are unknown.
  *   or_kd_build : structural invariants (leaf coverage of every triangle,
  *       split = box midpoint, right-child = next free index, leaf cut-off),
  *       hand-derived trees on tiny meshes, bbox max = last node.

@@ -202,3 +202,43 @@ def test_float_offset_is_exact_below_2_24(oracle):
     b = oracle.kd_find(t, m.xy(), m.elems(), pts, float_offset=True)
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
+
+
+def _shift_leaves(t, shift):
+    """The same tree with its leaf index array behind `shift` entries of
+    padding (every leaf offset grows by `shift`)."""
+    w = t.nodes.reshape(-1, 2).copy()
+    leaf = (w[:, 0] & 3) == 2
+    w[leaf, 1] += np.uint32(shift)
+    pad = np.zeros(shift, np.int32)
+    return type(t)(w.reshape(-1), np.concatenate([pad, t.leaf_indices]), t.bbox, t.max_depth)
+
+
+def test_float_offset_decoding_above_2_24(oracle):
+    """Above 2^24 (and below 2^25) the float decoding rounds odd offsets to
+    an even neighbour.  On a structured cavity every leaf holds an even
+    number of triangles, so every offset is even: with the leaves moved by
+    2^24 + 2 the reference's decoding stays exact and no point moves (the
+    2048^2 cavity's case: leaves of 12 or 16 triangles in an array of
+    33.5M < 2^25 entries), while a move by 2^24 + 1 makes every offset odd
+    and the windows one entry off place points elsewhere."""
+    from navier_stokes_fem_amd import mesh
+
+    m = mesh.cavity(20)
+    t = oracle.kd_build(m.xy(), m.elems())
+    w = t.nodes.reshape(-1, 2)
+    leaf = (w[:, 0] & 3) == 2
+    assert ((w[leaf, 0] >> 2) % 2 == 0).all() and (w[leaf, 1] % 2 == 0).all()
+    pts = np.random.default_rng(2).uniform(0.0, 1.0, size=(4000, 2)).astype(np.float32)
+    base = oracle.kd_find(t, m.xy(), m.elems(), pts)
+    moved = {}
+    for shift in (2 ** 24 + 2, 2 ** 24 + 1):
+        ts = _shift_leaves(t, shift)
+        exact = oracle.kd_find(ts, m.xy(), m.elems(), pts)
+        for u, v in zip(exact, base):
+            np.testing.assert_array_equal(u, v)  # integer decoding: the same tree
+        rounded = oracle.kd_find(ts, m.xy(), m.elems(), pts, float_offset=True)
+        moved[shift] = int((rounded[0] != base[0]).sum())
+    print(f"points placed elsewhere by the float decoding: {moved}")
+    assert moved[2 ** 24 + 2] == 0
+    assert moved[2 ** 24 + 1] > 0
