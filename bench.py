@@ -96,9 +96,9 @@ def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
 
 
 def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
-             coarse_rtol=1e-6, velocity_projection=0, coarse_levels=()):
+             coarse_rtol=1e-6, velocity_projection=0, coarse_levels=(), rtol=1e-8):
     sim.reset()
-    sim.configure(precision=form, rtol=1e-8, pair_velocity=True)
+    sim.configure(precision=form, rtol=rtol, pair_velocity=True)
     sim.set_warm_start(warm)
     sim.set_coarse_space(coarse, rtol=coarse_rtol)
     if coarse is not None:
@@ -180,7 +180,10 @@ def cpu_baseline(m, nu, dt, gpu_iters_per_step, sample_iters):
     velocity node, the right-hand-side products, and `sample_iters` CG
     iterations of each matrix (timed as the difference between two runs so
     set-up costs cancel).  A step's CPU time is extrapolated with the
-    per-step CG iteration counts of the GPU FP64 run."""
+    per-step CG iteration counts of the GPU FP64 run.  The same sample runs
+    once more on one thread (half the CG iterations), for the 1-thread
+    figure the reference also publishes (tex/Chapters/04_implementation_
+    details.tex:104)."""
     import numpy as np
 
     from navier_stokes_fem_amd import device as D
@@ -195,37 +198,46 @@ def cpu_baseline(m, nu, dt, gpu_iters_per_step, sample_iters):
     tree = D.build_kdtree(m)
     ot = O.Tree(tree.nodes, tree.leaf_indices, tree.bbox, tree.max_depth)
     V = m.node_count
-    rng = np.random.default_rng(5)
-    u = rng.uniform(-1, 1, V)
-    v = rng.uniform(-1, 1, V)
-    t = time.perf_counter()
-    O.advect_f64(ot, m.xy(), m.elems(), u, v, dt)
-    t_adv = time.perf_counter() - t
-    log(f"cpu baseline: advect {t_adv:.2f} s")
     csr = lambda k: a[k][:3]
-    t = time.perf_counter()
-    O.spmv(csr("velocity_mass"), u)
-    O.spmv(csr("velocity_mass"), v)
-    O.spmv(csr("velocity_divergence"), np.concatenate([u, v]))
-    O.spmv(csr("pressure_divergence"), rng.uniform(-1, 1, m.pressure_count))
-    t_rhs = time.perf_counter() - t
-    per_iter = {}
-    for k in ("diffusion", "pressure_stiffness", "velocity_mass"):
-        A = csr(k)
-        n = len(A[0]) - 1
-        b = rng.uniform(-1, 1, n)
-        x0 = np.zeros(n)
-        t = time.perf_counter()
-        O.cg(A, b, x0, max_iter=2, rtol=0.0, atol=0.0)
-        t2 = time.perf_counter() - t
-        t = time.perf_counter()
-        O.cg(A, b, x0, max_iter=2 + sample_iters, rtol=0.0, atol=0.0)
-        t22 = time.perf_counter() - t
-        per_iter[k] = max(t22 - t2, 1e-9) / sample_iters
-        log(f"cpu baseline: {k} {per_iter[k] * 1e3:.2f} ms/iteration")
     its = gpu_iters_per_step
-    step = (t_adv + t_rhs + (its[0] + its[1]) * per_iter["diffusion"] +
-            its[2] * per_iter["pressure_stiffness"] + (its[3] + its[4]) * per_iter["velocity_mass"])
+
+    def sample(nthreads, iters):
+        O.set_threads(nthreads)
+        rng = np.random.default_rng(5)
+        u = rng.uniform(-1, 1, V)
+        v = rng.uniform(-1, 1, V)
+        t = time.perf_counter()
+        O.advect_f64(ot, m.xy(), m.elems(), u, v, dt)
+        t_adv = time.perf_counter() - t
+        log(f"cpu baseline ({nthreads} threads): advect {t_adv:.2f} s")
+        t = time.perf_counter()
+        O.spmv(csr("velocity_mass"), u)
+        O.spmv(csr("velocity_mass"), v)
+        O.spmv(csr("velocity_divergence"), np.concatenate([u, v]))
+        O.spmv(csr("pressure_divergence"), rng.uniform(-1, 1, m.pressure_count))
+        t_rhs = time.perf_counter() - t
+        per_iter = {}
+        for k in ("diffusion", "pressure_stiffness", "velocity_mass"):
+            A = csr(k)
+            n = len(A[0]) - 1
+            b = rng.uniform(-1, 1, n)
+            x0 = np.zeros(n)
+            t = time.perf_counter()
+            O.cg(A, b, x0, max_iter=2, rtol=0.0, atol=0.0)
+            t2 = time.perf_counter() - t
+            t = time.perf_counter()
+            O.cg(A, b, x0, max_iter=2 + iters, rtol=0.0, atol=0.0)
+            t22 = time.perf_counter() - t
+            per_iter[k] = max(t22 - t2, 1e-9) / iters
+            log(f"cpu baseline ({nthreads} threads): {k} {per_iter[k] * 1e3:.2f} ms/iteration")
+        step = (t_adv + t_rhs + (its[0] + its[1]) * per_iter["diffusion"] +
+                its[2] * per_iter["pressure_stiffness"] +
+                (its[3] + its[4]) * per_iter["velocity_mass"])
+        return t_adv, t_rhs, per_iter, step
+
+    t_adv, t_rhs, per_iter, step = sample(threads, sample_iters)
+    t_adv1, t_rhs1, per_iter1, step1 = sample(1, max(1, sample_iters // 2))
+    O.set_threads(threads)
     return {
         "value": 1.0 / step,
         "unit": "steps/s",
@@ -238,6 +250,11 @@ def cpu_baseline(m, nu, dt, gpu_iters_per_step, sample_iters):
                    + f"); step time extrapolated with the GPU FP64 run's iterations/step "
                    f"{[round(x, 1) for x in its]}"),
         "seconds_per_step": step,
+        "one_thread": {"value": 1.0 / step1, "seconds_per_step": step1,
+                       "advect_s": round(t_adv1, 3), "rhs_s": round(t_rhs1, 3),
+                       "ms_per_iteration": {k: round(v * 1e3, 2) for k, v in per_iter1.items()},
+                       "sample": f"the same sample on 1 thread, {max(1, sample_iters // 2)} CG "
+                                 "iterations per matrix"},
     }
 
 
@@ -583,6 +600,11 @@ def main():
     ap.add_argument("--n", type=int, default=2048, help="cavity squares per side")
     ap.add_argument("--form", choices=["both", "fp32_refined", "fp64"], default="both")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tight", action="store_true",
+                    help="skip the run solved to 1e-12 that the forms' accuracy is measured against")
+    ap.add_argument("--no-attribution", action="store_true",
+                    help="skip the attribution runs (the chosen form without the coarse "
+                         "correction, and with the previous-pressure guess)")
     ap.add_argument("--cpu-sample-iters", type=int, default=10)
     ap.add_argument("--warm", choices=["previous", "extrapolate", "projection"],
                     default="projection",
@@ -666,7 +688,8 @@ def main():
     log(f"device {info}")
     t = time.perf_counter()
     sim = Simulation(m, nu, dt, device=dev, rtol=1e-8, precision="fp64", pair_velocity=True)
-    log(f"simulation set-up (assembly + KD tree + upload) {time.perf_counter() - t:.1f} s")
+    setup_s = time.perf_counter() - t
+    log(f"simulation set-up (velocity order, device assembly, KD tree, tiles) {setup_s:.1f} s")
     n_p = m.pressure_count
     nnz_p = sim.matrix_info(0)[2]
 
@@ -691,13 +714,16 @@ def main():
     # attribution: the same mixed solver without the coarse correction, and
     # with the reference's previous-pressure guess alone
     attribution = {}
-    if "fp32_refined" in forms and coarse is not None and args.warm != "previous":
+    if args.no_attribution:
+        pass
+    elif "fp32_refined" in forms and coarse is not None and args.warm != "previous":
         wall_a, st_a = run_form(sim, "fp32_refined", args.warmup, args.steps, n_p, nnz_p,
                                 warm=args.warm)
         attribution[f"fp32_refined_{args.warm}_guess"] = (wall_a, st_a)
         log(f"fp32_refined ({args.warm} guess, no coarse correction): "
             f"{args.steps / wall_a:.3f} steps/s")
-    if "fp32_refined" in forms and (args.warm != "previous" or coarse is not None):
+    if not args.no_attribution and "fp32_refined" in forms and (args.warm != "previous" or
+                                                                 coarse is not None):
         wall_a, st_a = run_form(sim, "fp32_refined", args.warmup, args.steps, n_p, nnz_p,
                                 warm="previous")
         attribution["fp32_refined_previous_guess"] = (wall_a, st_a)
@@ -708,18 +734,32 @@ def main():
     wall, st = results[chosen]
     value = args.steps / wall
     mixed = chosen == "fp32_refined"
-    # dominant kernel: the pressure CG launch (k_cg, 1 column), one per step
+    # dominant kernel: the pressure CG launch (k_cg, 1 column), one per step.
+    # achieved = the textbook iteration's bytes (matrix once with int32
+    # columns, row_start, 11 vector words per row; w = 4 for the FP32 inner
+    # iteration, 8 for FP64) x iterations / kernel time; the build's own
+    # count, with the FP64 solution and the reliable updates' residual
+    # passes, is reported beside it
     p_iters = st["cg_iterations"][2]
     p_secs = st["cg_seconds"][2]
+    w = 4 if mixed else 8
+    textbook = (4 + w) * nnz_p + (4 + 11 * w) * n_p
+    bytes_textbook = textbook * p_iters
     bytes_p = cg_bytes(nnz_p, n_p, p_iters, st["cg_refinements"][2] + args.steps - 1, mixed)
-    achieved = bytes_p / max(p_secs, 1e-12) / 1e9
+    achieved = bytes_textbook / max(p_secs, 1e-12) / 1e9
     traffic = (MEASURED_BYTES_PER_ITERATION[mixed] * p_iters / args.steps
                if args.n == 2048 else None)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                 "kernel": "k_cg<1,1024,%s> (pressure CG, persistent)" % ("true" if mixed else "false"),
-                "launches": args.steps, "bytes_per_launch": bytes_p / args.steps,
-                "seconds_per_launch": p_secs / args.steps}
+                "launches": args.steps, "iterations_per_launch": p_iters / args.steps,
+                "bytes_per_iteration": textbook,
+                "bytes_per_launch": bytes_textbook / args.steps,
+                "seconds_per_launch": p_secs / args.steps,
+                "with_refinement_passes": {
+                    "bytes_per_launch": bytes_p / args.steps,
+                    "achieved": round(bytes_p / max(p_secs, 1e-12) / 1e9, 1),
+                    "frac": round(bytes_p / max(p_secs, 1e-12) / 1e9 / PEAK_HBM_GBS, 4)}}
     out = {
         "metric": "steps/sec at 1e-8 rel. residual, chosen vs plain FP64 CG; achieved HBM GB/s",
         "value": round(value, 4),
@@ -765,6 +805,25 @@ def main():
         r_, c_, z_ = sim.matrix_info(slot)
         out["spmv"][name] = spmv_bench(dev, sim, slot, r_, c_, z_)
         log(f"spmv {name}: {out['spmv'][name]}")
+    out["setup_seconds"] = round(setup_s, 2)
+    # accuracy at equal tolerance: every form's distance to the same steps
+    # solved to 1e-12 (the chosen form's solver chain at 1e-12: its FP64
+    # residual makes it as exact as an FP64 solve, at a fraction of the time)
+    if not args.no_tight and "fp32_refined" in forms:
+        t = time.perf_counter()
+        w_t, st_t = run_form(sim, "fp32_refined", args.warmup, args.steps, n_p, nnz_p,
+                             warm=args.warm, coarse=coarse, coarse_rtol=min(args.coarse_rtol, 1e-8),
+                             velocity_projection=args.velocity_projection, coarse_levels=levels,
+                             rtol=1e-12)
+        log(f"tight reference (1e-12) {time.perf_counter() - t:.1f} s, max residuals "
+            f"{st_t['max_relative_residual']}")
+        tight = st_t["fields"]
+        out["error_vs_tight_solve"] = {
+            "reference": f"the chosen form's chain solved to 1e-12 (coarse solves to "
+                         f"{min(args.coarse_rtol, 1e-8):g}), same {args.warmup + args.steps} "
+                         f"steps from rest; relative L2 per field",
+            **{form: {k: v["rel_l2"] for k, v in field_difference(s_["fields"], tight).items()}
+               for form, (w_, s_) in list(results.items()) + list(attribution.items())}}
     if "fp64" in results and "fp32_refined" in results:
         out["speedup_vs_fp64"] = round((args.steps / results["fp32_refined"][0]) /
                                        (args.steps / results["fp64"][0]), 3)

@@ -247,10 +247,11 @@ int nsfem_advect_f64(nsfem_device *dev, int n, const nsfem_buffer *u,
 int nsfem_locate(nsfem_device *dev, int n, const float *points, int32_t *element,
                  float *xi, float *eta, int32_t *nearest);
 
-/* ------------------------------------------------- FEM assembly (host) */
+/* ------------------------------------------------------- FEM assembly */
 /* The five device matrices of NavierStokesAssembly::assembleAllMatrices
  * (include/assembly.h:1302-1475) in FP64 from float node coordinates, plus
- * the constant Dirichlet right-hand-side corrections.  Host only. */
+ * the constant Dirichlet right-hand-side corrections, on host threads
+ * (nsfem_assemble) or on the GPU (nsfem_assemble_device). */
 typedef struct nsfem_assembly nsfem_assembly;
 #define NSFEM_ASM_VELOCITY_MASS 0
 #define NSFEM_ASM_DIFFUSION 1
@@ -265,6 +266,19 @@ int nsfem_assemble(int node_count, int pressure_count, const float *xy,
                    int p_bc_count, const int32_t *p_bc_nodes,
                    const double *p_bc_values, double viscosity, double dt,
                    int threads, nsfem_assembly **out);
+/* The same assembly on the GPU (one thread per matrix row, contributions
+ * summed in ascending element order as the reference inserts its
+ * triplets): bit-identical matrices and corrections, returned as host
+ * copies in the same handle type.  A node may belong to at most 16 elements
+ * (NSFEM_INVALID_ARGUMENT beyond).  Simulations assemble this way on the
+ * device without the host copies (nsfem_sim_options.assembly). */
+int nsfem_assemble_device(nsfem_device *dev, int node_count, int pressure_count,
+                          const float *xy, int element_count, const int32_t *elements,
+                          int vel_bc_count, const int32_t *vel_bc_nodes,
+                          const double *vel_bc_u, const double *vel_bc_v,
+                          int p_bc_count, const int32_t *p_bc_nodes,
+                          const double *p_bc_values, double viscosity, double dt,
+                          nsfem_assembly **out);
 int nsfem_assembly_destroy(nsfem_assembly *a);
 int nsfem_assembly_matrix_info(const nsfem_assembly *a, int which, int *rows,
                                int *cols, int64_t *nnz);
@@ -293,7 +307,10 @@ typedef struct {
                                  last step's column iteration counts stay within
                                  2x (+8) of each other, else one after the other */
     int32_t threads;          /* host threads for assembly / KD build, 0 = all */
+    int32_t assembly;         /* NSFEM_ASSEMBLY_DEVICE (0, default) or _HOST */
 } nsfem_sim_options;
+#define NSFEM_ASSEMBLY_DEVICE 0   /* matrices assembled on the GPU */
+#define NSFEM_ASSEMBLY_HOST 1     /* assembled on host threads, then uploaded */
 
 typedef struct {
     int32_t steps;

@@ -25,6 +25,8 @@ STAGNATION = -9
 
 PRECISION_FP64 = 0
 PRECISION_FP32_REFINED = 1
+ASSEMBLY_DEVICE = 0
+ASSEMBLY_HOST = 1
 
 WARM_PREVIOUS = 0
 WARM_ZERO = 1
@@ -59,7 +61,7 @@ class CgStats(C.Structure):
 class SimOptions(C.Structure):
     _fields_ = [("viscosity", C.c_double), ("dt", C.c_double), ("rtol", C.c_double),
                 ("max_iterations", C.c_int32), ("precision", C.c_int32),
-                ("pair_velocity", C.c_int32), ("threads", C.c_int32)]
+                ("pair_velocity", C.c_int32), ("threads", C.c_int32), ("assembly", C.c_int32)]
 
 
 class StepStats(C.Structure):
@@ -141,6 +143,8 @@ _SIGNATURES = {
     "nsfem_locate": ([vp, C.c_int, vp, vp, vp, vp, vp], C.c_int),
     "nsfem_assemble": ([C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, vp, vp,
                         dbl, dbl, C.c_int, PP], C.c_int),
+    "nsfem_assemble_device": ([vp, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, vp,
+                               C.c_int, vp, vp, dbl, dbl, PP], C.c_int),
     "nsfem_assembly_destroy": ([vp], C.c_int),
     "nsfem_assembly_matrix_info": ([vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                     C.POINTER(i64)], C.c_int),

@@ -64,6 +64,25 @@ void upload_csr(Device &d, Matrix &m, int rows, int cols, int64_t nnz, const int
     m.ready = true;
 }
 
+void adopt_csr(Device &d, Matrix &m, DeviceCsr &c) {
+    std::lock_guard<std::mutex> lk(m.mtx);
+    d.set_current();
+    m.ready = false;
+    m.rows = c.rows;
+    m.cols = c.cols;
+    m.nnz = c.nnz;
+    m.row_start = std::move(c.rs);
+    m.col = std::move(c.col);
+    m.val = std::move(c.val);
+    m.val32.release();
+    std::vector<int32_t> hrs((size_t)m.rows + 1);
+    NSFEM_HIP(hipMemcpy(hrs.data(), m.row_start.ptr, sizeof(int32_t) * hrs.size(),
+                        hipMemcpyDeviceToHost));
+    build_tiles(m, hrs);
+    if (m.rows == m.cols && m.nnz > 0) ensure_val32(d, m);
+    m.ready = true;
+}
+
 void upload_tree(Device &d, KdTreeGpu &t, int32_t tn, const uint32_t *w, int64_t lc,
                  const int32_t *leaf, int V, const float *xy, int E, const int32_t *el) {
     NSFEM_REQUIRE(tn > 0 && V > 0 && E > 0 && lc >= 0, NSFEM_INVALID_ARGUMENT,
@@ -730,6 +749,55 @@ int nsfem_assemble(int V, int P, const float *xy, int E, const int32_t *el, int 
         auto *a = new nsfem_assembly();
         try {
             assemble_host(V, P, xy, E, el, nvb, vb, gu, gv, npb, pb, gp, nu, dt, threads, a->impl);
+        } catch (...) {
+            delete a;
+            throw;
+        }
+        *out = a;
+    });
+}
+
+int nsfem_assemble_device(nsfem_device *h, int V, int P, const float *xy, int E,
+                          const int32_t *el, int nvb, const int32_t *vb, const double *gu,
+                          const double *gv, int npb, const int32_t *pb, const double *gp,
+                          double nu, double dt, nsfem_assembly **out) {
+    return guard([&] {
+        Device &d = dev(h);
+        NSFEM_REQUIRE(out && xy && el, NSFEM_INVALID_ARGUMENT, "null argument");
+        NSFEM_REQUIRE((nvb == 0 || (vb && gu && gv)) && (npb == 0 || (pb && gp)),
+                      NSFEM_INVALID_ARGUMENT, "null boundary arrays");
+        std::lock_guard<std::mutex> lk(d.mtx);
+        DeviceAssembly da;
+        assemble_device(d, V, P, xy, E, el, nvb, vb, gu, gv, npb, pb, gp, nu, dt, nullptr,
+                        nullptr, da);
+        auto *a = new nsfem_assembly();
+        try {
+            for (int k = 0; k < 5; ++k) {
+                HostCsr &hc = a->impl.m[k];
+                DeviceCsr &dc = da.m[k];
+                hc.rows = dc.rows;
+                hc.cols = dc.cols;
+                hc.rs.resize((size_t)dc.rows + 1);
+                hc.col.resize((size_t)dc.nnz);
+                hc.val.resize((size_t)dc.nnz);
+                NSFEM_HIP(hipMemcpy(hc.rs.data(), dc.rs.ptr, sizeof(int32_t) * hc.rs.size(),
+                                    hipMemcpyDeviceToHost));
+                if (dc.nnz) {
+                    NSFEM_HIP(hipMemcpy(hc.col.data(), dc.col.ptr, sizeof(int32_t) * dc.nnz,
+                                        hipMemcpyDeviceToHost));
+                    NSFEM_HIP(hipMemcpy(hc.val.data(), dc.val.ptr, sizeof(double) * dc.nnz,
+                                        hipMemcpyDeviceToHost));
+                }
+            }
+            a->impl.c_u.resize(V);
+            a->impl.c_v.resize(V);
+            a->impl.c_p.resize(P);
+            NSFEM_HIP(hipMemcpy(a->impl.c_u.data(), da.c_u.ptr, sizeof(double) * V,
+                                hipMemcpyDeviceToHost));
+            NSFEM_HIP(hipMemcpy(a->impl.c_v.data(), da.c_v.ptr, sizeof(double) * V,
+                                hipMemcpyDeviceToHost));
+            NSFEM_HIP(hipMemcpy(a->impl.c_p.data(), da.c_p.ptr, sizeof(double) * P,
+                                hipMemcpyDeviceToHost));
         } catch (...) {
             delete a;
             throw;

@@ -325,6 +325,26 @@ void assemble_host(int V, int P, const float *xy, int E, const int32_t *el, int 
                    const int32_t *pb, const double *gp, double nu, double dt, int threads,
                    HostAssembly &out);
 
+// Device assembly (assembly.hip): the same five matrices and corrections on
+// the GPU, bit-identical to assemble_host.  With vorder/vpos (new -> original
+// and original -> new velocity index) the velocity side comes out renumbered.
+struct DeviceCsr {
+    int rows = 0, cols = 0;
+    int64_t nnz = 0;
+    DeviceArray<int32_t> rs, col;
+    DeviceArray<double> val;
+};
+struct DeviceAssembly {
+    DeviceCsr m[5];  // NSFEM_ASM_* order
+    DeviceArray<double> c_u, c_v, c_p;
+};
+void assemble_device(Device &d, int V, int P, const float *xy, int E, const int32_t *el, int nvb,
+                     const int32_t *vb, const double *gu, const double *gv, int npb,
+                     const int32_t *pb, const double *gp, double nu, double dt,
+                     const int32_t *vorder, const int32_t *vpos, DeviceAssembly &out);
+// Takes over a device CSR as a matrix (tiles built, FP32 copy made).
+void adopt_csr(Device &d, Matrix &m, DeviceCsr &c);
+
 // Validate and upload a host CSR into a device slot (builds the tile table).
 void upload_csr(Device &d, Matrix &m, int rows, int cols, int64_t nnz, const int32_t *rs,
                 const int32_t *col, const double *val);

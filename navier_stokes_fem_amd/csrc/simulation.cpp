@@ -243,18 +243,35 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
     NSFEM_REQUIRE(opt.precision == NSFEM_PRECISION_FP64 ||
                       opt.precision == NSFEM_PRECISION_FP32_REFINED,
                   NSFEM_INVALID_ARGUMENT, "simulation: unknown precision");
-    HostAssembly A;
-    assemble_host(V, P, xy, E, el, nvb, vb, gu, gv, npb, pb, gp, opt.viscosity, opt.dt,
-                  opt.threads, A);
-    HostKdTree tree;
-    kd_build_host(V, xy, E, el, -1, -1, tree);
+    NSFEM_REQUIRE(opt.assembly == NSFEM_ASSEMBLY_DEVICE || opt.assembly == NSFEM_ASSEMBLY_HOST,
+                  NSFEM_INVALID_ARGUMENT, "simulation: unknown assembly option");
+    const bool on_device = opt.assembly == NSFEM_ASSEMBLY_DEVICE;
+    const auto t_start = std::chrono::steady_clock::now();
+    auto lap = [&t_start] {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    };
+    double t_asm = 0.0, t_tree = 0.0, t_order = 0.0;
     // renumber the velocity nodes (velocity_order): matrices, right-hand-side
     // corrections, Dirichlet nodes, coordinates and elements
     const int threads = opt.threads > 0 ? opt.threads
                                         : (int)std::max(1u, std::thread::hardware_concurrency());
     std::vector<int32_t> vorder = velocity_order(V, xy), vpos((size_t)V);
     for (int i = 0; i < V; ++i) vpos[vorder[i]] = i;
-    {
+    t_order = lap();
+    HostAssembly A;
+    DeviceAssembly DA;
+    if (on_device) {
+        assemble_device(d, V, P, xy, E, el, nvb, vb, gu, gv, npb, pb, gp, opt.viscosity, opt.dt,
+                        vorder.data(), vpos.data(), DA);
+    } else {
+        assemble_host(V, P, xy, E, el, nvb, vb, gu, gv, npb, pb, gp, opt.viscosity, opt.dt,
+                      opt.threads, A);
+    }
+    t_asm = lap();
+    HostKdTree tree;
+    kd_build_host(V, xy, E, el, -1, -1, tree);
+    t_tree = lap();
+    if (!on_device) {
         NSFEM_REQUIRE(A.m[0].rows == V && A.m[0].cols == V && A.m[1].rows == V &&
                           A.m[1].cols == V && A.m[3].rows == P && A.m[3].cols == 2 * V &&
                           A.m[4].rows == 2 * V && A.m[4].cols == P,
@@ -293,6 +310,10 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
     const int slot_of[5] = {NSFEM_VELOCITY_MASS, NSFEM_DIFFUSION, NSFEM_PRESSURE_STIFFNESS,
                             NSFEM_VELOCITY_DIVERGENCE, NSFEM_PRESSURE_DIVERGENCE};
     for (int k = 0; k < 5; ++k) {
+        if (on_device) {
+            adopt_csr(d, s->mats[slot_of[k]], DA.m[k]);
+            continue;
+        }
         HostCsr &c = A.m[k];
         upload_csr(d, s->mats[slot_of[k]], c.rows, c.cols, (int64_t)c.col.size(), c.rs.data(),
                    c.col.data(), c.val.data());
@@ -312,11 +333,18 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
     s->du.resize(2 * (size_t)V);
     s->prhs.resize(P);
     s->p.resize(P);
-    std::vector<double> cuv(2 * (size_t)V);
-    std::copy(A.c_u.begin(), A.c_u.end(), cuv.begin());
-    std::copy(A.c_v.begin(), A.c_v.end(), cuv.begin() + V);
-    put(d, s->c_uv, cuv.data(), cuv.size());
-    put(d, s->c_p, A.c_p.data(), A.c_p.size());
+    if (on_device) {
+        s->c_uv.resize(2 * (size_t)V);
+        launch_copy(d, V, DA.c_u.ptr, s->c_uv.ptr);
+        launch_copy(d, V, DA.c_v.ptr, s->c_uv.ptr + V);
+        s->c_p = std::move(DA.c_p);
+    } else {
+        std::vector<double> cuv(2 * (size_t)V);
+        std::copy(A.c_u.begin(), A.c_u.end(), cuv.begin());
+        std::copy(A.c_v.begin(), A.c_v.end(), cuv.begin() + V);
+        put(d, s->c_uv, cuv.data(), cuv.size());
+        put(d, s->c_p, A.c_p.data(), A.c_p.size());
+    }
     std::vector<int32_t> vi, pi;
     std::vector<double> vu, vv, pv, dummy;
     last_wins(nvb, vb, gu, gv, vi, vu, vv);
@@ -341,6 +369,12 @@ Simulation *simulation_create(Device &d, int V, int P, const float *xy, int E,
     s->vorder.swap(vorder);
     s->vpos.swap(vpos);
     simulation_reset(*s);
+    if (std::getenv("NSFEM_SETUP_VERBOSE"))
+        std::fprintf(stderr,
+                     "[nsfem] simulation set-up: velocity order %.3f s, assembly (%s) %.3f s, "
+                     "KD tree %.3f s, tiles/uploads %.3f s, total %.3f s\n",
+                     t_order, on_device ? "device" : "host", t_asm - t_order, t_tree - t_asm,
+                     lap() - t_tree, lap());
     {
         std::lock_guard<std::mutex> lk(d.buf_mtx);
         d.sims.insert(s);

@@ -112,20 +112,28 @@ def build_kdtree(mesh, max_depth: int = -1, min_leaf_size: int = 16) -> KdTree:
         L.nsfem_kdtree_destroy(h)
 
 
-def assemble(mesh, viscosity: float, dt: float, threads: int = 0) -> dict:
-    """assembleAllMatrices (include/assembly.h:1302-1475) on the host, FP64.
-    Returns CSR triples (row_start, col, val) for ``velocity_mass``,
-    ``diffusion``, ``pressure_stiffness``, ``velocity_divergence``,
-    ``pressure_divergence`` and the Dirichlet corrections ``c_u, c_v, c_p``."""
+def assemble(mesh, viscosity: float, dt: float, threads: int = 0, device=None) -> dict:
+    """assembleAllMatrices (include/assembly.h:1302-1475), FP64: on the host
+    threads, or on the GPU of ``device`` (a :class:`GPUSimulationDevice`;
+    bit-identical results).  Returns CSR triples (row_start, col, val) for
+    ``velocity_mass``, ``diffusion``, ``pressure_stiffness``,
+    ``velocity_divergence``, ``pressure_divergence`` and the Dirichlet
+    corrections ``c_u, c_v, c_p``."""
     L = N.lib()
     h = C.c_void_p()
     xy, el = mesh.xy(), mesh.elems()
     vb, gu, gv = _c(mesh.vel_bc_nodes, np.int32), _c(mesh.vel_bc_u, np.float64), _c(mesh.vel_bc_v, np.float64)
     pb, gp = _c(mesh.p_bc_nodes, np.int32), _c(mesh.p_bc_values, np.float64)
-    N.check(L.nsfem_assemble(mesh.node_count, mesh.pressure_count, N.ptr(xy), mesh.element_count,
-                             N.ptr(el), len(vb), N.ptr(vb), N.ptr(gu), N.ptr(gv), len(pb),
-                             N.ptr(pb), N.ptr(gp), float(viscosity), float(dt), int(threads),
-                             C.byref(h)))
+    if device is None:
+        N.check(L.nsfem_assemble(mesh.node_count, mesh.pressure_count, N.ptr(xy),
+                                 mesh.element_count, N.ptr(el), len(vb), N.ptr(vb), N.ptr(gu),
+                                 N.ptr(gv), len(pb), N.ptr(pb), N.ptr(gp), float(viscosity),
+                                 float(dt), int(threads), C.byref(h)))
+    else:
+        N.check(L.nsfem_assemble_device(device.handle, mesh.node_count, mesh.pressure_count,
+                                        N.ptr(xy), mesh.element_count, N.ptr(el), len(vb),
+                                        N.ptr(vb), N.ptr(gu), N.ptr(gv), len(pb), N.ptr(pb),
+                                        N.ptr(gp), float(viscosity), float(dt), C.byref(h)))
     try:
         out = {}
         names = ("velocity_mass", "diffusion", "pressure_stiffness", "velocity_divergence",

@@ -20,7 +20,7 @@ __all__ = ["Simulation", "step_batch", "autotune_batch"]
 class Simulation:
     def __init__(self, mesh, viscosity: float, dt: float, device: GPUSimulationDevice = None,
                  rtol: float = 1e-8, precision: str = "fp64", pair_velocity: bool = True,
-                 max_iterations: int = -1, threads: int = 0):
+                 max_iterations: int = -1, threads: int = 0, assembly: str = "device"):
         self.mesh = mesh
         self.device = device if device is not None else GPUSimulationDevice(0)
         self.options = N.SimOptions(viscosity=float(viscosity), dt=float(dt), rtol=float(rtol),
@@ -28,7 +28,9 @@ class Simulation:
                                     precision={"fp64": N.PRECISION_FP64,
                                                "fp32_refined": N.PRECISION_FP32_REFINED}[precision],
                                     pair_velocity=int(bool(pair_velocity)),
-                                    threads=int(threads))
+                                    threads=int(threads),
+                                    assembly={"device": N.ASSEMBLY_DEVICE,
+                                              "host": N.ASSEMBLY_HOST}[assembly])
         xy, el = mesh.xy(), mesh.elems()
         vb = np.ascontiguousarray(mesh.vel_bc_nodes, np.int32)
         gu = np.ascontiguousarray(mesh.vel_bc_u, np.float64)
