@@ -346,15 +346,7 @@ int nsfem_free_matrix(nsfem_device *h, int s) {
         std::lock_guard<std::mutex> lk(m.mtx);
         d.set_current();
         NSFEM_HIP(hipStreamSynchronize(d.stream));
-        m.ready = false;
-        m.row_start.release();
-        m.col.release();
-        m.val.release();
-        m.val32.release();
-        m.tiles.release();
-        m.wtiles.release();
-        m.block_tiles.release();
-        m.block_tiles_grid = 0;
+        matrix_release(m);
         m.rows = m.cols = 0;
         m.nnz = 0;
     });

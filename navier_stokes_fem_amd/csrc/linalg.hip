@@ -171,7 +171,8 @@ __global__ void k_col_delta(int n_wtiles, const int4 *__restrict__ meta,
 bool col_delta(const Matrix &m, int n_tiles, const int4 *meta, DeviceArray<int16_t> &out) {
     out.release();
     if (m.nnz <= 0 || m.rows != m.cols || n_tiles <= 0) return false;
-    out.resize((size_t)m.nnz);
+    out.resize((size_t)m.nnz + 8);  // 8 zero offsets of padding for group loads
+    NSFEM_HIP(hipMemset(out.ptr + m.nnz, 0, 8 * sizeof(int16_t)));
     DeviceArray<int> bad;
     bad.resize(1);
     NSFEM_HIP(hipMemset(bad.ptr, 0, sizeof(int)));
@@ -187,6 +188,12 @@ bool col_delta(const Matrix &m, int n_tiles, const int4 *meta, DeviceArray<int16
 
 // wave tiles: the persistent CG's offsets
 void build_col_delta(Matrix &m) { m.dcol_ok = col_delta(m, m.n_wtiles, m.wmeta.ptr, m.dcol); }
+
+// Group loads (8 nonzeros per lane from a multiple of 8) need the 16-bit
+// offsets (padded by 8) and every multi-row tile within kStageNnz nonzeros.
+void set_group_loads(Matrix &m) {
+    m.groups_ok = m.dcol_ok && m.nnz > 0 && m.max_multirow_nz <= kStageNnz;
+}
 
 }  // namespace
 
@@ -208,7 +215,7 @@ void build_tiles(Matrix &m, const std::vector<int32_t> &rs) {
                             hipMemcpyHostToDevice));
         m.bcol_ok = col_delta(m, m.n_tiles, dmeta.ptr, m.bcol);
     }
-    build_wave_tiles(m, kWaveRows, kWaveNnz);
+    build_wave_tiles(m, kWaveRows, kStageNnz);
 }
 
 void build_wave_tiles(Matrix &m, int rows, int nnz) {
@@ -239,6 +246,7 @@ void build_wave_tiles(Matrix &m, int rows, int nnz) {
     m.host_block_tiles.reserve(kMaxGridBlocks + 1);
     m.block_tiles_grid = 0;
     build_col_delta(m);
+    set_group_loads(m);
 }
 
 // Persistent blocks own contiguous wave-tile ranges balanced by (nnz + rows).
@@ -272,9 +280,12 @@ __global__ void k_to_float(int64_t n, const double *__restrict__ a, float *__res
 }
 }  // namespace
 
+// The FP32 copy of the values, with 8 zero entries of padding (the group-load
+// product loads 8 values per lane from a multiple of 8).
 void ensure_val32(Device &d, Matrix &m) {
-    if (m.val32.ptr && m.val32.count == (size_t)m.nnz) return;
-    m.val32.resize((size_t)m.nnz);
+    if (m.val32.ptr && m.val32.count == (size_t)m.nnz + 8) return;
+    m.val32.resize((size_t)m.nnz + 8);
+    NSFEM_HIP(hipMemsetAsync(m.val32.ptr + m.nnz, 0, 8 * sizeof(float), d.stream));
     if (m.nnz) {
         const int grid = (int)std::min<int64_t>((m.nnz + 255) / 256, 65536);
         k_to_float<<<grid, 256, 0, d.stream>>>(m.nnz, m.val.ptr, m.val32.ptr);
@@ -887,6 +898,65 @@ struct WTile {
     int r0, r1, k0, nz;
 };
 
+// Sum of a row's products p[ra..rb) in column order, four LDS reads in
+// flight at a time (the adds stay sequential: bit-identical to one by one).
+__device__ __forceinline__ double row_sum(const double *p, int ra, int rb) {
+    double s = 0.0;
+    int k = ra;
+    for (; k + 4 <= rb; k += 4) {
+        const double a0 = p[k], a1 = p[k + 1], a2 = p[k + 2], a3 = p[k + 3];
+        s += a0;
+        s += a1;
+        s += a2;
+        s += a3;
+    }
+    for (; k < rb; ++k) s += p[k];
+    return s;
+}
+
+// The same for NC interleaved columns (products [k][NC]), added to s[c];
+// two nonzeros' products in flight at a time, each column in column order.
+template <int NC>
+__device__ __forceinline__ void row_sums(const double *p, int ra, int rb, double (&s)[NC]) {
+    if constexpr (NC == 1) {
+        s[0] += row_sum(p, ra, rb);
+    } else {
+        int k = ra;
+        for (; k + 2 <= rb; k += 2) {
+            double a[2][NC];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) a[j][c] = p[(k + j) * NC + c];
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s[c] += a[j][c];
+        }
+        for (; k < rb; ++k)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) s[c] += p[k * NC + c];
+    }
+}
+
+// Tile metadata: read from global memory, or from the block's copy in LDS
+// (MetaLds: indexed from the block's first tile), which keeps the per-tile
+// descriptor read out of the vector-memory counter the gathers wait on.
+struct MetaLds {
+    const int4 *m;
+    int t0;
+};
+__device__ __forceinline__ WTile wave_tile(const int4 *meta, int t);
+__device__ __forceinline__ WTile wave_tile(const MetaLds &meta, int t) {
+    const int4 m = meta.m[t - meta.t0];
+    WTile w;
+    w.r0 = __builtin_amdgcn_readfirstlane(m.x);
+    w.r1 = __builtin_amdgcn_readfirstlane(m.y);
+    w.k0 = __builtin_amdgcn_readfirstlane(m.z);
+    w.nz = __builtin_amdgcn_readfirstlane(m.w);
+    return w;
+}
+
 __device__ __forceinline__ WTile wave_tile(const int4 *meta, int t) {
     const int4 m = meta[t];
     WTile w;
@@ -945,8 +1015,9 @@ __device__ __forceinline__ int col_at(const CgArgs &A, int r0, int k) {
 // acc[2NC+c] = (FP32 r).(FP32 r).
 // PACKED (one-column mixed solves): r goes to the packed state `st`; SETUP
 // also clears its p and q.
-template <int NC, int BS, bool MIXED, bool SETUP, typename IT, bool PACKED = false>
-__device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave,
+template <int NC, int BS, bool MIXED, bool SETUP, typename IT, bool PACKED = false,
+          typename MP = const int4 *>
+__device__ void residual_phase(const CgArgs &A, MP meta, int T0, int T1, double *lds_wave,
                                double (&acc)[3 * NC], void *st = nullptr) {
     static_assert(!PACKED || (NC == 1 && MIXED), "packed state: one FP32 column");
     using VT = typename std::conditional<MIXED, float, double>::type;
@@ -956,7 +1027,7 @@ __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave
     VT *r = (VT *)A.r;
     VT *pold = (VT *)A.p0;
     for (int t = T0 + wave; t < T1; t += NW) {
-        const WTile w = wave_tile(A.meta, t);
+        const WTile w = wave_tile(meta, t);
         const int row = w.r0 + lane;
         const bool has_row = row < w.r1;
         const int rowc = has_row ? row : w.r0;
@@ -998,12 +1069,7 @@ __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave
                 }
             }
             __syncwarp();
-            if (has_row) {
-                for (int k = ra; k < rb; ++k) {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) s[c] += lds_wave[k * NC + c];
-                }
-            }
+            if (has_row) row_sums<NC>(lds_wave, ra, rb, s);
             __syncwarp();
         } else {  // a single row longer than a tile: wave-wide sum
 #pragma unroll
@@ -1046,8 +1112,8 @@ __device__ void residual_phase(const CgArgs &A, int T0, int T1, double *lds_wave
 //   deferred FP64 solution update x += pend * p_old, delta += (p_new, q).
 // The products and row sums accumulate in FP64 in both precisions (the
 // product of two FP32 numbers is exact in FP64); only q is stored as VT.
-template <int NC, int BS, bool MIXED, typename IT>
-__device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
+template <int NC, int BS, bool MIXED, typename IT, typename MP>
+__device__ void phase_a(const CgArgs &A, MP meta, int T0, int T1, double *lds_wave,
                         const typename std::conditional<MIXED, float, double>::type *pold,
                         typename std::conditional<MIXED, float, double>::type *pnew,
                         const typename std::conditional<MIXED, float, double>::type (&beta)[NC],
@@ -1073,7 +1139,7 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
     // (a valid column of the tile); a tile without nonzeros gathers nothing.
     // Item slices wholly beyond a tile's nonzeros issue no loads (a 64-row
     // tile of a 7-point matrix fills 7 of its 8 slices): 5% faster here.
-    WTile w = wave_tile(A.meta, t);
+    WTile w = wave_tile(meta, t);
     int cn[kWItems];
     {
         const int kf = w.nz > 0 ? w.k0 : 0;
@@ -1091,7 +1157,7 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
         // 1. next tile: descriptor and column words
         const int tn = t + NW;
         if (tn < T1) {
-            w = wave_tile(A.meta, tn);
+            w = wave_tile(meta, tn);
             const int kf = w.nz > 0 ? w.k0 : 0;
 #pragma unroll
             for (int i = 0; i < kWItems; ++i) {
@@ -1152,10 +1218,7 @@ __device__ void phase_a(const CgArgs &A, int T0, int T1, double *lds_wave,
             __syncwarp();
             const int ra = rsa - cur.k0;
             const int rb = has_row ? rsb - cur.k0 : ra;
-            for (int k = ra; k < rb; ++k) {
-#pragma unroll
-                for (int c = 0; c < NC; ++c) s[c] += lds_wave[k * NC + c];
-            }
+            row_sums<NC>(lds_wave, ra, rb, s);
             __syncwarp();
         } else {
             // a single row longer than a tile: wave-wide sum
@@ -1237,13 +1300,13 @@ __device__ void flush_x(const CgArgs &A, int R0, int R1, const VT *p, const doub
 // The persistent CG run by one block of a solve that owns `nb` blocks: `bid`
 // is the block's index among them (reduction slots), `L` its logical index
 // (tile ownership).  prod_raw / lds are the block's LDS.
-template <int NC, int BS, bool MIXED, typename IT>
-__device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const int nb, const int L,
-                                        double *prod_raw, double *lds) {
+template <int NC, int BS, bool MIXED, typename IT, typename MP>
+__device__ __forceinline__ void cg_body_m(const CgArgs &A, MP meta, const int bid, const int nb,
+                                          const int L, double *prod_raw, double *lds) {
     using VT = typename std::conditional<MIXED, float, double>::type;
     const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
     const int R0 = A.tiles[T0], R1 = A.tiles[T1];
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     double *lds64 = prod_raw + wave * kWaveNnz * NC;
     VT *pbuf[2] = {(VT *)A.p0, (VT *)A.p1};
     unsigned epoch = 0;
@@ -1253,7 +1316,7 @@ __device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const in
     double acc[3 * NC];
 #pragma unroll
     for (int k = 0; k < 3 * NC; ++k) acc[k] = 0.0;
-    residual_phase<NC, BS, MIXED, true, IT>(A, T0, T1, lds64, acc);
+    residual_phase<NC, BS, MIXED, true, IT, false, MP>(A, meta, T0, T1, lds64, acc);
     if (!grid_sum<3 * NC, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
     {
         double tol2[NC], gamma[NC], rmax2[NC];
@@ -1302,7 +1365,7 @@ __device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const in
 #pragma unroll
             for (int c = 0; c < NC; ++c) delta[c] = 0.0;
             CG_TRACE_STAMP(k, 0);
-            phase_a<NC, BS, MIXED, IT>(A, T0, T1, lds64, pold, pnew, beta, pend, has_pend, delta);
+            phase_a<NC, BS, MIXED, IT, MP>(A, meta, T0, T1, lds64, pold, pnew, beta, pend, has_pend, delta);
             CG_TRACE_MAX(k, 1);
             if (!grid_sum<NC, BS>(A.sync, delta, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
             CG_TRACE_STAMP(k, 2);
@@ -1368,7 +1431,7 @@ __device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const in
                 double ra[3 * NC];
 #pragma unroll
                 for (int kk = 0; kk < 3 * NC; ++kk) ra[kk] = 0.0;
-                residual_phase<NC, BS, MIXED, false, IT>(A, T0, T1, lds64, ra);
+                residual_phase<NC, BS, MIXED, false, IT, false, MP>(A, meta, T0, T1, lds64, ra);
                 if (!grid_sum<3 * NC, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
                 ++refines;
 #pragma unroll
@@ -1443,6 +1506,26 @@ timeout:
     }
 }
 
+// Tile descriptors of a block: copied into LDS once per launch when they fit
+// (one-column solves; the two-column solves' product slices fill the LDS).
+constexpr int kMetaLds = 2048;
+
+template <int NC, int BS, bool MIXED, typename IT>
+__device__ __forceinline__ void cg_body(const CgArgs &A, const int bid, const int nb, const int L,
+                                        double *prod_raw, double *lds) {
+    if constexpr (NC == 1) {
+        __shared__ int4 tmeta[kMetaLds];
+        const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
+        if (T1 - T0 <= kMetaLds) {
+            for (int i = threadIdx.x; i < T1 - T0; i += BS) tmeta[i] = A.meta[T0 + i];
+            __syncthreads();
+            cg_body_m<NC, BS, MIXED, IT>(A, MetaLds{tmeta, T0}, bid, nb, L, prod_raw, lds);
+            return;
+        }
+    }
+    cg_body_m<NC, BS, MIXED, IT>(A, A.meta, bid, nb, L, prod_raw, lds);
+}
+
 // ---------------------------------------------------------------------
 // One-column mixed solves: single-reduction iteration over the packed FP32
 // row state.
@@ -1466,8 +1549,9 @@ __device__ __forceinline__ float next_r(float alpha, float r, float q) {
 
 // WI: nonzeros per lane per tile (the host's bound on every multi-row tile's
 // nonzeros over 64); a single longer row is summed wave-wide.
-template <int BS, typename IT, int WI>
-__device__ void phase_f1(const CgArgs &A, int T0, int T1, double *lds_wave, const void *st_old,
+template <int BS, typename IT, int WI, typename MP>
+__device__ void phase_f1(const CgArgs &A, MP meta, int T0, int T1, double *lds_wave,
+                         const void *st_old,
                          void *st_new, const float alpha, const float beta, const double pend,
                          const bool has_pend, double (&sums)[4]) {
     constexpr int NW = BS / 64;
@@ -1478,7 +1562,7 @@ __device__ void phase_f1(const CgArgs &A, int T0, int T1, double *lds_wave, cons
     if (t >= T1) return;
     // column words of the next tile stay raw until the tile is consumed;
     // lanes beyond a tile read its first nonzero (a valid column)
-    WTile w = wave_tile(A.meta, t);
+    WTile w = wave_tile(meta, t);
     int cn[WI];
     {
         const int kf = w.nz > 0 ? w.k0 : 0;
@@ -1495,7 +1579,7 @@ __device__ void phase_f1(const CgArgs &A, int T0, int T1, double *lds_wave, cons
         for (int i = 0; i < WI; ++i) cj[i] = col_of<IT>(cn[i], cur.r0);
         const int tn = t + NW;
         if (tn < T1) {
-            w = wave_tile(A.meta, tn);
+            w = wave_tile(meta, tn);
             const int kf = w.nz > 0 ? w.k0 : 0;
 #pragma unroll
             for (int i = 0; i < WI; ++i) {
@@ -1538,7 +1622,7 @@ __device__ void phase_f1(const CgArgs &A, int T0, int T1, double *lds_wave, cons
             __syncwarp();
             const int ra = rsa - cur.k0;
             const int rb = has_row ? rsb - cur.k0 : ra;
-            for (int k = ra; k < rb; ++k) s += lds_wave[k];
+            s = row_sum(lds_wave, ra, rb);
             __syncwarp();
         } else {
             // a single row longer than a tile: wave-wide sum
@@ -1546,6 +1630,111 @@ __device__ void phase_f1(const CgArgs &A, int T0, int T1, double *lds_wave, cons
             for (int k = lane; k < cur.nz; k += 64) {
                 const RowState g = ld_state(st_old, col_at<IT>(A, cur.r0, cur.k0 + k));
                 part += (double)val[cur.k0 + k] *
+                        (double)__builtin_fmaf(beta, g.p, next_r(alpha, g.r, g.q));
+            }
+            s = wave_sum(part);
+        }
+        if (has_row) {
+            RowState sn;
+            sn.r = next_r(alpha, so.r, so.q);
+            sn.p = __builtin_fmaf(beta, so.p, sn.r);
+            sn.q = (float)s;
+            const double rd = (double)sn.r, qd = (double)sn.q;
+            sums[0] += (double)sn.p * s;
+            sums[1] += rd * rd;
+            sums[2] += rd * qd;
+            sums[3] += qd * qd;
+            st_state(st_new, row, sn);
+            if (has_pend) A.x[row] = __builtin_fma(pend, (double)so.p, xo);
+        }
+    }
+}
+
+// The same fused phase with group loads (one-column mixed solves on
+// matrices with 16-bit offsets).  Each lane owns 8 consecutive nonzeros of a
+// wave tile, counted from the tile's first nonzero rounded down to a
+// multiple of 8, so a tile's values take two 16-byte loads per lane and its
+// column words one, instead of one load per nonzero each.  Entries of the
+// neighbouring tiles in the first and last group are computed too (their
+// offsets, decoded against this tile's first row, are clamped into the
+// matrix) into the slice's 8 entries of padding or its unused tail, and are
+// never summed.  The next tile's column words and values load while the
+// current tile's gathers are in flight.  Products and sums are those of
+// phase_f1 (p_k with the owner's two roundings, FP64 products of FP32
+// factors, row sums in column order), so the iterates are identical.
+template <int BS, typename MP>
+__device__ void phase_f1g(const CgArgs &A, MP meta, int T0, int T1, double *lds_wave,
+                          const void *st_old, void *st_new, const float alpha, const float beta,
+                          const double pend, const bool has_pend, double (&sums)[4]) {
+    constexpr int NW = BS / 64;
+    typedef short i16x8 __attribute__((ext_vector_type(8)));
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int t = T0 + wave;
+    if (t >= T1) return;
+    const int nmax = A.n - 1;
+    struct Pre {  // a tile's group: column words and values
+        i16x8 w;
+        f32x4 v0, v1;
+    };
+    auto fetch = [&](const WTile &w, Pre &p) {
+        const int kk = (w.k0 & ~7) + 8 * lane;
+        const int kc = kk < w.k0 + w.nz ? kk : (w.k0 & ~7);  // idle lanes: the first group
+        p.w = ld_stream(reinterpret_cast<const i16x8 *>(A.dcol + kc));
+        p.v0 = ld_stream(reinterpret_cast<const f32x4 *>(A.val32 + kc));
+        p.v1 = ld_stream(reinterpret_cast<const f32x4 *>(A.val32 + kc + 4));
+    };
+    WTile w = wave_tile(meta, t);
+    Pre pn;
+    if (w.nz > 0 && w.nz <= kStageNnz) fetch(w, pn);
+    for (; t < T1; t += NW) {
+        const WTile cur = w;
+        const Pre pc = pn;
+        const int tn = t + NW;
+        if (tn < T1) {
+            w = wave_tile(meta, tn);
+            if (w.nz > 0 && w.nz <= kStageNnz) fetch(w, pn);
+        }
+        const int row = cur.r0 + lane;
+        const bool has_row = row < cur.r1;
+        const int rowc = has_row ? row : cur.r0;
+        const int rsa = A.rs[rowc];
+        const int rsb = A.rs[rowc + 1];
+        const RowState so = ld_state(st_old, rowc);
+        double xo = 0.0;
+        if (has_pend) xo = A.x[rowc];
+        double s = 0.0;
+        if (cur.nz == 0) {
+            // rows without nonzeros: q = 0
+        } else if (cur.nz <= kStageNnz) {
+            const int kk = (cur.k0 & ~7) + 8 * lane;
+            RowState g[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                g[e] = ld_state(st_old, min(max(cur.r0 + (int)pc.w[e], 0), nmax));
+            if (kk < cur.k0 + cur.nz) {
+                double *dst = lds_wave + (kk - cur.k0);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float v = e < 4 ? pc.v0[e] : pc.v1[e - 4];
+                    dst[e] = (double)v *
+                             (double)__builtin_fmaf(beta, g[e].p, next_r(alpha, g[e].r, g[e].q));
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const int ra = rsa - cur.k0;
+            const int rb = has_row ? rsb - cur.k0 : ra;
+            s = row_sum(lds_wave, ra, rb);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            // a single row longer than a tile: wave-wide sum
+            double part = 0.0;
+            for (int k = lane; k < cur.nz; k += 64) {
+                const RowState g = ld_state(st_old, col_at<int16_t>(A, cur.r0, cur.k0 + k));
+                part += (double)A.val32[cur.k0 + k] *
                         (double)__builtin_fmaf(beta, g.p, next_r(alpha, g.r, g.q));
             }
             s = wave_sum(part);
@@ -1663,15 +1852,20 @@ __device__ void ctl_end(CgCtl1 &C) {
 // The one-column mixed persistent CG run by one block (arguments as
 // cg_body).  The recursive residual is replaced by the FP64 b - A x whenever
 // it has fallen by rdelta (reliable updates).
-template <int BS, typename IT, int WI>
-__device__ __forceinline__ void cg_body1(const CgArgs &A, const int bid, const int nb, const int L,
-                                         double *prod_raw, double *lds) {
+template <int BS, typename IT, int WI, typename MP>
+__device__ __forceinline__ void cg_body1_m(const CgArgs &A, MP meta, const int bid, const int nb,
+                                           const int L, double *prod_raw, double *lds) {
     constexpr bool MIXED = true;  // residual_phase precision
+    // WI 1: group loads (16-bit offsets only); else WI items per lane
+    constexpr bool GROUPS = WI == 1;
+    static_assert(!GROUPS || std::is_same<IT, int16_t>::value, "group loads read 16-bit words");
     __shared__ CgCtl1 C;
     const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
     const int R0 = A.tiles[T0], R1 = A.tiles[T1];
-    const int wave = threadIdx.x >> 6;
-    double *lds64 = prod_raw + wave * kWaveNnz;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // the group-load product writes up to 7 entries before a tile's first
+    // nonzero: each wave's slice starts 8 entries in
+    double *lds64 = prod_raw + wave * (kWaveNnz + (GROUPS ? 8 : 0)) + (GROUPS ? 8 : 0);
     unsigned epoch = 0;
     const bool t0 = threadIdx.x == 0;
     const bool writer = (L == 0 && t0);
@@ -1679,7 +1873,7 @@ __device__ __forceinline__ void cg_body1(const CgArgs &A, const int bid, const i
 
     {
         double acc[3] = {0.0, 0.0, 0.0};
-        residual_phase<1, BS, MIXED, true, IT, true>(A, T0, T1, lds64, acc, A.st[0]);
+        residual_phase<1, BS, MIXED, true, IT, true, MP>(A, meta, T0, T1, lds64, acc, A.st[0]);
         if (!grid_sum<3, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
         if (t0) {
             const double bn = sqrt(acc[1]), r0n = sqrt(acc[0]);
@@ -1710,8 +1904,12 @@ __device__ __forceinline__ void cg_body1(const CgArgs &A, const int bid, const i
         const bool has_pend = C.has_pend != 0;
         double sums[4] = {0.0, 0.0, 0.0, 0.0};
         CG_TRACE_STAMP(C.k, 0);
-        phase_f1<BS, IT, WI>(A, T0, T1, lds64, A.st[cur], A.st[nx], alpha, beta, pend, has_pend,
-                             sums);
+        if constexpr (GROUPS)
+            phase_f1g<BS, MP>(A, meta, T0, T1, lds64, A.st[cur], A.st[nx], alpha, beta, pend,
+                              has_pend, sums);
+        else
+            phase_f1<BS, IT, WI, MP>(A, meta, T0, T1, lds64, A.st[cur], A.st[nx], alpha, beta,
+                                     pend, has_pend, sums);
         CG_TRACE_MAX(C.k, 1);
         if (!grid_sum<4, BS>(A.sync, sums, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
         CG_TRACE_STAMP(C.k, 2);
@@ -1727,7 +1925,7 @@ __device__ __forceinline__ void cg_body1(const CgArgs &A, const int bid, const i
             double dummy[1] = {0.0};
             if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
             double ra[3] = {0.0, 0.0, 0.0};
-            residual_phase<1, BS, MIXED, false, IT, true>(A, T0, T1, lds64, ra, A.st[nx]);
+            residual_phase<1, BS, MIXED, false, IT, true, MP>(A, meta, T0, T1, lds64, ra, A.st[nx]);
             if (!grid_sum<3, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
             if (t0) {
                 ctl_after_update(C, A, ra);
@@ -1752,13 +1950,27 @@ timeout:
     if (writer) A.res[0].status = NSFEM_BARRIER_TIMEOUT;
 }
 
+template <int BS, typename IT, int WI>
+__device__ __forceinline__ void cg_body1(const CgArgs &A, const int bid, const int nb, const int L,
+                                         double *prod_raw, double *lds) {
+    __shared__ int4 tmeta[kMetaLds];
+    const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
+    if (T1 - T0 <= kMetaLds) {
+        for (int i = threadIdx.x; i < T1 - T0; i += BS) tmeta[i] = A.meta[T0 + i];
+        __syncthreads();
+        cg_body1_m<BS, IT, WI>(A, MetaLds{tmeta, T0}, bid, nb, L, prod_raw, lds);
+    } else {
+        cg_body1_m<BS, IT, WI>(A, A.meta, bid, nb, L, prod_raw, lds);
+    }
+}
+
 // One block per CU: BS / 256 waves per SIMD (4 for 1024 threads, 128 VGPRs
 // per wave; 2 for 512 threads, up to 256 VGPRs).
 // WI (items per lane of a tile) < 8 is used by the one-column mixed solve only.
 template <int NC, int BS, bool MIXED, typename IT, int WI>
 __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
     constexpr int NW = BS / 64;
-    __shared__ double prod_raw[NW * kWaveNnz * NC];
+    __shared__ double prod_raw[NW * (kWaveNnz * NC + (WI == 1 ? 8 : 0))];
     __shared__ double lds[kMaxRed * NW];
     const int L = xcd_swizzle(blockIdx.x, gridDim.x);
     if constexpr (NC == 1 && MIXED)
@@ -1775,7 +1987,7 @@ template <int NC, int BS, bool MIXED, typename IT, int WI>
 __global__ __launch_bounds__(BS, BS / 256) void k_cg_batch(const CgArgs *__restrict__ args,
                                                            const int2 *__restrict__ map) {
     constexpr int NW = BS / 64;
-    __shared__ double prod_raw[NW * kWaveNnz * NC];
+    __shared__ double prod_raw[NW * (kWaveNnz * NC + (WI == 1 ? 8 : 0))];
     __shared__ double lds[kMaxRed * NW];
     const int2 m = map[blockIdx.x];
     const int inst = __builtin_amdgcn_readfirstlane(m.x);
@@ -1788,18 +2000,34 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg_batch(const CgArgs *__restr
 }
 
 constexpr int kCgBlock = 1024;
+// One-column mixed solves may run 768-thread blocks: 12 waves per CU with
+// 168 VGPRs each instead of 16 with 128, which the tile loop needs to hold
+// a prefetched tile without spilling (a scratch reload inside the loop waits
+// for every load in flight, the prefetch included).
+constexpr int kCgBlockWide = 768;
 
-template <int NC, bool MIXED, typename IT, int WI>
+// Block size of a solve: one-column mixed solves run the wide-register
+// blocks unless NSFEM_CG_BLOCK=1024 (read at every solve; measured at the
+// 2048^2 cavity, 768 threads with group loads: pressure 95.9 us and v
+// diffusion 550 us per iteration against 98.8 / 564 us for 1024 threads
+// with one load per nonzero per lane).
+int cg_block(int nc, bool mixed) {
+    if (!(nc == 1 && mixed)) return kCgBlock;
+    const char *e = std::getenv("NSFEM_CG_BLOCK");
+    return (e && std::atoi(e) == kCgBlock) ? kCgBlock : kCgBlockWide;
+}
+
+template <int NC, bool MIXED, typename IT, int WI, int BS>
 void cg_launch(Device &d, const CgArgs &args, int grid) {
     void *kargs[] = {(void *)&args};
-    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg<NC, kCgBlock, MIXED, IT, WI>,
-                                         dim3(grid), dim3(kCgBlock), kargs, 0, d.stream));
+    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg<NC, BS, MIXED, IT, WI>, dim3(grid),
+                                         dim3(BS), kargs, 0, d.stream));
 }
-template <int NC, bool MIXED, typename IT, int WI>
+template <int NC, bool MIXED, typename IT, int WI, int BS>
 int cg_max_grid(Device &d) {
     int per_cu = 0;
-    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, k_cg<NC, kCgBlock, MIXED, IT, WI>, kCgBlock, 0));
+    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg<NC, BS, MIXED, IT, WI>,
+                                                           BS, 0));
     return std::max(1, std::min(per_cu * d.cu_count, kMaxGridBlocks));
 }
 
@@ -1825,9 +2053,11 @@ void cg_dispatch_wi(int nc, bool mixed, bool idx16, F &f) {
         }
     }
 }
+// wi 1: the one-column mixed iteration with group loads (16-bit offsets)
 template <typename F>
 void cg_dispatch(int nc, bool mixed, bool idx16, int wi, F &&f) {
-    if (wi <= 4) cg_dispatch_wi<4>(nc, mixed, idx16, f);
+    if (wi == 1 && nc == 1 && mixed && idx16) f.template run<1, true, int16_t, 1>();
+    else if (wi <= 4) cg_dispatch_wi<4>(nc, mixed, idx16, f);
     else if (wi <= 7) cg_dispatch_wi<7>(nc, mixed, idx16, f);
     else cg_dispatch_wi<kWItems>(nc, mixed, idx16, f);
 }
@@ -1839,18 +2069,49 @@ int items_per_lane(const Matrix &m) {
     return need <= 4 ? 4 : need <= 7 ? 7 : kWItems;
 }
 
+// Form of the one-column mixed iteration on matrices with 16-bit offsets:
+// NSFEM_CG_PHASE = items (one load per nonzero per lane) or groups (8
+// nonzeros per lane); without it, groups where `groups_default` (the
+// single solves' wide blocks) and the matrix allows them.  Read at every
+// solve (comparison runs switch it in one process).
+int phase_form(const Matrix &m, bool groups_default) {
+    const char *e = std::getenv("NSFEM_CG_PHASE");
+    const std::string v = e ? e : "";
+    if (v == "items") return -1;
+    if (v == "groups" || (v.empty() && groups_default)) return m.groups_ok ? 1 : -1;
+    return -1;
+}
+
 struct CgMaxGrid {
     Device &d;
     int out;
+    int bs = kCgBlock;
     template <int NC, bool MX, typename IT, int WI>
-    void run() { out = cg_max_grid<NC, MX, IT, WI>(d); }
+    void run() {
+        if constexpr (NC == 1 && MX) {
+            if (bs == kCgBlockWide) {
+                out = cg_max_grid<NC, MX, IT, WI, kCgBlockWide>(d);
+                return;
+            }
+        }
+        out = cg_max_grid<NC, MX, IT, WI, kCgBlock>(d);
+    }
 };
 struct CgLaunch {
     Device &d;
     const CgArgs &a;
     int grid;
+    int bs = kCgBlock;
     template <int NC, bool MX, typename IT, int WI>
-    void run() { cg_launch<NC, MX, IT, WI>(d, a, grid); }
+    void run() {
+        if constexpr (NC == 1 && MX) {
+            if (bs == kCgBlockWide) {
+                cg_launch<NC, MX, IT, WI, kCgBlockWide>(d, a, grid);
+                return;
+            }
+        }
+        cg_launch<NC, MX, IT, WI, kCgBlock>(d, a, grid);
+    }
 };
 
 
@@ -1968,13 +2229,16 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
     Matrix &m = *req.m;
     const int n = m.rows, nc = req.ncol;
     if (mixed) ensure_val32(d, m);
-    // 1024 threads per block (16 waves per CU, 128 VGPRs each): the loop is
-    // bound by gather latency, so waves in flight matter more than
-    // registers; 512 threads (8 waves, up to 256 VGPRs) was slower on every
-    // measured solve.
+    // 1024 threads per block (16 waves per CU, 128 VGPRs each) for FP64
+    // and two-column solves: the loop is bound by gather latency, so waves
+    // in flight matter more than registers; 512 threads (8 waves, up to 256
+    // VGPRs) was slower on every measured solve.  One-column mixed solves
+    // run 768 (cg_block).
     const bool idx16 = m.dcol_ok && want_idx16();
-    const int wi = items_per_lane(m);
-    CgMaxGrid mg{d, 0};
+    const int bs = cg_block(nc, mixed);
+    const int form = idx16 && mixed && nc == 1 ? phase_form(m, bs == kCgBlockWide) : -1;
+    const int wi = form >= 0 ? form : items_per_lane(m);
+    CgMaxGrid mg{d, 0, bs};
     cg_dispatch(nc, mixed, idx16, wi, mg);
     const int maxg = mg.out;
     int want = req.grid;
@@ -1984,7 +2248,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
         if (env) want = std::atoi(env);
     }
     int grid = want > 0 ? std::min(want, maxg)
-                        : std::min(maxg, std::max(1, m.n_wtiles / (2 * (kCgBlock / 64))));
+                        : std::min(maxg, std::max(1, m.n_wtiles / (2 * (bs / 64))));
     grid = std::max(1, std::min(grid, std::max(1, m.n_wtiles)));
     if (std::getenv("NSFEM_CG_VERBOSE"))
         std::fprintf(stderr, "[nsfem] cg rows=%d cols=%d wave tiles=%d grid=%d maxgrid=%d mixed=%d idx16=%d\n",
@@ -2022,7 +2286,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
         NSFEM_HIP(hipEventRecord(d.ev_start, d.stream));
         k_sync_reset<<<(ngroups + 255) / 256, 256, 0, d.stream>>>(a.sync, ngroups);
         NSFEM_HIP(hipGetLastError());
-        CgLaunch ln{d, a, grid};
+        CgLaunch ln{d, a, grid, bs};
         cg_dispatch(nc, mixed, idx16, wi, ln);
         NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
         NSFEM_HIP(hipMemcpyAsync(P.res, ws.res, sizeof(CgColumnResult) * nc,
@@ -2245,11 +2509,16 @@ int run_cg_batch(Device &d, const std::vector<CgRequest> &reqs, nsfem_cg_stats *
         NSFEM_REQUIRE(nc == 1 || nc == 2, NSFEM_INVALID_ARGUMENT, "columns must be 1 or 2");
     }
     bool idx16 = want_idx16();
+    // batched launches run 1024-thread blocks: one load per nonzero per
+    // lane unless NSFEM_CG_PHASE says otherwise
+    int form = mixed && nc == 1 ? phase_form(*reqs[0].m, false) : -1;
     int wi = 4;
     for (const CgRequest &r : reqs) {
         idx16 = idx16 && r.m->dcol_ok;
+        if (phase_form(*r.m, false) != form) form = -1;
         wi = std::max(wi, items_per_lane(*r.m));
     }
+    if (form >= 0 && idx16) wi = form;
     CgBatchMaxGrid mg{d, 0};
     cg_dispatch(nc, mixed, idx16, wi, mg);
     const int maxg = mg.out;

@@ -103,7 +103,11 @@ constexpr int kBlock = 256;
 constexpr int kTileRows = 256;
 constexpr int kTileNnz = 2048;
 constexpr int kWaveRows = 64;   // wave tiles of the persistent CG
-constexpr int kWaveNnz = 512;
+constexpr int kWaveNnz = 512;   // LDS room of a wave tile's products
+// default cap on a wave tile's nonzeros: a tile's nonzeros from its first
+// one rounded down to a multiple of 8 then fill at most 64 groups of 8 (one
+// group per lane in the group-load product)
+constexpr int kStageNnz = kWaveNnz - 8;
 constexpr int kMaxGridBlocks = 4096;
 
 struct Matrix {
@@ -135,6 +139,9 @@ struct Matrix {
     // the same for the product's tiles (offsets from the tile's first row)
     DeviceArray<int16_t> bcol;
     bool bcol_ok = false;
+    // group loads (8 nonzeros per lane) need the 16-bit offsets (padded by
+    // 8) and every multi-row tile within kStageNnz nonzeros
+    bool groups_ok = false;
     bool ready = false;
     std::mutex mtx;
 };
@@ -146,6 +153,7 @@ inline void matrix_release(Matrix &m) {
     m.dcol_ok = false;
     m.bcol.release();
     m.bcol_ok = false;
+    m.groups_ok = false;
     m.block_tiles_grid = 0;
     m.ready = false;
 }
