@@ -633,16 +633,25 @@ void launch_copy(Device &d, int n, const double *a, double *y) {
 // and so do sparse reliable updates (reliable_delta 0.1: +33%); with x and
 // the accumulations in FP64 and reliable_delta 0.5 the count equals FP64 CG's.
 //
-// Grid reduction: partial sums are stored per block; arrivals count on one
-// counter per group of kGroup blocks, the last arrival of a group adds the
-// group's partials in block order and arrives at the top counter, the last
-// top arrival adds the group sums in group order and releases every group.
-// The summation order is fixed, so results are deterministic.
+// Grid reduction of a solve over at most kFlatMax blocks: each block posts
+// its partial sums and an epoch flag, then every block waits for all flags
+// and adds all partials itself, in block order -- one write and one round of
+// polling on the critical path, and every block holds the same bits.
+// Larger grids use a tree: arrivals count on one counter per group of
+// kGroup blocks, the last arrival of a group adds the group's partials in
+// block order and arrives at the top counter, the last top arrival adds the
+// group sums in group order and releases every group.  Either way the
+// summation order is fixed, so results are deterministic.  Measured on the
+// 512x128 channel's pressure matrix (65,656 rows), per reduction: flat
+// 2.8-5.5 us against the tree's 5.1-10 us at 16-64 blocks, about even at
+// 128, and the tree ahead at 256 (the flags of all blocks sit on a few
+// cache lines that every block polls).
 
 namespace {
 
 constexpr int kGroup = 16;
 constexpr int kMaxGroups = kMaxGridBlocks / kGroup;
+constexpr int kFlatMax = 128;
 
 struct alignas(256) SyncState {
     unsigned int arrive_top;
@@ -655,6 +664,11 @@ struct alignas(256) SyncState {
     double pad2[24];
     double gpartial[kMaxRed][kMaxGroups];
     double partial[kMaxRed][kMaxGridBlocks];
+    // flat reduction: per block the last epoch whose partials are posted,
+    // and the partials by epoch parity (a block overwrites a parity's slots
+    // only after every block has posted the next epoch, i.e. has read them)
+    unsigned int flag[kFlatMax];
+    double fpart[2][kMaxRed][kFlatMax];
 };
 
 #ifdef NSFEM_CG_TRACE
@@ -720,14 +734,17 @@ struct CgArgs {
     SyncState *sync;
     CgColumnResult *res;
     int nblocks;  // blocks of this solve (batched launches)
+    int flat;     // flat grid reduction for solves of <= kFlatMax blocks
 };
 
+// Launched with at least kFlatMax threads.
 __global__ void k_sync_reset(SyncState *s, int ngroups) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         s->arrive_top = 0;
         s->abort = 0;
     }
+    if (i < kFlatMax) s->flag[i] = 0;
     if (i < ngroups) {
         s->arrive_group[i][0] = 0;
         s->release_group[i][0] = 0;
@@ -755,14 +772,80 @@ __device__ __forceinline__ double ld_relaxed(const double *p) {
 // blocks wrote.  The last arrival at a counter (the group's, then the top
 // one) reads the counter's release sequence and performs an acquire fence
 // before it adds the other blocks' partials.
+//
+// Flat form (nb <= kFlatMax): thread 0 posts the block's partials at the
+// coherence point and then its flag with an agent-scope release (the
+// block's stores, completed to L2 by every wave's s_waitcnt before the
+// block barrier, are written back with it).  Wave 0 polls every block's
+// flag with relaxed agent-scope loads, performs one acquire fence once all
+// are posted, and adds the partials in block order (lane i takes blocks i,
+// i + 64, ...; then a fixed butterfly), so every block computes the same
+// sum.  A block posts epoch e + 1 only after its wave 0 has read epoch e's
+// partials, which is what makes two parity slots enough.
+template <int K, int BS>
+__device__ bool grid_sum_flat(SyncState *S, double (&v)[K], const unsigned epoch,
+                              unsigned long long timeout, const int b, const int nb) {
+    static_assert(K <= kMaxRed, "reduction width");
+    __shared__ double s_sum[K];
+    __shared__ int s_ok;
+    const int par = epoch & 1u;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            __hip_atomic_store(&S->fpart[par][k][b], v[k], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&S->flag[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int ok = 1;
+        const unsigned long long t0 = wall_clock64();
+        for (;;) {
+            bool mine = true;
+            for (int i = lane; i < nb; i += 64)
+                mine = mine && __hip_atomic_load(&S->flag[i], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) >= epoch;
+            if (__all(mine)) break;
+            if (__any(__hip_atomic_load(&S->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                      0u)) {
+                ok = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > timeout) {
+                if (lane == 0)
+                    __hip_atomic_store(&S->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double t = 0.0;
+            for (int i = lane; i < nb; i += 64) t += ld_relaxed(&S->fpart[par][k][i]);
+            t = wave_sum(t);
+            if (lane == 0) s_sum[k] = t;
+        }
+        if (lane == 0) s_ok = ok;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = s_sum[k];
+    return s_ok != 0;
+}
+
 template <int K, int BS>
 __device__ bool grid_sum(SyncState *S, double (&v)[K], unsigned &epoch, double *lds,
-                         unsigned long long timeout, const int b, const int nb) {
+                         unsigned long long timeout, const int b, const int nb, const bool flat) {
     // Each flag is written by thread 0 before a __syncthreads and read by
     // the block after it; none is rewritten until the next call, which
     // every thread reaches only after passing several more barriers.
     __shared__ int s_group_last, s_top_last, s_ok;
     block_sum<K, BS>(v, lds);
+    if (flat && nb <= kFlatMax) return grid_sum_flat<K, BS>(S, v, ++epoch, timeout, b, nb);
     const int g = b / kGroup;
     const int ng = (nb + kGroup - 1) / kGroup;
     const int gsize = min(kGroup, nb - g * kGroup);
@@ -1317,7 +1400,7 @@ __device__ __forceinline__ void cg_body_m(const CgArgs &A, MP meta, const int bi
 #pragma unroll
     for (int k = 0; k < 3 * NC; ++k) acc[k] = 0.0;
     residual_phase<NC, BS, MIXED, true, IT, false, MP>(A, meta, T0, T1, lds64, acc);
-    if (!grid_sum<3 * NC, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+    if (!grid_sum<3 * NC, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
     {
         double tol2[NC], gamma[NC], rmax2[NC];
         bool active[NC];
@@ -1367,7 +1450,7 @@ __device__ __forceinline__ void cg_body_m(const CgArgs &A, MP meta, const int bi
             CG_TRACE_STAMP(k, 0);
             phase_a<NC, BS, MIXED, IT, MP>(A, meta, T0, T1, lds64, pold, pnew, beta, pend, has_pend, delta);
             CG_TRACE_MAX(k, 1);
-            if (!grid_sum<NC, BS>(A.sync, delta, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+            if (!grid_sum<NC, BS>(A.sync, delta, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
             CG_TRACE_STAMP(k, 2);
             double alpha[NC];
             VT alpha_v[NC];
@@ -1389,7 +1472,7 @@ __device__ __forceinline__ void cg_body_m(const CgArgs &A, MP meta, const int bi
             for (int c = 0; c < NC; ++c) g[c] = 0.0;
             phase_b<NC, BS, VT>(A, R0, R1, alpha_v, g);
             CG_TRACE_MAX(k, 3);
-            if (!grid_sum<NC, BS>(A.sync, g, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+            if (!grid_sum<NC, BS>(A.sync, g, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
             CG_TRACE_STAMP(k, 4);
             ++k;
             bool want_ru = false;
@@ -1427,12 +1510,12 @@ __device__ __forceinline__ void cg_body_m(const CgArgs &A, MP meta, const int bi
 #pragma unroll
                 for (int c = 0; c < NC; ++c) pend[c] = 0.0;
                 double dummy[1] = {0.0};
-                if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+                if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
                 double ra[3 * NC];
 #pragma unroll
                 for (int kk = 0; kk < 3 * NC; ++kk) ra[kk] = 0.0;
                 residual_phase<NC, BS, MIXED, false, IT, false, MP>(A, meta, T0, T1, lds64, ra);
-                if (!grid_sum<3 * NC, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+                if (!grid_sum<3 * NC, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
                 ++refines;
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
@@ -1874,7 +1957,7 @@ __device__ __forceinline__ void cg_body1_m(const CgArgs &A, MP meta, const int b
     {
         double acc[3] = {0.0, 0.0, 0.0};
         residual_phase<1, BS, MIXED, true, IT, true, MP>(A, meta, T0, T1, lds64, acc, A.st[0]);
-        if (!grid_sum<3, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+        if (!grid_sum<3, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
         if (t0) {
             const double bn = sqrt(acc[1]), r0n = sqrt(acc[0]);
             const double ref = bn > 0.0 ? bn : r0n;
@@ -1911,7 +1994,7 @@ __device__ __forceinline__ void cg_body1_m(const CgArgs &A, MP meta, const int b
             phase_f1<BS, IT, WI, MP>(A, meta, T0, T1, lds64, A.st[cur], A.st[nx], alpha, beta,
                                      pend, has_pend, sums);
         CG_TRACE_MAX(C.k, 1);
-        if (!grid_sum<4, BS>(A.sync, sums, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+        if (!grid_sum<4, BS>(A.sync, sums, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
         CG_TRACE_STAMP(C.k, 2);
         if (t0) {
             ctl_after_phase(C, A, sums);
@@ -1923,10 +2006,10 @@ __device__ __forceinline__ void cg_body1_m(const CgArgs &A, MP meta, const int b
             // recursive residual: r_k := b - A x, alpha := 0
             flush_x1<BS>(A, R0, R1, A.st[nx], C.pend);
             double dummy[1] = {0.0};
-            if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+            if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
             double ra[3] = {0.0, 0.0, 0.0};
             residual_phase<1, BS, MIXED, false, IT, true, MP>(A, meta, T0, T1, lds64, ra, A.st[nx]);
-            if (!grid_sum<3, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb)) goto timeout;
+            if (!grid_sum<3, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
             if (t0) {
                 ctl_after_update(C, A, ra);
                 ctl_end(C);
@@ -2203,6 +2286,9 @@ void set_cg_params(const Device &d, const CgRequest &req, int n, CgArgs &a) {
     a.ru_gap = ru_gap_env > 0 ? ru_gap_env : (req.ru_gap > 0 ? req.ru_gap : 20000);
     a.ru_min_gap = ru_min_gap_env >= 0 ? ru_min_gap_env : 0;
     a.timeout_ticks = (unsigned long long)d.wall_khz * 1000ull * 20ull;  // 20 s per barrier
+    // the flat grid reduction unless NSFEM_CG_FLAT=0 (read at every solve)
+    const char *flat = std::getenv("NSFEM_CG_FLAT");
+    a.flat = !(flat && std::atoi(flat) == 0);
 }
 
 // Packed FP32 {r, p, q} rows of a one-column mixed solve: two copies of 3
