@@ -567,7 +567,8 @@ def bench_learned_channels(dev, args):
         held = mesh.channel(nx, ny, seed=seed, umax=1.25)
         r = L.evaluate_warm_start(dev, held, nets["solution"][0], steps=20, warmup=10, dt=5e-4,
                                   modes=("previous", "zero", "learned", "learned_or_previous",
-                                         "learned_correction"),
+                                         "learned_correction", "learned_search", "projection",
+                                         "learned_projection"),
                                   correction_net=nets["correction"][0])
         evals[f"{nx}x{ny}"] = r
         for mode, v in r.items():
@@ -577,10 +578,18 @@ def bench_learned_channels(dev, args):
     best_mode = min(evals["1024x256"], key=lambda k: evals["1024x256"][k]["pressure_iterations_per_step"])
     for net, _, _ in nets.values():
         net.close()
-    return _line("pressure CG iterations/step with the learned warm start vs previous-step guess "
-                 "(1024x256 held-out channel)",
-                 evals["1024x256"]["learned_correction"]["pressure_iterations_per_step"],
-                 "iterations/step", 20, 1e3 / evals["1024x256"]["learned_correction"]["steps_per_s"],
+    ev = evals["1024x256"]
+    gains = {size: {"learned_search_vs_previous":
+                    1 - r["learned_search"]["pressure_iterations_per_step"]
+                    / r["previous"]["pressure_iterations_per_step"],
+                    "learned_projection_vs_projection":
+                    1 - r["learned_projection"]["pressure_iterations_per_step"]
+                    / r["projection"]["pressure_iterations_per_step"]}
+             for size, r in evals.items()}
+    return _line("pressure CG iterations/step with the learned warm start (A-norm-optimal step "
+                 "from the previous pressure) vs previous-step guess (1024x256 held-out channel)",
+                 ev["learned_search"]["pressure_iterations_per_step"],
+                 "iterations/step", 20, 1e3 / ev["learned_search"]["steps_per_s"],
                  {"workload": "channel 512x128 training meshes (50 seeds x 400 steps = 20000 pairs, "
                               "inflow max ~ U(0.5,2)), "
                               "held-out seeds at 512x128 and 1024x256, nu=1e-3, dt=5e-4",
@@ -589,7 +598,7 @@ def bench_learned_channels(dev, args):
                  training={k: {"pairs": v[2], "loss_per_epoch": v[1],
                                "held_out_loss": held_loss[k]} for k, v in nets.items()},
                  evaluation=evals, previous_guess_iterations=base["pressure_iterations_per_step"],
-                 best_mode=best_mode)
+                 network_gain=gains, best_mode=best_mode)
 
 
 def main():

@@ -446,6 +446,11 @@ int nsfem_gnn_train_step(nsfem_gnn *gnn, nsfem_gnn_graph *graph,
 #define NSFEM_WARM_PROJECTION 6          /* A-norm projection of the solution onto
                                             the span of past pressures (Fischer);
                                             basis depth NSFEM_PROJECTION_DEPTH */
+#define NSFEM_WARM_LEARNED_SEARCH 7      /* p_prev + c network(b - A p_prev), c the
+                                            A-norm-optimal step along the prediction
+                                            (never worse than p_prev in the A-norm) */
+#define NSFEM_WARM_LEARNED_PROJECTION 8  /* the same step from the projection guess
+                                            (mode 6) instead of p_prev */
 int nsfem_simulation_set_warm_start(nsfem_simulation *sim, int mode, nsfem_gnn *gnn);
 /* Initial guess of the diffusion solves: the advected velocity (the
  * reference's, include/assembly.h:1521-1531) plus the A-norm projection of

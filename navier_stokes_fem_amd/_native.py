@@ -35,6 +35,8 @@ WARM_LEARNED_OR_PREVIOUS = 3
 WARM_LEARNED_CORRECTION = 4
 WARM_EXTRAPOLATE = 5
 WARM_PROJECTION = 6
+WARM_LEARNED_SEARCH = 7
+WARM_LEARNED_PROJECTION = 8
 GNN_PARAM_COUNT = 4481
 
 

@@ -470,6 +470,17 @@ __global__ void k_select(int n, const double *__restrict__ scal, const double *_
     if (i < n) dst[i] = src[i];
 }
 
+// x += c d with c = scal[0] / scal[1] = (d . r) / (d . A d), the step along
+// d that minimises the A-norm error of x (r = b - A x); no step unless
+// d . A d > 0 and c is finite
+__global__ void k_line_step(int n, const double *__restrict__ scal, const double *__restrict__ dir,
+                            double *__restrict__ x) {
+    const double den = scal[1], c = scal[0] / den;
+    if (!(den > 0.0) || !(c == c) || c - c != 0.0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = __builtin_fma(c, dir[i], x[i]);
+}
+
 inline int blocks_for(int64_t work, int bs = kRB) { return (int)((work + bs - 1) / bs); }
 
 }  // namespace gnn
@@ -758,6 +769,19 @@ void select_better_guess(Device &d, Matrix &A, const double *b, const double *sr
     launch_spmv(d, A, src, b, tmp, 1);
     launch_dot_accumulate(d, n, tmp, tmp, scal2 + 1);
     gnn::k_select<<<gnn::blocks_for(n), gnn::kRB, 0, d.stream>>>(n, scal2, src, dst);
+    NSFEM_HIP(hipGetLastError());
+}
+
+// x += c d, c the A-norm-optimal step along d from x (r = b - A x on entry;
+// no host sync); w: n doubles of scratch (receives A d), scal2: 2 doubles
+void line_search_step(Device &d, Matrix &A, const double *r, const double *dir, double *x,
+                      double *w, double *scal2) {
+    const int n = A.rows;
+    launch_fill(d, 2, 0.0, scal2);
+    launch_spmv(d, A, dir, nullptr, w, 0);
+    launch_dot_accumulate(d, n, dir, r, scal2);
+    launch_dot_accumulate(d, n, dir, w, scal2 + 1);
+    gnn::k_line_step<<<gnn::blocks_for(n), gnn::kRB, 0, d.stream>>>(n, scal2, dir, x);
     NSFEM_HIP(hipGetLastError());
 }
 

@@ -126,6 +126,7 @@ struct Matrix {
     std::vector<int32_t> host_rs;            // row_start (wave tiles are rebuilt from it)
     int wave_rows = kWaveRows, wave_nnz = kWaveNnz;  // current wave-tile shape
     int max_multirow_nz = 0;  // most nonzeros of a wave tile with more than one row
+    int max_tile_nz = 0;      // most nonzeros of any wave tile
     // ownership of wave tiles by persistent blocks, cached per grid size:
     // both arrays hold kMaxGridBlocks + 1 entries from the tiling on, so a
     // change of grid size copies and never allocates
@@ -142,6 +143,7 @@ struct Matrix {
     // group loads (8 nonzeros per lane) need the 16-bit offsets (padded by
     // 8) and every multi-row tile within kStageNnz nonzeros
     bool groups_ok = false;
+    bool pipe_ok = false;  // and every tile within kStageNnz (pipelined form)
     bool ready = false;
     std::mutex mtx;
 };
@@ -154,6 +156,7 @@ inline void matrix_release(Matrix &m) {
     m.bcol.release();
     m.bcol_ok = false;
     m.groups_ok = false;
+    m.pipe_ok = false;
     m.block_tiles_grid = 0;
     m.ready = false;
 }
@@ -402,6 +405,11 @@ void gnn_train_step(GnnNet &g, GnnGraph &gr, const double *b, const double *p, f
 // tmp: A.rows doubles, scal2: 2 doubles
 void select_better_guess(Device &d, Matrix &A, const double *b, const double *src, double *dst,
                          double *tmp, double *scal2);
+// x += c dir with c = (dir . r) / (dir . A dir), r = b - A x: the step along
+// dir that minimises the A-norm error (none unless dir . A dir > 0; no host
+// sync); w: A.rows doubles of scratch, scal2: 2 doubles
+void line_search_step(Device &d, Matrix &A, const double *r, const double *dir, double *x,
+                      double *w, double *scal2);
 inline void gnn_release(GnnNet &g) {
     g.params.release(); g.grads.release(); g.m.release(); g.v.release();
     g.wt1.release(); g.wt2.release(); g.wr2.release(); g.wt3.release(); g.wr3.release();

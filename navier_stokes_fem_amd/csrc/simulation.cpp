@@ -90,7 +90,7 @@ struct Simulation {
     GnnGraph *graph = nullptr;  // owned, built on first use of a learned mode
     std::vector<double> p_area;  // lumped P1 area of each pressure node
     std::vector<float> p_xy;     // coordinates of the pressure nodes
-    DeviceArray<double> pguess, ptmp, pscal;
+    DeviceArray<double> pguess, ptmp, pscal, pw;
     DeviceArray<double> p_older, p_keep;  // p_(n-1) and p_n for the extrapolation
     bool have_older = false;
     // projection warm start of the pressure solve
@@ -1131,13 +1131,27 @@ void pressure_guess_base(Simulation &s) {
         select_better_guess(d, A, s.prhs.ptr, s.pguess.ptr, s.p.ptr, s.ptmp.ptr, s.pscal.ptr);
         return;
     }
+    case NSFEM_WARM_LEARNED_SEARCH:
+    case NSFEM_WARM_LEARNED_PROJECTION: {
+        // from the previous pressure (or its projection onto past
+        // pressures): r = b - A x, then the network's correction d = net(r)
+        // taken with the A-norm-optimal step, x += c d: no worse than x in
+        // the A-norm whatever the prediction
+        Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+        if (s.warm_mode == NSFEM_WARM_LEARNED_PROJECTION)
+            projection_guess(*s.dev, s.proj, s.prhs.ptr, s.p.ptr);
+        launch_spmv(d, A, s.p.ptr, s.prhs.ptr, s.ptmp.ptr, 1);
+        gnn_predict(*s.gnn, *s.graph, s.ptmp.ptr, s.pguess.ptr);
+        line_search_step(d, A, s.ptmp.ptr, s.pguess.ptr, s.p.ptr, s.pw.ptr, s.pscal.ptr);
+        return;
+    }
     }
 }
 
 // after the pressure solve: the extrapolating mode's p_n becomes p_(n-1);
 // the projecting mode extends its basis
 void pressure_after_solve(Simulation &s) {
-    if (s.warm_mode == NSFEM_WARM_PROJECTION) {
+    if (s.warm_mode == NSFEM_WARM_PROJECTION || s.warm_mode == NSFEM_WARM_LEARNED_PROJECTION) {
         projection_update(*s.dev, s.proj, s.mats[NSFEM_PRESSURE_STIFFNESS], s.p.ptr);
         return;
     }
@@ -1149,10 +1163,12 @@ void pressure_after_solve(Simulation &s) {
 }  // namespace
 
 void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
-    NSFEM_REQUIRE(mode >= NSFEM_WARM_PREVIOUS && mode <= NSFEM_WARM_PROJECTION,
+    NSFEM_REQUIRE(mode >= NSFEM_WARM_PREVIOUS && mode <= NSFEM_WARM_LEARNED_PROJECTION,
                   NSFEM_INVALID_ARGUMENT, "unknown warm-start mode");
     const bool learned = mode == NSFEM_WARM_LEARNED || mode == NSFEM_WARM_LEARNED_OR_PREVIOUS ||
-                         mode == NSFEM_WARM_LEARNED_CORRECTION;
+                         mode == NSFEM_WARM_LEARNED_CORRECTION ||
+                         mode == NSFEM_WARM_LEARNED_SEARCH ||
+                         mode == NSFEM_WARM_LEARNED_PROJECTION;
     NSFEM_REQUIRE(!learned || (g && g->dev == s.dev), NSFEM_INVALID_ARGUMENT,
                   "a learned warm start needs a network on the simulation's device");
     Device &d = *s.dev;
@@ -1172,16 +1188,19 @@ void simulation_set_warm_start(Simulation &s, int mode, GnnNet *g) {
                                    s.p_area.data(), s.p_xy.data());
     }
     if (mode == NSFEM_WARM_LEARNED_OR_PREVIOUS || mode == NSFEM_WARM_LEARNED_CORRECTION ||
-        mode == NSFEM_WARM_EXTRAPOLATE) {
+        mode == NSFEM_WARM_EXTRAPOLATE || mode == NSFEM_WARM_LEARNED_SEARCH ||
+        mode == NSFEM_WARM_LEARNED_PROJECTION) {
         s.pguess.resize(s.P);
         s.ptmp.resize(s.P);
         s.pscal.resize(2);
     }
+    if (mode == NSFEM_WARM_LEARNED_SEARCH || mode == NSFEM_WARM_LEARNED_PROJECTION)
+        s.pw.resize(s.P);
     if (mode == NSFEM_WARM_EXTRAPOLATE) {
         s.p_older.resize(s.P);
         s.p_keep.resize(s.P);
     }
-    if (mode == NSFEM_WARM_PROJECTION) {
+    if (mode == NSFEM_WARM_PROJECTION || mode == NSFEM_WARM_LEARNED_PROJECTION) {
         // basis size: NSFEM_PROJECTION_DEPTH (default 16), 2 x depth + 3
         // pressure vectors of device memory
         static const int depth_env = [] {

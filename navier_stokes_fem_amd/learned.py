@@ -281,7 +281,8 @@ def evaluate_warm_start(device, mesh, net, modes=("previous", "zero", "learned",
         sim.set_warm_start("previous")
         if warmup:
             sim.step(warmup)
-        model = correction_net if mode == "learned_correction" else net
+        model = (correction_net if mode in ("learned_correction", "learned_search",
+                                             "learned_projection") else net)
         sim.set_warm_start(mode, model if mode.startswith("learned") else None)  # noqa
         device.synchronize()
         t = time.perf_counter()

@@ -96,7 +96,9 @@ class Simulation:
     WARM_MODES = {"previous": N.WARM_PREVIOUS, "zero": N.WARM_ZERO, "learned": N.WARM_LEARNED,
                   "learned_or_previous": N.WARM_LEARNED_OR_PREVIOUS,
                   "learned_correction": N.WARM_LEARNED_CORRECTION,
-                  "extrapolate": N.WARM_EXTRAPOLATE, "projection": N.WARM_PROJECTION}
+                  "extrapolate": N.WARM_EXTRAPOLATE, "projection": N.WARM_PROJECTION,
+                  "learned_search": N.WARM_LEARNED_SEARCH,
+                  "learned_projection": N.WARM_LEARNED_PROJECTION}
 
     def set_warm_start(self, mode: str = "previous", net=None):
         """Initial guess of the pressure solve: ``previous`` (the reference's
@@ -104,9 +106,12 @@ class Simulation:
         right-hand side), ``learned_or_previous`` (the smaller residual) or
         ``learned_correction`` (previous pressure + the network's correction
         from the previous pressure's residual, if that lowers the residual),
-        ``extrapolate`` (2 p_n - p_(n-1), if that lowers the residual) or
+        ``extrapolate`` (2 p_n - p_(n-1), if that lowers the residual),
         ``projection`` (the A-norm projection of the solution onto the span of
-        past pressures, Fischer 1998)."""
+        past pressures, Fischer 1998), ``learned_search`` (previous pressure
+        plus the network's correction taken with the A-norm-optimal step:
+        never worse than the previous pressure in the A-norm) or
+        ``learned_projection`` (the same step from the projection guess)."""
         N.check(N.lib().nsfem_simulation_set_warm_start(
             self.handle, self.WARM_MODES[mode], net.handle if net is not None else None))
 
