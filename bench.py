@@ -430,12 +430,12 @@ def bench_channel512(dev, args):
     steps = 200
     m = mesh.channel(512, 128, seed=2)
     sim = Simulation(m, 1e-3, 5e-4, device=dev, rtol=1e-8, precision="fp64")
-    res = {}
+    res, fields = {}, {}
     for form, prec, warm in (("fp64", "fp64", "previous"),
                              ("fp32_refined", "fp32_refined", "previous"),
                              ("fp32_refined_projection", "fp32_refined", "projection")):
         sim.reset()
-        sim.configure(precision=prec)
+        sim.configure(precision=prec, rtol=1e-8)
         sim.set_warm_start(warm)
         dev.synchronize()
         t = time.perf_counter()
@@ -444,8 +444,23 @@ def bench_channel512(dev, args):
         res[form] = {"steps_per_s": round(steps / wall, 2),
                      "cg_iterations_per_step": [x / steps for x in st["cg_iterations"]],
                      "max_relative_residual": st["max_relative_residual"]}
+        fields[form] = sim.fields()
         log(f"channel512 {form}: {steps / wall:.1f} steps/s, iterations/step {res[form]['cg_iterations_per_step']}")
+    # the same 200 steps in FP64 with every solve to 1e-12: each form's
+    # distance to it per field (rel. L2), and the mixed forms' difference
+    # from the FP64 run per field
+    sim.reset()
+    sim.configure(precision="fp64", rtol=1e-12)
+    sim.set_warm_start("previous")
+    sim.step(steps)
+    tight = sim.fields()
     sim.close()
+    for form in res:
+        res[form]["error_vs_tight_solve"] = {k: v["rel_l2"] for k, v in
+                                             field_difference(fields[form], tight).items()}
+        if form != "fp64":
+            res[form]["field_difference_vs_fp64"] = field_difference(fields[form], fields["fp64"])
+        log(f"channel512 {form}: rel. L2 vs the 1e-12 run {res[form]['error_vs_tight_solve']}")
     best = max(res, key=lambda f: res[f]["steps_per_s"])
     return _line("steps/sec, 512x128 channel, 1e-8 rel. residual", res[best]["steps_per_s"],
                  "steps/s", steps, 1e3 / res[best]["steps_per_s"],

@@ -190,11 +190,9 @@ bool col_delta(const Matrix &m, int n_tiles, const int4 *meta, DeviceArray<int16
 void build_col_delta(Matrix &m) { m.dcol_ok = col_delta(m, m.n_wtiles, m.wmeta.ptr, m.dcol); }
 
 // Group loads (8 nonzeros per lane from a multiple of 8) need the 16-bit
-// offsets (padded by 8) and every multi-row tile within kStageNnz nonzeros;
-// the pipelined form also every single-row tile.
+// offsets (padded by 8) and every multi-row tile within kStageNnz nonzeros.
 void set_group_loads(Matrix &m) {
     m.groups_ok = m.dcol_ok && m.nnz > 0 && m.max_multirow_nz <= kStageNnz;
-    m.pipe_ok = m.groups_ok && m.max_tile_nz <= kStageNnz;
 }
 
 }  // namespace
@@ -237,11 +235,9 @@ void build_wave_tiles(Matrix &m, int rows, int nnz) {
                         hipMemcpyHostToDevice));
     std::vector<int4> meta((size_t)std::max(m.n_wtiles, 1));
     m.max_multirow_nz = 0;
-    m.max_tile_nz = 0;
     for (int i = 0; i < m.n_wtiles; ++i) {
         meta[i] = make_int4(w[i], w[i + 1], rs[w[i]], rs[w[i + 1]] - rs[w[i]]);
         if (w[i + 1] - w[i] > 1) m.max_multirow_nz = std::max(m.max_multirow_nz, meta[i].w);
-        m.max_tile_nz = std::max(m.max_tile_nz, meta[i].w);
     }
     m.wmeta.resize(meta.size());
     NSFEM_HIP(hipMemcpy(m.wmeta.ptr, meta.data(), meta.size() * sizeof(int4),
@@ -1856,134 +1852,6 @@ __device__ void phase_f1g(const CgArgs &A, MP meta, int T0, int T1, double *lds_
     }
 }
 
-// phase_f1g with a two-tile software pipeline (matrices whose every tile
-// holds at most kStageNnz nonzeros): while a wave consumes tile
-// t (products, row sums, state store), the gathers, values and own-row
-// loads of its next tile t + NW and the column words of t + 2 NW are in
-// flight.  Per tile the loads are issued oldest-first in the order they are
-// used (words of t + 2 NW; then, the words of t + NW being in, its gathers;
-// then tile t's results are awaited), so that each wait of the in-order
-// vector-memory counter leaves the younger tiles' loads in flight.  The
-// loop is unrolled by two over two register sets, so no register that a
-// load in flight will write is ever copied (a copy would wait for it).
-// Same products, same row sums in the same order: bit-identical to
-// phase_f1g.
-template <int BS, typename MP>
-__device__ void phase_f1p(const CgArgs &A, MP meta, int T0, int T1, double *lds_wave,
-                          const void *st_old, void *st_new, const float alpha, const float beta,
-                          const double pend, const bool has_pend, double (&sums)[4]) {
-    constexpr int NW = BS / 64;
-    typedef short i16x8 __attribute__((ext_vector_type(8)));
-    typedef float f32x4 __attribute__((ext_vector_type(4)));
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int t = T0 + wave;
-    if (t >= T1) return;
-    const int nmax = A.n - 1;
-    struct Set {      // the vector loads of one tile
-        i16x8 cw;     // column words of the lane's group
-        f32x4 v0, v1; // values of the group
-        RowState g[8];  // gathered neighbour states
-        int rsa, rsb;   // the lane's own row: row starts, state, solution
-        RowState so;
-        double xo;
-    };
-    auto group = [&](const WTile &w) {
-        const int kk = (w.k0 & ~7) + 8 * lane;
-        return kk < w.k0 + w.nz ? kk : (w.k0 & ~7);  // idle lanes: the first group
-    };
-    // every load is issued for every tile (idle lanes and empty or long
-    // tiles read valid clamped addresses), so the number of loads between
-    // two uses is the same on every path and the compiler's counter waits
-    // stay partial
-    auto words = [&](const WTile &w, Set &S) {
-        S.cw = ld_stream(reinterpret_cast<const i16x8 *>(A.dcol + group(w)));
-    };
-    auto rest = [&](const WTile &w, Set &S) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-            S.g[e] = ld_state(st_old, min(max(w.r0 + (int)S.cw[e], 0), nmax));
-        const int kc = group(w);
-        S.v0 = ld_stream(reinterpret_cast<const f32x4 *>(A.val32 + kc));
-        S.v1 = ld_stream(reinterpret_cast<const f32x4 *>(A.val32 + kc + 4));
-        const int row = w.r0 + lane;
-        const int rowc = row < w.r1 ? row : w.r0;
-        S.rsa = A.rs[rowc];
-        S.rsb = A.rs[rowc + 1];
-        S.so = ld_state(st_old, rowc);
-        S.xo = A.x[rowc];
-    };
-    // tiles without nonzeros need no branch: no lane stages a product and
-    // every row sum is empty
-    auto consume = [&](const WTile &w, const Set &S) {
-        const int row = w.r0 + lane;
-        const bool has_row = row < w.r1;
-        const int kk = (w.k0 & ~7) + 8 * lane;
-        if (kk < w.k0 + w.nz) {
-            double *dst = lds_wave + (kk - w.k0);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float v = e < 4 ? S.v0[e] : S.v1[e - 4];
-                dst[e] = (double)v * (double)__builtin_fmaf(beta, S.g[e].p,
-                                                            next_r(alpha, S.g[e].r, S.g[e].q));
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        const int ra = S.rsa - w.k0;
-        const int rb = has_row ? S.rsb - w.k0 : ra;
-        const double s = row_sum(lds_wave, ra, rb);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (has_row) {
-            RowState sn;
-            sn.r = next_r(alpha, S.so.r, S.so.q);
-            sn.p = __builtin_fmaf(beta, S.so.p, sn.r);
-            sn.q = (float)s;
-            const double rd = (double)sn.r, qd = (double)sn.q;
-            sums[0] += (double)sn.p * s;
-            sums[1] += rd * rd;
-            sums[2] += rd * qd;
-            sums[3] += qd * qd;
-            st_state(st_new, row, sn);
-            A.x[row] = has_pend ? __builtin_fma(pend, (double)S.so.p, S.xo) : S.xo;
-        }
-    };
-    // the loads in steady-state order: words of t, words of t + NW, rest of
-    // t (tiles beyond the block's last read its last tile again, unused)
-    Set SA, SB;
-    const int tl = T1 - 1;
-    WTile wc = wave_tile(meta, t), wn = wave_tile(meta, min(t + NW, tl));
-    words(wc, SA);
-    words(wn, SB);
-    __builtin_amdgcn_sched_barrier(0);
-    rest(wc, SA);
-    __builtin_amdgcn_sched_barrier(0);
-    // S holds tile t (all loads issued) and receives the words of t + 2 NW;
-    // O holds the words of t + NW and receives the rest of its loads
-    // (scheduling barriers keep the issue order: a words load sunk below the
-    // gathers would make the next wait for it also wait for them)
-    auto step = [&](Set &S, Set &O) __attribute__((always_inline)) {
-        const WTile wnn = wave_tile(meta, min(t + 2 * NW, tl));
-        words(wnn, S);
-        __builtin_amdgcn_sched_barrier(0);
-        rest(wn, O);
-        __builtin_amdgcn_sched_barrier(0);
-        consume(wc, S);
-        wc = wn;
-        wn = wnn;
-        t += NW;
-    };
-    // the wave's tiles in pairs (no exit between the two halves), then an
-    // odd last one
-    const int ntiles = (T1 - t + NW - 1) / NW;
-    for (int i = 1; i < ntiles; i += 2) {
-        step(SA, SB);
-        step(SB, SA);
-    }
-    if (ntiles & 1) step(SA, SB);
-}
-
 // x += pend p over the block's rows (p from the packed state)
 template <int BS>
 __device__ void flush_x1(const CgArgs &A, int R0, int R1, const void *st, double pend) {
@@ -2086,7 +1954,7 @@ __device__ __forceinline__ void cg_body1_m(const CgArgs &A, MP meta, const int b
                                            const int L, double *prod_raw, double *lds) {
     constexpr bool MIXED = true;  // residual_phase precision
     // WI 1: group loads (16-bit offsets only); else WI items per lane
-    constexpr bool GROUPS = WI == 1 || WI == 2;
+    constexpr bool GROUPS = WI == 1;
     static_assert(!GROUPS || std::is_same<IT, int16_t>::value, "group loads read 16-bit words");
     __shared__ CgCtl1 C;
     const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
@@ -2133,10 +2001,7 @@ __device__ __forceinline__ void cg_body1_m(const CgArgs &A, MP meta, const int b
         const bool has_pend = C.has_pend != 0;
         double sums[4] = {0.0, 0.0, 0.0, 0.0};
         CG_TRACE_STAMP(C.k, 0);
-        if constexpr (WI == 2)
-            phase_f1p<BS, MP>(A, meta, T0, T1, lds64, A.st[cur], A.st[nx], alpha, beta, pend,
-                              has_pend, sums);
-        else if constexpr (GROUPS)
+        if constexpr (GROUPS)
             phase_f1g<BS, MP>(A, meta, T0, T1, lds64, A.st[cur], A.st[nx], alpha, beta, pend,
                               has_pend, sums);
         else
@@ -2201,7 +2066,7 @@ __device__ __forceinline__ void cg_body1(const CgArgs &A, const int bid, const i
 template <int NC, int BS, bool MIXED, typename IT, int WI>
 __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
     constexpr int NW = BS / 64;
-    __shared__ double prod_raw[NW * (kWaveNnz * NC + (WI == 1 || WI == 2 ? 8 : 0))];
+    __shared__ double prod_raw[NW * (kWaveNnz * NC + (WI == 1 ? 8 : 0))];
     __shared__ double lds[kMaxRed * NW];
     const int L = xcd_swizzle(blockIdx.x, gridDim.x);
     if constexpr (NC == 1 && MIXED)
@@ -2218,7 +2083,7 @@ template <int NC, int BS, bool MIXED, typename IT, int WI>
 __global__ __launch_bounds__(BS, BS / 256) void k_cg_batch(const CgArgs *__restrict__ args,
                                                            const int2 *__restrict__ map) {
     constexpr int NW = BS / 64;
-    __shared__ double prod_raw[NW * (kWaveNnz * NC + (WI == 1 || WI == 2 ? 8 : 0))];
+    __shared__ double prod_raw[NW * (kWaveNnz * NC + (WI == 1 ? 8 : 0))];
     __shared__ double lds[kMaxRed * NW];
     const int2 m = map[blockIdx.x];
     const int inst = __builtin_amdgcn_readfirstlane(m.x);
@@ -2241,7 +2106,8 @@ constexpr int kCgBlockWide = 768;
 // blocks unless NSFEM_CG_BLOCK=1024 (read at every solve; measured at the
 // 2048^2 cavity, 768 threads with group loads: pressure 95.9 us and v
 // diffusion 550 us per iteration against 98.8 / 564 us for 1024 threads
-// with one load per nonzero per lane).
+// with one load per nonzero per lane).  The two-column mixed solve was
+// slower in 768-thread blocks (mass 1,195 against 1,044 us per iteration).
 int cg_block(int nc, bool mixed) {
     if (!(nc == 1 && mixed)) return kCgBlock;
     const char *e = std::getenv("NSFEM_CG_BLOCK");
@@ -2284,12 +2150,10 @@ void cg_dispatch_wi(int nc, bool mixed, bool idx16, F &f) {
         }
     }
 }
-// wi 1 / 2: the one-column mixed iteration with group loads (16-bit
-// offsets), plain or with the two-tile pipeline
+// wi 1: the one-column mixed iteration with group loads (16-bit offsets)
 template <typename F>
 void cg_dispatch(int nc, bool mixed, bool idx16, int wi, F &&f) {
     if (wi == 1 && nc == 1 && mixed && idx16) f.template run<1, true, int16_t, 1>();
-    else if (wi == 2 && nc == 1 && mixed && idx16) f.template run<1, true, int16_t, 2>();
     else if (wi <= 4) cg_dispatch_wi<4>(nc, mixed, idx16, f);
     else if (wi <= 7) cg_dispatch_wi<7>(nc, mixed, idx16, f);
     else cg_dispatch_wi<kWItems>(nc, mixed, idx16, f);
@@ -2303,15 +2167,14 @@ int items_per_lane(const Matrix &m) {
 }
 
 // Form of the one-column mixed iteration on matrices with 16-bit offsets:
-// NSFEM_CG_PHASE = items (one load per nonzero per lane), groups (8
-// nonzeros per lane) or pipe (groups, two tiles in flight); without it, groups where `groups_default` (the
+// NSFEM_CG_PHASE = items (one load per nonzero per lane) or groups (8
+// nonzeros per lane); without it, groups where `groups_default` (the
 // single solves' wide blocks) and the matrix allows them.  Read at every
 // solve (comparison runs switch it in one process).
 int phase_form(const Matrix &m, bool groups_default) {
     const char *e = std::getenv("NSFEM_CG_PHASE");
     const std::string v = e ? e : "";
     if (v == "items") return -1;
-    if (v == "pipe") return m.pipe_ok ? 2 : -1;
     if (v == "groups" || (v.empty() && groups_default)) return m.groups_ok ? 1 : -1;
     return -1;
 }

@@ -126,7 +126,6 @@ struct Matrix {
     std::vector<int32_t> host_rs;            // row_start (wave tiles are rebuilt from it)
     int wave_rows = kWaveRows, wave_nnz = kWaveNnz;  // current wave-tile shape
     int max_multirow_nz = 0;  // most nonzeros of a wave tile with more than one row
-    int max_tile_nz = 0;      // most nonzeros of any wave tile
     // ownership of wave tiles by persistent blocks, cached per grid size:
     // both arrays hold kMaxGridBlocks + 1 entries from the tiling on, so a
     // change of grid size copies and never allocates
@@ -143,7 +142,6 @@ struct Matrix {
     // group loads (8 nonzeros per lane) need the 16-bit offsets (padded by
     // 8) and every multi-row tile within kStageNnz nonzeros
     bool groups_ok = false;
-    bool pipe_ok = false;  // and every tile within kStageNnz (pipelined form)
     bool ready = false;
     std::mutex mtx;
 };
@@ -156,7 +154,6 @@ inline void matrix_release(Matrix &m) {
     m.bcol.release();
     m.bcol_ok = false;
     m.groups_ok = false;
-    m.pipe_ok = false;
     m.block_tiles_grid = 0;
     m.ready = false;
 }
