@@ -65,10 +65,10 @@ def cg_bytes(nnz, n, iters, refinements, mixed, columns=1):
 
 
 # HBM bytes per pressure-CG iteration measured with rocprofv3 FETCH_SIZE and
-# WRITE_SIZE (scaled by a k_axpy calibration) on this workload; source:
-# profiles/k_cg_cavity2048_hbm_traffic.md.  roofline.traffic = this figure
-# x the iterations of the launch.
-MEASURED_BYTES_PER_ITERATION = {False: 721.9e6, True: 457.7e6}
+# WRITE_SIZE (scaled by a k_axpy calibration) on this workload, reliable
+# updates included; source: profiles/k_cg_cavity2048_hbm_traffic.md.
+# roofline.traffic = this figure x the iterations of the launch.
+MEASURED_BYTES_PER_ITERATION = {False: 721.9e6, True: 435.4e6}
 
 
 def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
@@ -691,12 +691,17 @@ def main():
         n = 1 << 26
         x = dev.buffer(np.ones(n))
         y = dev.buffer(np.zeros(n))
+        dev.saxpy(n, 0.5, x, y)  # first touch
+        dev.synchronize()
+        t = time.perf_counter()
         for _ in range(5):
             dev.saxpy(n, 0.5, x, y)
         dev.synchronize()
+        wall = time.perf_counter() - t
         got = y.download(np.float64, 4)
         print(json.dumps({"calibration": "saxpy", "elements": n, "launches": 5,
                           "fetch_bytes_per_launch": 16 * n, "write_bytes_per_launch": 8 * n,
+                          "stream_GBs": round(5 * 24 * n / wall / 1e9, 1),
                           "check": float(got[0])}), flush=True)
         x.close()
         y.close()
@@ -775,7 +780,9 @@ def main():
                if args.n == 2048 else None)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "kernel": "k_cg<1,1024,%s> (pressure CG, persistent)" % ("true" if mixed else "false"),
+                "kernel": ("k_cg<1, 768, true, short, 1> (pressure CG, FP32 group loads, persistent)"
+                           if mixed else
+                           "k_cg<1, 1024, false, short, 8> (pressure CG, FP64, persistent)"),
                 "launches": args.steps, "iterations_per_launch": p_iters / args.steps,
                 "bytes_per_iteration": textbook,
                 "bytes_per_launch": bytes_textbook / args.steps,

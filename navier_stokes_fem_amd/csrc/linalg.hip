@@ -2350,9 +2350,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
     int grid = want > 0 ? std::min(want, maxg)
                         : std::min(maxg, std::max(1, m.n_wtiles / (2 * (bs / 64))));
     grid = std::max(1, std::min(grid, std::max(1, m.n_wtiles)));
-    if (std::getenv("NSFEM_CG_VERBOSE"))
-        std::fprintf(stderr, "[nsfem] cg rows=%d cols=%d wave tiles=%d grid=%d maxgrid=%d mixed=%d idx16=%d\n",
-                     n, nc, m.n_wtiles, grid, maxg, (int)mixed, (int)idx16);
+    const bool verbose = std::getenv("NSFEM_CG_VERBOSE") != nullptr;
     ensure_block_tiles(d, m, grid);
 
     const CgWorkspace ws = fixed ? *fixed : device_workspace(d, n, nc, mixed);
@@ -2383,6 +2381,12 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
     const int ngroups = (grid + kGroup - 1) / kGroup;
 
     for (int attempt = 0; attempt < 3; ++attempt) {
+        if (verbose)  // one line per launch, in launch order
+            std::fprintf(stderr,
+                         "[nsfem] cg launch matrix=%p rows=%d cols=%d wave tiles=%d grid=%d "
+                         "maxgrid=%d mixed=%d idx16=%d form=%d block=%d attempt=%d\n",
+                         (const void *)&m, n, nc, m.n_wtiles, grid, maxg, (int)mixed, (int)idx16,
+                         wi, bs, attempt);
         NSFEM_HIP(hipEventRecord(d.ev_start, d.stream));
         k_sync_reset<<<(ngroups + 255) / 256, 256, 0, d.stream>>>(a.sync, ngroups);
         NSFEM_HIP(hipGetLastError());
