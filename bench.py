@@ -64,11 +64,13 @@ def cg_bytes(nnz, n, iters, refinements, mixed, columns=1):
     return iters * it + (refinements + 1) * res
 
 
-# HBM bytes per pressure-CG iteration measured with rocprofv3 FETCH_SIZE and
-# WRITE_SIZE (scaled by a k_axpy calibration) on this workload, reliable
-# updates included; source: profiles/k_cg_cavity2048_hbm_traffic.md.
-# roofline.traffic = this figure x the iterations of the launch.
-MEASURED_BYTES_PER_ITERATION = {False: 721.9e6, True: 435.4e6}
+# HBM bytes per CG iteration (mixed?, solve) measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE (scaled by a k_axpy calibration) on this
+# workload, reliable updates included; source:
+# profiles/k_cg_cavity2048_hbm_traffic.md.  roofline.traffic = this figure x
+# the iterations of a launch.
+MEASURED_BYTES_PER_ITERATION = {(False, "pressure"): 721.9e6, (True, "pressure"): 435.4e6,
+                                (True, "diffusion v"): 2723.5e6, (True, "mass u+v"): 4771.2e6}
 
 
 def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
@@ -96,14 +98,16 @@ def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
 
 
 def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
-             coarse_rtol=1e-6, velocity_projection=0, coarse_levels=(), rtol=1e-8):
+             coarse_rtol=1e-6, velocity_projection=0, coarse_levels=(), rtol=1e-8,
+             pressure_multigrid=False, pair_velocity=True):
     sim.reset()
-    sim.configure(precision=form, rtol=rtol, pair_velocity=True)
+    sim.configure(precision=form, rtol=rtol, pair_velocity=pair_velocity)
     sim.set_warm_start(warm)
     sim.set_coarse_space(coarse, rtol=coarse_rtol)
     if coarse is not None:
         for level in coarse_levels:
             sim.add_coarse_level(level)
+    sim.set_pressure_multigrid(pressure_multigrid and coarse is not None)
     sim.set_velocity_projection(velocity_projection)
     for i in range(warmup):
         sim.step(1)
@@ -116,7 +120,8 @@ def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
         if agg is None:
             agg = st
         else:
-            for k in ("cg_iterations", "cg_seconds", "cg_refinements"):
+            for k in ("cg_iterations", "cg_seconds", "cg_refinements", "cg_launches",
+                      "cg_launch_iterations"):
                 agg[k] = [a + b for a, b in zip(agg[k], st[k])]
             agg["max_relative_residual"] = [max(a, b) for a, b in zip(agg["max_relative_residual"],
                                                                        st["max_relative_residual"])]
@@ -423,7 +428,10 @@ def bench_cavity32(dev, args):
 def bench_channel512(dev, args):
     """512x128 channel past the disk (jitter seed 2), nu = 1e-3, dt = 5e-4,
     200 steps in each form: plain FP64, FP32 + FP64 refinement with the
-    previous-pressure guess, and with the projection guess."""
+    previous-pressure guess, with the projection guess, and with the
+    projection guess and the pressure solved by CG preconditioned with a
+    V-cycle over the channel's grid hierarchy (holes and outlet nodes
+    kept out of the coarse spaces)."""
     from navier_stokes_fem_amd import mesh
     from navier_stokes_fem_amd.simulation import Simulation
 
@@ -431,12 +439,20 @@ def bench_channel512(dev, args):
     m = mesh.channel(512, 128, seed=2)
     sim = Simulation(m, 1e-3, 5e-4, device=dev, rtol=1e-8, precision="fp64")
     res, fields = {}, {}
+    hierarchy = mesh.grid_hierarchy(m, min_size=8)
     for form, prec, warm in (("fp64", "fp64", "previous"),
                              ("fp32_refined", "fp32_refined", "previous"),
-                             ("fp32_refined_projection", "fp32_refined", "projection")):
+                             ("fp32_refined_projection", "fp32_refined", "projection"),
+                             ("fp32_refined_multigrid", "fp32_refined", "projection")):
         sim.reset()
         sim.configure(precision=prec, rtol=1e-8)
         sim.set_warm_start(warm)
+        multigrid = form == "fp32_refined_multigrid"
+        sim.set_coarse_space(hierarchy[0] if multigrid else None)
+        if multigrid:
+            for level in hierarchy[1:]:
+                sim.add_coarse_level(level)
+            sim.set_pressure_multigrid(True)
         dev.synchronize()
         t = time.perf_counter()
         st = sim.step(steps)
@@ -450,6 +466,7 @@ def bench_channel512(dev, args):
     # distance to it per field (rel. L2), and the mixed forms' difference
     # from the FP64 run per field
     sim.reset()
+    sim.set_coarse_space(None)
     sim.configure(precision="fp64", rtol=1e-12)
     sim.set_warm_start("previous")
     sim.step(steps)
@@ -642,6 +659,15 @@ def main():
                          "P1 interpolation from the (n/ratio)^2 cavity (0 = off)")
     ap.add_argument("--coarse-rtol", type=float, default=1e-5,
                     help="relative tolerance of the coarse solves")
+    ap.add_argument("--pair-velocity", type=int, default=1, choices=[0, 1],
+                    help="fp32_refined form: solve the u and v columns of a velocity system as "
+                         "one two-column CG while their iteration counts stay balanced (1) or "
+                         "one after the other (0)")
+    ap.add_argument("--pressure-multigrid", action=argparse.BooleanOptionalAction, default=True,
+                    help="fp32_refined form (default on): solve the pressure by FP64 CG "
+                         "preconditioned by one V-cycle over the fine grid, the coarse space and "
+                         "its levels; off: CG on the pressure with the coarse correction of its "
+                         "guess")
     ap.add_argument("--coarse-levels", type=int, default=-1,
                     help="multigrid levels below the coarse space (each halves the grid; -1 = "
                          "halve down to the 16x16 grid); with levels the coarse system is solved "
@@ -737,7 +763,9 @@ def main():
                             warm=args.warm if mixed_form else "previous",
                             coarse=coarse if mixed_form else None, coarse_rtol=args.coarse_rtol,
                             velocity_projection=args.velocity_projection if mixed_form else 0,
-                            coarse_levels=levels)
+                            coarse_levels=levels,
+                            pressure_multigrid=args.pressure_multigrid and mixed_form,
+                            pair_velocity=bool(args.pair_velocity) or not mixed_form)
         results[form] = (wall, st)
         log(f"{form}: {args.steps / wall:.3f} steps/s")
     # attribution: the same mixed solver without the coarse correction, and
@@ -763,34 +791,55 @@ def main():
     wall, st = results[chosen]
     value = args.steps / wall
     mixed = chosen == "fp32_refined"
-    # dominant kernel: the pressure CG launch (k_cg, 1 column), one per step.
-    # achieved = the textbook iteration's bytes (matrix once with int32
-    # columns, row_start, 11 vector words per row; w = 4 for the FP32 inner
-    # iteration, 8 for FP64) x iterations / kernel time; the build's own
-    # count, with the FP64 solution and the reliable updates' residual
-    # passes, is reported beside it
-    p_iters = st["cg_iterations"][2]
-    p_secs = st["cg_seconds"][2]
+    # dominant kernel: the persistent CG kernel of the solve kind with the
+    # most device time in the chosen form's timed steps (the V-cycle
+    # preconditioned pressure solve is host-orchestrated from several
+    # kernels and is left out).  achieved = the textbook iteration's bytes
+    # (matrix once with int32 columns, row_start, 11 vector words per row
+    # and column; w = 4 for the FP32 inner iteration, 8 for FP64) x the
+    # launches' iterations / their HIP-event time; the build's own count,
+    # with the FP64 solution and the reliable updates' residual passes, is
+    # reported beside it
+    kinds = [("diffusion u", 0, 1), ("diffusion v", 1, 1), ("pressure", 2, 1),
+             ("mass u+v", 3, 2), ("mass v", 4, 1)]
+    pmg = mixed and args.pressure_multigrid
+    n_v, nnz_v = sim.matrix_info(2)[0], sim.matrix_info(2)[2]
+    cand = [k for k in kinds if not (pmg and k[1] == 2) and st["cg_launches"][k[1]] > 0]
+    name, k, cols = max(cand, key=lambda k: st["cg_seconds"][k[1]])
+    # a paired u+v launch has 2 columns; the u entry mixes both when pairing varied
+    if k == 0 and st["cg_launches"][1] < st["cg_launches"][0]:
+        cols = 2
+    n_k, nnz_k = (n_p, nnz_p) if k == 2 else (n_v, nnz_v)
+    launches = st["cg_launches"][k]
+    k_iters = st["cg_launch_iterations"][k]
+    k_secs = st["cg_seconds"][k]
     w = 4 if mixed else 8
-    textbook = (4 + w) * nnz_p + (4 + 11 * w) * n_p
-    bytes_textbook = textbook * p_iters
-    bytes_p = cg_bytes(nnz_p, n_p, p_iters, st["cg_refinements"][2] + args.steps - 1, mixed)
-    achieved = bytes_textbook / max(p_secs, 1e-12) / 1e9
-    traffic = (MEASURED_BYTES_PER_ITERATION[mixed] * p_iters / args.steps
-               if args.n == 2048 else None)
+    textbook = (4 + w) * nnz_k + (4 + 11 * w * cols) * n_k
+    bytes_textbook = textbook * k_iters
+    bytes_k = cg_bytes(nnz_k, n_k, k_iters, st["cg_refinements"][k] + launches - 1, mixed,
+                       columns=cols)
+    achieved = bytes_textbook / max(k_secs, 1e-12) / 1e9
+    measured = MEASURED_BYTES_PER_ITERATION.get((mixed, name)) if args.n == 2048 else None
+    traffic = measured * k_iters / launches if measured else None
+    if cols == 2:
+        kernel = "k_cg<2, 1024, %s, short, 8>" % ("true" if mixed else "false")
+    elif mixed:
+        kernel = "k_cg<1, 768, true, short, 1>"
+    else:
+        kernel = "k_cg<1, 1024, false, short, 8>"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "kernel": ("k_cg<1, 768, true, short, 1> (pressure CG, FP32 group loads, persistent)"
-                           if mixed else
-                           "k_cg<1, 1024, false, short, 8> (pressure CG, FP64, persistent)"),
-                "launches": args.steps, "iterations_per_launch": p_iters / args.steps,
+                "kernel": f"{kernel} ({name} CG, persistent)", "solve": name,
+                "launches": launches, "iterations_per_launch": k_iters / launches,
                 "bytes_per_iteration": textbook,
-                "bytes_per_launch": bytes_textbook / args.steps,
-                "seconds_per_launch": p_secs / args.steps,
+                "bytes_per_launch": bytes_textbook / launches,
+                "seconds_per_launch": k_secs / launches,
                 "with_refinement_passes": {
-                    "bytes_per_launch": bytes_p / args.steps,
-                    "achieved": round(bytes_p / max(p_secs, 1e-12) / 1e9, 1),
-                    "frac": round(bytes_p / max(p_secs, 1e-12) / 1e9 / PEAK_HBM_GBS, 4)}}
+                    "bytes_per_launch": bytes_k / launches,
+                    "achieved": round(bytes_k / max(k_secs, 1e-12) / 1e9, 1),
+                    "frac": round(bytes_k / max(k_secs, 1e-12) / 1e9 / PEAK_HBM_GBS, 4)},
+                "seconds_per_step_by_solve": {
+                    kn: round(st["cg_seconds"][kk] / args.steps, 5) for kn, kk, _ in kinds}}
     out = {
         "metric": "steps/sec at 1e-8 rel. residual, chosen vs plain FP64 CG; achieved HBM GB/s",
         "value": round(value, 4),
@@ -808,7 +857,9 @@ def main():
                                f"({m.element_count} P2-P1 triangles, {m.node_count} velocity / "
                                f"{m.pressure_count} pressure nodes), nu=1e-3, dt=1e-3, "
                                f"rel tol 1e-8 per solve",
-                   "form": chosen, "pair_velocity": True, "parallelism": "single GPU",
+                   "form": chosen,
+                   "pair_velocity": bool(args.pair_velocity) if mixed else True,
+                   "parallelism": "single GPU",
                    "diffusion_initial_guess": (
                        f"advected velocity + projection of the correction (depth "
                        f"{args.velocity_projection})" if chosen == "fp32_refined" and
@@ -818,8 +869,18 @@ def main():
                                      f"{args.n // args.coarse_ratio}^2 cavity, coarse system "
                                      + (f"by V-cycles over {len(levels)} more levels"
                                         if levels else "by CG")
-                                     + f" to {args.coarse_rtol:g})" if coarse is not None else ""))
-                       if chosen == "fp32_refined" else "previous")},
+                                     + f" to {args.coarse_rtol:g})"
+                                     if coarse is not None and not args.pressure_multigrid
+                                     else ""))
+                       if chosen == "fp32_refined" else "previous"),
+                   "pressure_solve": (
+                       f"FP64 CG preconditioned by one V-cycle (damped Jacobi 2+2 sweeps on "
+                       f"the fine grid, the P1 space of the {args.n // args.coarse_ratio}^2 "
+                       f"cavity and {len(levels)} more levels, the deepest by its dense "
+                       f"inverse), to the same 1e-8 relative residual"
+                       if chosen == "fp32_refined" and args.pressure_multigrid and coarse is not None
+                       else ("FP32 + FP64 refinement CG" if chosen == "fp32_refined"
+                             else "FP64 CG"))},
         "roofline": roofline,
     }
     for form, (w, s) in list(results.items()) + list(attribution.items()):
@@ -845,7 +906,8 @@ def main():
         w_t, st_t = run_form(sim, "fp32_refined", args.warmup, args.steps, n_p, nnz_p,
                              warm=args.warm, coarse=coarse, coarse_rtol=min(args.coarse_rtol, 1e-8),
                              velocity_projection=args.velocity_projection, coarse_levels=levels,
-                             rtol=1e-12)
+                             rtol=1e-12, pressure_multigrid=args.pressure_multigrid,
+                             pair_velocity=bool(args.pair_velocity))
         log(f"tight reference (1e-12) {time.perf_counter() - t:.1f} s, max residuals "
             f"{st_t['max_relative_residual']}")
         tight = st_t["fields"]

@@ -324,6 +324,11 @@ typedef struct {
                                    (V-cycles when multigrid levels are added) */
     double coarse_seconds;      /* their time (device time of the CG launches;
                                    wall time of the V-cycles) */
+    int64_t cg_launches[5];     /* solves whose time is in cg_seconds[k]: a paired
+                                   two-column solve counts under the u entry (and
+                                   a batch's shared launch under every instance) */
+    int64_t cg_launch_iterations[5]; /* their iterations (a paired solve: the
+                                   larger column's) */
 } nsfem_step_stats;
 
 int nsfem_simulation_create(nsfem_device *dev, int node_count,

@@ -71,7 +71,8 @@ class StepStats(C.Structure):
                 ("max_relative_residual", C.c_double * 5), ("seconds", C.c_double),
                 ("advect_seconds", C.c_double), ("cg_seconds", C.c_double * 5),
                 ("cg_refinements", C.c_int64 * 5), ("coarse_iterations", C.c_int64),
-                ("coarse_seconds", C.c_double)]
+                ("coarse_seconds", C.c_double), ("cg_launches", C.c_int64 * 5),
+                ("cg_launch_iterations", C.c_int64 * 5)]
 
     def as_dict(self):
         return {"steps": self.steps, "cg_iterations": list(self.cg_iterations),
@@ -80,7 +81,9 @@ class StepStats(C.Structure):
                 "cg_seconds": list(self.cg_seconds),
                 "cg_refinements": list(self.cg_refinements),
                 "coarse_iterations": self.coarse_iterations,
-                "coarse_seconds": self.coarse_seconds}
+                "coarse_seconds": self.coarse_seconds,
+                "cg_launches": list(self.cg_launches),
+                "cg_launch_iterations": list(self.cg_launch_iterations)}
 
 
 class BatchTuning(C.Structure):

@@ -1362,6 +1362,16 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
             const double t[5] = {o[0].seconds, pair_d ? 0.0 : o[1].seconds, o[2].seconds,
                                  o[3].seconds, pair_m ? 0.0 : o[4].seconds};
             for (int k = 0; k < 5; ++k) stats->cg_seconds[k] += t[k];
+            const int64_t li[5] = {
+                pair_d ? std::max(o[0].iters[0], o[0].iters[1]) : o[0].iters[0],
+                pair_d ? 0 : o[1].iters[0], o[2].iters[0],
+                pair_m ? std::max(o[3].iters[0], o[3].iters[1]) : o[3].iters[0],
+                pair_m ? 0 : o[4].iters[0]};
+            const int64_t ln[5] = {1, pair_d ? 0 : 1, 1, 1, pair_m ? 0 : 1};
+            for (int k = 0; k < 5; ++k) {
+                stats->cg_launches[k] += ln[k];
+                stats->cg_launch_iterations[k] += li[k];
+            }
             stats->coarse_iterations += coarse_iters;
             stats->coarse_seconds += coarse_seconds;
         }
@@ -1511,6 +1521,17 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
                 o.cg_seconds[0] += sec[0];
                 o.cg_seconds[2] += sec[1];
                 o.cg_seconds[3] += sec[2];
+                const int64_t li[5] = {std::max<int64_t>(sd[2 * i].iterations,
+                                                         sd[2 * i + 1].iterations),
+                                       0, sp_[2 * i].iterations,
+                                       std::max<int64_t>(sm[2 * i].iterations,
+                                                         sm[2 * i + 1].iterations),
+                                       0};
+                const int64_t ln[5] = {1, 0, 1, 1, 0};
+                for (int k = 0; k < 5; ++k) {
+                    o.cg_launches[k] += ln[k];
+                    o.cg_launch_iterations[k] += li[k];
+                }
             }
         }
     }

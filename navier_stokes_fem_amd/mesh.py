@@ -181,6 +181,107 @@ def cavity_prolongation(n: int, ratio: int):
     return rs.astype(np.int32), cols.astype(np.int32), w.astype(np.float64), (nc + 1) ** 2
 
 
+def _coarse_grid_level(grid: np.ndarray, extent, dirichlet: np.ndarray, ratio: int):
+    """The coarse level of a structured grid (node positions `grid`, logical
+    extents `extent`, Dirichlet flags): every coarse position (I*ratio,
+    J*ratio) that carries a node, in raster order, with its coarse logical
+    position and Dirichlet flag; plus which of them are coarse unknowns."""
+    gi = grid[:, 0].astype(np.int64)
+    gj = grid[:, 1].astype(np.int64)
+    nx, ny = extent
+    lut = -np.ones((ny + 1) * (nx + 1), np.int64)
+    lut[gj * (nx + 1) + gi] = np.arange(len(gi))
+    mx, my = nx // ratio, ny // ratio
+    cI, cJ = np.meshgrid(np.arange(mx + 1), np.arange(my + 1), indexing="xy")
+    cf = lut[(cJ.reshape(-1) * ratio) * (nx + 1) + cI.reshape(-1) * ratio]
+    exists = cf >= 0
+    valid = exists.copy()
+    valid[exists] = ~dirichlet[cf[exists]]
+    return cI.reshape(-1), cJ.reshape(-1), cf, exists, valid
+
+
+def _grid_interpolation(grid, extent, dirichlet, ratio):
+    gi = grid[:, 0].astype(np.int64)
+    gj = grid[:, 1].astype(np.int64)
+    P = len(gi)
+    nx, ny = extent
+    if nx % ratio or ny % ratio:
+        raise ValueError("ratio must divide the grid's extents")
+    cI, cJ, cf, exists, valid = _coarse_grid_level(grid, extent, dirichlet, ratio)
+    mx, my = nx // ratio, ny // ratio
+    cid = -np.ones(len(cf), np.int64)
+    cid[valid] = np.arange(int(valid.sum()))
+    I = np.minimum(gi // ratio, mx - 1)
+    J = np.minimum(gj // ratio, my - 1)
+    a = (gi - I * ratio) / ratio
+    b = (gj - J * ratio) / ratio
+    c00 = J * (mx + 1) + I
+    c10, c01 = c00 + 1, c00 + mx + 1
+    c11 = c01 + 1
+    lower = a >= b
+    corners = np.stack([c00, np.where(lower, c10, c11), np.where(lower, c11, c01)], axis=1)
+    w = np.stack([np.where(lower, 1 - a, 1 - b), np.where(lower, a - b, a),
+                  np.where(lower, b, b - a)], axis=1)
+    w = w * exists[corners]
+    tot = w.sum(axis=1, keepdims=True)
+    w = np.divide(w, tot, out=np.zeros_like(w), where=tot > 0)
+    w = w * valid[corners]
+    w[dirichlet] = 0.0
+    rows = np.repeat(np.arange(P), 3)
+    cols = cid[corners].reshape(-1)
+    w = w.reshape(-1)
+    keep = w > 0
+    rows, cols, w = rows[keep], cols[keep], w[keep]
+    order = np.lexsort((cols, rows))
+    rows, cols, w = rows[order], cols[order], w[order]
+    rs = np.zeros(P + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=P), out=rs[1:])
+    return rs.astype(np.int32), cols.astype(np.int32), w, int(valid.sum())
+
+
+def grid_hierarchy(mesh: Mesh, levels: int = -1, min_size: int = 16):
+    """Prolongations of a multigrid hierarchy on a mesh built on a
+    structured grid (``mesh.grid``: cavity, channel): the first is
+    ``grid_prolongation(mesh, 2)`` (for ``Simulation.set_coarse_space``),
+    each further one interpolates from the next coarser grid onto the
+    previous level's coarse unknowns (for ``Simulation.add_coarse_level``)
+    with the same rules: holes rescale, positions of pressure Dirichlet
+    nodes stay zero.  ``levels`` further levels (< 0: halve while both
+    extents stay even and the smaller one above ``min_size``)."""
+    g = mesh.grid
+    if g is None:
+        raise ValueError("the mesh carries no structured grid")
+    grid = np.asarray(g, np.int64)
+    extent = (int(grid[:, 0].max()), int(grid[:, 1].max()))
+    dirichlet = np.zeros(mesh.pressure_count, bool)
+    dirichlet[np.asarray(mesh.p_bc_nodes, np.int64)] = True
+    out = [_grid_interpolation(grid, extent, dirichlet, 2)]
+    count = 0
+    while levels < 0 or count < levels:
+        # the level just built, as a grid: its existing coarse positions
+        cI, cJ, cf, exists, valid = _coarse_grid_level(grid, extent, dirichlet, 2)
+        extent = (extent[0] // 2, extent[1] // 2)
+        grid = np.stack([cI[exists], cJ[exists]], axis=1)
+        dirichlet = ~valid[exists]
+        if extent[0] % 2 or extent[1] % 2:
+            if levels >= 0:
+                raise ValueError("cannot halve the coarse grid again")
+            break
+        if levels < 0 and min(extent) // 2 < min_size:
+            break
+        rs, col, val, nc = _grid_interpolation(grid, extent, dirichlet, 2)
+        # rows of the level's coarse unknowns only (raster order, as their ids)
+        keep = np.nonzero(~dirichlet)[0]
+        cnt = np.diff(rs)[keep]
+        idx = np.concatenate([np.arange(rs[k], rs[k + 1]) for k in keep]) if len(keep) else \
+            np.zeros(0, np.int64)
+        nrs = np.zeros(len(keep) + 1, np.int64)
+        np.cumsum(cnt, out=nrs[1:])
+        out.append((nrs.astype(np.int32), col[idx], val[idx], nc))
+        count += 1
+    return out
+
+
 def grid_prolongation(mesh: Mesh, ratio: int):
     """Interpolation from a coarse grid onto the pressure nodes of a mesh
     built on a structured grid (``mesh.grid``: cavity, channel), for

@@ -108,6 +108,46 @@ def test_grid_prolongation_matches_the_cavity_operator_away_from_the_pin():
         mesh.grid_prolongation(m, 3)
 
 
+def test_grid_hierarchy_on_the_channel():
+    """The channel's multigrid hierarchy: the first level is
+    grid_prolongation(mesh, 2); each further prolongation maps the next
+    coarser grid onto the level above's unknowns (rows = their count, every
+    column used); rows whose coarse triangle has all its corners reproduce
+    linear functions of the logical coordinates (the coarse unknowns'
+    positions recovered from the identity rows); no row sums above 1."""
+    m = mesh.channel(128, 32, seed=5)
+    H = mesh.grid_hierarchy(m, min_size=4)
+    first = mesh.grid_prolongation(m, 2)
+    for a, b in zip(H[0], first):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    assert [h[3] for h in H] == sorted([h[3] for h in H], reverse=True) and len(H) == 3
+    pos = m.grid.astype(float)  # logical positions of the level above's unknowns
+    above = m.pressure_count
+    for rs, col, val, nc in H:
+        assert len(rs) == above + 1 and col.min() >= 0 and col.max() < nc
+        assert np.all(np.bincount(col, minlength=nc) > 0)
+        P = sp.csr_matrix((val, col, rs), shape=(above, nc))
+        sums = P.sum(axis=1).A1
+        assert sums.max() <= 1.0 + 1e-12
+        r, c = np.nonzero(P.toarray() == 1.0)
+        cpos = np.full((nc, 2), np.nan)
+        cpos[c] = pos[r]
+        assert not np.isnan(cpos).any()
+        # away from the disk (logical centre ~(11.6, 15.6), where weights of
+        # missing corners are rescaled) and from the outlet column
+        span = 2.0 * np.abs(np.diff(np.unique(cpos[:, 0]))).min()
+        far = (np.hypot(pos[:, 0] - 11.6, pos[:, 1] - 15.6) > 6 + 2 * span) & \
+              (pos[:, 0] < 128 - 2 * span) & np.isclose(sums, 1.0)
+        for a0, a1, a2 in ((0.5, 2.0, -1.0), (1.0, 0.0, 0.3)):
+            np.testing.assert_allclose((P @ (a0 + cpos @ [a1, a2]))[far],
+                                       (a0 + pos @ [a1, a2])[far], rtol=0, atol=1e-9)
+        assert far.sum() > 0.6 * above
+        pos, above = cpos, nc
+    with pytest.raises(ValueError):
+        # 64 x 16 -> 32 x 8 -> 16 x 4 -> 8 x 2 -> 4 x 1, which cannot halve again
+        mesh.grid_hierarchy(mesh.channel(64, 16, seed=5), levels=4)
+
+
 def test_coarse_levels_chain():
     """The multigrid levels below a coarse space: each prolongation maps the
     next coarser cavity's nodes onto the level above (rows = the size of the

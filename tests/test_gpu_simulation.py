@@ -178,7 +178,6 @@ def test_multigrid_coarse_levels(gpu, grids):
     assert 0 < s1["coarse_iterations"] < 0.5 * s0["coarse_iterations"]
 
 
-@pytest.mark.skip(reason="V-cycle preconditioned pressure CG: built, not yet run on a GPU")
 def test_vcycle_preconditioned_pressure_cg(gpu):
     """The pressure solve as CG preconditioned by one V-cycle (128^2 cavity,
     coarse space from 64^2, levels 32^2 and 16^2): every solve meets the
@@ -207,7 +206,8 @@ def test_vcycle_preconditioned_pressure_cg(gpu):
             assert sim.step(1)["cg_iterations"][2] > 15
         sim.close()
     (s0, f0), (s1, f1) = out[False], out[True]
-    print(f"pressure iterations plain {s0['cg_iterations'][2]}, V-cycle PCG {s1['cg_iterations'][2]}")
+    print(f"V-cycle preconditioned pressure CG: pressure iterations plain {s0['cg_iterations'][2]}, "
+          f"V-cycle PCG {s1['cg_iterations'][2]}", flush=True)
     assert max(s1["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
     for a, b in zip(f1, f0):
         assert _rel(a, b) < 1e-6
