@@ -70,7 +70,7 @@ def cg_bytes(nnz, n, iters, refinements, mixed, columns=1):
 # profiles/k_cg_cavity2048_hbm_traffic.md.  roofline.traffic = this figure x
 # the iterations of a launch.
 MEASURED_BYTES_PER_ITERATION = {(False, "pressure"): 721.9e6, (True, "pressure"): 435.4e6,
-                                (True, "diffusion v"): 2723.5e6, (True, "mass u+v"): 4771.2e6}
+                                (True, "diffusion v"): 2723.9e6, (True, "mass u+v"): 4772.8e6}
 
 
 def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
