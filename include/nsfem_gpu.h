@@ -308,6 +308,11 @@ typedef struct {
                                  2x (+8) of each other, else one after the other */
     int32_t threads;          /* host threads for assembly / KD build, 0 = all */
     int32_t assembly;         /* NSFEM_ASSEMBLY_DEVICE (0, default) or _HOST */
+    int32_t velocity_jacobi;  /* refined precision: Jacobi-preconditioned CG (CG on
+                                 D^-1/2 A D^-1/2, stopped on the unscaled residual
+                                 like the plain solve) for the mass solves (bit 0)
+                                 and the diffusion solves (bit 1); 0 = off
+                                 (default) */
 } nsfem_sim_options;
 #define NSFEM_ASSEMBLY_DEVICE 0   /* matrices assembled on the GPU */
 #define NSFEM_ASSEMBLY_HOST 1     /* assembled on host threads, then uploaded */

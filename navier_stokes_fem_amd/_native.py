@@ -63,7 +63,8 @@ class CgStats(C.Structure):
 class SimOptions(C.Structure):
     _fields_ = [("viscosity", C.c_double), ("dt", C.c_double), ("rtol", C.c_double),
                 ("max_iterations", C.c_int32), ("precision", C.c_int32),
-                ("pair_velocity", C.c_int32), ("threads", C.c_int32), ("assembly", C.c_int32)]
+                ("pair_velocity", C.c_int32), ("threads", C.c_int32), ("assembly", C.c_int32),
+                ("velocity_jacobi", C.c_int32)]
 
 
 class StepStats(C.Structure):

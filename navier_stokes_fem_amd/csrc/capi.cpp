@@ -43,6 +43,9 @@ void upload_csr(Device &d, Matrix &m, int rows, int cols, int64_t nnz, const int
     m.col.resize((size_t)std::max<int64_t>(nnz, 1));
     m.val.resize((size_t)std::max<int64_t>(nnz, 1));
     m.val32.release();
+    m.jscale.release();
+    m.jval32.release();
+    m.jacobi_built = m.jacobi_ok = false;
     NSFEM_HIP(hipMemcpy(m.row_start.ptr, rs, sizeof(int32_t) * ((size_t)rows + 1),
                         hipMemcpyHostToDevice));
     if (nnz) {
@@ -75,6 +78,9 @@ void adopt_csr(Device &d, Matrix &m, DeviceCsr &c) {
     m.col = std::move(c.col);
     m.val = std::move(c.val);
     m.val32.release();
+    m.jscale.release();
+    m.jval32.release();
+    m.jacobi_built = m.jacobi_ok = false;
     std::vector<int32_t> hrs((size_t)m.rows + 1);
     NSFEM_HIP(hipMemcpy(hrs.data(), m.row_start.ptr, sizeof(int32_t) * hrs.size(),
                         hipMemcpyDeviceToHost));

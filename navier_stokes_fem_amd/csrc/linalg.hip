@@ -293,6 +293,70 @@ void ensure_val32(Device &d, Matrix &m) {
     }
 }
 
+// --------------------------------------------------------- Jacobi scaling
+namespace {
+
+// scale[r] = 1 / sqrt(a_rr) (the row's diagonal entries summed); a row
+// without a positive diagonal sets *bad (and keeps scale 1)
+__global__ void k_jacobi_scale(int n, const int32_t *__restrict__ rs,
+                               const int32_t *__restrict__ col, const double *__restrict__ val,
+                               double *__restrict__ scale, int32_t *__restrict__ bad) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double d = 0.0;
+    for (int32_t k = rs[r]; k < rs[r + 1]; ++k)
+        if (col[k] == r) d += val[k];
+    if (d > 0.0) {
+        scale[r] = 1.0 / sqrt(d);
+    } else {
+        scale[r] = 1.0;
+        *bad = 1;
+    }
+}
+
+// out[k] = (float)(scale[row] * a_k * scale[col_k]): D^-1/2 A D^-1/2 in FP32
+__global__ void k_jacobi_values(int n, const int32_t *__restrict__ rs,
+                                const int32_t *__restrict__ col, const double *__restrict__ val,
+                                const double *__restrict__ scale, float *__restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const double sr = scale[r];
+    for (int32_t k = rs[r]; k < rs[r + 1]; ++k) out[k] = (float)(sr * val[k] * scale[col[k]]);
+}
+}  // namespace
+
+// The Jacobi-scaled FP32 values and the row scales of a square matrix
+// (built once; 8 zero entries of padding as val32).  False when a row has
+// no positive diagonal entry: the solve then runs unscaled.
+bool ensure_jacobi(Device &d, Matrix &m) {
+    if (m.jacobi_built) return m.jacobi_ok;
+    m.jacobi_built = true;
+    m.jacobi_ok = false;
+    if (m.rows != m.cols || m.rows == 0) return false;
+    m.jscale.resize((size_t)m.rows);
+    DeviceArray<int32_t> bad;
+    bad.resize(1);
+    NSFEM_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(int32_t), d.stream));
+    const int grid = (m.rows + 255) / 256;
+    k_jacobi_scale<<<grid, 256, 0, d.stream>>>(m.rows, m.row_start.ptr, m.col.ptr, m.val.ptr,
+                                               m.jscale.ptr, bad.ptr);
+    NSFEM_HIP(hipGetLastError());
+    int32_t h = 0;
+    NSFEM_HIP(hipMemcpyAsync(&h, bad.ptr, sizeof(h), hipMemcpyDeviceToHost, d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    if (h) {
+        m.jscale.release();
+        return false;
+    }
+    m.jval32.resize((size_t)m.nnz + 8);
+    NSFEM_HIP(hipMemsetAsync(m.jval32.ptr + m.nnz, 0, 8 * sizeof(float), d.stream));
+    k_jacobi_values<<<grid, 256, 0, d.stream>>>(m.rows, m.row_start.ptr, m.col.ptr, m.val.ptr,
+                                                m.jscale.ptr, m.jval32.ptr);
+    NSFEM_HIP(hipGetLastError());
+    m.jacobi_ok = true;
+    return true;
+}
+
 // ------------------------------------------------------------- CSR product
 namespace {
 
@@ -735,7 +799,18 @@ struct CgArgs {
     CgColumnResult *res;
     int nblocks;  // blocks of this solve (batched launches)
     int flat;     // flat grid reduction for solves of <= kFlatMax blocks
+    // Jacobi scaling (mixed solves): 1/sqrt(diag A) per row, or null.  The
+    // FP32 iteration runs on D^-1/2 A D^-1/2 (val32 holds it) with r, p, q
+    // in that scaled space; x, b and the FP64 residual passes stay unscaled
+    // (r_scaled = scale * (b - A x)), so convergence is decided on the
+    // unscaled residual, as without scaling.
+    const double *scale;
 };
+
+// The FP64 step of x_row along the scaled search direction: x = D^-1/2 y.
+__device__ __forceinline__ double x_step(const CgArgs &A, double pend, int row) {
+    return A.scale ? pend * A.scale[row] : pend;
+}
 
 // Launched with at least kFlatMax threads.
 __global__ void k_sync_reset(SyncState *s, int ngroups) {
@@ -1179,12 +1254,13 @@ __device__ void residual_phase(const CgArgs &A, MP meta, int T0, int T1, double 
         }
         if (has_row) {
             VecN<NC, VT> rv, zero;
+            const double sc = A.scale ? A.scale[row] : 1.0;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 const double rr = bb[c] - s[c];
                 acc[c] += rr * rr;
                 if (SETUP) acc[NC + c] += bb[c] * bb[c];
-                rv.v[c] = (VT)rr;
+                rv.v[c] = (VT)(A.scale ? rr * sc : rr);
                 if (MIXED) acc[2 * NC + c] += (double)rv.v[c] * (double)rv.v[c];
                 zero.v[c] = (VT)0;
             }
@@ -1344,7 +1420,8 @@ __device__ void phase_a(const CgArgs &A, MP meta, int T0, int T1, double *lds_wa
             if (has_pend) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c)
-                    A.x[(size_t)c * A.n + row] = __builtin_fma(pend[c], (double)pv.v[c], xo[c]);
+                    A.x[(size_t)c * A.n + row] =
+                        __builtin_fma(x_step(A, pend[c], row), (double)pv.v[c], xo[c]);
             }
         }
     }
@@ -1388,7 +1465,7 @@ __device__ void flush_x(const CgArgs &A, int R0, int R1, const VT *p, const doub
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             double *xp = A.x + (size_t)c * A.n + row;
-            *xp = __builtin_fma(pend[c], (double)pv.v[c], *xp);
+            *xp = __builtin_fma(x_step(A, pend[c], row), (double)pv.v[c], *xp);
         }
     }
 }
@@ -1415,7 +1492,7 @@ __device__ __forceinline__ void cg_body_m(const CgArgs &A, MP meta, const int bi
     residual_phase<NC, BS, MIXED, true, IT, false, MP>(A, meta, T0, T1, lds64, acc);
     if (!grid_sum<3 * NC, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
     {
-        double tol2[NC], gamma[NC], rmax2[NC];
+        double tol2[NC], gamma[NC], rmax2[NC], rho[NC];
         bool active[NC];
         int iters[NC], status[NC], refines = 0;
         VT beta[NC];
@@ -1431,7 +1508,11 @@ __device__ __forceinline__ void cg_body_m(const CgArgs &A, MP meta, const int bi
             active[c] = acc[c] > tol2[c];
             any = any || active[c];
             gamma[c] = MIXED ? acc[2 * NC + c] : acc[c];
-            rmax2[c] = acc[c];
+            // the drop triggers compare the recursion with the true residual
+            // in the iteration's (scaled) space; rho converts a scaled norm
+            // into the unscaled one the tolerance is on
+            rmax2[c] = A.scale ? acc[2 * NC + c] : acc[c];
+            rho[c] = A.scale && acc[2 * NC + c] > 0.0 ? acc[c] / acc[2 * NC + c] : 1.0;
             last_rr[c] = acc[c];
             iters[c] = 0;
             status[c] = NSFEM_OK;
@@ -1501,7 +1582,7 @@ __device__ __forceinline__ void cg_body_m(const CgArgs &A, MP meta, const int bi
                         // rdelta^2 at once (FP32 rounding must not pile up
                         // over a steep fall of the residual)
                         const double d2 = A.rdelta * A.rdelta;
-                        dropped[c] = g[c] <= tol2[c] ||
+                        dropped[c] = g[c] * rho[c] <= tol2[c] ||
                                      (g[c] < d2 * rmax2[c] &&
                                       (k - last_ru >= A.ru_min_gap || g[c] < d2 * d2 * rmax2[c]));
                         want_ru = want_ru || dropped[c];
@@ -1538,7 +1619,9 @@ __device__ __forceinline__ void cg_body_m(const CgArgs &A, MP meta, const int bi
                             active[c] = false;
                         } else {
                             g[c] = ra[2 * NC + c];
-                            rmax2[c] = ra[c];
+                            rmax2[c] = A.scale ? ra[2 * NC + c] : ra[c];
+                            rho[c] = A.scale && ra[2 * NC + c] > 0.0 ? ra[c] / ra[2 * NC + c]
+                                                                     : 1.0;
                             // when the recursion reports a drop by rdelta,
                             // the FP64 residual must fall below
                             // (1 + rdelta)/2 of its best value so far; three
@@ -1742,7 +1825,7 @@ __device__ void phase_f1(const CgArgs &A, MP meta, int T0, int T1, double *lds_w
             sums[2] += rd * qd;
             sums[3] += qd * qd;
             st_state(st_new, row, sn);
-            if (has_pend) A.x[row] = __builtin_fma(pend, (double)so.p, xo);
+            if (has_pend) A.x[row] = __builtin_fma(x_step(A, pend, row), (double)so.p, xo);
         }
     }
 }
@@ -1847,7 +1930,7 @@ __device__ void phase_f1g(const CgArgs &A, MP meta, int T0, int T1, double *lds_
             sums[2] += rd * qd;
             sums[3] += qd * qd;
             st_state(st_new, row, sn);
-            if (has_pend) A.x[row] = __builtin_fma(pend, (double)so.p, xo);
+            if (has_pend) A.x[row] = __builtin_fma(x_step(A, pend, row), (double)so.p, xo);
         }
     }
 }
@@ -1856,14 +1939,14 @@ __device__ void phase_f1g(const CgArgs &A, MP meta, int T0, int T1, double *lds_
 template <int BS>
 __device__ void flush_x1(const CgArgs &A, int R0, int R1, const void *st, double pend) {
     for (int row = R0 + (int)threadIdx.x; row < R1; row += BS)
-        A.x[row] = __builtin_fma(pend, (double)ld_state(st, row).p, A.x[row]);
+        A.x[row] = __builtin_fma(x_step(A, pend, row), (double)ld_state(st, row).p, A.x[row]);
 }
 
 // Control state of a one-column solve, in LDS: thread 0 updates it after
 // every grid reduction and the block reads the step coefficients from it,
 // so the bookkeeping holds no registers during the phases.
 struct CgCtl1 {
-    double tol2, gamma, rmax2, last_rr, ru_rr;
+    double tol2, gamma, rmax2, last_rr, ru_rr, rho;
     double pend, alpha, beta, g;
     int iters, status, stall, active, dropped;
     int k, last_ru, refines, ru, has_pend, out_of_iters;
@@ -1890,7 +1973,7 @@ __device__ void ctl_after_phase(CgCtl1 &C, const CgArgs &A, const double (&sums)
             C.gamma = gk;
             C.iters = k;
             const double d2 = A.rdelta * A.rdelta;
-            dropped = g <= C.tol2 ||
+            dropped = g * C.rho <= C.tol2 ||
                       (g < d2 * C.rmax2 && (k - C.last_ru >= A.ru_min_gap || g < d2 * d2 * C.rmax2));
         }
     }
@@ -1919,7 +2002,8 @@ __device__ void ctl_after_update(CgCtl1 &C, const CgArgs &A, const double (&ra)[
         C.active = 0;
     } else {
         C.g = ra[2];
-        C.rmax2 = ra[0];
+        C.rmax2 = A.scale ? ra[2] : ra[0];
+        C.rho = A.scale && ra[2] > 0.0 ? ra[0] / ra[2] : 1.0;
         if (C.dropped) {
             const double sr = 0.5 * (1.0 + A.rdelta);
             C.stall = ra[0] > sr * sr * C.ru_rr ? C.stall + 1 : 0;
@@ -1980,6 +2064,10 @@ __device__ __forceinline__ void cg_body1_m(const CgArgs &A, MP meta, const int b
             C.active = acc[0] > C.tol2;
             C.gamma = acc[2];
             C.rmax2 = C.last_rr = C.ru_rr = acc[0];
+            // drop triggers in the iteration's (scaled) space; rho converts
+            // a scaled norm into the unscaled one the tolerance is on
+            if (A.scale) C.rmax2 = acc[2];
+            C.rho = A.scale && acc[2] > 0.0 ? acc[0] / acc[2] : 1.0;
             C.g = 0.0;
             C.iters = 0;
             C.status = NSFEM_OK;
@@ -2329,6 +2417,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
     Matrix &m = *req.m;
     const int n = m.rows, nc = req.ncol;
     if (mixed) ensure_val32(d, m);
+    const bool jacobi = mixed && req.jacobi && ensure_jacobi(d, m);
     // 1024 threads per block (16 waves per CU, 128 VGPRs each) for FP64
     // and two-column solves: the loop is bound by gather latency, so waves
     // in flight matter more than registers; 512 threads (8 waves, up to 256
@@ -2366,7 +2455,8 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
     a.col = m.col.ptr;
     a.dcol = m.dcol.ptr;
     a.val64 = m.val.ptr;
-    a.val32 = m.val32.ptr;
+    a.val32 = jacobi ? m.jval32.ptr : m.val32.ptr;
+    a.scale = jacobi ? m.jscale.ptr : nullptr;
     a.b = req.b;
     a.x = req.x;
     unsigned char *w = ws.vectors;

@@ -142,6 +142,11 @@ struct Matrix {
     // group loads (8 nonzeros per lane) need the 16-bit offsets (padded by
     // 8) and every multi-row tile within kStageNnz nonzeros
     bool groups_ok = false;
+    // Jacobi scaling of mixed solves: 1/sqrt(diag) per row and the FP32
+    // values of D^-1/2 A D^-1/2 (padded by 8), built on first use
+    DeviceArray<double> jscale;
+    DeviceArray<float> jval32;
+    bool jacobi_built = false, jacobi_ok = false;
     bool ready = false;
     std::mutex mtx;
 };
@@ -154,6 +159,9 @@ inline void matrix_release(Matrix &m) {
     m.bcol.release();
     m.bcol_ok = false;
     m.groups_ok = false;
+    m.jscale.release();
+    m.jval32.release();
+    m.jacobi_built = m.jacobi_ok = false;
     m.block_tiles_grid = 0;
     m.ready = false;
 }
@@ -247,6 +255,7 @@ void build_tiles(Matrix &m, const std::vector<int32_t> &row_start);
 void build_wave_tiles(Matrix &m, int rows, int nnz);
 void ensure_block_tiles(Device &d, Matrix &m, int grid);
 void ensure_val32(Device &d, Matrix &m);
+bool ensure_jacobi(Device &d, Matrix &m);
 
 void launch_spmv(Device &d, Matrix &m, const double *x, const double *lhs, double *y,
                  int mode);
@@ -267,6 +276,7 @@ struct CgRequest {
     double reliable_delta = 0.5;
     int ru_gap = 0;   // mixed: forced reliable update after this many iterations (0 = 20000)
     int grid = 0;     // 0 = automatic
+    bool jacobi = false;  // mixed: CG on D^-1/2 A D^-1/2 (Jacobi-preconditioned CG)
 };
 // Runs the persistent CG; fills stats[ncol].  Returns the worst status.
 int run_cg(Device &d, const CgRequest &req, nsfem_cg_stats *stats);

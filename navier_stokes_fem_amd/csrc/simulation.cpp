@@ -132,6 +132,11 @@ struct Simulation {
     // last step's iteration counts of the u and v columns of the diffusion
     // (0) and mass (1) solves: they decide whether the next step pairs them
     int64_t vel_iters[2][2] = {{0, 0}, {0, 0}};
+    // measured device seconds per iteration of the diffusion (0) and mass
+    // (1) solves run one column at a time (sec1: both launches over their
+    // iterations) and as one two-column launch (sec2: over its iterations);
+    // 0 = not measured yet
+    double pair_sec1[2] = {0.0, 0.0}, pair_sec2[2] = {0.0, 0.0};
     // internal order of the velocity nodes (velocity_order): vorder[new] =
     // caller's index, vpos[caller's index] = new
     std::vector<int32_t> vorder, vpos;
@@ -391,6 +396,7 @@ void simulation_reset(Simulation &s) {
     s.cproj.dim = 0;
     s.vproj[0].dim = s.vproj[1].dim = 0;
     for (auto &row : s.vel_iters) row[0] = row[1] = 0;
+    for (int k = 0; k < 2; ++k) s.pair_sec1[k] = s.pair_sec2[k] = 0.0;
     launch_fill(d, 2 * s.V, 0.0, s.uv.ptr);
     launch_fill(d, s.P, 0.0, s.p.ptr);
     launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, s.uv.ptr);
@@ -436,6 +442,8 @@ void solve(Simulation &s, int slot, const double *b, double *x, int ncol, SolveO
     rq.rtol = s.opt.rtol;
     rq.atol = 0.0;
     rq.precision = s.opt.precision;
+    rq.jacobi = (slot == NSFEM_VELOCITY_MASS && (s.opt.velocity_jacobi & 1)) ||
+                (slot == NSFEM_DIFFUSION && (s.opt.velocity_jacobi & 2));
     set_update_policy(rq, slot);
     nsfem_cg_stats st[2]{};
     const int status = run_cg(d, rq, st);
@@ -1268,11 +1276,33 @@ void diffusion_after_solve(Simulation &s) {
 // columns are solved one after the other.  On the 2048^2 cavity the
 // diffusion columns take ~28 and ~124 iterations (separately 107 ms a step
 // against 120 paired), the mass columns ~28 and ~24 (paired 34 ms against 45).
-bool pair_columns(const Simulation &s, int sys) {
+// Whether the next step solves a velocity system's u and v columns as one
+// two-column CG.  Once both ways have been timed on this simulation, by
+// cost: pairing takes max(a, b) two-column iterations, the separate solves
+// a + b one-column ones, a and b the last step's counts (on the 2048^2
+// cavity's P2 matrices a two-column iteration costs 1.95 one-column ones,
+// so only nearly equal counts pay; on small systems, bound by launch and
+// synchronisation latency, pairing pays more often).  Before that, counts
+// within 2x (+8) of each other pair, and a system measured only paired is
+// run apart once to time it.
+bool pair_columns(Simulation &s, int sys) {
     if (!s.opt.pair_velocity) return false;
     const int64_t a = s.vel_iters[sys][0], b = s.vel_iters[sys][1];
     if (a <= 0 || b <= 0) return true;
-    return std::max(a, b) <= 2 * std::min(a, b) + 8;
+    const double t1 = s.pair_sec1[sys], t2 = s.pair_sec2[sys];
+    if (t1 > 0.0 && t2 > 0.0) return t2 * (double)std::max(a, b) <= t1 * (double)(a + b);
+    const bool close = std::max(a, b) <= 2 * std::min(a, b) + 8;
+    return close && !(t2 > 0.0 && t1 == 0.0);
+}
+
+// Records the timing of a velocity system's step for pair_columns
+// (exponential average, weight 1/2).
+void record_pairing(Simulation &s, int sys, bool paired, const SolveOut &u, const SolveOut &v) {
+    double &t = paired ? s.pair_sec2[sys] : s.pair_sec1[sys];
+    const int64_t its = paired ? std::max(u.iters[0], u.iters[1]) : u.iters[0] + v.iters[0];
+    if (its <= 0) return;
+    const double sec = (paired ? u.seconds : u.seconds + v.seconds) / (double)its;
+    t = t > 0.0 ? 0.5 * (t + sec) : sec;
 }
 
 void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
@@ -1313,6 +1343,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         }
         s.vel_iters[0][0] = o[0].iters[0];
         s.vel_iters[0][1] = o[1].iters[0];
+        record_pairing(s, 0, pair_d, o[0], o[1]);
         if (s.vproj[0].max > 0) diffusion_after_solve(s);
         // 3. pressure
         launch_spmv(d, BD, s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
@@ -1340,6 +1371,7 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         }
         s.vel_iters[1][0] = o[3].iters[0];
         s.vel_iters[1][1] = o[4].iters[0];
+        record_pairing(s, 1, pair_m, o[3], o[4]);
         // 5. update and Dirichlet values
         launch_add(d, 2 * V, s.du.ptr, s.uv.ptr);
         launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, u);
@@ -1817,6 +1849,9 @@ int nsfem_simulation_set_options(nsfem_simulation *s, const nsfem_sim_options *o
         s->impl->opt.max_iterations = opt->max_iterations;
         s->impl->opt.precision = opt->precision;
         s->impl->opt.pair_velocity = opt->pair_velocity;
+        s->impl->opt.velocity_jacobi = opt->velocity_jacobi;
+        // the solves' cost per iteration may change: time both ways again
+        for (int k = 0; k < 2; ++k) s->impl->pair_sec1[k] = s->impl->pair_sec2[k] = 0.0;
     });
 }
 

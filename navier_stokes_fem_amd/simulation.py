@@ -20,7 +20,8 @@ __all__ = ["Simulation", "step_batch", "autotune_batch"]
 class Simulation:
     def __init__(self, mesh, viscosity: float, dt: float, device: GPUSimulationDevice = None,
                  rtol: float = 1e-8, precision: str = "fp64", pair_velocity: bool = True,
-                 max_iterations: int = -1, threads: int = 0, assembly: str = "device"):
+                 max_iterations: int = -1, threads: int = 0, assembly: str = "device",
+                 velocity_jacobi: bool = False):
         self.mesh = mesh
         self.device = device if device is not None else GPUSimulationDevice(0)
         self.options = N.SimOptions(viscosity=float(viscosity), dt=float(dt), rtol=float(rtol),
@@ -30,7 +31,8 @@ class Simulation:
                                     pair_velocity=int(bool(pair_velocity)),
                                     threads=int(threads),
                                     assembly={"device": N.ASSEMBLY_DEVICE,
-                                              "host": N.ASSEMBLY_HOST}[assembly])
+                                              "host": N.ASSEMBLY_HOST}[assembly],
+                                    velocity_jacobi=self._jacobi_bits(velocity_jacobi))
         xy, el = mesh.xy(), mesh.elems()
         vb = np.ascontiguousarray(mesh.vel_bc_nodes, np.int32)
         gu = np.ascontiguousarray(mesh.vel_bc_u, np.float64)
@@ -61,9 +63,23 @@ class Simulation:
         a = [None if x is None else np.ascontiguousarray(x, np.float64) for x in (u, v, p)]
         N.check(N.lib().nsfem_simulation_set_fields(self.handle, *[N.ptr(x) for x in a]))
 
-    def configure(self, rtol=None, precision=None, pair_velocity=None, max_iterations=None):
-        """Change solver options (viscosity and dt are fixed at creation)."""
+    @staticmethod
+    def _jacobi_bits(v) -> int:
+        """``velocity_jacobi``: False / "none", "mass", "diffusion", True /
+        "both" -> the option's bits (1 = mass, 2 = diffusion)."""
+        if isinstance(v, str):
+            return {"none": 0, "mass": 1, "diffusion": 2, "both": 3}[v]
+        return 3 if v else 0
+
+    def configure(self, rtol=None, precision=None, pair_velocity=None, max_iterations=None,
+                  velocity_jacobi=None):
+        """Change solver options (viscosity and dt are fixed at creation).
+        ``velocity_jacobi``: in the refined precision, the mass and/or
+        diffusion solves as Jacobi-preconditioned CG (same stopping rule):
+        False / "none", "mass", "diffusion", True / "both"."""
         o = self.options
+        if velocity_jacobi is not None:
+            o.velocity_jacobi = self._jacobi_bits(velocity_jacobi)
         if rtol is not None:
             o.rtol = float(rtol)
         if precision is not None:
