@@ -99,10 +99,15 @@ def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
 
 def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
              coarse_rtol=1e-6, velocity_projection=0, coarse_levels=(), rtol=1e-8,
-             pressure_multigrid=False, pair_velocity=True, velocity_jacobi=False):
+             pressure_multigrid=False, pair_velocity=True, velocity_jacobi=False,
+             velocity_multigrid=0):
+    """Steps `sim` in one form after `warmup` untimed steps.  A nonzero
+    velocity_multigrid needs the simulation's velocity hierarchy set."""
     sim.reset()
     sim.configure(precision=form, rtol=rtol, pair_velocity=pair_velocity,
                   velocity_jacobi=velocity_jacobi)
+    if velocity_multigrid or getattr(sim, "_velocity_sizes", None):
+        sim.set_velocity_multigrid(None, mode=velocity_multigrid)
     sim.set_warm_start(warm)
     sim.set_coarse_space(coarse, rtol=coarse_rtol)
     if coarse is not None:
@@ -122,7 +127,8 @@ def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
             agg = st
         else:
             for k in ("cg_iterations", "cg_seconds", "cg_refinements", "cg_launches",
-                      "cg_launch_iterations"):
+                      "cg_launch_iterations", "vcycle_solves", "vcycle_iterations",
+                      "vcycle_seconds"):
                 agg[k] = [a + b for a, b in zip(agg[k], st[k])]
             agg["max_relative_residual"] = [max(a, b) for a, b in zip(agg["max_relative_residual"],
                                                                        st["max_relative_residual"])]
@@ -674,6 +680,12 @@ def main():
                          "preconditioned by one V-cycle over the fine grid, the coarse space and "
                          "its levels; off: CG on the pressure with the coarse correction of its "
                          "guess")
+    ap.add_argument("--velocity-multigrid", type=int, default=2, choices=[0, 1, 2],
+                    help="fp32_refined form: the diffusion columns by FP64 CG preconditioned with "
+                         "one V-cycle over the velocity nodes, the P1 vertex space and its grid "
+                         "halvings (2, default: both columns -- at 2048^2 the v column takes 6-9 "
+                         "iterations a step instead of ~90; 1: each column the way measured "
+                         "faster; 0: off)")
     ap.add_argument("--coarse-levels", type=int, default=-1,
                     help="multigrid levels below the coarse space (each halves the grid; -1 = "
                          "halve down to the 16x16 grid); with levels the coarse system is solved "
@@ -761,6 +773,13 @@ def main():
         log(f"prolongation from the {args.n // args.coarse_ratio}^2 cavity "
             f"({time.perf_counter() - t:.1f} s)")
     levels = coarse_levels(args.n // args.coarse_ratio, args.coarse_levels) if coarse else []
+    if args.velocity_multigrid:
+        t = time.perf_counter()
+        vhier = mesh.velocity_hierarchy(m)
+        t1 = time.perf_counter()
+        sim.set_velocity_multigrid(vhier, mode=0)
+        log(f"velocity hierarchy {[h[3] for h in vhier]}: prolongations {t1 - t:.1f} s, "
+            f"Galerkin products {time.perf_counter() - t1:.1f} s")
     results = {}
     forms = ["fp64", "fp32_refined"] if args.form == "both" else [args.form]
     for form in forms:
@@ -772,7 +791,8 @@ def main():
                             coarse_levels=levels,
                             pressure_multigrid=args.pressure_multigrid and mixed_form,
                             pair_velocity=bool(args.pair_velocity) or not mixed_form,
-                            velocity_jacobi=args.velocity_jacobi if mixed_form else "none")
+                            velocity_jacobi=args.velocity_jacobi if mixed_form else "none",
+                            velocity_multigrid=args.velocity_multigrid if mixed_form else 0)
         results[form] = (wall, st)
         log(f"{form}: {args.steps / wall:.3f} steps/s")
     # attribution: the same mixed solver without the coarse correction, and
@@ -809,17 +829,20 @@ def main():
     # reported beside it
     kinds = [("diffusion u", 0, 1), ("diffusion v", 1, 1), ("pressure", 2, 1),
              ("mass u+v", 3, 2), ("mass v", 4, 1)]
-    pmg = mixed and args.pressure_multigrid
     n_v, nnz_v = sim.matrix_info(2)[0], sim.matrix_info(2)[2]
-    cand = [k for k in kinds if not (pmg and k[1] == 2) and st["cg_launches"][k[1]] > 0]
-    name, k, cols = max(cand, key=lambda k: st["cg_seconds"][k[1]])
+    # the CG-kernel launches of each kind: all solves less the V-cycle ones
+    plain_launches = [a - b for a, b in zip(st["cg_launches"], st["vcycle_solves"])]
+    plain_iters = [a - b for a, b in zip(st["cg_launch_iterations"], st["vcycle_iterations"])]
+    plain_secs = [a - b for a, b in zip(st["cg_seconds"], st["vcycle_seconds"])]
+    cand = [k for k in kinds if plain_launches[k[1]] > 0]
+    name, k, cols = max(cand, key=lambda k: plain_secs[k[1]])
     # a paired u+v launch has 2 columns; the u entry mixes both when pairing varied
     if k == 0 and st["cg_launches"][1] < st["cg_launches"][0]:
         cols = 2
     n_k, nnz_k = (n_p, nnz_p) if k == 2 else (n_v, nnz_v)
-    launches = st["cg_launches"][k]
-    k_iters = st["cg_launch_iterations"][k]
-    k_secs = st["cg_seconds"][k]
+    launches = plain_launches[k]
+    k_iters = plain_iters[k]
+    k_secs = plain_secs[k]
     w = 4 if mixed else 8
     textbook = (4 + w) * nnz_k + (4 + 11 * w * cols) * n_k
     bytes_textbook = textbook * k_iters
@@ -846,7 +869,10 @@ def main():
                     "achieved": round(bytes_k / max(k_secs, 1e-12) / 1e9, 1),
                     "frac": round(bytes_k / max(k_secs, 1e-12) / 1e9 / PEAK_HBM_GBS, 4)},
                 "seconds_per_step_by_solve": {
-                    kn: round(st["cg_seconds"][kk] / args.steps, 5) for kn, kk, _ in kinds}}
+                    kn: round(st["cg_seconds"][kk] / args.steps, 5) for kn, kk, _ in kinds},
+                "vcycle_seconds_per_step_by_solve": {
+                    kn: round(st["vcycle_seconds"][kk] / args.steps, 5) for kn, kk, _ in kinds
+                    if st["vcycle_solves"][kk]}}
     out = {
         "metric": "steps/sec at 1e-8 rel. residual, chosen vs plain FP64 CG; achieved HBM GB/s",
         "value": round(value, 4),
@@ -887,6 +913,15 @@ def main():
                        f"inverse), to the same 1e-8 relative residual"
                        if chosen == "fp32_refined" and args.pressure_multigrid and coarse is not None
                        else ("FP32 + FP64 refinement CG" if chosen == "fp32_refined"
+                             else "FP64 CG")),
+                   "diffusion_solve": (
+                       ("FP64 CG preconditioned by one V-cycle (damped Jacobi 2+2 sweeps on "
+                        "the P2 nodes, the P1 vertex space and its grid halvings, the deepest by "
+                        "its dense inverse) for "
+                        + ("each column when measured faster than the FP32 + FP64 refinement CG"
+                           if args.velocity_multigrid == 1 else "both columns"))
+                       if chosen == "fp32_refined" and args.velocity_multigrid
+                       else ("FP32 + FP64 refinement CG" if chosen == "fp32_refined"
                              else "FP64 CG"))},
         "roofline": roofline,
     }
@@ -897,6 +932,8 @@ def main():
                      "cg_refinements": s["cg_refinements"],
                      "coarse_iterations": s["coarse_iterations"],
                      "coarse_seconds": round(s["coarse_seconds"], 4),
+                     "vcycle_solves": s["vcycle_solves"],
+                     "vcycle_iterations": s["vcycle_iterations"],
                      "advect_seconds": round(s["advect_seconds"], 4)}
     # SpMV micro-benchmark on the assembled pressure and diffusion matrices
     out["spmv"] = {}
@@ -915,7 +952,8 @@ def main():
                              velocity_projection=args.velocity_projection, coarse_levels=levels,
                              rtol=1e-12, pressure_multigrid=args.pressure_multigrid,
                              pair_velocity=bool(args.pair_velocity),
-                             velocity_jacobi=args.velocity_jacobi)
+                             velocity_jacobi=args.velocity_jacobi,
+                             velocity_multigrid=args.velocity_multigrid)
         log(f"tight reference (1e-12) {time.perf_counter() - t:.1f} s, max residuals "
             f"{st_t['max_relative_residual']}")
         tight = st_t["fields"]

@@ -334,6 +334,11 @@ typedef struct {
                                    a batch's shared launch under every instance) */
     int64_t cg_launch_iterations[5]; /* their iterations (a paired solve: the
                                    larger column's) */
+    int64_t vcycle_solves[5];   /* of cg_launches[k], the solves by V-cycle
+                                   preconditioned CG (host-driven products and
+                                   smoothing sweeps, no CG-kernel launch) */
+    int64_t vcycle_iterations[5]; /* of cg_launch_iterations[k], theirs */
+    double vcycle_seconds[5];     /* of cg_seconds[k], theirs (wall time) */
 } nsfem_step_stats;
 
 int nsfem_simulation_create(nsfem_device *dev, int node_count,
@@ -506,6 +511,37 @@ int nsfem_simulation_add_coarse_level(nsfem_simulation *sim, int count, int64_t 
  * counterpart in the reference, whose pressure CG is unpreconditioned
  * (include/assembly.h:608-615). */
 int nsfem_simulation_set_pressure_multigrid(nsfem_simulation *sim, int enable);
+/* Multigrid hierarchy below the velocity (P2) space for the diffusion
+ * solves.  P has one row per velocity node, in the caller's node order, and
+ * coarse_count columns (CSR, ascending columns); the rows of velocity
+ * Dirichlet nodes must be empty (so every coarse correction vanishes on the
+ * velocity boundary); A_1 = P^T A_diffusion P is formed here, on the host.
+ * Setting it switches the diffusion solves to mode 1 of
+ * nsfem_simulation_set_velocity_multigrid; coarse_count = 0 removes the
+ * hierarchy (and the mode).  A call that fails part way leaves none.  No
+ * counterpart in the reference, whose GPU diffusion CG is unpreconditioned
+ * (include/assembly.h:528-538; its CPU CG can take an IC0 preconditioner,
+ * :539-551). */
+int nsfem_simulation_set_velocity_coarse_space(nsfem_simulation *sim, int coarse_count,
+                                               int64_t nnz, const int32_t *row_start,
+                                               const int32_t *col, const double *val);
+/* Adds a level below the deepest level of the velocity hierarchy: P has one
+ * row per node of the level above and `count` columns; the new level's
+ * matrix is P^T A_above P (a level of at most 600 nodes is solved by its
+ * dense inverse, else by CG to 1e-10).  No counterpart in the reference. */
+int nsfem_simulation_add_velocity_coarse_level(nsfem_simulation *sim, int count, int64_t nnz,
+                                               const int32_t *row_start, const int32_t *col,
+                                               const double *val);
+/* Which diffusion columns (u, v) are solved by CG preconditioned with one
+ * V-cycle over the velocity nodes and the hierarchy (damped Jacobi, 2 sweeps
+ * before and after on every level, z zeroed on the velocity Dirichlet rows),
+ * in FP64 whatever the precision option, to the same relative residual from
+ * the same guess: 0 none, 1 each column the way (plain or V-cycle) whose
+ * solves of that column have taken less time so far, the slower way timed
+ * again every 16 steps, 2 both columns.  Needs the hierarchy for modes 1 and
+ * 2; while it is on the diffusion columns are never paired, and batches
+ * reject the simulation.  No counterpart in the reference. */
+int nsfem_simulation_set_velocity_multigrid(nsfem_simulation *sim, int mode);
 /* Copies the last step's pressure right-hand side and pressure into device
  * buffers (either may be NULL): training pairs for the network. */
 int nsfem_simulation_copy_pressure_system(nsfem_simulation *sim, nsfem_buffer *rhs,

@@ -73,7 +73,8 @@ class StepStats(C.Structure):
                 ("advect_seconds", C.c_double), ("cg_seconds", C.c_double * 5),
                 ("cg_refinements", C.c_int64 * 5), ("coarse_iterations", C.c_int64),
                 ("coarse_seconds", C.c_double), ("cg_launches", C.c_int64 * 5),
-                ("cg_launch_iterations", C.c_int64 * 5)]
+                ("cg_launch_iterations", C.c_int64 * 5), ("vcycle_solves", C.c_int64 * 5),
+                ("vcycle_iterations", C.c_int64 * 5), ("vcycle_seconds", C.c_double * 5)]
 
     def as_dict(self):
         return {"steps": self.steps, "cg_iterations": list(self.cg_iterations),
@@ -84,7 +85,10 @@ class StepStats(C.Structure):
                 "coarse_iterations": self.coarse_iterations,
                 "coarse_seconds": self.coarse_seconds,
                 "cg_launches": list(self.cg_launches),
-                "cg_launch_iterations": list(self.cg_launch_iterations)}
+                "cg_launch_iterations": list(self.cg_launch_iterations),
+                "vcycle_solves": list(self.vcycle_solves),
+                "vcycle_iterations": list(self.vcycle_iterations),
+                "vcycle_seconds": list(self.vcycle_seconds)}
 
 
 class BatchTuning(C.Structure):
@@ -174,6 +178,11 @@ _SIGNATURES = {
                                           C.c_int),
     "nsfem_simulation_add_coarse_level": ([vp, C.c_int, C.c_int64, vp, vp, vp], C.c_int),
     "nsfem_simulation_set_pressure_multigrid": ([vp, C.c_int], C.c_int),
+    "nsfem_simulation_set_velocity_coarse_space": ([vp, C.c_int, C.c_int64, vp, vp, vp],
+                                                   C.c_int),
+    "nsfem_simulation_add_velocity_coarse_level": ([vp, C.c_int, C.c_int64, vp, vp, vp],
+                                                   C.c_int),
+    "nsfem_simulation_set_velocity_multigrid": ([vp, C.c_int], C.c_int),
     "nsfem_simulation_set_velocity_projection": ([vp, C.c_int], C.c_int),
     "nsfem_simulation_copy_pressure_system": ([vp, vp, vp], C.c_int),
     "nsfem_gnn_create": ([vp, C.c_uint64, PP], C.c_int),

@@ -72,6 +72,25 @@ struct MgLevel {
     DeviceArray<double> dinv, b, x, r;
 };
 
+// The levels below a system (lv[0] the next coarser one), the host CSR of
+// the deepest level's matrix (for the next level's Galerkin product) and,
+// for a deepest level of at most kMgDenseMax nodes, its dense inverse.
+struct MgLevels {
+    std::vector<std::unique_ptr<MgLevel>> lv;
+    Matrix deep_inv;  // as CSR
+    bool deep_ok = false;
+    std::vector<int32_t> deep_rs, deep_col;
+    std::vector<double> deep_val;
+    void release() {
+        lv.clear();
+        matrix_release(deep_inv);
+        deep_ok = false;
+        deep_rs.clear();
+        deep_col.clear();
+        deep_val.clear();
+    }
+};
+
 struct Simulation {
     Device *dev = nullptr;
     int V = 0, P = 0;
@@ -108,20 +127,27 @@ struct Simulation {
     Matrix cP, cPT, cA;
     DeviceArray<double> cy, cr, ce;
     ProjBasis cproj;  // projection guess of the coarse solves
-    // multigrid levels below the coarse grid (empty: the coarse system is
-    // solved by CG); the coarse level's 1/diag and residual; the host CSR of
-    // the deepest level's matrix, for the next level's Galerkin product
-    std::vector<std::unique_ptr<MgLevel>> mg;
-    Matrix deep_inv;  // dense inverse of the deepest level's matrix (as CSR)
-    bool deep_inv_ok = false;
+    // multigrid levels below the coarse grid (none: the coarse system is
+    // solved by CG); the coarse level's 1/diag and residual
+    MgLevels mg;
     // pressure CG preconditioned by one V-cycle over the fine grid, the
     // coarse space and its levels: 1/diag of the pressure matrix, the CG
     // vectors r, z, p, q, a smoothing residual t and a dot scratch
     bool pmg = false;
     DeviceArray<double> pdinv, pr, pz, pp, pq, pt, pdot;
     DeviceArray<double> cdinv, cres;
-    std::vector<int32_t> deep_rs, deep_col;
-    std::vector<double> deep_val;
+    // diffusion CG preconditioned by one V-cycle over the velocity nodes,
+    // the velocity coarse space (vmg.lv[0]) and the levels below it: mode
+    // (0 off, 1 per column by measured cost, 2 every column), 1/diag of the
+    // diffusion matrix, the CG vectors r, z, p, q, the fine smoothing
+    // residual t and a dot scratch; per column the measured seconds of its
+    // solves each way (exponential averages; 0 = not measured) and the steps
+    // since each way last ran
+    MgLevels vmg;
+    int vmg_mode = 0;
+    DeviceArray<double> vdinv, vr, vz, vp, vq, vt, vdot;
+    double vmg_sec[2][2] = {{0.0, 0.0}, {0.0, 0.0}};  // [column][plain, V-cycle]
+    int vmg_age[2][2] = {{0, 0}, {0, 0}};
     // projection guess of the diffusion solves' corrections u - u~ (u~ = the
     // advected velocity), one basis per column; vdel: the guessed, then the
     // true, corrections
@@ -666,17 +692,21 @@ constexpr int kMgSweeps = 2;
 constexpr double kMgOmega = 0.8;
 constexpr int kMgMaxCycles = 30;
 
-void vcycle(Simulation &s, int l, const double *b, double *x) {
-    Device &d = *s.dev;
-    const int L = (int)s.mg.size();
-    Matrix &A = l == 0 ? s.cA : s.mg[l - 1]->A;
-    const int n = l == 0 ? s.nc : s.mg[l - 1]->n;
-    if (l == L && s.deep_inv_ok) {
-        launch_spmv(d, s.deep_inv, b, nullptr, x, 0);
+// One V-cycle on A x = b over a level (A of n rows, its 1/diag, a residual
+// vector r) and the levels below it, m.lv[l] the next coarser, from x = 0
+// (zero: x is cleared here and the first sweep needs no product) or from
+// the x given: kMgSweeps damped Jacobi sweeps, the correction from the level
+// below on the restricted residual, as many sweeps after.  The deepest level
+// (l = m.lv.size()) is solved by its dense inverse when it has one, else by
+// FP64 CG to 1e-10 from zero.
+void mg_cycle(Device &d, Matrix &A, int n, const double *dinv, double *r, MgLevels &m, size_t l,
+              const double *b, double *x, bool zero = true) {
+    if (l == m.lv.size() && m.deep_ok) {
+        launch_spmv(d, m.deep_inv, b, nullptr, x, 0);
         return;
     }
-    if (l == L) {
-        launch_fill(d, n, 0.0, x);
+    if (zero || l == m.lv.size()) launch_fill(d, n, 0.0, x);
+    if (l == m.lv.size()) {
         CgRequest rq{};
         rq.m = &A;
         rq.b = b;
@@ -689,23 +719,31 @@ void vcycle(Simulation &s, int l, const double *b, double *x) {
         run_cg(d, rq, &st);
         return;
     }
-    const double *dinv = l == 0 ? s.cdinv.ptr : s.mg[l - 1]->dinv.ptr;
-    double *r = l == 0 ? s.cres.ptr : s.mg[l - 1]->r.ptr;
     for (int k = 0; k < kMgSweeps; ++k) {
+        if (k == 0 && zero) {
+            launch_jacobi(d, n, kMgOmega, dinv, b, x);
+            continue;
+        }
         launch_spmv(d, A, x, b, r, 1);
         launch_jacobi(d, n, kMgOmega, dinv, r, x);
     }
     launch_spmv(d, A, x, b, r, 1);
-    MgLevel &below = *s.mg[l];
+    MgLevel &below = *m.lv[l];
     launch_spmv(d, below.PT, r, nullptr, below.b.ptr, 0);
-    launch_fill(d, below.n, 0.0, below.x.ptr);
-    vcycle(s, l + 1, below.b.ptr, below.x.ptr);
+    mg_cycle(d, below.A, below.n, below.dinv.ptr, below.r.ptr, m, l + 1, below.b.ptr,
+             below.x.ptr);
     launch_spmv(d, below.P, below.x.ptr, nullptr, r, 0);
     launch_add(d, n, r, x);
     for (int k = 0; k < kMgSweeps; ++k) {
         launch_spmv(d, A, x, b, r, 1);
         launch_jacobi(d, n, kMgOmega, dinv, r, x);
     }
+}
+
+// One V-cycle from the pressure coarse level down, from zero or from the x
+// given.
+void vcycle(Simulation &s, const double *b, double *x, bool zero) {
+    mg_cycle(*s.dev, s.cA, s.nc, s.cdinv.ptr, s.cres.ptr, s.mg, 0, b, x, zero);
 }
 
 // V-cycles on A_c e = cr from the current e; returns the status, the cycle
@@ -721,7 +759,7 @@ int mg_solve(Simulation &s, int64_t *cycles, double *seconds) {
         status = NSFEM_OK;  // e = projection guess of a zero right-hand side
     } else {
         while (k < kMgMaxCycles) {
-            vcycle(s, 0, s.cr.ptr, s.ce.ptr);
+            vcycle(s, s.cr.ptr, s.ce.ptr, false);
             ++k;
             launch_spmv(d, s.cA, s.ce.ptr, s.cr.ptr, s.cres.ptr, 1);
             const double rr = dev_dot(d, s.cproj, s.cres.ptr, s.cres.ptr);
@@ -781,7 +819,7 @@ void coarse_finish(Simulation &s, int status, int64_t iterations, double seconds
 
 void coarse_correct(Simulation &s) {
     coarse_prepare(s);
-    if (!s.mg.empty()) {
+    if (!s.mg.lv.empty()) {
         int64_t cycles = 0;
         double sec = 0.0;
         const int status = mg_solve(s, &cycles, &sec);
@@ -812,8 +850,7 @@ void fine_vcycle(Simulation &s, const double *r, double *z) {
     }
     launch_spmv(d, A, z, r, s.pt.ptr, 1);
     launch_spmv(d, s.cPT, s.pt.ptr, nullptr, s.cr.ptr, 0);
-    launch_fill(d, s.nc, 0.0, s.ce.ptr);
-    vcycle(s, 0, s.cr.ptr, s.ce.ptr);
+    vcycle(s, s.cr.ptr, s.ce.ptr, true);
     launch_spmv(d, s.cP, s.ce.ptr, nullptr, s.pt.ptr, 0);
     launch_add(d, n, s.pt.ptr, z);
     for (int k = 0; k < kMgSweeps; ++k) {
@@ -823,46 +860,48 @@ void fine_vcycle(Simulation &s, const double *r, double *z) {
     launch_scatter_value(d, s.npb, s.pb_idx.ptr, 0.0, z);
 }
 
-// a . b over n entries (one host synchronisation)
-double vec_dot(Simulation &s, int n, const double *a, const double *b) {
-    Device &d = *s.dev;
+// a . b over n entries, through a one-entry device scratch (one host
+// synchronisation)
+double vec_dot(Device &d, int n, const double *a, const double *b, double *scratch) {
     double h = 0.0;
-    launch_fill(d, 1, 0.0, s.pdot.ptr);
-    launch_dot_accumulate(d, n, a, b, s.pdot.ptr);
-    NSFEM_HIP(hipMemcpyAsync(&h, s.pdot.ptr, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    launch_fill(d, 1, 0.0, scratch);
+    launch_dot_accumulate(d, n, a, b, scratch);
+    NSFEM_HIP(hipMemcpyAsync(&h, scratch, sizeof(double), hipMemcpyDeviceToHost, d.stream));
     NSFEM_HIP(hipStreamSynchronize(d.stream));
     return h;
 }
 
-// The pressure solve as CG preconditioned by fine_vcycle, in FP64, from the
-// guess in p to ||b - A x|| <= rtol ||b||; convergence is decided on a
-// recomputed b - A x, which then replaces the recursive residual.
-void pcg_pressure(Simulation &s, SolveOut &o) {
-    Device &d = *s.dev;
-    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
-    const int n = s.P;
+// Vectors of a preconditioned CG over n rows: residual, preconditioned
+// residual, direction, product, and a one-entry dot scratch.
+struct PcgWork {
+    double *r, *z, *p, *q, *dot;
+};
+
+// CG preconditioned by z = precond(r) (symmetric positive definite), in
+// FP64, on A x = b from the x given, to ||b - A x|| <= rtol ||b||;
+// convergence is decided on a recomputed b - A x, which then replaces the
+// recursive residual.  Fills o's column `c` and seconds; returns the status.
+template <class Pre>
+int pcg_run(Device &d, Matrix &A, int n, const double *b, double *x, const PcgWork &w,
+            double rtol, int64_t max_iterations, Pre precond, SolveOut &o, int c) {
     const auto t0 = std::chrono::steady_clock::now();
-    double *x = s.p.ptr, *r = s.pr.ptr, *z = s.pz.ptr, *p = s.pp.ptr, *q = s.pq.ptr;
-    const double *b = s.prhs.ptr;
-    // the Dirichlet rows (identity rows) start at their values, so their
-    // residual is zero and z, p stay zero there
-    launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, x);
-    const double bb = vec_dot(s, n, b, b);
+    double *r = w.r, *z = w.z, *p = w.p, *q = w.q;
+    const double bb = vec_dot(d, n, b, b, w.dot);
     launch_spmv(d, A, x, b, r, 1);
-    double rr = vec_dot(s, n, r, r);
+    double rr = vec_dot(d, n, r, r, w.dot);
     const double ref = bb > 0.0 ? bb : rr;
-    const double tol2 = s.opt.rtol * s.opt.rtol * ref;
-    const int64_t max_iter = s.opt.max_iterations > 0 ? s.opt.max_iterations : n;
+    const double tol2 = rtol * rtol * ref;
+    const int64_t max_iter = max_iterations > 0 ? max_iterations : n;
     int64_t k = 0;
     int status = NSFEM_OK;
     if (rr > tol2) {
         status = NSFEM_MAX_ITERATIONS_REACHED;
-        fine_vcycle(s, r, z);
+        precond(r, z);
         launch_copy(d, n, z, p);
-        double rz = vec_dot(s, n, r, z);
+        double rz = vec_dot(d, n, r, z, w.dot);
         while (k < max_iter) {
             launch_spmv(d, A, p, nullptr, q, 0);
-            const double pq = vec_dot(s, n, p, q);
+            const double pq = vec_dot(d, n, p, q, w.dot);
             if (!(pq > 0.0) || !(rz > 0.0)) {
                 status = NSFEM_BREAKDOWN;
                 break;
@@ -871,29 +910,70 @@ void pcg_pressure(Simulation &s, SolveOut &o) {
             launch_axpy(d, n, alpha, p, x);
             launch_axpy(d, n, -alpha, q, r);
             ++k;
-            rr = vec_dot(s, n, r, r);
+            rr = vec_dot(d, n, r, r, w.dot);
             if (rr <= tol2) {
                 launch_spmv(d, A, x, b, r, 1);
-                rr = vec_dot(s, n, r, r);
+                rr = vec_dot(d, n, r, r, w.dot);
                 if (rr <= tol2) {
                     status = NSFEM_OK;
                     break;
                 }
             }
-            fine_vcycle(s, r, z);
-            const double rz_new = vec_dot(s, n, r, z);
+            precond(r, z);
+            const double rz_new = vec_dot(d, n, r, z, w.dot);
             const double beta = rz_new / rz;
             rz = rz_new;
             launch_axpby(d, n, 1.0, beta, z, p, p);
         }
     }
+    o.iters[c] = k;
+    o.refines[c] = 0;
+    o.rel[c] = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    o.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return status;
+}
+
+// The pressure solve as CG preconditioned by fine_vcycle from the guess in
+// p.  The Dirichlet rows (identity rows) start at their values, so their
+// residual is zero and z, p stay zero there.
+void pcg_pressure(Simulation &s, SolveOut &o) {
+    Device &d = *s.dev;
+    launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.p.ptr);
+    const PcgWork w{s.pr.ptr, s.pz.ptr, s.pp.ptr, s.pq.ptr, s.pdot.ptr};
+    const int status =
+        pcg_run(d, s.mats[NSFEM_PRESSURE_STIFFNESS], s.P, s.prhs.ptr, s.p.ptr, w, s.opt.rtol,
+                s.opt.max_iterations, [&](const double *r, double *z) { fine_vcycle(s, r, z); },
+                o, 0);
     if (status != NSFEM_OK)
         fail(status, "V-cycle preconditioned CG on the pressure stiffness matrix did not "
                      "converge (status " + std::to_string(status) + ")");
-    o.iters[0] = k;
-    o.refines[0] = 0;
-    o.rel[0] = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
-    o.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// z = M r for the diffusion solves: one V-cycle from zero over the velocity
+// nodes (damped Jacobi), the velocity coarse space and its levels, zeroed
+// on the velocity Dirichlet rows (whose residual is zero: identity rows,
+// their columns eliminated, the solve starting at the boundary values; the
+// prolongation's rows there are empty).  Symmetric as fine_vcycle.
+void velocity_vcycle(Simulation &s, const double *r, double *z) {
+    Device &d = *s.dev;
+    mg_cycle(d, s.mats[NSFEM_DIFFUSION], s.V, s.vdinv.ptr, s.vt.ptr, s.vmg, 0, r, z);
+    launch_scatter_value(d, s.nvb, s.vb_idx.ptr, 0.0, z);
+}
+
+// Column c (0: u, 1: v) of the diffusion solve by CG preconditioned with
+// velocity_vcycle, from the guess in the column of uv.
+void pcg_velocity(Simulation &s, int c, SolveOut &o) {
+    Device &d = *s.dev;
+    const size_t off = (size_t)c * s.V;
+    double *x = s.uv.ptr + off;
+    launch_scatter(d, s.nvb, s.vb_idx.ptr, c ? s.vb_v.ptr : s.vb_u.ptr, x);
+    const PcgWork w{s.vr.ptr, s.vz.ptr, s.vp.ptr, s.vq.ptr, s.vdot.ptr};
+    const int status = pcg_run(
+        d, s.mats[NSFEM_DIFFUSION], s.V, s.rhs.ptr + off, x, w, s.opt.rtol, s.opt.max_iterations,
+        [&](const double *r, double *z) { velocity_vcycle(s, r, z); }, o, 0);
+    if (status != NSFEM_OK)
+        fail(status, "V-cycle preconditioned CG on the diffusion matrix did not converge "
+                     "(status " + std::to_string(status) + ")");
 }
 
 }  // namespace
@@ -923,15 +1003,10 @@ void release_coarse_space(Simulation &s) {
     s.cr.release();
     s.ce.release();
     s.cproj.release();
-    s.mg.clear();
-    matrix_release(s.deep_inv);
-    s.deep_inv_ok = false;
+    s.mg.release();
     s.cdinv.release();
     s.cres.release();
     release_pressure_multigrid(s);
-    s.deep_rs.clear();
-    s.deep_col.clear();
-    s.deep_val.clear();
     s.has_coarse = false;
     s.nc = 0;
 }
@@ -992,9 +1067,9 @@ void build_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
     s.cres.resize(nc);
     s.cproj.reserve(nc, 16);
     upload_inverse_diagonal(d, nc, crs, ccol, cval, s.cdinv);
-    s.deep_rs.swap(crs);
-    s.deep_col.swap(ccol);
-    s.deep_val.swap(cval);
+    s.mg.deep_rs.swap(crs);
+    s.mg.deep_col.swap(ccol);
+    s.mg.deep_val.swap(cval);
     s.nc = nc;
     s.coarse_rtol = rtol;
     s.coarse_iters = 0;
@@ -1007,21 +1082,58 @@ void build_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
 // on the host by Cholesky) applied as a dense CSR product; larger ones by CG.
 constexpr int kMgDenseMax = 600;
 
-void deep_dense_inverse(Simulation &s) {
-    matrix_release(s.deep_inv);
-    s.deep_inv_ok = false;
-    const int n = s.mg.back()->n;
+namespace {
+// The dense inverse of m's deepest level when it has at most kMgDenseMax
+// nodes and is positive definite (else the deepest level keeps CG).
+void dense_inverse(Device &d, MgLevels &m) {
+    matrix_release(m.deep_inv);
+    m.deep_ok = false;
+    const int n = m.lv.back()->n;
     if (n > kMgDenseMax) return;
     std::vector<double> inv;
-    if (!spd_inverse(n, s.deep_rs.data(), s.deep_col.data(), s.deep_val.data(), inv))
-        return;  // not positive definite: the deepest level keeps CG
+    if (!spd_inverse(n, m.deep_rs.data(), m.deep_col.data(), m.deep_val.data(), inv)) return;
     std::vector<int32_t> rs((size_t)n + 1), col((size_t)n * n);
     for (int r = 0; r <= n; ++r) rs[r] = r * n;
     for (int r = 0; r < n; ++r)
         for (int c = 0; c < n; ++c) col[(size_t)r * n + c] = c;
-    upload_csr(*s.dev, s.deep_inv, n, n, (int64_t)n * n, rs.data(), col.data(), inv.data());
-    s.deep_inv_ok = true;
+    upload_csr(d, m.deep_inv, n, n, (int64_t)n * n, rs.data(), col.data(), inv.data());
+    m.deep_ok = true;
 }
+
+// Appends a level below m's deepest: P (`above` rows, `count` columns; the
+// upload validates it) maps the new level's nodes to the level above, whose
+// matrix is the host CSR (ars, acol, aval); the new matrix P^T A_above P
+// (formed on `threads` host threads) becomes m's deepest.
+void append_level(Device &d, MgLevels &m, int above, const std::vector<int32_t> &ars,
+                  const std::vector<int32_t> &acol, const std::vector<double> &aval, int count,
+                  int64_t nnz, const int32_t *rs, const int32_t *col, const double *val,
+                  int threads) {
+    std::unique_ptr<MgLevel> lv(new MgLevel());
+    lv->n = count;
+    upload_csr(d, lv->P, above, count, nnz, rs, col, val);
+    std::vector<int32_t> trs, tcol, crs, ccol;
+    std::vector<double> tval, cval;
+    csr_transpose(above, count, rs, col, val, trs, tcol, tval);
+    galerkin_product(above, ars, acol, aval, count, rs, col, val, trs, tcol, tval, threads, crs,
+                     ccol, cval);
+    upload_csr(d, lv->PT, count, above, nnz, trs.data(), tcol.data(), tval.data());
+    upload_csr(d, lv->A, count, count, (int64_t)ccol.size(), crs.data(), ccol.data(), cval.data());
+    upload_inverse_diagonal(d, count, crs, ccol, cval, lv->dinv);
+    lv->b.resize(count);
+    lv->x.resize(count);
+    lv->r.resize(count);
+    m.deep_rs.swap(crs);
+    m.deep_col.swap(ccol);
+    m.deep_val.swap(cval);
+    m.lv.push_back(std::move(lv));
+    dense_inverse(d, m);
+}
+
+int host_threads(const Simulation &s) {
+    return s.opt.threads > 0 ? s.opt.threads
+                             : (int)std::max(1u, std::thread::hardware_concurrency());
+}
+}  // namespace
 
 // Appends a multigrid level below the deepest one: P maps the new level's
 // count nodes to the deepest level's (coarse grid first), A_new = P^T A_deep
@@ -1032,31 +1144,12 @@ void simulation_add_coarse_level(Simulation &s, int count, int64_t nnz, const in
     d.set_current();
     NSFEM_REQUIRE(s.has_coarse, NSFEM_INVALID_ARGUMENT,
                   "coarse level: set a coarse space first");
-    const int above = s.mg.empty() ? s.nc : s.mg.back()->n;
+    const int above = s.mg.lv.empty() ? s.nc : s.mg.lv.back()->n;
     NSFEM_REQUIRE(count > 0 && count < above && nnz > 0 && rs && col && val,
                   NSFEM_INVALID_ARGUMENT,
                   "coarse level: need 0 < count < the level above's size and a non-empty P");
-    std::unique_ptr<MgLevel> lv(new MgLevel());
-    lv->n = count;
-    upload_csr(d, lv->P, above, count, nnz, rs, col, val);  // validates P
-    std::vector<int32_t> trs, tcol, crs, ccol;
-    std::vector<double> tval, cval;
-    csr_transpose(above, count, rs, col, val, trs, tcol, tval);
-    const int threads = s.opt.threads > 0 ? s.opt.threads
-                                          : (int)std::max(1u, std::thread::hardware_concurrency());
-    galerkin_product(above, s.deep_rs, s.deep_col, s.deep_val, count, rs, col, val, trs, tcol,
-                     tval, threads, crs, ccol, cval);
-    upload_csr(d, lv->PT, count, above, nnz, trs.data(), tcol.data(), tval.data());
-    upload_csr(d, lv->A, count, count, (int64_t)ccol.size(), crs.data(), ccol.data(), cval.data());
-    upload_inverse_diagonal(d, count, crs, ccol, cval, lv->dinv);
-    lv->b.resize(count);
-    lv->x.resize(count);
-    lv->r.resize(count);
-    s.deep_rs.swap(crs);
-    s.deep_col.swap(ccol);
-    s.deep_val.swap(cval);
-    s.mg.push_back(std::move(lv));
-    deep_dense_inverse(s);
+    append_level(d, s.mg, above, s.mg.deep_rs, s.mg.deep_col, s.mg.deep_val, count, nnz, rs, col,
+                 val, host_threads(s));
 }
 
 // Switches the pressure solve to CG preconditioned by one V-cycle over the
@@ -1088,6 +1181,121 @@ void simulation_set_pressure_multigrid(Simulation &s, int enable) {
     s.pt.resize(s.P);
     s.pdot.resize(1);
     s.pmg = true;
+}
+
+namespace {
+// Drops the velocity hierarchy and the V-cycle preconditioned diffusion
+// solves.
+void release_velocity_multigrid(Simulation &s) {
+    s.vmg.release();
+    s.vmg_mode = 0;
+    for (DeviceArray<double> *a : {&s.vdinv, &s.vr, &s.vz, &s.vp, &s.vq, &s.vt, &s.vdot})
+        a->release();
+    for (int c = 0; c < 2; ++c)
+        for (int w = 0; w < 2; ++w) {
+            s.vmg_sec[c][w] = 0.0;
+            s.vmg_age[c][w] = 0;
+        }
+}
+
+// The host CSR (row starts, columns, values) of a simulation matrix.
+void host_matrix(Matrix &A, std::vector<int32_t> &col, std::vector<double> &val) {
+    col.resize((size_t)A.nnz);
+    val.resize((size_t)A.nnz);
+    if (!A.nnz) return;
+    NSFEM_HIP(hipMemcpy(col.data(), A.col.ptr, col.size() * sizeof(int32_t),
+                        hipMemcpyDeviceToHost));
+    NSFEM_HIP(hipMemcpy(val.data(), A.val.ptr, val.size() * sizeof(double),
+                        hipMemcpyDeviceToHost));
+}
+
+void build_velocity_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
+                                 const int32_t *col, const double *val) {
+    Device &d = *s.dev;
+    const int V = s.V;
+    NSFEM_REQUIRE(rs[0] == 0 && rs[V] == nnz, NSFEM_INVALID_ARGUMENT,
+                  "velocity coarse space: row starts must run from 0 to nnz");
+    for (int r = 0; r < V; ++r)
+        NSFEM_REQUIRE(rs[r] <= rs[r + 1], NSFEM_INVALID_ARGUMENT,
+                      "velocity coarse space: row starts must not decrease");
+    // rows into the internal (lattice) order of the velocity nodes
+    std::vector<int32_t> prs((size_t)V + 1, 0), pcol((size_t)nnz);
+    std::vector<double> pval((size_t)nnz);
+    for (int r = 0; r < V; ++r) {
+        const int src = s.vorder.empty() ? r : s.vorder[r];
+        const int32_t len = rs[src + 1] - rs[src];
+        std::copy(col + rs[src], col + rs[src] + len, pcol.begin() + prs[r]);
+        std::copy(val + rs[src], val + rs[src] + len, pval.begin() + prs[r]);
+        prs[r + 1] = prs[r] + len;
+    }
+    std::vector<int32_t> vb((size_t)s.nvb);
+    if (s.nvb)
+        NSFEM_HIP(hipMemcpy(vb.data(), s.vb_idx.ptr, vb.size() * sizeof(int32_t),
+                            hipMemcpyDeviceToHost));
+    for (int32_t r : vb)
+        NSFEM_REQUIRE(prs[r] == prs[r + 1], NSFEM_INVALID_ARGUMENT,
+                      "velocity coarse space: the rows of velocity Dirichlet nodes must be empty");
+    Matrix &A = s.mats[NSFEM_DIFFUSION];
+    std::vector<int32_t> acol;
+    std::vector<double> aval;
+    host_matrix(A, acol, aval);
+    upload_inverse_diagonal(d, V, A.host_rs, acol, aval, s.vdinv);
+    append_level(d, s.vmg, V, A.host_rs, acol, aval, nc, nnz, prs.data(), pcol.data(),
+                 pval.data(), host_threads(s));
+    for (DeviceArray<double> *a : {&s.vr, &s.vz, &s.vp, &s.vq, &s.vt}) a->resize((size_t)V);
+    s.vdot.resize(1);
+    s.vmg_mode = 1;
+}
+}  // namespace
+
+// Sets (nc > 0) or removes (nc <= 0) the multigrid hierarchy below the
+// velocity space: P (V x nc, rows in the caller's node order) maps its
+// first coarse space onto the velocity nodes, and the diffusion solves
+// switch to V-cycle preconditioned CG (mode 1: per column by measured cost).
+// A failure part way leaves no hierarchy.
+void simulation_set_velocity_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
+                                          const int32_t *col, const double *val) {
+    Device &d = *s.dev;
+    d.set_current();
+    if (nc > 0)
+        NSFEM_REQUIRE(nc < s.V && nnz > 0 && rs && col && val, NSFEM_INVALID_ARGUMENT,
+                      "velocity coarse space: need 0 < coarse count < velocity count and a "
+                      "non-empty P");
+    release_velocity_multigrid(s);
+    if (nc <= 0) return;
+    try {
+        build_velocity_coarse_space(s, nc, nnz, rs, col, val);
+    } catch (...) {
+        release_velocity_multigrid(s);
+        throw;
+    }
+}
+
+// Appends a level below the deepest level of the velocity hierarchy.
+void simulation_add_velocity_coarse_level(Simulation &s, int count, int64_t nnz,
+                                          const int32_t *rs, const int32_t *col,
+                                          const double *val) {
+    Device &d = *s.dev;
+    d.set_current();
+    NSFEM_REQUIRE(!s.vmg.lv.empty(), NSFEM_INVALID_ARGUMENT,
+                  "velocity coarse level: set a velocity coarse space first");
+    const int above = s.vmg.lv.back()->n;
+    NSFEM_REQUIRE(count > 0 && count < above && nnz > 0 && rs && col && val,
+                  NSFEM_INVALID_ARGUMENT,
+                  "velocity coarse level: need 0 < count < the level above's size and a "
+                  "non-empty P");
+    append_level(d, s.vmg, above, s.vmg.deep_rs, s.vmg.deep_col, s.vmg.deep_val, count, nnz, rs,
+                 col, val, host_threads(s));
+}
+
+// Which diffusion columns run V-cycle preconditioned CG: 0 none, 1 each
+// column by measured cost, 2 both.
+void simulation_set_velocity_multigrid(Simulation &s, int mode) {
+    NSFEM_REQUIRE(mode >= 0 && mode <= 2, NSFEM_INVALID_ARGUMENT,
+                  "velocity multigrid: mode must be 0, 1 or 2");
+    NSFEM_REQUIRE(mode == 0 || !s.vmg.lv.empty(), NSFEM_INVALID_ARGUMENT,
+                  "velocity multigrid: set a velocity coarse space first");
+    s.vmg_mode = mode;
 }
 
 void pressure_guess_base(Simulation &s);
@@ -1305,6 +1513,32 @@ void record_pairing(Simulation &s, int sys, bool paired, const SolveOut &u, cons
     t = t > 0.0 ? 0.5 * (t + sec) : sec;
 }
 
+// Whether column c of the diffusion solve runs V-cycle preconditioned CG
+// this step: never without a hierarchy or in mode 0, always in mode 2; in
+// mode 1 the way whose solves of this column took less time so far
+// (exponential averages; the V-cycle first, then the plain CG once so that
+// both are measured), except that the slower way runs again once it has not
+// run for kVmgRetry steps (the columns' needs drift as the flow develops).
+constexpr int kVmgRetry = 16;
+
+bool velocity_pcg(const Simulation &s, int c) {
+    if (s.vmg_mode == 0 || s.vmg.lv.empty()) return false;
+    if (s.vmg_mode == 2) return true;
+    const double plain = s.vmg_sec[c][0], mg = s.vmg_sec[c][1];
+    if (mg == 0.0) return true;
+    if (plain == 0.0) return false;
+    const int faster = mg <= plain ? 1 : 0;
+    return (s.vmg_age[c][1 - faster] >= kVmgRetry ? 1 - faster : faster) == 1;
+}
+
+void record_velocity_way(Simulation &s, int c, bool mg, double seconds) {
+    const int w = mg ? 1 : 0;
+    double &t = s.vmg_sec[c][w];
+    t = t > 0.0 ? 0.5 * (t + seconds) : seconds;
+    s.vmg_age[c][w] = 0;
+    ++s.vmg_age[c][1 - w];
+}
+
 void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
     Device &d = *s.dev;
     d.set_current();
@@ -1331,19 +1565,29 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         launch_copy(d, 2 * V, s.utv.ptr, s.uv.ptr);
         SolveOut o[5]{};
         if (s.vproj[0].max > 0) diffusion_guess(s);
-        const bool pair_d = pair_columns(s, 0);
+        const bool vmg_on = s.vmg_mode != 0 && !s.vmg.lv.empty();
+        const bool pcg[2] = {velocity_pcg(s, 0), velocity_pcg(s, 1)};
+        const bool pair_d = !vmg_on && pair_columns(s, 0);
         if (pair_d) {
             solve(s, NSFEM_DIFFUSION, s.rhs.ptr, s.uv.ptr, 2, o[0]);
             o[1].iters[0] = o[0].iters[1];
             o[1].refines[0] = o[0].refines[1];
             o[1].rel[0] = o[0].rel[1];
         } else {
-            solve(s, NSFEM_DIFFUSION, s.rhs.ptr, u, 1, o[0]);
-            solve(s, NSFEM_DIFFUSION, s.rhs.ptr + V, v, 1, o[1]);
+            for (int c = 0; c < 2; ++c) {
+                if (pcg[c])
+                    pcg_velocity(s, c, o[c]);
+                else
+                    solve(s, NSFEM_DIFFUSION, s.rhs.ptr + (size_t)c * V, c ? v : u, 1, o[c]);
+            }
         }
         s.vel_iters[0][0] = o[0].iters[0];
         s.vel_iters[0][1] = o[1].iters[0];
-        record_pairing(s, 0, pair_d, o[0], o[1]);
+        if (vmg_on) {
+            for (int c = 0; c < 2; ++c) record_velocity_way(s, c, pcg[c], o[c].seconds);
+        } else {
+            record_pairing(s, 0, pair_d, o[0], o[1]);
+        }
         if (s.vproj[0].max > 0) diffusion_after_solve(s);
         // 3. pressure
         launch_spmv(d, BD, s.uv.ptr, s.c_p.ptr, s.prhs.ptr, 2);
@@ -1404,6 +1648,13 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
                 stats->cg_launches[k] += ln[k];
                 stats->cg_launch_iterations[k] += li[k];
             }
+            const bool vc[3] = {pcg[0], pcg[1], s.pmg && s.has_coarse};
+            for (int k = 0; k < 3; ++k) {
+                if (!vc[k]) continue;
+                stats->vcycle_solves[k] += 1;
+                stats->vcycle_iterations[k] += o[k].iters[0];
+                stats->vcycle_seconds[k] += o[k].seconds;
+            }
             stats->coarse_iterations += coarse_iters;
             stats->coarse_seconds += coarse_seconds;
         }
@@ -1431,6 +1682,8 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
                       "a batch needs one precision");
         NSFEM_REQUIRE(!s->pmg, NSFEM_INVALID_ARGUMENT,
                       "the V-cycle preconditioned pressure solve steps one simulation at a time");
+        NSFEM_REQUIRE(s->vmg_mode == 0, NSFEM_INVALID_ARGUMENT,
+                      "the V-cycle preconditioned diffusion solves step one simulation at a time");
         NSFEM_REQUIRE(s->vproj[0].max == 0, NSFEM_INVALID_ARGUMENT,
                       "the velocity projection of the diffusion guesses steps one simulation "
                       "at a time");
@@ -1491,7 +1744,7 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
             pressure_guess_base(s);
             s.coarse_iters = 0;
             s.coarse_seconds = 0.0;
-            if (s.has_coarse && !s.mg.empty()) {
+            if (s.has_coarse && !s.mg.lv.empty()) {
                 coarse_correct(s);  // multigrid: its own V-cycles
             } else if (s.has_coarse) {
                 coarse_prepare(s);
@@ -1738,12 +1991,11 @@ void simulation_detach(Simulation *s) {
     s->vproj[0].release();
     s->vproj[1].release();
     s->vdel.release();
+    release_velocity_multigrid(*s);
     matrix_release(s->cP);
     matrix_release(s->cPT);
     matrix_release(s->cA);
-    s->mg.clear();
-    matrix_release(s->deep_inv);
-    s->deep_inv_ok = false;
+    s->mg.release();
     release_pressure_multigrid(*s);
     s->cdinv.release();
     s->cres.release();
@@ -1852,6 +2104,7 @@ int nsfem_simulation_set_options(nsfem_simulation *s, const nsfem_sim_options *o
         s->impl->opt.velocity_jacobi = opt->velocity_jacobi;
         // the solves' cost per iteration may change: time both ways again
         for (int k = 0; k < 2; ++k) s->impl->pair_sec1[k] = s->impl->pair_sec2[k] = 0.0;
+        for (auto &t : s->impl->vmg_sec) t[0] = t[1] = 0.0;
     });
 }
 
@@ -1894,6 +2147,37 @@ int nsfem_simulation_add_coarse_level(nsfem_simulation *s, int count, int64_t nn
                       "null simulation or destroyed device");
         std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
         simulation_add_coarse_level(*s->impl, count, nnz, row_start, col, val);
+    });
+}
+
+int nsfem_simulation_set_velocity_coarse_space(nsfem_simulation *s, int coarse_count,
+                                               int64_t nnz, const int32_t *row_start,
+                                               const int32_t *col, const double *val) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_set_velocity_coarse_space(*s->impl, coarse_count, nnz, row_start, col, val);
+    });
+}
+
+int nsfem_simulation_add_velocity_coarse_level(nsfem_simulation *s, int count, int64_t nnz,
+                                               const int32_t *row_start, const int32_t *col,
+                                               const double *val) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_add_velocity_coarse_level(*s->impl, count, nnz, row_start, col, val);
+    });
+}
+
+int nsfem_simulation_set_velocity_multigrid(nsfem_simulation *s, int mode) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_set_velocity_multigrid(*s->impl, mode);
     });
 }
 

@@ -282,6 +282,82 @@ def grid_hierarchy(mesh: Mesh, levels: int = -1, min_size: int = 16):
     return out
 
 
+def _grid_levels(grid, extent, dirichlet, levels, min_size):
+    """Prolongations below a structured level (node positions `grid`, logical
+    `extent`, Dirichlet flags), each from the next coarser grid onto the
+    level's non-Dirichlet nodes in their order (rows), as grid_hierarchy's
+    deeper levels."""
+    out = []
+    count = 0
+    while levels < 0 or count < levels:
+        if extent[0] % 2 or extent[1] % 2:
+            if levels >= 0:
+                raise ValueError("cannot halve the coarse grid again")
+            break
+        if levels < 0 and min(extent) // 2 < min_size:
+            break
+        rs, col, val, nc = _grid_interpolation(grid, extent, dirichlet, 2)
+        keep = np.nonzero(~dirichlet)[0]
+        cnt = np.diff(rs)[keep]
+        idx = np.concatenate([np.arange(rs[k], rs[k + 1]) for k in keep]) if len(keep) else \
+            np.zeros(0, np.int64)
+        nrs = np.zeros(len(keep) + 1, np.int64)
+        np.cumsum(cnt, out=nrs[1:])
+        out.append((nrs.astype(np.int32), col[idx], val[idx], nc))
+        cI, cJ, cf, exists, valid = _coarse_grid_level(grid, extent, dirichlet, 2)
+        extent = (extent[0] // 2, extent[1] // 2)
+        grid = np.stack([cI[exists], cJ[exists]], axis=1)
+        dirichlet = ~valid[exists]
+        count += 1
+    return out
+
+
+def velocity_hierarchy(mesh: Mesh, levels: int = -1, min_size: int = 16):
+    """Prolongations of a multigrid hierarchy below the velocity (P2) space
+    of a mesh built on a structured grid, for
+    ``Simulation.set_velocity_coarse_space`` (the first) and
+    ``add_velocity_coarse_level`` (the rest).  The first maps the P1 space
+    of the mesh's vertices without velocity Dirichlet values onto the P2
+    nodes (a vertex keeps its value, an edge midpoint takes half of each
+    endpoint's; Dirichlet nodes get empty rows and Dirichlet vertices no
+    column, so every coarse function vanishes on the velocity boundary);
+    the others halve the vertex grid as grid_hierarchy does (holes
+    rescale, Dirichlet positions stay zero), ``levels`` of them (< 0: while
+    both extents stay even and the smaller one above ``min_size``).  Rows
+    are in the mesh's node order."""
+    g = mesh.grid
+    if g is None:
+        raise ValueError("the mesh carries no structured grid")
+    V, P = mesh.node_count, mesh.pressure_count
+    vdir = np.zeros(V, bool)
+    vdir[np.asarray(mesh.vel_bc_nodes, np.int64)] = True
+    free = ~vdir[:P]
+    vid = -np.ones(P, np.int64)
+    vid[free] = np.arange(int(free.sum()))
+    el = np.asarray(mesh.elements, np.int64)
+    mids = el[:, 3:6].reshape(-1)
+    ends = np.stack([el[:, [1, 2]], el[:, [0, 2]], el[:, [0, 1]]], axis=1).reshape(-1, 2)
+    um, first = np.unique(mids, return_index=True)
+    e = ends[first]
+    rows = [np.nonzero(free)[0]]
+    cols = [vid[free]]
+    vals = [np.ones(int(free.sum()))]
+    for k in (0, 1):
+        ok = ~vdir[um] & free[e[:, k]]
+        rows.append(um[ok])
+        cols.append(vid[e[ok, k]])
+        vals.append(np.full(int(ok.sum()), 0.5))
+    rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    rs = np.zeros(V + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=V), out=rs[1:])
+    out = [(rs.astype(np.int32), cols.astype(np.int32), vals, int(free.sum()))]
+    grid = np.asarray(g, np.int64)
+    extent = (int(grid[:, 0].max()), int(grid[:, 1].max()))
+    return out + _grid_levels(grid, extent, vdir[:P], levels, min_size)
+
+
 def grid_prolongation(mesh: Mesh, ratio: int):
     """Interpolation from a coarse grid onto the pressure nodes of a mesh
     built on a structured grid (``mesh.grid``: cavity, channel), for

@@ -190,6 +190,42 @@ class Simulation:
         space; single-simulation stepping only)."""
         N.check(N.lib().nsfem_simulation_set_pressure_multigrid(self.handle, int(bool(enable))))
 
+    def set_velocity_multigrid(self, hierarchy=None, mode: int = 1):
+        """Solve the diffusion columns by CG preconditioned with one V-cycle
+        over the velocity nodes and ``hierarchy`` (a list of prolongations
+        (row_start, col, val, count): the first with one row per velocity
+        node, zero rows on the velocity Dirichlet nodes, each next one with a
+        row per node of the level above, e.g. ``mesh.velocity_hierarchy``);
+        the library forms the Galerkin matrices.  ``mode``: 1 = each column
+        the way (plain CG or V-cycle) measured faster, 2 = both columns by
+        V-cycle, 0 = keep the hierarchy but solve plainly.  ``hierarchy=None``
+        keeps the hierarchy set before and changes the mode only."""
+        lib = N.lib()
+        if hierarchy is None:
+            N.check(lib.nsfem_simulation_set_velocity_multigrid(self.handle, int(mode)))
+            return
+        above = self.mesh.node_count
+        self._velocity_sizes = []
+        for k, (rs, col, val, nc) in enumerate(hierarchy):
+            rs = np.ascontiguousarray(rs, np.int32)
+            col = np.ascontiguousarray(col, np.int32)
+            val = np.ascontiguousarray(val, np.float64)
+            if len(rs) != above + 1:
+                raise ValueError(f"prolongation {k} needs one row per node of the level above "
+                                 f"({above})")
+            fn = (lib.nsfem_simulation_set_velocity_coarse_space if k == 0 else
+                  lib.nsfem_simulation_add_velocity_coarse_level)
+            N.check(fn(self.handle, int(nc), len(col), N.ptr(rs), N.ptr(col), N.ptr(val)))
+            self._velocity_sizes.append(int(nc))
+            above = int(nc)
+        N.check(lib.nsfem_simulation_set_velocity_multigrid(self.handle, int(mode)))
+
+    def remove_velocity_multigrid(self):
+        """Drop the velocity hierarchy: the diffusion solves are plain again."""
+        N.check(N.lib().nsfem_simulation_set_velocity_coarse_space(self.handle, 0, 0, None, None,
+                                                                   None))
+        self._velocity_sizes = []
+
     def copy_pressure_system(self, rhs=None, p=None):
         """Copy the last step's pressure right-hand side and solution into
         device buffers (``DeviceBuffer`` of pressure-count doubles)."""

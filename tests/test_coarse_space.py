@@ -299,3 +299,97 @@ def test_vcycle_preconditioned_cg_premise():
     assert info == 0 and np.linalg.norm(b - A @ x) <= 1e-8 * np.linalg.norm(b) * 1.01
     spl.cg(A, b, rtol=1e-8, maxiter=5000, callback=count("plain"))
     assert counts["pcg"] <= 12 and counts["plain"] > 100
+
+
+def _velocity_vcycle(levels, Ps, dinvs, deep, l, b, omega=0.8, sweeps=2):
+    """One V-cycle from zero (damped Jacobi, as many sweeps before as after,
+    restriction = P^T, the deepest level solved exactly)."""
+    if l == len(levels) - 1:
+        return deep @ b
+    A = levels[l]
+    x = np.zeros_like(b)
+    for _ in range(sweeps):
+        x += omega * dinvs[l] * (b - A @ x)
+    e = _velocity_vcycle(levels, Ps, dinvs, deep, l + 1, Ps[l].T @ (b - A @ x))
+    x += Ps[l] @ e
+    for _ in range(sweeps):
+        x += omega * dinvs[l] * (b - A @ x)
+    return x
+
+
+@pytest.mark.parametrize("make", [lambda: mesh.cavity(32), lambda: mesh.channel(64, 16, seed=3)],
+                         ids=["cavity32", "channel64x16"])
+def test_velocity_hierarchy(make):
+    """The velocity hierarchy's first prolongation maps P1 functions on the
+    vertices without velocity Dirichlet values onto the P2 nodes: a vertex
+    keeps its value, an edge midpoint takes the mean of its endpoints (so a
+    linear function is reproduced wherever no Dirichlet vertex takes part),
+    Dirichlet rows are empty and every column is used; the levels below
+    are grid_hierarchy's halvings of the vertex grid; every Galerkin matrix
+    of the diffusion matrix (nu dt / h^2 as on the 2048^2 cavity) is
+    positive definite, and CG preconditioned by one V-cycle over the levels
+    (the scheme of the device's diffusion solve) needs at most 16 iterations
+    (cavity 8, channel with its disk 14), under a third of the plain CG's."""
+    from navier_stokes_fem_amd.device import assemble
+
+    m = make()
+    H = mesh.velocity_hierarchy(m, min_size=4)
+    V, P = m.node_count, m.pressure_count
+    vdir = np.zeros(V, bool)
+    vdir[m.vel_bc_nodes] = True
+    rs, col, val, nc = H[0]
+    assert len(rs) == V + 1 and nc == int((~vdir[:P]).sum())
+    P21 = sp.csr_matrix((val, col, rs), shape=(V, nc))
+    assert np.all(np.diff(rs)[vdir] == 0) and np.all(np.bincount(col, minlength=nc) > 0)
+    free = np.nonzero(~vdir[:P])[0]
+    lin = 0.3 + m.nodes[:, 0].astype(float) * 1.5 - m.nodes[:, 1].astype(float) * 0.7
+    got = P21 @ lin[free]
+    el = np.asarray(m.elements)
+    ends = {}
+    for t in el:
+        for mid, a, b in ((t[3], t[1], t[2]), (t[4], t[0], t[2]), (t[5], t[0], t[1])):
+            ends[int(mid)] = (int(a), int(b))
+    clean = np.zeros(V, bool)
+    clean[free] = True
+    for mid, (a, b) in ends.items():
+        clean[mid] = not vdir[mid] and not vdir[a] and not vdir[b]
+    np.testing.assert_allclose(got[clean], ((lin[[ends.get(i, (i, i))[0] for i in range(V)]] +
+                                             lin[[ends.get(i, (i, i))[1] for i in range(V)]]) / 2)[clean],
+                               rtol=0, atol=1e-6)
+    assert clean.sum() > 0.8 * V
+    # the levels below: grid_hierarchy's structure on the vertex grid
+    above = nc
+    for rs_, col_, val_, c_ in H[1:]:
+        assert len(rs_) == above + 1 and np.all(np.bincount(col_, minlength=c_) > 0)
+        assert sp.csr_matrix((val_, col_, rs_), shape=(above, c_)).sum(axis=1).max() <= 1 + 1e-12
+        above = c_
+    h = 1.0 / 32
+    nu = 1e-3 * (h * 2048) ** 2
+    rs_a, col_a, val_a, shape = assemble(m, nu, 1e-3)["diffusion"]
+    A = sp.csr_matrix((val_a, col_a, rs_a), shape=shape)
+    Ps = [sp.csr_matrix((v_, c_, r_), shape=(len(r_) - 1, n_)) for r_, c_, v_, n_ in H]
+    levels = [A]
+    for Pk in Ps:
+        levels.append((Pk.T @ levels[-1] @ Pk).tocsr())
+    for L in levels[1:]:
+        assert abs(L - L.T).max() <= 1e-12 * abs(L).max()
+        assert np.linalg.eigvalsh(L.toarray()).min() > 0 if L.shape[0] <= 1200 else \
+            L.diagonal().min() > 0
+    dinvs = [1.0 / L.diagonal() for L in levels]
+    deep = np.linalg.inv(levels[-1].toarray())
+    rng = np.random.default_rng(1)
+    x_true = rng.standard_normal(V)
+    x_true[vdir] = 0.0
+    b = A @ x_true
+    counts = {}
+    for name, M in (("plain", None),
+                    ("vcycle", spl.LinearOperator(A.shape, dtype=np.float64, matvec=lambda r: (
+                        lambda z: np.where(vdir, 0.0, z))(_velocity_vcycle(
+                            levels, Ps, dinvs, deep, 0, np.asarray(r, np.float64).ravel()))))):
+        it = [0]
+        x, info = spl.cg(A, b, rtol=1e-8, atol=0.0, M=M, maxiter=2000,
+                         callback=lambda xk: it.__setitem__(0, it[0] + 1))
+        assert info == 0 and np.linalg.norm(b - A @ x) <= 1e-8 * np.linalg.norm(b) * 1.01
+        counts[name] = it[0]
+    print("diffusion CG iterations", counts)
+    assert counts["vcycle"] <= 16 and counts["plain"] > 3 * counts["vcycle"]

@@ -4,8 +4,9 @@ projected away with its pressure, a solid-body rotation stays put with the
 centripetal pressure, the step is first order in dt, mirror symmetry holds
 and the Dirichlet values are exact -- for the plain FP64 method, the FP32 +
 FP64 refinement form, the chosen form (projection guess, coarse-grid
-correction, velocity projection) and the refinement form with
-Jacobi-preconditioned velocity solves.  These do not go through the oracle: they
+correction, velocity projection), the refinement form with
+Jacobi-preconditioned velocity solves and the diffusion solved by V-cycle
+preconditioned CG (on the meshes built on a grid).  These do not go through the oracle: they
 pin the device step (navier_stokes_fem_amd/csrc/simulation.cpp, after
 include/assembly.h:1516-1554) directly.
 """
@@ -16,7 +17,7 @@ import step_cases as S
 
 pytestmark = pytest.mark.gpu
 
-FORMS = ("fp64", "fp32_refined", "chosen", "jacobi")
+FORMS = ("fp64", "fp32_refined", "chosen", "jacobi", "vcycle")
 
 
 def gpu_stepper(dev, form, rtol=1e-10):
@@ -34,6 +35,8 @@ def gpu_stepper(dev, form, rtol=1e-10):
             sim = Simulation(m, nu, dt, device=dev, rtol=rtol,
                              precision="fp64" if form == "fp64" else "fp32_refined",
                              velocity_jacobi=form == "jacobi")
+            if form == "vcycle" and m.grid is not None:
+                sim.set_velocity_multigrid(M.velocity_hierarchy(m, min_size=4), mode=2)
             if form == "chosen":
                 sim.set_warm_start("projection")
                 sim.set_velocity_projection(8)
