@@ -100,7 +100,7 @@ def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
 def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
              coarse_rtol=1e-6, velocity_projection=0, coarse_levels=(), rtol=1e-8,
              pressure_multigrid=False, pair_velocity=True, velocity_jacobi=False,
-             velocity_multigrid=0):
+             velocity_multigrid=0, mass_projection=0):
     """Steps `sim` in one form after `warmup` untimed steps.  A nonzero
     velocity_multigrid needs the simulation's velocity hierarchy set."""
     sim.reset()
@@ -115,6 +115,7 @@ def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
             sim.add_coarse_level(level)
     sim.set_pressure_multigrid(pressure_multigrid and coarse is not None)
     sim.set_velocity_projection(velocity_projection)
+    sim.set_mass_projection(mass_projection)
     for i in range(warmup):
         sim.step(1)
         log(f"{form}: warmup step {i + 1}/{warmup} done")
@@ -686,6 +687,11 @@ def main():
                          "halvings (2, default: both columns -- at 2048^2 the v column takes 6-9 "
                          "iterations a step instead of ~90; 1: each column the way measured "
                          "faster; 0: off)")
+    ap.add_argument("--mass-projection", type=int, default=1,
+                    help="fp32_refined form: mass solves started from the A-norm projection of "
+                         "the velocity correction onto this many past ones (default 1: at "
+                         "2048^2 the mass iterations fall from 16 to 11 a step, 2 and 4 gain no "
+                         "more; 0 = the reference's guess, the velocity)")
     ap.add_argument("--coarse-levels", type=int, default=-1,
                     help="multigrid levels below the coarse space (each halves the grid; -1 = "
                          "halve down to the 16x16 grid); with levels the coarse system is solved "
@@ -792,7 +798,8 @@ def main():
                             pressure_multigrid=args.pressure_multigrid and mixed_form,
                             pair_velocity=bool(args.pair_velocity) or not mixed_form,
                             velocity_jacobi=args.velocity_jacobi if mixed_form else "none",
-                            velocity_multigrid=args.velocity_multigrid if mixed_form else 0)
+                            velocity_multigrid=args.velocity_multigrid if mixed_form else 0,
+                            mass_projection=args.mass_projection if mixed_form else 0)
         results[form] = (wall, st)
         log(f"{form}: {args.steps / wall:.3f} steps/s")
     # attribution: the same mixed solver without the coarse correction, and
@@ -914,6 +921,11 @@ def main():
                        if chosen == "fp32_refined" and args.pressure_multigrid and coarse is not None
                        else ("FP32 + FP64 refinement CG" if chosen == "fp32_refined"
                              else "FP64 CG")),
+                   "mass_initial_guess": (
+                       f"projection of the velocity correction onto its last "
+                       f"{args.mass_projection} (A-norm, mass matrix)"
+                       if chosen == "fp32_refined" and args.mass_projection
+                       else "the velocity (the reference's)"),
                    "diffusion_solve": (
                        ("FP64 CG preconditioned by one V-cycle (damped Jacobi 2+2 sweeps on "
                         "the P2 nodes, the P1 vertex space and its grid halvings, the deepest by "
@@ -953,7 +965,8 @@ def main():
                              rtol=1e-12, pressure_multigrid=args.pressure_multigrid,
                              pair_velocity=bool(args.pair_velocity),
                              velocity_jacobi=args.velocity_jacobi,
-                             velocity_multigrid=args.velocity_multigrid)
+                             velocity_multigrid=args.velocity_multigrid,
+                             mass_projection=args.mass_projection)
         log(f"tight reference (1e-12) {time.perf_counter() - t:.1f} s, max residuals "
             f"{st_t['max_relative_residual']}")
         tight = st_t["fields"]

@@ -474,6 +474,15 @@ int nsfem_simulation_set_warm_start(nsfem_simulation *sim, int mode, nsfem_gnn *
  * Costs 2 x (2 depth + 3) velocity vectors of device memory.  No
  * counterpart in the reference. */
 int nsfem_simulation_set_velocity_projection(nsfem_simulation *sim, int depth);
+/* Initial guess of the mass solves (the velocity corrections du, M du =
+ * B_p^T p): the A-norm projection of du onto the span of the last `depth`
+ * solutions of each velocity column (Fischer's projection on the mass
+ * matrix) instead of the reference's guess, the velocity itself
+ * (include/assembly.h:642-656); 0 = off (the reference's).  The solves stop
+ * at the same relative residual.  Costs 2 x (2 depth + 3) velocity vectors
+ * of device memory; single-simulation stepping only.  No counterpart in the
+ * reference. */
+int nsfem_simulation_set_mass_projection(nsfem_simulation *sim, int depth);
 /* Coarse-grid correction of the pressure solve's initial guess, applied
  * after the warm-start mode's guess x0: with the prolongation P (one row per
  * pressure node, coarse_count columns, CSR with ascending columns) and

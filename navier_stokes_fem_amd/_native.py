@@ -184,6 +184,7 @@ _SIGNATURES = {
                                                    C.c_int),
     "nsfem_simulation_set_velocity_multigrid": ([vp, C.c_int], C.c_int),
     "nsfem_simulation_set_velocity_projection": ([vp, C.c_int], C.c_int),
+    "nsfem_simulation_set_mass_projection": ([vp, C.c_int], C.c_int),
     "nsfem_simulation_copy_pressure_system": ([vp, vp, vp], C.c_int),
     "nsfem_gnn_create": ([vp, C.c_uint64, PP], C.c_int),
     "nsfem_gnn_destroy": ([vp], C.c_int),

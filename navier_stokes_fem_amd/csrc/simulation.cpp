@@ -152,6 +152,9 @@ struct Simulation {
     // advected velocity), one basis per column; vdel: the guessed, then the
     // true, corrections
     ProjBasis vproj[2];
+    // projection guess of the mass solves' velocity corrections du, one
+    // basis per column (empty: the reference's guess, the velocity itself)
+    ProjBasis mproj[2];
     DeviceArray<double> vdel;
     int64_t coarse_iters = 0;
     double coarse_seconds = 0.0;
@@ -421,6 +424,7 @@ void simulation_reset(Simulation &s) {
     s.proj.dim = 0;
     s.cproj.dim = 0;
     s.vproj[0].dim = s.vproj[1].dim = 0;
+    s.mproj[0].dim = s.mproj[1].dim = 0;
     for (auto &row : s.vel_iters) row[0] = row[1] = 0;
     for (int k = 0; k < 2; ++k) s.pair_sec1[k] = s.pair_sec2[k] = 0.0;
     launch_fill(d, 2 * s.V, 0.0, s.uv.ptr);
@@ -991,6 +995,18 @@ void simulation_set_velocity_projection(Simulation &s, int depth) {
     s.vproj[0].reserve(s.V, depth);
     s.vproj[1].reserve(s.V, depth);
     s.vdel.resize(2 * (size_t)s.V);
+}
+
+void simulation_set_mass_projection(Simulation &s, int depth) {
+    NSFEM_REQUIRE(depth >= 0 && depth <= 256, NSFEM_INVALID_ARGUMENT,
+                  "mass projection depth must lie in [0, 256]");
+    s.dev->set_current();
+    for (ProjBasis &B : s.mproj) {
+        if (depth == 0)
+            B.release();
+        else
+            B.reserve(s.V, depth);
+    }
 }
 
 namespace {
@@ -1603,6 +1619,9 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         // 4. velocity correction
         launch_spmv(d, PD, s.p.ptr, nullptr, s.rhs.ptr, 0);
         launch_copy(d, 2 * V, s.uv.ptr, s.du.ptr);
+        if (s.mproj[0].max > 0)
+            for (int c = 0; c < 2; ++c)
+                projection_guess(d, s.mproj[c], s.rhs.ptr + (size_t)c * V, s.du.ptr + (size_t)c * V);
         const bool pair_m = pair_columns(s, 1);
         if (pair_m) {
             solve(s, NSFEM_VELOCITY_MASS, s.rhs.ptr, s.du.ptr, 2, o[3]);
@@ -1615,6 +1634,8 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         }
         s.vel_iters[1][0] = o[3].iters[0];
         s.vel_iters[1][1] = o[4].iters[0];
+        if (s.mproj[0].max > 0)
+            for (int c = 0; c < 2; ++c) projection_update(d, s.mproj[c], M, s.du.ptr + (size_t)c * V);
         record_pairing(s, 1, pair_m, o[3], o[4]);
         // 5. update and Dirichlet values
         launch_add(d, 2 * V, s.du.ptr, s.uv.ptr);
@@ -1687,6 +1708,8 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
         NSFEM_REQUIRE(s->vproj[0].max == 0, NSFEM_INVALID_ARGUMENT,
                       "the velocity projection of the diffusion guesses steps one simulation "
                       "at a time");
+        NSFEM_REQUIRE(s->mproj[0].max == 0, NSFEM_INVALID_ARGUMENT,
+                      "the projection of the mass guesses steps one simulation at a time");
     }
     d.set_current();
     const size_t I = S.size();
@@ -1990,6 +2013,8 @@ void simulation_detach(Simulation *s) {
     s->cproj.release();
     s->vproj[0].release();
     s->vproj[1].release();
+    s->mproj[0].release();
+    s->mproj[1].release();
     s->vdel.release();
     release_velocity_multigrid(*s);
     matrix_release(s->cP);
@@ -2196,6 +2221,15 @@ int nsfem_simulation_set_velocity_projection(nsfem_simulation *s, int depth) {
                       "null simulation or destroyed device");
         std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
         simulation_set_velocity_projection(*s->impl, depth);
+    });
+}
+
+int nsfem_simulation_set_mass_projection(nsfem_simulation *s, int depth) {
+    return guard([&] {
+        NSFEM_REQUIRE(s && s->impl && s->impl->dev, NSFEM_INVALID_ARGUMENT,
+                      "null simulation or destroyed device");
+        std::lock_guard<std::mutex> lk(s->impl->dev->mtx);
+        simulation_set_mass_projection(*s->impl, depth);
     });
 }
 

@@ -144,6 +144,12 @@ class Simulation:
         corrections (0 = the advected velocity alone, the reference's)."""
         N.check(N.lib().nsfem_simulation_set_velocity_projection(self.handle, int(depth)))
 
+    def set_mass_projection(self, depth: int = 2):
+        """Start each mass solve (the velocity correction) from the A-norm
+        projection onto its last ``depth`` solutions instead of the
+        reference's guess, the velocity itself (0 = the reference's)."""
+        N.check(N.lib().nsfem_simulation_set_mass_projection(self.handle, int(depth)))
+
     def set_coarse_space(self, prolongation=None, rtol: float = 1e-6):
         """Coarse-grid correction of every pressure solve's initial guess:
         ``prolongation`` = (row_start, col, val, coarse_count), one row per

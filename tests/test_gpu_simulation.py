@@ -506,3 +506,40 @@ def test_velocity_multigrid_per_column_choice_and_errors(gpu):
     with pytest.raises(N.NsfemError):
         sim.set_velocity_multigrid(None, mode=1)
     sim.close()
+
+
+def test_mass_projection(gpu):
+    """The mass solves started from the A-norm projection of the velocity
+    correction onto its past solutions: every solve meets the tolerance,
+    the mass solves take fewer iterations than from the reference's guess
+    (the velocity), the other solves are untouched in kind, and the fields
+    are as close to a tightly solved run; depth 0 restores the reference's
+    guess, a negative depth is rejected and a batch holding the simulation
+    is rejected."""
+    from navier_stokes_fem_amd import _native as N
+    from navier_stokes_fem_amd import mesh
+    from navier_stokes_fem_amd.simulation import Simulation, step_batch
+
+    m = mesh.cavity(64)
+    out = {}
+    for form, depth, rtol, prec in (("tight", 0, 1e-12, "fp64"), ("plain", 0, 1e-8, "fp32_refined"),
+                                    ("projection", 4, 1e-8, "fp32_refined")):
+        sim = Simulation(m, 1e-3, 1e-3, device=gpu, rtol=rtol, precision=prec)
+        sim.set_mass_projection(depth)
+        st = sim.step(10)
+        out[form] = (st, sim.fields())
+        if depth:
+            with pytest.raises(N.NsfemError):
+                step_batch([sim], 1)
+            with pytest.raises(N.NsfemError):
+                sim.set_mass_projection(-1)
+            sim.set_mass_projection(0)
+            assert max(sim.step(1)["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+        sim.close()
+    (s0, f0), (s1, f1), (_, ft) = out["plain"], out["projection"], out["tight"]
+    print("mass iterations reference guess", s0["cg_iterations"][3:], "projection",
+          s1["cg_iterations"][3:], flush=True)
+    assert max(s1["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+    assert sum(s1["cg_iterations"][3:]) < 0.85 * sum(s0["cg_iterations"][3:])
+    for a, b, t in zip(f1, f0, ft):
+        assert _rel(a, t) <= 2.0 * _rel(b, t) + 1e-7
