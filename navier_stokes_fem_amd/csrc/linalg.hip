@@ -41,6 +41,7 @@ constexpr int kGroups = kTileNnz / (4 * kBlock);
 static_assert(kGroups * 4 * kBlock == kTileNnz, "tile must be whole groups per thread");
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 // Workgroups are dispatched round-robin over the 8 XCDs; this maps the
 // hardware block id to a logical id so that each XCD (and its L2) works on
@@ -364,17 +365,20 @@ namespace {
 // MODE 2: y = A x - lhs.
 // IT = int16_t reads the columns as 16-bit offsets from the tile's first row
 // (bcol, square matrices whose offsets fit): 2 bytes less per nonzero.
-template <int MODE, typename IT>
+// T = double: the FP64 values; T = float: the FP32 copy (val32) with FP32
+// vectors and sums (the multigrid smoothing of mixed-precision solves).
+template <int MODE, typename IT, typename T>
 __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_t nnz,
                                                  const int32_t *__restrict__ tiles,
                                                  const int32_t *__restrict__ rs,
                                                  const int32_t *__restrict__ col,
                                                  const int16_t *__restrict__ bcol,
-                                                 const double *__restrict__ val,
-                                                 const double *__restrict__ x,
-                                                 const double *lhs, double *y) {
-    __shared__ __align__(16) double prod[kTileNnz];
-    __shared__ double red[kWaves];
+                                                 const T *__restrict__ val,
+                                                 const T *__restrict__ x,
+                                                 const T *lhs, T *y) {
+    constexpr bool F64 = std::is_same<T, double>::value;
+    __shared__ __align__(16) T prod[kTileNnz];
+    __shared__ T red[kWaves];
     const int t = xcd_swizzle(blockIdx.x, gridDim.x);
     if (t >= n_tiles) return;
     const int tid = threadIdx.x;
@@ -389,13 +393,13 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_
         const int ka = k0 & ~3;
         const int ng = (k0 + nz - ka + 3) >> 2;
         i32x4 c[kGroups];
-        f64x2 v[kGroups][2];
+        T v[kGroups][4];
 #pragma unroll
         for (int g = 0; g < kGroups; ++g) {
             const int grp = tid + g * kBlock;
             c[g] = i32x4{0, 0, 0, 0};
-            v[g][0] = f64x2{0.0, 0.0};
-            v[g][1] = f64x2{0.0, 0.0};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[g][e] = T(0);
             if (grp < ng) {
                 const int64_t base = ka + 4 * (int64_t)grp;
                 if (base + 4 <= nnz) {
@@ -409,21 +413,33 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_
                     } else {
                         c[g] = ld_stream(reinterpret_cast<const i32x4 *>(col + base));
                     }
-                    v[g][0] = ld_stream(reinterpret_cast<const f64x2 *>(val + base));
-                    v[g][1] = ld_stream(reinterpret_cast<const f64x2 *>(val + base + 2));
+                    if constexpr (F64) {
+                        const f64x2 a = ld_stream(reinterpret_cast<const f64x2 *>(val + base));
+                        const f64x2 b = ld_stream(reinterpret_cast<const f64x2 *>(val + base + 2));
+                        v[g][0] = a.x;
+                        v[g][1] = a.y;
+                        v[g][2] = b.x;
+                        v[g][3] = b.y;
+                    } else {
+                        const f32x4_t a = ld_stream(reinterpret_cast<const f32x4_t *>(val + base));
+                        v[g][0] = a.x;
+                        v[g][1] = a.y;
+                        v[g][2] = a.z;
+                        v[g][3] = a.w;
+                    }
                 } else {
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         const bool in = base + e < nnz;
                         c[g][e] = in ? col[base + e] : 0;
-                        v[g][e >> 1][e & 1] = in ? val[base + e] : 0.0;
+                        v[g][e] = in ? val[base + e] : T(0);
                     }
                 }
             }
         }
         // columns transposed through the wave's own slice of prod (256
-        // products = 2 KB, the 256 columns take 1 KB of it), so that each
-        // gather instruction covers 64 consecutive nonzeros
+        // products; the 256 columns take 1 KB of it), so that each gather
+        // instruction covers 64 consecutive nonzeros
         const int lane = tid & 63;
 #pragma unroll
         for (int g = 0; g < kGroups; ++g) {
@@ -436,7 +452,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_
                 int cc[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) cc[j] = cs[lane + 64 * j];
-                double xv[4];
+                T xv[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) xv[j] = x[cc[j]];
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -445,34 +461,43 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_
                 for (int j = 0; j < 4; ++j) prod[chunk + lane + 64 * j] = xv[j];
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
-                f64x2 p0 = *reinterpret_cast<const f64x2 *>(prod + chunk + 4 * lane);
-                f64x2 p1 = *reinterpret_cast<const f64x2 *>(prod + chunk + 4 * lane + 2);
-                p0.x = v[g][0].x * p0.x;
-                p0.y = v[g][0].y * p0.y;
-                p1.x = v[g][1].x * p1.x;
-                p1.y = v[g][1].y * p1.y;
-                *reinterpret_cast<f64x2 *>(prod + chunk + 4 * lane) = p0;
-                *reinterpret_cast<f64x2 *>(prod + chunk + 4 * lane + 2) = p1;
+                if constexpr (F64) {
+                    f64x2 p0 = *reinterpret_cast<const f64x2 *>(prod + chunk + 4 * lane);
+                    f64x2 p1 = *reinterpret_cast<const f64x2 *>(prod + chunk + 4 * lane + 2);
+                    p0.x = v[g][0] * p0.x;
+                    p0.y = v[g][1] * p0.y;
+                    p1.x = v[g][2] * p1.x;
+                    p1.y = v[g][3] * p1.y;
+                    *reinterpret_cast<f64x2 *>(prod + chunk + 4 * lane) = p0;
+                    *reinterpret_cast<f64x2 *>(prod + chunk + 4 * lane + 2) = p1;
+                } else {
+                    f32x4_t p = *reinterpret_cast<const f32x4_t *>(prod + chunk + 4 * lane);
+                    p.x = v[g][0] * p.x;
+                    p.y = v[g][1] * p.y;
+                    p.z = v[g][2] * p.z;
+                    p.w = v[g][3] * p.w;
+                    *reinterpret_cast<f32x4_t *>(prod + chunk + 4 * lane) = p;
+                }
             }
         }
         __syncthreads();
         const int row = r0 + tid;
         if (row < r1) {
             const int a = rs[row] - ka, b = rs[row + 1] - ka;
-            double s = 0.0;
+            T s = T(0);
             for (int k = a; k < b; ++k) s += prod[k];
             if (MODE == 0) y[row] = s;
             if (MODE == 1) y[row] = lhs[row] - s;
             if (MODE == 2) y[row] = s - lhs[row];
         }
     } else {  // one row longer than a tile
-        double s = 0.0;
+        T s = T(0);
         for (int k = tid; k < nz; k += kBlock) s += val[k0 + k] * x[col[k0 + k]];
         s = wave_sum(s);
         if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
         if (tid == 0) {
-            double tot = 0.0;
+            T tot = T(0);
             for (int i = 0; i < kWaves; ++i) tot += red[i];
             if (MODE == 0) y[r0] = tot;
             if (MODE == 1) y[r0] = lhs[r0] - tot;
@@ -530,21 +555,32 @@ __global__ void k_scatter_value(int count, const int32_t *__restrict__ idx, doub
     if (i < count) dst[idx[i]] = v;
 }
 
-__global__ void k_add(int n, const double *__restrict__ a, double *__restrict__ y) {
+template <typename T>
+__global__ void k_add(int n, const T *__restrict__ a, T *__restrict__ y) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         y[i] = a[i] + y[i];
 }
 
 // x += w * dinv .* r (a damped Jacobi update from the residual r)
-__global__ void k_jacobi(int n, double w, const double *__restrict__ dinv,
-                         const double *__restrict__ r, double *__restrict__ x) {
+template <typename T>
+__global__ void k_jacobi(int n, T w, const T *__restrict__ dinv, const T *__restrict__ r,
+                         T *__restrict__ x) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         x[i] = __builtin_fma(w * dinv[i], r[i], x[i]);
 }
 
-__global__ void k_fill(int n, double v, double *__restrict__ y) {
+// y = a rounded to (or widened from) y's type
+template <typename A, typename Y>
+__global__ void k_convert(int n, const A *__restrict__ a, Y *__restrict__ y) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (Y)a[i];
+}
+
+template <typename T>
+__global__ void k_fill(int n, T v, T *__restrict__ y) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         y[i] = v;
@@ -564,24 +600,34 @@ inline int ew_grid(int n) { return std::max(1, std::min((n + kBlock - 1) / kBloc
 }  // namespace
 
 namespace {
-template <int MODE, typename IT>
-void spmv_launch(Device &d, Matrix &m, const double *x, const double *lhs, double *y) {
-    k_spmv<MODE, IT><<<m.n_tiles, kBlock, 0, d.stream>>>(m.n_tiles, m.rows, m.nnz, m.tiles.ptr,
-                                                          m.row_start.ptr, m.col.ptr,
-                                                          m.bcol.ptr, m.val.ptr, x, lhs, y);
+template <typename T>
+const T *values(Matrix &m);
+template <>
+const double *values<double>(Matrix &m) {
+    return m.val.ptr;
+}
+template <>
+const float *values<float>(Matrix &m) {
+    return m.val32.ptr;
 }
 
-template <int MODE>
-void spmv_launch(Device &d, Matrix &m, const double *x, const double *lhs, double *y) {
+template <int MODE, typename IT, typename T>
+void spmv_launch(Device &d, Matrix &m, const T *x, const T *lhs, T *y) {
+    k_spmv<MODE, IT, T><<<m.n_tiles, kBlock, 0, d.stream>>>(
+        m.n_tiles, m.rows, m.nnz, m.tiles.ptr, m.row_start.ptr, m.col.ptr, m.bcol.ptr,
+        values<T>(m), x, lhs, y);
+}
+
+template <int MODE, typename T>
+void spmv_launch(Device &d, Matrix &m, const T *x, const T *lhs, T *y) {
     if (m.bcol_ok && want_idx16())
         spmv_launch<MODE, int16_t>(d, m, x, lhs, y);
     else
         spmv_launch<MODE, int32_t>(d, m, x, lhs, y);
 }
-}  // namespace
 
-void launch_spmv(Device &d, Matrix &m, const double *x, const double *lhs, double *y,
-                 int mode) {
+template <typename T>
+void spmv_any(Device &d, Matrix &m, const T *x, const T *lhs, T *y, int mode) {
     if (m.rows == 0) return;
     if (mode == 0)
         spmv_launch<0>(d, m, x, lhs, y);
@@ -590,6 +636,20 @@ void launch_spmv(Device &d, Matrix &m, const double *x, const double *lhs, doubl
     else
         spmv_launch<2>(d, m, x, lhs, y);
     NSFEM_HIP(hipGetLastError());
+}
+}  // namespace
+
+void launch_spmv(Device &d, Matrix &m, const double *x, const double *lhs, double *y,
+                 int mode) {
+    spmv_any(d, m, x, lhs, y, mode);
+}
+
+// The FP32 product reads the FP32 copy of the values (val32, built by
+// ensure_val32 before the first call).
+void launch_spmv(Device &d, Matrix &m, const float *x, const float *lhs, float *y, int mode) {
+    NSFEM_REQUIRE(m.rows == 0 || m.val32.ptr, NSFEM_NOT_READY,
+                  "FP32 product without the FP32 copy of the values");
+    spmv_any(d, m, x, lhs, y, mode);
 }
 
 void launch_spmv_affine(Device &d, Matrix &m, const double *x, const double *c, double *y) {
@@ -647,15 +707,45 @@ void launch_add(Device &d, int n, const double *a, double *y) {
     NSFEM_HIP(hipGetLastError());
 }
 
+void launch_add(Device &d, int n, const float *a, float *y) {
+    if (n <= 0) return;
+    k_add<<<ew_grid(n), kBlock, 0, d.stream>>>(n, a, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
 void launch_jacobi(Device &d, int n, double w, const double *dinv, const double *r, double *x) {
     if (n <= 0) return;
     k_jacobi<<<ew_grid(n), kBlock, 0, d.stream>>>(n, w, dinv, r, x);
     NSFEM_HIP(hipGetLastError());
 }
 
+void launch_jacobi(Device &d, int n, double w, const float *dinv, const float *r, float *x) {
+    if (n <= 0) return;
+    k_jacobi<<<ew_grid(n), kBlock, 0, d.stream>>>(n, (float)w, dinv, r, x);
+    NSFEM_HIP(hipGetLastError());
+}
+
 void launch_fill(Device &d, int n, double v, double *y) {
     if (n <= 0) return;
     k_fill<<<ew_grid(n), kBlock, 0, d.stream>>>(n, v, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_fill(Device &d, int n, double v, float *y) {
+    if (n <= 0) return;
+    k_fill<<<ew_grid(n), kBlock, 0, d.stream>>>(n, (float)v, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_convert(Device &d, int n, const double *a, float *y) {
+    if (n <= 0) return;
+    k_convert<<<ew_grid(n), kBlock, 0, d.stream>>>(n, a, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_convert(Device &d, int n, const float *a, double *y) {
+    if (n <= 0) return;
+    k_convert<<<ew_grid(n), kBlock, 0, d.stream>>>(n, a, y);
     NSFEM_HIP(hipGetLastError());
 }
 

@@ -259,6 +259,8 @@ bool ensure_jacobi(Device &d, Matrix &m);
 
 void launch_spmv(Device &d, Matrix &m, const double *x, const double *lhs, double *y,
                  int mode);
+// the same in FP32 from the FP32 copy of the values (ensure_val32 first)
+void launch_spmv(Device &d, Matrix &m, const float *x, const float *lhs, float *y, int mode);
 void launch_axpy(Device &d, int n, double a, const double *x, double *y);
 void launch_axpby(Device &d, int n, double a, double b, const double *x,
                   const double *y, double *out);
@@ -310,8 +312,14 @@ void launch_scatter(Device &d, int count, const int32_t *idx, const double *val,
 void launch_scatter_value(Device &d, int count, const int32_t *idx, double v,
                           double *dst);  // dst[idx[i]] = v
 void launch_add(Device &d, int n, const double *a, double *y);  // y += a
+void launch_add(Device &d, int n, const float *a, float *y);
 // x += w * dinv .* r
 void launch_jacobi(Device &d, int n, double w, const double *dinv, const double *r, double *x);
+void launch_jacobi(Device &d, int n, double w, const float *dinv, const float *r, float *x);
+void launch_fill(Device &d, int n, double v, float *y);
+// y = a rounded to FP32 / widened to FP64
+void launch_convert(Device &d, int n, const double *a, float *y);
+void launch_convert(Device &d, int n, const float *a, double *y);
 void launch_copy(Device &d, int n, const double *src, double *dst);
 void launch_spmv_affine(Device &d, Matrix &m, const double *x, const double *c,
                         double *y);  // y = A x - c

@@ -70,6 +70,24 @@ struct MgLevel {
     int n = 0;
     Matrix A, P, PT;
     DeviceArray<double> dinv, b, x, r;
+    DeviceArray<float> dinv32, b32, x32, r32;  // the FP32 cycle's (built on first use)
+};
+
+// A level's vectors in a cycle's precision.
+template <typename T>
+struct LevelVectors;
+template <>
+struct LevelVectors<double> {
+    const double *dinv;
+    double *b, *x, *r;
+    explicit LevelVectors(MgLevel &l) : dinv(l.dinv.ptr), b(l.b.ptr), x(l.x.ptr), r(l.r.ptr) {}
+};
+template <>
+struct LevelVectors<float> {
+    const float *dinv;
+    float *b, *x, *r;
+    explicit LevelVectors(MgLevel &l)
+        : dinv(l.dinv32.ptr), b(l.b32.ptr), x(l.x32.ptr), r(l.r32.ptr) {}
 };
 
 // The levels below a system (lv[0] the next coarser one), the host CSR of
@@ -81,10 +99,12 @@ struct MgLevels {
     bool deep_ok = false;
     std::vector<int32_t> deep_rs, deep_col;
     std::vector<double> deep_val;
+    bool fp32_ok = false;  // the levels' FP32 vectors and values are built
     void release() {
         lv.clear();
         matrix_release(deep_inv);
         deep_ok = false;
+        fp32_ok = false;
         deep_rs.clear();
         deep_col.clear();
         deep_val.clear();
@@ -146,6 +166,7 @@ struct Simulation {
     MgLevels vmg;
     int vmg_mode = 0;
     DeviceArray<double> vdinv, vr, vz, vp, vq, vt, vdot;
+    DeviceArray<float> vdinv32, vb32, vz32, vt32;  // the FP32 cycle's fine level
     double vmg_sec[2][2] = {{0.0, 0.0}, {0.0, 0.0}};  // [column][plain, V-cycle]
     int vmg_age[2][2] = {{0, 0}, {0, 0}};
     // projection guess of the diffusion solves' corrections u - u~ (u~ = the
@@ -702,25 +723,32 @@ constexpr int kMgMaxCycles = 30;
 // the x given: kMgSweeps damped Jacobi sweeps, the correction from the level
 // below on the restricted residual, as many sweeps after.  The deepest level
 // (l = m.lv.size()) is solved by its dense inverse when it has one, else by
-// FP64 CG to 1e-10 from zero.
-void mg_cycle(Device &d, Matrix &A, int n, const double *dinv, double *r, MgLevels &m, size_t l,
-              const double *b, double *x, bool zero = true) {
+// FP64 CG to 1e-10 from zero.  T = float runs the whole cycle in FP32 (the
+// FP32 copies of every level's values and vectors, ensure_fp32_levels; it
+// needs the dense deepest level).
+template <typename T>
+void mg_cycle(Device &d, Matrix &A, int n, const T *dinv, T *r, MgLevels &m, size_t l,
+              const T *b, T *x, bool zero = true) {
     if (l == m.lv.size() && m.deep_ok) {
         launch_spmv(d, m.deep_inv, b, nullptr, x, 0);
         return;
     }
     if (zero || l == m.lv.size()) launch_fill(d, n, 0.0, x);
     if (l == m.lv.size()) {
-        CgRequest rq{};
-        rq.m = &A;
-        rq.b = b;
-        rq.x = x;
-        rq.ncol = 1;
-        rq.max_iter = -1;
-        rq.rtol = 1e-10;
-        rq.precision = NSFEM_PRECISION_FP64;
-        nsfem_cg_stats st{};
-        run_cg(d, rq, &st);
+        if constexpr (std::is_same<T, double>::value) {
+            CgRequest rq{};
+            rq.m = &A;
+            rq.b = b;
+            rq.x = x;
+            rq.ncol = 1;
+            rq.max_iter = -1;
+            rq.rtol = 1e-10;
+            rq.precision = NSFEM_PRECISION_FP64;
+            nsfem_cg_stats st{};
+            run_cg(d, rq, &st);
+        } else {
+            fail(NSFEM_NOT_READY, "an FP32 V-cycle needs a dense deepest level");
+        }
         return;
     }
     for (int k = 0; k < kMgSweeps; ++k) {
@@ -733,10 +761,10 @@ void mg_cycle(Device &d, Matrix &A, int n, const double *dinv, double *r, MgLeve
     }
     launch_spmv(d, A, x, b, r, 1);
     MgLevel &below = *m.lv[l];
-    launch_spmv(d, below.PT, r, nullptr, below.b.ptr, 0);
-    mg_cycle(d, below.A, below.n, below.dinv.ptr, below.r.ptr, m, l + 1, below.b.ptr,
-             below.x.ptr);
-    launch_spmv(d, below.P, below.x.ptr, nullptr, r, 0);
+    const LevelVectors<T> v(below);
+    launch_spmv(d, below.PT, r, nullptr, v.b, 0);
+    mg_cycle<T>(d, below.A, below.n, v.dinv, v.r, m, l + 1, v.b, v.x);
+    launch_spmv(d, below.P, v.x, nullptr, r, 0);
     launch_add(d, n, r, x);
     for (int k = 0; k < kMgSweeps; ++k) {
         launch_spmv(d, A, x, b, r, 1);
@@ -744,10 +772,25 @@ void mg_cycle(Device &d, Matrix &A, int n, const double *dinv, double *r, MgLeve
     }
 }
 
+// The FP32 copies of m's levels (values of A, P, P^T and the dense deepest
+// inverse; 1/diag; the cycle's vectors), built once.
+void ensure_fp32_levels(Device &d, MgLevels &m) {
+    if (m.fp32_ok) return;
+    for (auto &lp : m.lv) {
+        MgLevel &l = *lp;
+        for (Matrix *a : {&l.A, &l.P, &l.PT}) ensure_val32(d, *a);
+        l.dinv32.resize((size_t)l.n);
+        launch_convert(d, l.n, l.dinv.ptr, l.dinv32.ptr);
+        for (DeviceArray<float> *a : {&l.b32, &l.x32, &l.r32}) a->resize((size_t)l.n);
+    }
+    if (m.deep_ok) ensure_val32(d, m.deep_inv);
+    m.fp32_ok = true;
+}
+
 // One V-cycle from the pressure coarse level down, from zero or from the x
 // given.
 void vcycle(Simulation &s, const double *b, double *x, bool zero) {
-    mg_cycle(*s.dev, s.cA, s.nc, s.cdinv.ptr, s.cres.ptr, s.mg, 0, b, x, zero);
+    mg_cycle<double>(*s.dev, s.cA, s.nc, s.cdinv.ptr, s.cres.ptr, s.mg, 0, b, x, zero);
 }
 
 // V-cycles on A_c e = cr from the current e; returns the status, the cycle
@@ -957,10 +1000,32 @@ void pcg_pressure(Simulation &s, SolveOut &o) {
 // nodes (damped Jacobi), the velocity coarse space and its levels, zeroed
 // on the velocity Dirichlet rows (whose residual is zero: identity rows,
 // their columns eliminated, the solve starting at the boundary values; the
-// prolongation's rows there are empty).  Symmetric as fine_vcycle.
+// prolongation's rows there are empty).  Symmetric as fine_vcycle.  In the
+// refined precision the cycle runs in FP32 (r rounded to FP32, z widened):
+// the products and sweeps move about half the bytes, and the FP64 CG
+// around it still decides convergence on its FP64 residual.
+bool velocity_cycle_fp32(const Simulation &s) {
+    return s.opt.precision == NSFEM_PRECISION_FP32_REFINED && s.vmg.deep_ok;
+}
+
 void velocity_vcycle(Simulation &s, const double *r, double *z) {
     Device &d = *s.dev;
-    mg_cycle(d, s.mats[NSFEM_DIFFUSION], s.V, s.vdinv.ptr, s.vt.ptr, s.vmg, 0, r, z);
+    Matrix &A = s.mats[NSFEM_DIFFUSION];
+    const int V = s.V;
+    if (velocity_cycle_fp32(s)) {
+        if (!s.vmg.fp32_ok) {
+            ensure_fp32_levels(d, s.vmg);
+            ensure_val32(d, A);
+            s.vdinv32.resize((size_t)V);
+            launch_convert(d, V, s.vdinv.ptr, s.vdinv32.ptr);
+            for (DeviceArray<float> *a : {&s.vb32, &s.vz32, &s.vt32}) a->resize((size_t)V);
+        }
+        launch_convert(d, V, r, s.vb32.ptr);
+        mg_cycle<float>(d, A, V, s.vdinv32.ptr, s.vt32.ptr, s.vmg, 0, s.vb32.ptr, s.vz32.ptr);
+        launch_convert(d, V, s.vz32.ptr, z);
+    } else {
+        mg_cycle<double>(d, A, V, s.vdinv.ptr, s.vt.ptr, s.vmg, 0, r, z);
+    }
     launch_scatter_value(d, s.nvb, s.vb_idx.ptr, 0.0, z);
 }
 
@@ -1143,6 +1208,7 @@ void append_level(Device &d, MgLevels &m, int above, const std::vector<int32_t> 
     m.deep_val.swap(cval);
     m.lv.push_back(std::move(lv));
     dense_inverse(d, m);
+    m.fp32_ok = false;
 }
 
 int host_threads(const Simulation &s) {
@@ -1207,6 +1273,7 @@ void release_velocity_multigrid(Simulation &s) {
     s.vmg_mode = 0;
     for (DeviceArray<double> *a : {&s.vdinv, &s.vr, &s.vz, &s.vp, &s.vq, &s.vt, &s.vdot})
         a->release();
+    for (DeviceArray<float> *a : {&s.vdinv32, &s.vb32, &s.vz32, &s.vt32}) a->release();
     for (int c = 0; c < 2; ++c)
         for (int w = 0; w < 2; ++w) {
             s.vmg_sec[c][w] = 0.0;
