@@ -696,7 +696,7 @@ def main():
                     help="multigrid levels below the coarse space (each halves the grid; -1 = "
                          "halve down to the 16x16 grid); with levels the coarse system is solved "
                          "by V-cycles instead of CG")
-    ap.add_argument("--velocity-projection", type=int, default=8,
+    ap.add_argument("--velocity-projection", type=int, default=2,
                     help="fp32_refined form: diffusion guesses from the advected velocity plus the "
                          "projection of the correction onto this many past ones (0 = off: the "
                          "reference's guess; on, the fields are closer to a tightly solved run and "

@@ -138,24 +138,26 @@ struct Simulation {
     // two-column velocity solves (their P2 matrices have ~4x the rows)
     int batch_block_scale = 1;
     int batch_block_scale_v = 1;
-    // coarse-grid correction of the pressure guess: prolongation P (P x nc),
-    // its transpose and A_c = P^T A P; residual, coarse right-hand side and
-    // solution vectors; the last correction's CG iterations and seconds
+    // coarse-grid correction of the pressure guess: the coarse space is
+    // mg.lv[0] (prolongation P (P x nc), its transpose, A_c = P^T A P, 1/diag,
+    // and its vectors b (the restricted residual), x (the correction) and r);
+    // a pressure-sized residual vector; the last correction's CG iterations
+    // and seconds
     bool has_coarse = false;
     int nc = 0;
     double coarse_rtol = 1e-6;
-    Matrix cP, cPT, cA;
-    DeviceArray<double> cy, cr, ce;
+    DeviceArray<double> cy;
     ProjBasis cproj;  // projection guess of the coarse solves
-    // multigrid levels below the coarse grid (none: the coarse system is
-    // solved by CG); the coarse level's 1/diag and residual
+    // the coarse space (lv[0]) and the multigrid levels below it (none: the
+    // coarse system is solved by CG)
     MgLevels mg;
     // pressure CG preconditioned by one V-cycle over the fine grid, the
     // coarse space and its levels: 1/diag of the pressure matrix, the CG
-    // vectors r, z, p, q, a smoothing residual t and a dot scratch
+    // vectors r, z, p, q, a smoothing residual t and a dot scratch, and the
+    // FP32 cycle's fine-level vectors
     bool pmg = false;
     DeviceArray<double> pdinv, pr, pz, pp, pq, pt, pdot;
-    DeviceArray<double> cdinv, cres;
+    DeviceArray<float> pdinv32, pb32, pz32, pt32;
     // diffusion CG preconditioned by one V-cycle over the velocity nodes,
     // the velocity coarse space (vmg.lv[0]) and the levels below it: mode
     // (0 off, 1 per column by measured cost, 2 every column), 1/diag of the
@@ -187,6 +189,8 @@ struct Simulation {
     // iterations) and as one two-column launch (sec2: over its iterations);
     // 0 = not measured yet
     double pair_sec1[2] = {0.0, 0.0}, pair_sec2[2] = {0.0, 0.0};
+    // steps since each way (0: apart, 1: paired) last ran, per system
+    int pair_age[2][2] = {{0, 0}, {0, 0}};
     // internal order of the velocity nodes (velocity_order): vorder[new] =
     // caller's index, vpos[caller's index] = new
     std::vector<int32_t> vorder, vpos;
@@ -447,7 +451,10 @@ void simulation_reset(Simulation &s) {
     s.vproj[0].dim = s.vproj[1].dim = 0;
     s.mproj[0].dim = s.mproj[1].dim = 0;
     for (auto &row : s.vel_iters) row[0] = row[1] = 0;
-    for (int k = 0; k < 2; ++k) s.pair_sec1[k] = s.pair_sec2[k] = 0.0;
+    for (int k = 0; k < 2; ++k) {
+        s.pair_sec1[k] = s.pair_sec2[k] = 0.0;
+        s.pair_age[k][0] = s.pair_age[k][1] = 0;
+    }
     launch_fill(d, 2 * s.V, 0.0, s.uv.ptr);
     launch_fill(d, s.P, 0.0, s.p.ptr);
     launch_scatter(d, s.nvb, s.vb_idx.ptr, s.vb_u.ptr, s.uv.ptr);
@@ -681,6 +688,7 @@ void galerkin_product(int n, const std::vector<int32_t> &ars, const std::vector<
 
 void release_pressure_multigrid(Simulation &s) {
     s.pmg = false;
+    for (DeviceArray<float> *a : {&s.pdinv32, &s.pb32, &s.pz32, &s.pt32}) a->release();
     s.pdinv.release();
     s.pr.release();
     s.pz.release();
@@ -728,7 +736,8 @@ constexpr int kMgMaxCycles = 30;
 // needs the dense deepest level).
 template <typename T>
 void mg_cycle(Device &d, Matrix &A, int n, const T *dinv, T *r, MgLevels &m, size_t l,
-              const T *b, T *x, bool zero = true) {
+              const T *b, T *x, bool zero = true, int sweeps = kMgSweeps,
+              double omega = kMgOmega) {
     if (l == m.lv.size() && m.deep_ok) {
         launch_spmv(d, m.deep_inv, b, nullptr, x, 0);
         return;
@@ -751,13 +760,13 @@ void mg_cycle(Device &d, Matrix &A, int n, const T *dinv, T *r, MgLevels &m, siz
         }
         return;
     }
-    for (int k = 0; k < kMgSweeps; ++k) {
+    for (int k = 0; k < sweeps; ++k) {
         if (k == 0 && zero) {
-            launch_jacobi(d, n, kMgOmega, dinv, b, x);
+            launch_jacobi(d, n, omega, dinv, b, x);
             continue;
         }
         launch_spmv(d, A, x, b, r, 1);
-        launch_jacobi(d, n, kMgOmega, dinv, r, x);
+        launch_jacobi(d, n, omega, dinv, r, x);
     }
     launch_spmv(d, A, x, b, r, 1);
     MgLevel &below = *m.lv[l];
@@ -766,9 +775,9 @@ void mg_cycle(Device &d, Matrix &A, int n, const T *dinv, T *r, MgLevels &m, siz
     mg_cycle<T>(d, below.A, below.n, v.dinv, v.r, m, l + 1, v.b, v.x);
     launch_spmv(d, below.P, v.x, nullptr, r, 0);
     launch_add(d, n, r, x);
-    for (int k = 0; k < kMgSweeps; ++k) {
+    for (int k = 0; k < sweeps; ++k) {
         launch_spmv(d, A, x, b, r, 1);
-        launch_jacobi(d, n, kMgOmega, dinv, r, x);
+        launch_jacobi(d, n, omega, dinv, r, x);
     }
 }
 
@@ -787,18 +796,23 @@ void ensure_fp32_levels(Device &d, MgLevels &m) {
     m.fp32_ok = true;
 }
 
+// The pressure coarse space (present while has_coarse).
+MgLevel &coarse(Simulation &s) { return *s.mg.lv[0]; }
+
 // One V-cycle from the pressure coarse level down, from zero or from the x
 // given.
 void vcycle(Simulation &s, const double *b, double *x, bool zero) {
-    mg_cycle<double>(*s.dev, s.cA, s.nc, s.cdinv.ptr, s.cres.ptr, s.mg, 0, b, x, zero);
+    MgLevel &C = coarse(s);
+    mg_cycle<double>(*s.dev, C.A, s.nc, C.dinv.ptr, C.r.ptr, s.mg, 1, b, x, zero);
 }
 
 // V-cycles on A_c e = cr from the current e; returns the status, the cycle
 // count and the wall time (the cycles synchronise with the host).
 int mg_solve(Simulation &s, int64_t *cycles, double *seconds) {
     Device &d = *s.dev;
+    MgLevel &C = coarse(s);
     const auto t0 = std::chrono::steady_clock::now();
-    const double bb = dev_dot(d, s.cproj, s.cr.ptr, s.cr.ptr);
+    const double bb = dev_dot(d, s.cproj, C.b.ptr, C.b.ptr);
     const double tol2 = s.coarse_rtol * s.coarse_rtol * bb;
     int status = NSFEM_MAX_ITERATIONS_REACHED;
     int k = 0;
@@ -806,10 +820,10 @@ int mg_solve(Simulation &s, int64_t *cycles, double *seconds) {
         status = NSFEM_OK;  // e = projection guess of a zero right-hand side
     } else {
         while (k < kMgMaxCycles) {
-            vcycle(s, s.cr.ptr, s.ce.ptr, false);
+            vcycle(s, C.b.ptr, C.x.ptr, false);
             ++k;
-            launch_spmv(d, s.cA, s.ce.ptr, s.cr.ptr, s.cres.ptr, 1);
-            const double rr = dev_dot(d, s.cproj, s.cres.ptr, s.cres.ptr);
+            launch_spmv(d, C.A, C.x.ptr, C.b.ptr, C.r.ptr, 1);
+            const double rr = dev_dot(d, s.cproj, C.r.ptr, C.r.ptr);
             if (!std::isfinite(rr)) {
                 status = NSFEM_BREAKDOWN;
                 break;
@@ -834,17 +848,19 @@ int mg_solve(Simulation &s, int64_t *cycles, double *seconds) {
 void coarse_prepare(Simulation &s) {
     Device &d = *s.dev;
     Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+    MgLevel &C = coarse(s);
     launch_spmv(d, A, s.p.ptr, s.prhs.ptr, s.cy.ptr, 1);
-    launch_spmv(d, s.cPT, s.cy.ptr, nullptr, s.cr.ptr, 0);
-    launch_fill(d, s.nc, 0.0, s.ce.ptr);
-    projection_guess(d, s.cproj, s.cr.ptr, s.ce.ptr);
+    launch_spmv(d, C.PT, s.cy.ptr, nullptr, C.b.ptr, 0);
+    launch_fill(d, s.nc, 0.0, C.x.ptr);
+    projection_guess(d, s.cproj, C.b.ptr, C.x.ptr);
 }
 
 CgRequest coarse_request(Simulation &s) {
     CgRequest rq{};
-    rq.m = &s.cA;
-    rq.b = s.cr.ptr;
-    rq.x = s.ce.ptr;
+    MgLevel &C = coarse(s);
+    rq.m = &C.A;
+    rq.b = C.b.ptr;
+    rq.x = C.x.ptr;
     rq.ncol = 1;
     rq.max_iter = -1;
     rq.rtol = s.coarse_rtol;
@@ -858,15 +874,16 @@ void coarse_finish(Simulation &s, int status, int64_t iterations, double seconds
     s.coarse_iters = iterations;
     s.coarse_seconds = seconds;
     if (status != NSFEM_OK) return;
-    projection_update(d, s.cproj, s.cA, s.ce.ptr);
-    launch_spmv(d, s.cP, s.ce.ptr, nullptr, s.cy.ptr, 0);
+    MgLevel &C = coarse(s);
+    projection_update(d, s.cproj, C.A, C.x.ptr);
+    launch_spmv(d, C.P, C.x.ptr, nullptr, s.cy.ptr, 0);
     launch_add(d, s.P, s.cy.ptr, s.p.ptr);
     launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.p.ptr);
 }
 
 void coarse_correct(Simulation &s) {
     coarse_prepare(s);
-    if (!s.mg.lv.empty()) {
+    if (s.mg.lv.size() > 1) {
         int64_t cycles = 0;
         double sec = 0.0;
         const int status = mg_solve(s, &cycles, &sec);
@@ -885,24 +902,26 @@ void coarse_correct(Simulation &s) {
 // level solved exactly (or by CG to 1e-10) keep M symmetric, as
 // preconditioned CG needs.  z is zeroed on the pressure Dirichlet rows
 // (D M D with D the projection off them stays symmetric; the solve holds
-// those rows at their values).
+// those rows at their values).  In the refined precision the cycle runs in
+// FP32 when the deepest level has its dense inverse (r rounded to FP32, z
+// widened; the FP64 CG around it keeps its FP64 stopping rule).
 void fine_vcycle(Simulation &s, const double *r, double *z) {
     Device &d = *s.dev;
     Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
     const int n = s.P;
-    launch_fill(d, n, 0.0, z);
-    for (int k = 0; k < kMgSweeps; ++k) {
-        launch_spmv(d, A, z, r, s.pt.ptr, 1);
-        launch_jacobi(d, n, kMgOmega, s.pdinv.ptr, s.pt.ptr, z);
-    }
-    launch_spmv(d, A, z, r, s.pt.ptr, 1);
-    launch_spmv(d, s.cPT, s.pt.ptr, nullptr, s.cr.ptr, 0);
-    vcycle(s, s.cr.ptr, s.ce.ptr, true);
-    launch_spmv(d, s.cP, s.ce.ptr, nullptr, s.pt.ptr, 0);
-    launch_add(d, n, s.pt.ptr, z);
-    for (int k = 0; k < kMgSweeps; ++k) {
-        launch_spmv(d, A, z, r, s.pt.ptr, 1);
-        launch_jacobi(d, n, kMgOmega, s.pdinv.ptr, s.pt.ptr, z);
+    if (s.opt.precision == NSFEM_PRECISION_FP32_REFINED && s.mg.deep_ok) {
+        if (!s.mg.fp32_ok) {
+            ensure_fp32_levels(d, s.mg);
+            ensure_val32(d, A);
+            s.pdinv32.resize((size_t)n);
+            launch_convert(d, n, s.pdinv.ptr, s.pdinv32.ptr);
+            for (DeviceArray<float> *a : {&s.pb32, &s.pz32, &s.pt32}) a->resize((size_t)n);
+        }
+        launch_convert(d, n, r, s.pb32.ptr);
+        mg_cycle<float>(d, A, n, s.pdinv32.ptr, s.pt32.ptr, s.mg, 0, s.pb32.ptr, s.pz32.ptr);
+        launch_convert(d, n, s.pz32.ptr, z);
+    } else {
+        mg_cycle<double>(d, A, n, s.pdinv.ptr, s.pt.ptr, s.mg, 0, r, z);
     }
     launch_scatter_value(d, s.npb, s.pb_idx.ptr, 0.0, z);
 }
@@ -1004,6 +1023,14 @@ void pcg_pressure(Simulation &s, SolveOut &o) {
 // refined precision the cycle runs in FP32 (r rounded to FP32, z widened):
 // the products and sweeps move about half the bytes, and the FP64 CG
 // around it still decides convergence on its FP64 residual.
+// Jacobi weight of the sweeps on the P2 velocity nodes (the levels below
+// keep kMgOmega): D^-1 A of the diffusion matrix reaches ~2.2 (at weight 0.9
+// the 2048^2 cavity's v column needs 17 iterations instead of 6); measured
+// there per step, weight 0.6 / 0.65 / 0.7 / 0.8 took 22.9 / 21.4 / 20.8 /
+// 23.5 ms (6.3 / 5.9 / 5.7 / 6.5 iterations); three sweeps instead of two
+// save an iteration but cost more (23.5 ms at 0.7).
+constexpr double kVmgFineOmega = 0.7;
+
 bool velocity_cycle_fp32(const Simulation &s) {
     return s.opt.precision == NSFEM_PRECISION_FP32_REFINED && s.vmg.deep_ok;
 }
@@ -1021,10 +1048,12 @@ void velocity_vcycle(Simulation &s, const double *r, double *z) {
             for (DeviceArray<float> *a : {&s.vb32, &s.vz32, &s.vt32}) a->resize((size_t)V);
         }
         launch_convert(d, V, r, s.vb32.ptr);
-        mg_cycle<float>(d, A, V, s.vdinv32.ptr, s.vt32.ptr, s.vmg, 0, s.vb32.ptr, s.vz32.ptr);
+        mg_cycle<float>(d, A, V, s.vdinv32.ptr, s.vt32.ptr, s.vmg, 0, s.vb32.ptr, s.vz32.ptr, true,
+                        kMgSweeps, kVmgFineOmega);
         launch_convert(d, V, s.vz32.ptr, z);
     } else {
-        mg_cycle<double>(d, A, V, s.vdinv.ptr, s.vt.ptr, s.vmg, 0, r, z);
+        mg_cycle<double>(d, A, V, s.vdinv.ptr, s.vt.ptr, s.vmg, 0, r, z, true, kMgSweeps,
+                         kVmgFineOmega);
     }
     launch_scatter_value(d, s.nvb, s.vb_idx.ptr, 0.0, z);
 }
@@ -1073,91 +1102,6 @@ void simulation_set_mass_projection(Simulation &s, int depth) {
             B.reserve(s.V, depth);
     }
 }
-
-namespace {
-// Drops the coarse space, its levels and the V-cycle preconditioner.
-void release_coarse_space(Simulation &s) {
-    matrix_release(s.cP);
-    matrix_release(s.cPT);
-    matrix_release(s.cA);
-    s.cy.release();
-    s.cr.release();
-    s.ce.release();
-    s.cproj.release();
-    s.mg.release();
-    s.cdinv.release();
-    s.cres.release();
-    release_pressure_multigrid(s);
-    s.has_coarse = false;
-    s.nc = 0;
-}
-
-void build_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
-                        const int32_t *col, const double *val, double rtol);
-}  // namespace
-
-// Sets (nc > 0) or removes (nc <= 0) the coarse space.  A failure part way
-// leaves no coarse space at all (and no V-cycle preconditioner), never a
-// half-built one.
-void simulation_set_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
-                                 const int32_t *col, const double *val, double rtol) {
-    Device &d = *s.dev;
-    d.set_current();
-    if (nc > 0) {
-        NSFEM_REQUIRE(nc <= s.P && nnz > 0 && rs && col && val, NSFEM_INVALID_ARGUMENT,
-                      "coarse space: need 0 < coarse count <= pressure count and a non-empty P");
-        NSFEM_REQUIRE(rtol > 0.0 && rtol < 1.0, NSFEM_INVALID_ARGUMENT,
-                      "coarse space: the coarse tolerance must lie in (0, 1)");
-    }
-    release_coarse_space(s);
-    if (nc <= 0) return;
-    try {
-        build_coarse_space(s, nc, nnz, rs, col, val, rtol);
-    } catch (...) {
-        release_coarse_space(s);
-        throw;
-    }
-}
-
-namespace {
-void build_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
-                        const int32_t *col, const double *val, double rtol) {
-    Device &d = *s.dev;
-    upload_csr(d, s.cP, s.P, nc, nnz, rs, col, val);  // validates P
-    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
-    std::vector<int32_t> acol((size_t)A.nnz);
-    std::vector<double> aval((size_t)A.nnz);
-    if (A.nnz) {
-        NSFEM_HIP(hipMemcpy(acol.data(), A.col.ptr, acol.size() * sizeof(int32_t),
-                            hipMemcpyDeviceToHost));
-        NSFEM_HIP(hipMemcpy(aval.data(), A.val.ptr, aval.size() * sizeof(double),
-                            hipMemcpyDeviceToHost));
-    }
-    std::vector<int32_t> trs, tcol, crs, ccol;
-    std::vector<double> tval, cval;
-    csr_transpose(s.P, nc, rs, col, val, trs, tcol, tval);
-    const int threads = s.opt.threads > 0 ? s.opt.threads
-                                          : (int)std::max(1u, std::thread::hardware_concurrency());
-    galerkin_product(s.P, A.host_rs, acol, aval, nc, rs, col, val, trs, tcol, tval, threads, crs,
-                     ccol, cval);
-    upload_csr(d, s.cPT, nc, s.P, nnz, trs.data(), tcol.data(), tval.data());
-    upload_csr(d, s.cA, nc, nc, (int64_t)ccol.size(), crs.data(), ccol.data(), cval.data());
-    s.cy.resize(s.P);
-    s.cr.resize(nc);
-    s.ce.resize(nc);
-    s.cres.resize(nc);
-    s.cproj.reserve(nc, 16);
-    upload_inverse_diagonal(d, nc, crs, ccol, cval, s.cdinv);
-    s.mg.deep_rs.swap(crs);
-    s.mg.deep_col.swap(ccol);
-    s.mg.deep_val.swap(cval);
-    s.nc = nc;
-    s.coarse_rtol = rtol;
-    s.coarse_iters = 0;
-    s.coarse_seconds = 0.0;
-    s.has_coarse = true;
-}
-}  // namespace
 
 // Levels of at most this many nodes are solved by a dense inverse (formed
 // on the host by Cholesky) applied as a dense CSR product; larger ones by CG.
@@ -1215,6 +1159,74 @@ int host_threads(const Simulation &s) {
     return s.opt.threads > 0 ? s.opt.threads
                              : (int)std::max(1u, std::thread::hardware_concurrency());
 }
+// The host CSR (row starts, columns, values) of a simulation matrix.
+void host_matrix(Matrix &A, std::vector<int32_t> &col, std::vector<double> &val) {
+    col.resize((size_t)A.nnz);
+    val.resize((size_t)A.nnz);
+    if (!A.nnz) return;
+    NSFEM_HIP(hipMemcpy(col.data(), A.col.ptr, col.size() * sizeof(int32_t),
+                        hipMemcpyDeviceToHost));
+    NSFEM_HIP(hipMemcpy(val.data(), A.val.ptr, val.size() * sizeof(double),
+                        hipMemcpyDeviceToHost));
+}
+
+}  // namespace
+
+namespace {
+// Drops the coarse space, its levels and the V-cycle preconditioner.
+void release_coarse_space(Simulation &s) {
+    s.cy.release();
+    s.cproj.release();
+    s.mg.release();
+    release_pressure_multigrid(s);
+    s.has_coarse = false;
+    s.nc = 0;
+}
+
+void build_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
+                        const int32_t *col, const double *val, double rtol);
+}  // namespace
+
+// Sets (nc > 0) or removes (nc <= 0) the coarse space.  A failure part way
+// leaves no coarse space at all (and no V-cycle preconditioner), never a
+// half-built one.
+void simulation_set_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
+                                 const int32_t *col, const double *val, double rtol) {
+    Device &d = *s.dev;
+    d.set_current();
+    if (nc > 0) {
+        NSFEM_REQUIRE(nc <= s.P && nnz > 0 && rs && col && val, NSFEM_INVALID_ARGUMENT,
+                      "coarse space: need 0 < coarse count <= pressure count and a non-empty P");
+        NSFEM_REQUIRE(rtol > 0.0 && rtol < 1.0, NSFEM_INVALID_ARGUMENT,
+                      "coarse space: the coarse tolerance must lie in (0, 1)");
+    }
+    release_coarse_space(s);
+    if (nc <= 0) return;
+    try {
+        build_coarse_space(s, nc, nnz, rs, col, val, rtol);
+    } catch (...) {
+        release_coarse_space(s);
+        throw;
+    }
+}
+
+namespace {
+void build_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
+                        const int32_t *col, const double *val, double rtol) {
+    Device &d = *s.dev;
+    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+    std::vector<int32_t> acol;
+    std::vector<double> aval;
+    host_matrix(A, acol, aval);
+    append_level(d, s.mg, s.P, A.host_rs, acol, aval, nc, nnz, rs, col, val, host_threads(s));
+    s.cy.resize(s.P);
+    s.cproj.reserve(nc, 16);
+    s.nc = nc;
+    s.coarse_rtol = rtol;
+    s.coarse_iters = 0;
+    s.coarse_seconds = 0.0;
+    s.has_coarse = true;
+}
 }  // namespace
 
 // Appends a multigrid level below the deepest one: P maps the new level's
@@ -1226,7 +1238,7 @@ void simulation_add_coarse_level(Simulation &s, int count, int64_t nnz, const in
     d.set_current();
     NSFEM_REQUIRE(s.has_coarse, NSFEM_INVALID_ARGUMENT,
                   "coarse level: set a coarse space first");
-    const int above = s.mg.lv.empty() ? s.nc : s.mg.lv.back()->n;
+    const int above = s.mg.lv.back()->n;
     NSFEM_REQUIRE(count > 0 && count < above && nnz > 0 && rs && col && val,
                   NSFEM_INVALID_ARGUMENT,
                   "coarse level: need 0 < count < the level above's size and a non-empty P");
@@ -1279,17 +1291,6 @@ void release_velocity_multigrid(Simulation &s) {
             s.vmg_sec[c][w] = 0.0;
             s.vmg_age[c][w] = 0;
         }
-}
-
-// The host CSR (row starts, columns, values) of a simulation matrix.
-void host_matrix(Matrix &A, std::vector<int32_t> &col, std::vector<double> &val) {
-    col.resize((size_t)A.nnz);
-    val.resize((size_t)A.nnz);
-    if (!A.nnz) return;
-    NSFEM_HIP(hipMemcpy(col.data(), A.col.ptr, col.size() * sizeof(int32_t),
-                        hipMemcpyDeviceToHost));
-    NSFEM_HIP(hipMemcpy(val.data(), A.val.ptr, val.size() * sizeof(double),
-                        hipMemcpyDeviceToHost));
 }
 
 void build_velocity_coarse_space(Simulation &s, int nc, int64_t nnz, const int32_t *rs,
@@ -1575,25 +1576,37 @@ void diffusion_after_solve(Simulation &s) {
 // so only nearly equal counts pay; on small systems, bound by launch and
 // synchronisation latency, pairing pays more often).  Before that, counts
 // within 2x (+8) of each other pair, and a system measured only paired is
-// run apart once to time it.
+// run apart once to time it.  The way not taken runs again once it has not
+// run for kPairRetry steps: a first measurement can hold one-time costs
+// (the first launch of a kernel form, lazily built matrix copies), and the
+// counts drift as the flow develops.
+constexpr int kPairRetry = 16;
+
 bool pair_columns(Simulation &s, int sys) {
     if (!s.opt.pair_velocity) return false;
     const int64_t a = s.vel_iters[sys][0], b = s.vel_iters[sys][1];
     if (a <= 0 || b <= 0) return true;
     const double t1 = s.pair_sec1[sys], t2 = s.pair_sec2[sys];
-    if (t1 > 0.0 && t2 > 0.0) return t2 * (double)std::max(a, b) <= t1 * (double)(a + b);
+    if (t1 > 0.0 && t2 > 0.0) {
+        const bool pair = t2 * (double)std::max(a, b) <= t1 * (double)(a + b);
+        return s.pair_age[sys][pair ? 0 : 1] >= kPairRetry ? !pair : pair;
+    }
     const bool close = std::max(a, b) <= 2 * std::min(a, b) + 8;
     return close && !(t2 > 0.0 && t1 == 0.0);
 }
 
-// Records the timing of a velocity system's step for pair_columns
-// (exponential average, weight 1/2).
+// Records the timing of a velocity system's step for pair_columns: the
+// least seconds per iteration seen each way (a first launch of a kernel
+// form can carry its one-time loading in its event time; the cost per
+// iteration of a form on a matrix is otherwise steady).
 void record_pairing(Simulation &s, int sys, bool paired, const SolveOut &u, const SolveOut &v) {
+    s.pair_age[sys][paired ? 1 : 0] = 0;
+    ++s.pair_age[sys][paired ? 0 : 1];
     double &t = paired ? s.pair_sec2[sys] : s.pair_sec1[sys];
     const int64_t its = paired ? std::max(u.iters[0], u.iters[1]) : u.iters[0] + v.iters[0];
     if (its <= 0) return;
     const double sec = (paired ? u.seconds : u.seconds + v.seconds) / (double)its;
-    t = t > 0.0 ? 0.5 * (t + sec) : sec;
+    t = t > 0.0 ? std::min(t, sec) : sec;
 }
 
 // Whether column c of the diffusion solve runs V-cycle preconditioned CG
@@ -1617,7 +1630,8 @@ bool velocity_pcg(const Simulation &s, int c) {
 void record_velocity_way(Simulation &s, int c, bool mg, double seconds) {
     const int w = mg ? 1 : 0;
     double &t = s.vmg_sec[c][w];
-    t = t > 0.0 ? 0.5 * (t + seconds) : seconds;
+    // a way that had not run for kVmgRetry steps is measured afresh
+    t = t > 0.0 && s.vmg_age[c][w] < kVmgRetry ? 0.5 * (t + seconds) : seconds;
     s.vmg_age[c][w] = 0;
     ++s.vmg_age[c][1 - w];
 }
@@ -1834,7 +1848,7 @@ void simulation_step_batch(const std::vector<Simulation *> &S, int steps,
             pressure_guess_base(s);
             s.coarse_iters = 0;
             s.coarse_seconds = 0.0;
-            if (s.has_coarse && !s.mg.lv.empty()) {
+            if (s.has_coarse && s.mg.lv.size() > 1) {
                 coarse_correct(s);  // multigrid: its own V-cycles
             } else if (s.has_coarse) {
                 coarse_prepare(s);
@@ -2084,16 +2098,9 @@ void simulation_detach(Simulation *s) {
     s->mproj[1].release();
     s->vdel.release();
     release_velocity_multigrid(*s);
-    matrix_release(s->cP);
-    matrix_release(s->cPT);
-    matrix_release(s->cA);
     s->mg.release();
     release_pressure_multigrid(*s);
-    s->cdinv.release();
-    s->cres.release();
     s->cy.release();
-    s->cr.release();
-    s->ce.release();
     s->has_coarse = false;
     s->ptmp.release();
     s->pscal.release();
@@ -2195,7 +2202,10 @@ int nsfem_simulation_set_options(nsfem_simulation *s, const nsfem_sim_options *o
         s->impl->opt.pair_velocity = opt->pair_velocity;
         s->impl->opt.velocity_jacobi = opt->velocity_jacobi;
         // the solves' cost per iteration may change: time both ways again
-        for (int k = 0; k < 2; ++k) s->impl->pair_sec1[k] = s->impl->pair_sec2[k] = 0.0;
+        for (int k = 0; k < 2; ++k) {
+            s->impl->pair_sec1[k] = s->impl->pair_sec2[k] = 0.0;
+            s->impl->pair_age[k][0] = s->impl->pair_age[k][1] = 0;
+        }
         for (auto &t : s->impl->vmg_sec) t[0] = t[1] = 0.0;
     });
 }
