@@ -184,14 +184,16 @@ def test_vcycle_preconditioned_pressure_cg(gpu):
     tolerance, the fields match the plain CG run to 1e-6, the pressure
     solves take at most 15 iterations a step (the CPU premise: 7-9), no
     coarse correction runs, and a batch holding the simulation is rejected;
-    switching it off restores the plain solve."""
+    switching it off restores the plain solve.  In the refined precision
+    the cycles run in FP32 around the same FP64 CG."""
     from navier_stokes_fem_amd import mesh
     from navier_stokes_fem_amd.simulation import Simulation, step_batch
 
     m = mesh.cavity(128)
     out = {}
-    for pmg in (False, True):
-        sim = Simulation(m, 1e-3, 1e-3, device=gpu, rtol=1e-8, precision="fp64")
+    for pmg in (False, True, "fp32"):
+        sim = Simulation(m, 1e-3, 1e-3, device=gpu, rtol=1e-8,
+                         precision="fp32_refined" if pmg == "fp32" else "fp64")
         if pmg:
             sim.set_coarse_space(mesh.cavity_prolongation(128, 2), rtol=1e-6)
             for n in (64, 32):
@@ -199,19 +201,27 @@ def test_vcycle_preconditioned_pressure_cg(gpu):
             sim.set_pressure_multigrid(True)
         st = sim.step(8)
         out[pmg] = (st, sim.fields())
-        if pmg:
+        if pmg is True:
             with pytest.raises(Exception):
                 step_batch([sim], 1)
             sim.set_pressure_multigrid(False)
             assert sim.step(1)["cg_iterations"][2] > 15
         sim.close()
-    (s0, f0), (s1, f1) = out[False], out[True]
+    (s0, f0), (s1, f1), (s2, f2) = out[False], out[True], out["fp32"]
     print(f"V-cycle preconditioned pressure CG: pressure iterations plain {s0['cg_iterations'][2]}, "
-          f"V-cycle PCG {s1['cg_iterations'][2]}", flush=True)
-    assert max(s1["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+          f"V-cycle PCG {s1['cg_iterations'][2]}, FP32 cycles {s2['cg_iterations'][2]}", flush=True)
+    for st in (s1, s2):
+        assert max(st["max_relative_residual"]) <= 1e-8 * (1 + 1e-6)
+        assert st["cg_iterations"][2] <= 15 * 8 and st["coarse_iterations"] == 0
+        assert st["vcycle_solves"][2] == 8
     for a, b in zip(f1, f0):
         assert _rel(a, b) < 1e-6
-    assert s1["cg_iterations"][2] <= 15 * 8 and s1["coarse_iterations"] == 0
+    # the FP32 cycles (refined precision) precondition the same FP64 CG: the
+    # pressure takes at most one more iteration a step and the fields match
+    # as the mixed velocity solves allow
+    assert s2["cg_iterations"][2] <= s1["cg_iterations"][2] + 8
+    for a, b in zip(f2, f0):
+        assert _rel(a, b) < 1e-5
 
 
 def _errors_vs(fields, truth):

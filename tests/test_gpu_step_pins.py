@@ -5,8 +5,10 @@ centripetal pressure, the step is first order in dt, mirror symmetry holds
 and the Dirichlet values are exact -- for the plain FP64 method, the FP32 +
 FP64 refinement form, the chosen form (projection guess, coarse-grid
 correction, velocity projection), the refinement form with
-Jacobi-preconditioned velocity solves and the diffusion solved by V-cycle
-preconditioned CG (on the meshes built on a grid).  These do not go through the oracle: they
+Jacobi-preconditioned velocity solves, the diffusion solved by V-cycle
+preconditioned CG, and the whole chain of the 2048^2 headline (FP32
+V-cycles for the diffusion and the pressure, Jacobi mass solves, projection
+guesses of every solve) on the meshes built on a grid.  These do not go through the oracle: they
 pin the device step (navier_stokes_fem_amd/csrc/simulation.cpp, after
 include/assembly.h:1516-1554) directly.
 """
@@ -17,7 +19,7 @@ import step_cases as S
 
 pytestmark = pytest.mark.gpu
 
-FORMS = ("fp64", "fp32_refined", "chosen", "jacobi", "vcycle")
+FORMS = ("fp64", "fp32_refined", "chosen", "jacobi", "vcycle", "multigrid")
 
 
 def gpu_stepper(dev, form, rtol=1e-10):
@@ -34,9 +36,18 @@ def gpu_stepper(dev, form, rtol=1e-10):
             cache.clear()
             sim = Simulation(m, nu, dt, device=dev, rtol=rtol,
                              precision="fp64" if form == "fp64" else "fp32_refined",
-                             velocity_jacobi=form == "jacobi")
-            if form == "vcycle" and m.grid is not None:
+                             velocity_jacobi={"jacobi": True, "multigrid": "mass"}.get(form, False))
+            if form in ("vcycle", "multigrid") and m.grid is not None:
                 sim.set_velocity_multigrid(M.velocity_hierarchy(m, min_size=4), mode=2)
+            if form == "multigrid" and m.grid is not None:
+                H = M.grid_hierarchy(m, min_size=2)
+                sim.set_coarse_space(H[0], rtol=1e-8)
+                for level in H[1:]:
+                    sim.add_coarse_level(level)
+                sim.set_pressure_multigrid(True)
+                sim.set_warm_start("projection")
+                sim.set_velocity_projection(2)
+                sim.set_mass_projection(1)
             if form == "chosen":
                 sim.set_warm_start("projection")
                 sim.set_velocity_projection(8)
