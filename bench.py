@@ -149,14 +149,15 @@ def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
 def coarse_levels(n, count):
     """Prolongations of `count` multigrid levels below the n x n cavity's P1
     coarse space, each halving the grid (n must stay even); count < 0 halves
-    down to the 16 x 16 grid."""
+    down to the 32 x 32 grid (33^2 nodes: the library's dense deepest
+    level)."""
     from navier_stokes_fem_amd import mesh
 
     out = []
     if count < 0:
         count = 0
         k = n
-        while k > 16 and k % 2 == 0:
+        while k > 32 and k % 2 == 0:
             k //= 2
             count += 1
     for _ in range(count):
@@ -694,7 +695,7 @@ def main():
                          "more; 0 = the reference's guess, the velocity)")
     ap.add_argument("--coarse-levels", type=int, default=-1,
                     help="multigrid levels below the coarse space (each halves the grid; -1 = "
-                         "halve down to the 16x16 grid); with levels the coarse system is solved "
+                         "halve down to the 32x32 grid); with levels the coarse system is solved "
                          "by V-cycles instead of CG")
     ap.add_argument("--velocity-projection", type=int, default=2,
                     help="fp32_refined form: diffusion guesses from the advected velocity plus the "

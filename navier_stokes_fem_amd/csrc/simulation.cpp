@@ -1104,8 +1104,10 @@ void simulation_set_mass_projection(Simulation &s, int depth) {
 }
 
 // Levels of at most this many nodes are solved by a dense inverse (formed
-// on the host by Cholesky) applied as a dense CSR product; larger ones by CG.
-constexpr int kMgDenseMax = 600;
+// on the host by Cholesky, about a second at 33^2 nodes) applied as a dense
+// CSR product; larger ones by CG.  A dense 33^2 level replaces two halvings
+// whose two dozen launches per cycle are bound by launch latency.
+constexpr int kMgDenseMax = 1100;
 
 namespace {
 // The dense inverse of m's deepest level when it has at most kMgDenseMax

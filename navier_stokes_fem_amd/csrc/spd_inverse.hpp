@@ -1,7 +1,7 @@
 // Dense inverse of a small symmetric positive definite matrix given as CSR
 // (the lower triangle is read), by Cholesky A = L L^T and the solves
 // L y = e_c, L^T x = y for every column c.  Host code, O(n^3): meant for the
-// deepest multigrid level (a few hundred nodes).
+// deepest multigrid level (up to about a thousand nodes: about a second at 1089).
 #pragma once
 
 #include <cmath>
@@ -30,18 +30,28 @@ inline bool spd_inverse(int n, const int32_t *rs, const int32_t *col, const doub
             L[(size_t)i * n + j] = v / dj;
         }
     }
+    // L^T row-major, so that the backward solve reads rows too; the inverse
+    // is symmetric, so column c of it is stored as row c
+    std::vector<double> LT((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k <= i; ++k) LT[(size_t)k * n + i] = L[(size_t)i * n + k];
     inv.assign((size_t)n * n, 0.0);
     std::vector<double> y((size_t)n);
     for (int c = 0; c < n; ++c) {
-        for (int i = 0; i < n; ++i) {
+        double *x = inv.data() + (size_t)c * n;
+        // y = L^-1 e_c vanishes above row c
+        for (int i = 0; i < c; ++i) y[i] = 0.0;
+        for (int i = c; i < n; ++i) {
             double v = i == c ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) v -= L[(size_t)i * n + k] * y[k];
-            y[i] = v / L[(size_t)i * n + i];
+            const double *Li = L.data() + (size_t)i * n;
+            for (int k = c; k < i; ++k) v -= Li[k] * y[k];
+            y[i] = v / Li[i];
         }
         for (int i = n - 1; i >= 0; --i) {
             double v = y[i];
-            for (int k = i + 1; k < n; ++k) v -= L[(size_t)k * n + i] * inv[(size_t)k * n + c];
-            inv[(size_t)i * n + c] = v / L[(size_t)i * n + i];
+            const double *Ui = LT.data() + (size_t)i * n;
+            for (int k = i + 1; k < n; ++k) v -= Ui[k] * x[k];
+            x[i] = v / Ui[i];
         }
     }
     return true;

@@ -312,7 +312,7 @@ def _grid_levels(grid, extent, dirichlet, levels, min_size):
     return out
 
 
-def velocity_hierarchy(mesh: Mesh, levels: int = -1, min_size: int = 16):
+def velocity_hierarchy(mesh: Mesh, levels: int = -1, min_size: int = 32):
     """Prolongations of a multigrid hierarchy below the velocity (P2) space
     of a mesh built on a structured grid, for
     ``Simulation.set_velocity_coarse_space`` (the first) and
@@ -323,8 +323,10 @@ def velocity_hierarchy(mesh: Mesh, levels: int = -1, min_size: int = 16):
     column, so every coarse function vanishes on the velocity boundary);
     the others halve the vertex grid as grid_hierarchy does (holes
     rescale, Dirichlet positions stay zero), ``levels`` of them (< 0: while
-    both extents stay even and the smaller one above ``min_size``).  Rows
-    are in the mesh's node order."""
+    both extents stay even and the smaller one halves to at least
+    ``min_size``: a cavity ends on its 32^2 grid, whose 31^2 free vertices
+    the library solves by a dense inverse).  Rows are in the mesh's node
+    order."""
     g = mesh.grid
     if g is None:
         raise ValueError("the mesh carries no structured grid")

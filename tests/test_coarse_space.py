@@ -151,12 +151,12 @@ def test_grid_hierarchy_on_the_channel():
 def test_coarse_levels_chain():
     """The multigrid levels below a coarse space: each prolongation maps the
     next coarser cavity's nodes onto the level above (rows = the size of the
-    level above, columns = its own), halving down to the 16 x 16 grid, and
+    level above, columns = its own), halving down to the 32 x 32 grid, and
     P1 interpolation keeps each level exact on linear functions."""
     import bench
 
     levels = bench.coarse_levels(128, -1)
-    assert len(levels) == 3  # 128 -> 64 -> 32 -> 16
+    assert len(levels) == 2  # 128 -> 64 -> 32
     above = 129 ** 2
     k = 128
     for rs, col, val, count in levels:
