@@ -121,11 +121,13 @@ def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
         log(f"{form}: warmup step {i + 1}/{warmup} done")
     sim.device.synchronize()
     agg = None
+    per_step = []
     t0 = time.perf_counter()
     for i in range(steps):
         st = sim.step(1)
+        per_step.append(dict(st))
         if agg is None:
-            agg = st
+            agg = dict(st)
         else:
             for k in ("cg_iterations", "cg_seconds", "cg_refinements", "cg_launches",
                       "cg_launch_iterations", "vcycle_solves", "vcycle_iterations",
@@ -143,7 +145,39 @@ def run_form(sim, form, warmup, steps, n_p, nnz_p, warm="previous", coarse=None,
     sim.device.synchronize()
     wall = time.perf_counter() - t0
     agg["fields"] = sim.fields()
+    agg["per_step"] = per_step
     return wall, agg
+
+
+# CG-kernel launch classes of a step's stats: (name, stats index, columns,
+# test of a step's stats for a launch of that class).  A velocity system's
+# u entry holds its paired two-column launch when its v entry is empty; a
+# solve by V-cycle preconditioned CG launches no CG kernel.
+LAUNCH_CLASSES = [
+    ("diffusion u+v", 0, 2, lambda s: s["cg_launches"][0] and not s["cg_launches"][1]),
+    ("diffusion u", 0, 1, lambda s: s["cg_launches"][1] and not s["vcycle_solves"][0]),
+    ("diffusion v", 1, 1, lambda s: s["cg_launches"][1] and not s["vcycle_solves"][1]),
+    ("pressure", 2, 1, lambda s: s["cg_launches"][2] and not s["vcycle_solves"][2]),
+    ("mass u+v", 3, 2, lambda s: s["cg_launches"][3] and not s["cg_launches"][4]),
+    ("mass u", 3, 1, lambda s: s["cg_launches"][4]),
+    ("mass v", 4, 1, lambda s: s["cg_launches"][4]),
+]
+
+
+def launch_classes(per_step):
+    """Per class: [launches, iterations, seconds, reliable updates] summed
+    over the steps whose stats hold a launch of that class."""
+    out = {}
+    for name, k, cols, test in LAUNCH_CLASSES:
+        acc = [0, 0, 0.0, 0]
+        for s in per_step:
+            if test(s):
+                acc[0] += s["cg_launches"][k]
+                acc[1] += s["cg_launch_iterations"][k]
+                acc[2] += s["cg_seconds"][k]
+                acc[3] += s["cg_refinements"][k]
+        out[name] = (k, cols, acc)
+    return out
 
 
 def coarse_levels(n, count):
@@ -826,10 +860,13 @@ def main():
     wall, st = results[chosen]
     value = args.steps / wall
     mixed = chosen == "fp32_refined"
-    # dominant kernel: the persistent CG kernel of the solve kind with the
-    # most device time in the chosen form's timed steps (the V-cycle
-    # preconditioned pressure solve is host-orchestrated from several
-    # kernels and is left out).  achieved = the textbook iteration's bytes
+    # dominant kernel: the persistent CG kernel of the launch class (solve
+    # kind, paired or not) with the most device time in the chosen form's
+    # timed steps; the V-cycle preconditioned solves launch no CG kernel
+    # (their products and sweeps are reported beside it).  At 2048^2 the
+    # paired mass launch is the single kernel with the most time per step,
+    # ahead of the diffusion's FP32 fine-level product inside the V-cycles.
+    # achieved = the textbook iteration's bytes
     # (matrix once with int32 columns, row_start, 11 vector words per row
     # and column; w = 4 for the FP32 inner iteration, 8 for FP64) x the
     # launches' iterations / their HIP-event time; the build's own count,
@@ -838,23 +875,14 @@ def main():
     kinds = [("diffusion u", 0, 1), ("diffusion v", 1, 1), ("pressure", 2, 1),
              ("mass u+v", 3, 2), ("mass v", 4, 1)]
     n_v, nnz_v = sim.matrix_info(2)[0], sim.matrix_info(2)[2]
-    # the CG-kernel launches of each kind: all solves less the V-cycle ones
-    plain_launches = [a - b for a, b in zip(st["cg_launches"], st["vcycle_solves"])]
-    plain_iters = [a - b for a, b in zip(st["cg_launch_iterations"], st["vcycle_iterations"])]
-    plain_secs = [a - b for a, b in zip(st["cg_seconds"], st["vcycle_seconds"])]
-    cand = [k for k in kinds if plain_launches[k[1]] > 0]
-    name, k, cols = max(cand, key=lambda k: plain_secs[k[1]])
-    # a paired u+v launch has 2 columns; the u entry mixes both when pairing varied
-    if k == 0 and st["cg_launches"][1] < st["cg_launches"][0]:
-        cols = 2
+    classes = launch_classes(st["per_step"])
+    name = max((c for c in classes if classes[c][2][0] > 0), key=lambda c: classes[c][2][2])
+    k, cols, (launches, k_iters, k_secs, k_refines) = classes[name]
     n_k, nnz_k = (n_p, nnz_p) if k == 2 else (n_v, nnz_v)
-    launches = plain_launches[k]
-    k_iters = plain_iters[k]
-    k_secs = plain_secs[k]
     w = 4 if mixed else 8
     textbook = (4 + w) * nnz_k + (4 + 11 * w * cols) * n_k
     bytes_textbook = textbook * k_iters
-    bytes_k = cg_bytes(nnz_k, n_k, k_iters, st["cg_refinements"][k] + launches - 1, mixed,
+    bytes_k = cg_bytes(nnz_k, n_k, k_iters, k_refines + launches - 1, mixed,
                        columns=cols)
     achieved = bytes_textbook / max(k_secs, 1e-12) / 1e9
     measured = MEASURED_BYTES_PER_ITERATION.get((mixed, name)) if args.n == 2048 else None
