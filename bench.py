@@ -64,13 +64,15 @@ def cg_bytes(nnz, n, iters, refinements, mixed, columns=1):
     return iters * it + (refinements + 1) * res
 
 
-# HBM bytes per CG iteration (mixed?, solve) measured with rocprofv3
+# HBM bytes per CG iteration (mixed?, launch class) measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE (scaled by a k_axpy calibration) on this
-# workload, reliable updates included; source:
-# profiles/k_cg_cavity2048_hbm_traffic.md.  roofline.traffic = this figure x
-# the iterations of a launch.
+# workload, the launches' set-up and reliable-update residual passes
+# included; source: profiles/k_cg_cavity2048_hbm_traffic.md (the mass row
+# on the default command, ~10.6 iterations and ~2.9 FP64 passes a launch;
+# the others on the forms that still run those solves by the persistent
+# CG).  roofline.traffic = this figure x the iterations of a launch.
 MEASURED_BYTES_PER_ITERATION = {(False, "pressure"): 721.9e6, (True, "pressure"): 435.4e6,
-                                (True, "diffusion v"): 2723.9e6, (True, "mass u+v"): 4772.8e6}
+                                (True, "diffusion v"): 2723.9e6, (True, "mass u+v"): 5445.9e6}
 
 
 def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
