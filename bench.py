@@ -945,8 +945,8 @@ def main():
                                      else ""))
                        if chosen == "fp32_refined" else "previous"),
                    "pressure_solve": (
-                       f"FP64 CG preconditioned by one V-cycle (damped Jacobi 2+2 sweeps on "
-                       f"the fine grid, the P1 space of the {args.n // args.coarse_ratio}^2 "
+                       f"FP64 CG preconditioned by one FP32 V-cycle (damped Jacobi 2+2 sweeps "
+                       f"on the fine grid, the P1 space of the {args.n // args.coarse_ratio}^2 "
                        f"cavity and {len(levels)} more levels, the deepest by its dense "
                        f"inverse), to the same 1e-8 relative residual"
                        if chosen == "fp32_refined" and args.pressure_multigrid and coarse is not None
@@ -958,14 +958,20 @@ def main():
                        if chosen == "fp32_refined" and args.mass_projection
                        else "the velocity (the reference's)"),
                    "diffusion_solve": (
-                       ("FP64 CG preconditioned by one V-cycle (damped Jacobi 2+2 sweeps on "
-                        "the P2 nodes, the P1 vertex space and its grid halvings, the deepest by "
-                        "its dense inverse) for "
+                       ("FP64 CG preconditioned by one FP32 V-cycle (damped Jacobi 2+2 sweeps "
+                        "on the P2 nodes, the P1 vertex space and its grid halvings, the deepest "
+                        "by its dense inverse) for "
                         + ("each column when measured faster than the FP32 + FP64 refinement CG"
                            if args.velocity_multigrid == 1 else "both columns"))
                        if chosen == "fp32_refined" and args.velocity_multigrid
                        else ("FP32 + FP64 refinement CG" if chosen == "fp32_refined"
-                             else "FP64 CG"))},
+                             else "FP64 CG")),
+                   "mass_solve": (
+                       ("FP32 + FP64 refinement CG"
+                        + (" on D^-1/2 M D^-1/2 (Jacobi), the same unscaled stopping rule"
+                           if args.velocity_jacobi in ("mass", "both") else "")
+                        + ", u and v as one two-column solve while that is measured faster")
+                       if chosen == "fp32_refined" else "FP64 CG")},
         "roofline": roofline,
     }
     for form, (w, s) in list(results.items()) + list(attribution.items()):
