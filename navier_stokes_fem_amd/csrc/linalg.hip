@@ -362,7 +362,9 @@ bool ensure_jacobi(Device &d, Matrix &m) {
 namespace {
 
 // MODE 0: y = A x.  MODE 1: y = lhs - A x (y may alias lhs).
-// MODE 2: y = A x - lhs.
+// MODE 2: y = A x - lhs.  MODE 3: y = x + omega dinv (lhs - A x), a damped
+// Jacobi sweep into another vector (y aliases neither x nor lhs; the same
+// roundings as the residual product followed by k_jacobi).
 // IT = int16_t reads the columns as 16-bit offsets from the tile's first row
 // (bcol, square matrices whose offsets fit): 2 bytes less per nonzero.
 // T = double: the FP64 values; T = float: the FP32 copy (val32) with FP32
@@ -375,7 +377,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_
                                                  const int16_t *__restrict__ bcol,
                                                  const T *__restrict__ val,
                                                  const T *__restrict__ x,
-                                                 const T *lhs, T *y) {
+                                                 const T *lhs, T *y,
+                                                 const T *__restrict__ dinv, T omega) {
     constexpr bool F64 = std::is_same<T, double>::value;
     __shared__ __align__(16) T prod[kTileNnz];
     __shared__ T red[kWaves];
@@ -489,6 +492,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_
             if (MODE == 0) y[row] = s;
             if (MODE == 1) y[row] = lhs[row] - s;
             if (MODE == 2) y[row] = s - lhs[row];
+            if (MODE == 3) {
+                const T r = lhs[row] - s;
+                y[row] = __builtin_fma(omega * dinv[row], r, x[row]);
+            }
         }
     } else {  // one row longer than a tile
         T s = T(0);
@@ -502,6 +509,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_
             if (MODE == 0) y[r0] = tot;
             if (MODE == 1) y[r0] = lhs[r0] - tot;
             if (MODE == 2) y[r0] = tot - lhs[r0];
+            if (MODE == 3) {
+                const T r = lhs[r0] - tot;
+                y[r0] = __builtin_fma(omega * dinv[r0], r, x[r0]);
+            }
         }
     }
 }
@@ -571,6 +582,15 @@ __global__ void k_jacobi(int n, T w, const T *__restrict__ dinv, const T *__rest
         x[i] = __builtin_fma(w * dinv[i], r[i], x[i]);
 }
 
+// x = w * dinv .* b (the damped Jacobi sweep from x = 0)
+template <typename T>
+__global__ void k_jacobi0(int n, T w, const T *__restrict__ dinv, const T *__restrict__ b,
+                          T *__restrict__ x) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (w * dinv[i]) * b[i];
+}
+
 // y = a rounded to (or widened from) y's type
 template <typename A, typename Y>
 __global__ void k_convert(int n, const A *__restrict__ a, Y *__restrict__ y) {
@@ -612,18 +632,20 @@ const float *values<float>(Matrix &m) {
 }
 
 template <int MODE, typename IT, typename T>
-void spmv_launch(Device &d, Matrix &m, const T *x, const T *lhs, T *y) {
+void spmv_launch(Device &d, Matrix &m, const T *x, const T *lhs, T *y, const T *dinv,
+                 T omega) {
     k_spmv<MODE, IT, T><<<m.n_tiles, kBlock, 0, d.stream>>>(
         m.n_tiles, m.rows, m.nnz, m.tiles.ptr, m.row_start.ptr, m.col.ptr, m.bcol.ptr,
-        values<T>(m), x, lhs, y);
+        values<T>(m), x, lhs, y, dinv, omega);
 }
 
 template <int MODE, typename T>
-void spmv_launch(Device &d, Matrix &m, const T *x, const T *lhs, T *y) {
+void spmv_launch(Device &d, Matrix &m, const T *x, const T *lhs, T *y,
+                 const T *dinv = nullptr, T omega = T(0)) {
     if (m.bcol_ok && want_idx16())
-        spmv_launch<MODE, int16_t>(d, m, x, lhs, y);
+        spmv_launch<MODE, int16_t>(d, m, x, lhs, y, dinv, omega);
     else
-        spmv_launch<MODE, int32_t>(d, m, x, lhs, y);
+        spmv_launch<MODE, int32_t>(d, m, x, lhs, y, dinv, omega);
 }
 
 template <typename T>
@@ -650,6 +672,21 @@ void launch_spmv(Device &d, Matrix &m, const float *x, const float *lhs, float *
     NSFEM_REQUIRE(m.rows == 0 || m.val32.ptr, NSFEM_NOT_READY,
                   "FP32 product without the FP32 copy of the values");
     spmv_any(d, m, x, lhs, y, mode);
+}
+
+void launch_sweep(Device &d, Matrix &m, double w, const double *dinv, const double *x,
+                  const double *b, double *y) {
+    if (m.rows == 0) return;
+    spmv_launch<3>(d, m, x, b, y, dinv, w);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_sweep(Device &d, Matrix &m, double w, const float *dinv, const float *x,
+                  const float *b, float *y) {
+    if (m.rows == 0) return;
+    NSFEM_REQUIRE(m.val32.ptr, NSFEM_NOT_READY, "FP32 sweep without the FP32 copy of the values");
+    spmv_launch<3>(d, m, x, b, y, dinv, (float)w);
+    NSFEM_HIP(hipGetLastError());
 }
 
 void launch_spmv_affine(Device &d, Matrix &m, const double *x, const double *c, double *y) {
@@ -722,6 +759,18 @@ void launch_jacobi(Device &d, int n, double w, const double *dinv, const double 
 void launch_jacobi(Device &d, int n, double w, const float *dinv, const float *r, float *x) {
     if (n <= 0) return;
     k_jacobi<<<ew_grid(n), kBlock, 0, d.stream>>>(n, (float)w, dinv, r, x);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_jacobi0(Device &d, int n, double w, const double *dinv, const double *b, double *x) {
+    if (n <= 0) return;
+    k_jacobi0<<<ew_grid(n), kBlock, 0, d.stream>>>(n, w, dinv, b, x);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_jacobi0(Device &d, int n, double w, const float *dinv, const float *b, float *x) {
+    if (n <= 0) return;
+    k_jacobi0<<<ew_grid(n), kBlock, 0, d.stream>>>(n, (float)w, dinv, b, x);
     NSFEM_HIP(hipGetLastError());
 }
 

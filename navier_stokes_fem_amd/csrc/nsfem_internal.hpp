@@ -316,6 +316,14 @@ void launch_add(Device &d, int n, const float *a, float *y);
 // x += w * dinv .* r
 void launch_jacobi(Device &d, int n, double w, const double *dinv, const double *r, double *x);
 void launch_jacobi(Device &d, int n, double w, const float *dinv, const float *r, float *x);
+// x = w * dinv .* b (the sweep from x = 0)
+void launch_jacobi0(Device &d, int n, double w, const double *dinv, const double *b, double *x);
+void launch_jacobi0(Device &d, int n, double w, const float *dinv, const float *b, float *x);
+// y = x + w * dinv .* (b - A x) in one product (y aliases neither x nor b)
+void launch_sweep(Device &d, Matrix &m, double w, const double *dinv, const double *x,
+                  const double *b, double *y);
+void launch_sweep(Device &d, Matrix &m, double w, const float *dinv, const float *x,
+                  const float *b, float *y);
 void launch_fill(Device &d, int n, double v, float *y);
 // y = a rounded to FP32 / widened to FP64
 void launch_convert(Device &d, int n, const double *a, float *y);

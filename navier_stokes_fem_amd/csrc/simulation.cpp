@@ -731,9 +731,13 @@ constexpr int kMgMaxCycles = 30;
 
 // One V-cycle on A x = b over a level (A of n rows, its 1/diag, a residual
 // vector r) and the levels below it, m.lv[l] the next coarser, from x = 0
-// (zero: x is cleared here and the first sweep needs no product) or from
-// the x given: kMgSweeps damped Jacobi sweeps, the correction from the level
-// below on the restricted residual, as many sweeps after.  The deepest level
+// (zero: the first sweep needs no product) or from the x given: `sweeps`
+// damped Jacobi sweeps, the correction from the level below on the
+// restricted residual, as many sweeps after.  Each sweep with a product is
+// one fused launch (launch_sweep) from one of x and r into the other, so the
+// iterate alternates between them; the sweep from zero writes r, so that the
+// 2 sweeps - 1 moves after it end in x (from a given x: 2 sweeps moves).
+// The deepest level
 // (l = m.lv.size()) is solved by its dense inverse when it has one, else by
 // FP64 CG to 1e-10 from zero.  T = float runs the whole cycle in FP32 (the
 // FP32 copies of every level's values and vectors, ensure_fp32_levels; it
@@ -746,7 +750,7 @@ void mg_cycle(Device &d, Matrix &A, int n, const T *dinv, T *r, MgLevels &m, siz
         launch_spmv(d, m.deep_inv, b, nullptr, x, 0);
         return;
     }
-    if (zero || l == m.lv.size()) launch_fill(d, n, 0.0, x);
+    if (l == m.lv.size() || (zero && sweeps == 0)) launch_fill(d, n, 0.0, x);
     if (l == m.lv.size()) {
         if constexpr (std::is_same<T, double>::value) {
             CgRequest rq{};
@@ -764,25 +768,28 @@ void mg_cycle(Device &d, Matrix &A, int n, const T *dinv, T *r, MgLevels &m, siz
         }
         return;
     }
-    for (int k = 0; k < sweeps; ++k) {
-        if (k == 0 && zero) {
-            launch_jacobi(d, n, omega, dinv, b, x);
-            continue;
-        }
-        launch_spmv(d, A, x, b, r, 1);
-        launch_jacobi(d, n, omega, dinv, r, x);
+    // the iterate in cur, the other vector free; a fused sweep moves it across
+    T *cur = x, *spare = r;
+    auto sweep = [&] {
+        launch_sweep(d, A, omega, dinv, cur, b, spare);
+        std::swap(cur, spare);
+    };
+    if (zero && sweeps > 0) {
+        std::swap(cur, spare);
+        launch_jacobi0(d, n, omega, dinv, b, cur);
+        for (int k = 1; k < sweeps; ++k) sweep();
+    } else {
+        for (int k = 0; k < sweeps; ++k) sweep();
     }
-    launch_spmv(d, A, x, b, r, 1);
+    launch_spmv(d, A, cur, b, spare, 1);
     MgLevel &below = *m.lv[l];
     const LevelVectors<T> v(below);
-    launch_spmv(d, below.PT, r, nullptr, v.b, 0);
+    launch_spmv(d, below.PT, spare, nullptr, v.b, 0);
     mg_cycle<T>(d, below.A, below.n, v.dinv, v.r, m, l + 1, v.b, v.x);
-    launch_spmv(d, below.P, v.x, nullptr, r, 0);
-    launch_add(d, n, r, x);
-    for (int k = 0; k < sweeps; ++k) {
-        launch_spmv(d, A, x, b, r, 1);
-        launch_jacobi(d, n, omega, dinv, r, x);
-    }
+    launch_spmv(d, below.P, v.x, nullptr, spare, 0);
+    launch_add(d, n, spare, cur);
+    for (int k = 0; k < sweeps; ++k) sweep();
+    if (cur != x) fail(NSFEM_ERROR, "V-cycle: the sweeps did not end in the iterate's vector");
 }
 
 // The FP32 copies of m's levels (values of A, P, P^T and the dense deepest
