@@ -554,14 +554,64 @@ __global__ __launch_bounds__(kBlock) void k_dot_final(int nb, const double *__re
     if (threadIdx.x == 0) result[0] += s[0];
 }
 
+// The update of a CG with an FP32 preconditioner: x += a p, r -= a q, r32 =
+// r rounded to FP32 (the preconditioner's input), and the partial sums of
+// (r, r) per contiguous chunk in k_dot_partial's order (k_dot_final adds
+// them).
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int chunk, double a,
+                                                       const double *__restrict__ p,
+                                                       const double *__restrict__ q,
+                                                       double *__restrict__ x,
+                                                       double *__restrict__ r,
+                                                       float *__restrict__ r32,
+                                                       double *__restrict__ partial) {
+    __shared__ double lds[kWaves];
+    const int lo = blockIdx.x * chunk;
+    const int hi = min(n, lo + chunk);
+    double s[1] = {0.0};
+    for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
+        x[i] = __builtin_fma(a, p[i], x[i]);
+        const double ri = __builtin_fma(-a, q[i], r[i]);
+        r[i] = ri;
+        r32[i] = (float)ri;
+        s[0] = __builtin_fma(ri, ri, s[0]);
+    }
+    block_sum<1, kBlock>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// Partial sums of (r, z) with z in FP32, widened, per chunk.
+__global__ __launch_bounds__(kBlock) void k_dot_mixed_partial(int n, int chunk,
+                                                              const double *__restrict__ r,
+                                                              const float *__restrict__ z,
+                                                              double *__restrict__ partial) {
+    __shared__ double lds[kWaves];
+    const int lo = blockIdx.x * chunk;
+    const int hi = min(n, lo + chunk);
+    double s[1] = {0.0};
+    for (int i = lo + threadIdx.x; i < hi; i += kBlock)
+        s[0] = __builtin_fma(r[i], (double)z[i], s[0]);
+    block_sum<1, kBlock>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// p = z + b p with z in FP32 (the CG's new direction)
+__global__ void k_pcg_direction(int n, const float *__restrict__ z, double b,
+                                double *__restrict__ p) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = __builtin_fma(b, p[i], (double)z[i]);
+}
+
 __global__ void k_scatter(int count, const int32_t *__restrict__ idx,
                           const double *__restrict__ val, double *__restrict__ dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) dst[idx[i]] = val[i];
 }
 
-__global__ void k_scatter_value(int count, const int32_t *__restrict__ idx, double v,
-                                double *__restrict__ dst) {
+template <typename T>
+__global__ void k_scatter_value(int count, const int32_t *__restrict__ idx, T v,
+                                T *__restrict__ dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) dst[idx[i]] = v;
 }
@@ -726,6 +776,42 @@ void launch_dot_accumulate(Device &d, int n, const double *a, const double *b,
     dot_accumulate(d, n, a, b, result, d.dot_partial.ptr);
 }
 
+namespace {
+// blocks and chunk of the deterministic dot over n entries
+void dot_shape(int n, int &nb, int &chunk) {
+    nb = std::max(1, std::min((n + 4095) / 4096, kDotBlocks));
+    chunk = std::max(1, (n + nb - 1) / nb);
+}
+}  // namespace
+
+void launch_pcg_update(Device &d, int n, double a, const double *p, const double *q, double *x,
+                       double *r, float *r32, double *rr) {
+    if (n <= 0) return;
+    d.dot_partial.ensure(kDotBlocks);
+    int nb, chunk;
+    dot_shape(n, nb, chunk);
+    k_pcg_update<<<nb, kBlock, 0, d.stream>>>(n, chunk, a, p, q, x, r, r32, d.dot_partial.ptr);
+    k_dot_final<<<1, kBlock, 0, d.stream>>>(nb, d.dot_partial.ptr, rr);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_dot_mixed_accumulate(Device &d, int n, const double *r, const float *z,
+                                 double *result) {
+    if (n <= 0) return;
+    d.dot_partial.ensure(kDotBlocks);
+    int nb, chunk;
+    dot_shape(n, nb, chunk);
+    k_dot_mixed_partial<<<nb, kBlock, 0, d.stream>>>(n, chunk, r, z, d.dot_partial.ptr);
+    k_dot_final<<<1, kBlock, 0, d.stream>>>(nb, d.dot_partial.ptr, result);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_pcg_direction(Device &d, int n, const float *z, double b, double *p) {
+    if (n <= 0) return;
+    k_pcg_direction<<<ew_grid(n), kBlock, 0, d.stream>>>(n, z, b, p);
+    NSFEM_HIP(hipGetLastError());
+}
+
 void launch_scatter(Device &d, int count, const int32_t *idx, const double *val, double *dst) {
     if (count <= 0) return;
     k_scatter<<<(count + kBlock - 1) / kBlock, kBlock, 0, d.stream>>>(count, idx, val, dst);
@@ -735,6 +821,13 @@ void launch_scatter(Device &d, int count, const int32_t *idx, const double *val,
 void launch_scatter_value(Device &d, int count, const int32_t *idx, double v, double *dst) {
     if (count <= 0) return;
     k_scatter_value<<<(count + kBlock - 1) / kBlock, kBlock, 0, d.stream>>>(count, idx, v, dst);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_scatter_value(Device &d, int count, const int32_t *idx, double v, float *dst) {
+    if (count <= 0) return;
+    k_scatter_value<<<(count + kBlock - 1) / kBlock, kBlock, 0, d.stream>>>(count, idx, (float)v,
+                                                                            dst);
     NSFEM_HIP(hipGetLastError());
 }
 

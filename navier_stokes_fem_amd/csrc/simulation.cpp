@@ -913,27 +913,38 @@ void coarse_correct(Simulation &s) {
 // level solved exactly (or by CG to 1e-10) keep M symmetric, as
 // preconditioned CG needs.  z is zeroed on the pressure Dirichlet rows
 // (D M D with D the projection off them stays symmetric; the solve holds
-// those rows at their values).  In the refined precision the cycle runs in
-// FP32 when the deepest level has its dense inverse (r rounded to FP32, z
-// widened; the FP64 CG around it keeps its FP64 stopping rule).
+// those rows at their values).
 void fine_vcycle(Simulation &s, const double *r, double *z) {
     Device &d = *s.dev;
-    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+    mg_cycle<double>(d, s.mats[NSFEM_PRESSURE_STIFFNESS], s.P, s.pdinv.ptr, s.pt.ptr, s.mg, 0, r,
+                     z);
+    launch_scatter_value(d, s.npb, s.pb_idx.ptr, 0.0, z);
+}
+
+// In the refined precision, when the deepest level has its dense inverse,
+// the cycle runs in FP32 (pcg_run32 hands it r rounded to FP32 and uses z
+// widened; the FP64 CG keeps its FP64 stopping rule).
+bool pressure_cycle_fp32(const Simulation &s) {
+    return s.opt.precision == NSFEM_PRECISION_FP32_REFINED && s.mg.deep_ok;
+}
+
+// The FP32 copies of the pressure hierarchy and the fine level's FP32
+// vectors (r32 in, z32 out, the sweeps' second vector), built once.
+void ensure_pressure_fp32(Simulation &s) {
+    if (s.mg.fp32_ok && s.pdinv32.ptr) return;
+    Device &d = *s.dev;
     const int n = s.P;
-    if (s.opt.precision == NSFEM_PRECISION_FP32_REFINED && s.mg.deep_ok) {
-        if (!s.mg.fp32_ok) {
-            ensure_fp32_levels(d, s.mg);
-            ensure_val32(d, A);
-            s.pdinv32.resize((size_t)n);
-            launch_convert(d, n, s.pdinv.ptr, s.pdinv32.ptr);
-            for (DeviceArray<float> *a : {&s.pb32, &s.pz32, &s.pt32}) a->resize((size_t)n);
-        }
-        launch_convert(d, n, r, s.pb32.ptr);
-        mg_cycle<float>(d, A, n, s.pdinv32.ptr, s.pt32.ptr, s.mg, 0, s.pb32.ptr, s.pz32.ptr);
-        launch_convert(d, n, s.pz32.ptr, z);
-    } else {
-        mg_cycle<double>(d, A, n, s.pdinv.ptr, s.pt.ptr, s.mg, 0, r, z);
-    }
+    ensure_fp32_levels(d, s.mg);
+    ensure_val32(d, s.mats[NSFEM_PRESSURE_STIFFNESS]);
+    s.pdinv32.resize((size_t)n);
+    launch_convert(d, n, s.pdinv.ptr, s.pdinv32.ptr);
+    for (DeviceArray<float> *a : {&s.pb32, &s.pz32, &s.pt32}) a->resize((size_t)n);
+}
+
+void fine_vcycle32(Simulation &s, const float *r, float *z) {
+    Device &d = *s.dev;
+    mg_cycle<float>(d, s.mats[NSFEM_PRESSURE_STIFFNESS], s.P, s.pdinv32.ptr, s.pt32.ptr, s.mg, 0,
+                    r, z);
     launch_scatter_value(d, s.npb, s.pb_idx.ptr, 0.0, z);
 }
 
@@ -1010,6 +1021,79 @@ int pcg_run(Device &d, Matrix &A, int n, const double *b, double *x, const PcgWo
     return status;
 }
 
+// Reads a device scalar accumulated by the dot launchers (one host
+// synchronisation) and clears it for the next.
+double take_scalar(Device &d, double *dot) {
+    double h = 0.0;
+    NSFEM_HIP(hipMemcpyAsync(&h, dot, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    NSFEM_HIP(hipStreamSynchronize(d.stream));
+    launch_fill(d, 1, 0.0, dot);
+    return h;
+}
+
+// pcg_run with an FP32 preconditioner z32 = precond(r32): the update writes
+// r rounded to FP32 beside r and returns (r, r) from the same pass, and z
+// is used widened (in (r, z) and p = z + beta p), never stored in FP64.
+// The stopping rule is pcg_run's.
+template <class Pre>
+int pcg_run32(Device &d, Matrix &A, int n, const double *b, double *x, const PcgWork &w,
+              float *r32, float *z32, double rtol, int64_t max_iterations, Pre precond,
+              SolveOut &o, int c) {
+    const auto t0 = std::chrono::steady_clock::now();
+    double *r = w.r, *p = w.p, *q = w.q;
+    const double bb = vec_dot(d, n, b, b, w.dot);
+    launch_spmv(d, A, x, b, r, 1);
+    double rr = vec_dot(d, n, r, r, w.dot);
+    const double ref = bb > 0.0 ? bb : rr;
+    const double tol2 = rtol * rtol * ref;
+    const int64_t max_iter = max_iterations > 0 ? max_iterations : n;
+    int64_t k = 0;
+    int status = NSFEM_OK;
+    if (rr > tol2) {
+        status = NSFEM_MAX_ITERATIONS_REACHED;
+        launch_fill(d, 1, 0.0, w.dot);
+        launch_convert(d, n, r, r32);
+        precond(r32, z32);
+        launch_convert(d, n, z32, p);
+        launch_dot_mixed_accumulate(d, n, r, z32, w.dot);
+        double rz = take_scalar(d, w.dot);
+        while (k < max_iter) {
+            launch_spmv(d, A, p, nullptr, q, 0);
+            launch_dot_accumulate(d, n, p, q, w.dot);
+            const double pq = take_scalar(d, w.dot);
+            if (!(pq > 0.0) || !(rz > 0.0)) {
+                status = NSFEM_BREAKDOWN;
+                break;
+            }
+            const double alpha = rz / pq;
+            launch_pcg_update(d, n, alpha, p, q, x, r, r32, w.dot);
+            rr = take_scalar(d, w.dot);
+            ++k;
+            if (rr <= tol2) {
+                launch_spmv(d, A, x, b, r, 1);
+                launch_dot_accumulate(d, n, r, r, w.dot);
+                rr = take_scalar(d, w.dot);
+                if (rr <= tol2) {
+                    status = NSFEM_OK;
+                    break;
+                }
+                launch_convert(d, n, r, r32);
+            }
+            precond(r32, z32);
+            launch_dot_mixed_accumulate(d, n, r, z32, w.dot);
+            const double rz_new = take_scalar(d, w.dot);
+            const double beta = rz_new / rz;
+            rz = rz_new;
+            launch_pcg_direction(d, n, z32, beta, p);
+        }
+    }
+    o.iters[c] = k;
+    o.refines[c] = 0;
+    o.rel[c] = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    o.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return status;
+}
+
 // The pressure solve as CG preconditioned by fine_vcycle from the guess in
 // p.  The Dirichlet rows (identity rows) start at their values, so their
 // residual is zero and z, p stay zero there.
@@ -1017,10 +1101,17 @@ void pcg_pressure(Simulation &s, SolveOut &o) {
     Device &d = *s.dev;
     launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.p.ptr);
     const PcgWork w{s.pr.ptr, s.pz.ptr, s.pp.ptr, s.pq.ptr, s.pdot.ptr};
-    const int status =
-        pcg_run(d, s.mats[NSFEM_PRESSURE_STIFFNESS], s.P, s.prhs.ptr, s.p.ptr, w, s.opt.rtol,
-                s.opt.max_iterations, [&](const double *r, double *z) { fine_vcycle(s, r, z); },
-                o, 0);
+    Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
+    int status;
+    if (pressure_cycle_fp32(s)) {
+        ensure_pressure_fp32(s);
+        status = pcg_run32(d, A, s.P, s.prhs.ptr, s.p.ptr, w, s.pb32.ptr, s.pz32.ptr, s.opt.rtol,
+                           s.opt.max_iterations,
+                           [&](const float *r, float *z) { fine_vcycle32(s, r, z); }, o, 0);
+    } else {
+        status = pcg_run(d, A, s.P, s.prhs.ptr, s.p.ptr, w, s.opt.rtol, s.opt.max_iterations,
+                         [&](const double *r, double *z) { fine_vcycle(s, r, z); }, o, 0);
+    }
     if (status != NSFEM_OK)
         fail(status, "V-cycle preconditioned CG on the pressure stiffness matrix did not "
                      "converge (status " + std::to_string(status) + ")");
@@ -1031,9 +1122,9 @@ void pcg_pressure(Simulation &s, SolveOut &o) {
 // on the velocity Dirichlet rows (whose residual is zero: identity rows,
 // their columns eliminated, the solve starting at the boundary values; the
 // prolongation's rows there are empty).  Symmetric as fine_vcycle.  In the
-// refined precision the cycle runs in FP32 (r rounded to FP32, z widened):
-// the products and sweeps move about half the bytes, and the FP64 CG
-// around it still decides convergence on its FP64 residual.
+// refined precision the cycle runs in FP32 (velocity_vcycle32 under
+// pcg_run32): the products and sweeps move about half the bytes, and the
+// FP64 CG around it still decides convergence on its FP64 residual.
 // Jacobi weight of the sweeps on the P2 velocity nodes (the levels below
 // keep kMgOmega): D^-1 A of the diffusion matrix reaches ~2.2 (at weight 0.9
 // the 2048^2 cavity's v column needs 17 iterations instead of 6); measured
@@ -1048,24 +1139,28 @@ bool velocity_cycle_fp32(const Simulation &s) {
 
 void velocity_vcycle(Simulation &s, const double *r, double *z) {
     Device &d = *s.dev;
-    Matrix &A = s.mats[NSFEM_DIFFUSION];
+    mg_cycle<double>(d, s.mats[NSFEM_DIFFUSION], s.V, s.vdinv.ptr, s.vt.ptr, s.vmg, 0, r, z, true,
+                     kMgSweeps, kVmgFineOmega);
+    launch_scatter_value(d, s.nvb, s.vb_idx.ptr, 0.0, z);
+}
+
+// The FP32 copies of the velocity hierarchy and the P2 level's FP32 vectors
+// (r32 in, z32 out, the sweeps' second vector), built once.
+void ensure_velocity_fp32(Simulation &s) {
+    if (s.vmg.fp32_ok && s.vdinv32.ptr) return;
+    Device &d = *s.dev;
     const int V = s.V;
-    if (velocity_cycle_fp32(s)) {
-        if (!s.vmg.fp32_ok) {
-            ensure_fp32_levels(d, s.vmg);
-            ensure_val32(d, A);
-            s.vdinv32.resize((size_t)V);
-            launch_convert(d, V, s.vdinv.ptr, s.vdinv32.ptr);
-            for (DeviceArray<float> *a : {&s.vb32, &s.vz32, &s.vt32}) a->resize((size_t)V);
-        }
-        launch_convert(d, V, r, s.vb32.ptr);
-        mg_cycle<float>(d, A, V, s.vdinv32.ptr, s.vt32.ptr, s.vmg, 0, s.vb32.ptr, s.vz32.ptr, true,
-                        kMgSweeps, kVmgFineOmega);
-        launch_convert(d, V, s.vz32.ptr, z);
-    } else {
-        mg_cycle<double>(d, A, V, s.vdinv.ptr, s.vt.ptr, s.vmg, 0, r, z, true, kMgSweeps,
-                         kVmgFineOmega);
-    }
+    ensure_fp32_levels(d, s.vmg);
+    ensure_val32(d, s.mats[NSFEM_DIFFUSION]);
+    s.vdinv32.resize((size_t)V);
+    launch_convert(d, V, s.vdinv.ptr, s.vdinv32.ptr);
+    for (DeviceArray<float> *a : {&s.vb32, &s.vz32, &s.vt32}) a->resize((size_t)V);
+}
+
+void velocity_vcycle32(Simulation &s, const float *r, float *z) {
+    Device &d = *s.dev;
+    mg_cycle<float>(d, s.mats[NSFEM_DIFFUSION], s.V, s.vdinv32.ptr, s.vt32.ptr, s.vmg, 0, r, z,
+                    true, kMgSweeps, kVmgFineOmega);
     launch_scatter_value(d, s.nvb, s.vb_idx.ptr, 0.0, z);
 }
 
@@ -1077,9 +1172,17 @@ void pcg_velocity(Simulation &s, int c, SolveOut &o) {
     double *x = s.uv.ptr + off;
     launch_scatter(d, s.nvb, s.vb_idx.ptr, c ? s.vb_v.ptr : s.vb_u.ptr, x);
     const PcgWork w{s.vr.ptr, s.vz.ptr, s.vp.ptr, s.vq.ptr, s.vdot.ptr};
-    const int status = pcg_run(
-        d, s.mats[NSFEM_DIFFUSION], s.V, s.rhs.ptr + off, x, w, s.opt.rtol, s.opt.max_iterations,
-        [&](const double *r, double *z) { velocity_vcycle(s, r, z); }, o, 0);
+    Matrix &A = s.mats[NSFEM_DIFFUSION];
+    int status;
+    if (velocity_cycle_fp32(s)) {
+        ensure_velocity_fp32(s);
+        status = pcg_run32(d, A, s.V, s.rhs.ptr + off, x, w, s.vb32.ptr, s.vz32.ptr, s.opt.rtol,
+                           s.opt.max_iterations,
+                           [&](const float *r, float *z) { velocity_vcycle32(s, r, z); }, o, 0);
+    } else {
+        status = pcg_run(d, A, s.V, s.rhs.ptr + off, x, w, s.opt.rtol, s.opt.max_iterations,
+                         [&](const double *r, double *z) { velocity_vcycle(s, r, z); }, o, 0);
+    }
     if (status != NSFEM_OK)
         fail(status, "V-cycle preconditioned CG on the diffusion matrix did not converge "
                      "(status " + std::to_string(status) + ")");
