@@ -559,10 +559,13 @@ void projection_guess(Device &d, ProjBasis &B, const double *b, double *x) {
 }
 
 // After the solve x: the correction dx = x - guess (the whole solution after
-// a restart), A-orthogonalised against X by Gram-Schmidt applied twice, is
-// normalised in the A-norm and appended; a full basis restarts from the
-// latest solution.  A correction that the basis already spans (A-norm below
-// 1e-12 of its own before orthogonalisation) is dropped.
+// a restart), A-orthogonalised against X by classical Gram-Schmidt, a
+// second pass only when the first cancelled more than half of its A-norm
+// squared ("twice is enough": after a pass that keeps more, the loss of
+// A-orthogonality is of the order of the rounding), is normalised in the
+// A-norm and appended; a full basis restarts from the latest solution.  A
+// correction that the basis already spans (A-norm below 1e-12 of its own
+// before orthogonalisation) is dropped.
 void projection_update(Device &d, ProjBasis &B, Matrix &A, const double *x) {
     const int n = B.n;
     if (B.dim == B.max) B.dim = 0;
@@ -576,14 +579,17 @@ void projection_update(Device &d, ProjBasis &B, Matrix &A, const double *x) {
     const double n0 = dev_dot(d, B, dx, w);
     if (!(n0 > 0.0)) return;
     std::vector<double> c(std::max(l, 1));
+    double n1 = n0, before = n0;
     for (int pass = 0; pass < 2 && l > 0; ++pass) {
         proj_dots(d, B, l, w, c.data());  // X_i . A dx = (A X_i) . dx
         for (int i = 0; i < l; ++i) {
             launch_axpy(d, n, -c[i], B.x.ptr + (size_t)i * n, dx);
             launch_axpy(d, n, -c[i], B.ax.ptr + (size_t)i * n, w);
         }
+        n1 = dev_dot(d, B, dx, w);
+        if (n1 > 0.5 * before) break;
+        before = n1;
     }
-    const double n1 = dev_dot(d, B, dx, w);
     if (!(n1 > 1e-12 * n0)) return;
     const double inv = 1.0 / std::sqrt(n1);
     launch_axpby(d, n, inv, 0.0, dx, dx, B.x.ptr + (size_t)l * n);
