@@ -158,6 +158,10 @@ struct Simulation {
     bool pmg = false;
     DeviceArray<double> pdinv, pr, pz, pp, pq, pt, pdot;
     DeviceArray<float> pdinv32, pb32, pz32, pt32;
+    // the FP32 cycles of the pressure and the diffusion solves (fixed
+    // kernels on fixed vectors), captured in a HIP graph on their first run
+    // and replayed; dropped with their hierarchies
+    hipGraphExec_t pcycle_graph = nullptr, vcycle_graph = nullptr;
     // diffusion CG preconditioned by one V-cycle over the velocity nodes,
     // the velocity coarse space (vmg.lv[0]) and the levels below it: mode
     // (0 off, 1 per column by measured cost, 2 every column), 1/diag of the
@@ -696,8 +700,11 @@ void galerkin_product(int n, const std::vector<int32_t> &ars, const std::vector<
     }
 }
 
+void drop_graph(hipGraphExec_t &g);
+
 void release_pressure_multigrid(Simulation &s) {
     s.pmg = false;
+    drop_graph(s.pcycle_graph);
     for (DeviceArray<float> *a : {&s.pdinv32, &s.pb32, &s.pz32, &s.pt32}) a->release();
     s.pdinv.release();
     s.pr.release();
@@ -913,6 +920,44 @@ void coarse_correct(Simulation &s) {
     coarse_finish(s, status, st.iterations, st.seconds);
 }
 
+void drop_graph(hipGraphExec_t &g) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+}
+
+// Runs `launches` on the device's stream through a graph captured on the
+// first call and replayed after it: the launches must be the same kernels
+// with the same arguments every call, and allocate, copy, clear memory and
+// synchronise nothing (an FP32 cycle over a fixed hierarchy and fixed
+// vectors: products, sweeps, transfers and the dense deepest solve).  A
+// graph replaces ~100 launches of the cycle, most of them on levels so
+// small that their launch latency is their cost.  NSFEM_VCYCLE_GRAPH=0
+// runs the launches directly (read at each call).
+template <class F>
+void run_captured(Device &d, hipGraphExec_t &exec, F launches) {
+    const char *e = std::getenv("NSFEM_VCYCLE_GRAPH");
+    if (e && std::atoi(e) == 0) {
+        launches();
+        return;
+    }
+    if (!exec) {
+        NSFEM_HIP(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            launches();
+        } catch (...) {
+            if (hipStreamEndCapture(d.stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        NSFEM_HIP(hipStreamEndCapture(d.stream, &g));
+        const hipError_t err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (err != hipSuccess) exec = nullptr;
+        NSFEM_HIP(err);
+    }
+    NSFEM_HIP(hipGraphLaunch(exec, d.stream));
+}
+
 // z = M r: one V-cycle from zero over the fine pressure grid (damped Jacobi
 // as on the coarse levels), the coarse space and its levels.  As many sweeps
 // after as before, restriction = transposed prolongation and the deepest
@@ -947,11 +992,14 @@ void ensure_pressure_fp32(Simulation &s) {
     for (DeviceArray<float> *a : {&s.pb32, &s.pz32, &s.pt32}) a->resize((size_t)n);
 }
 
-void fine_vcycle32(Simulation &s, const float *r, float *z) {
+// z = M r in FP32 on the fixed vectors r = pb32, z = pz32 (captured)
+void fine_vcycle32(Simulation &s) {
     Device &d = *s.dev;
-    mg_cycle<float>(d, s.mats[NSFEM_PRESSURE_STIFFNESS], s.P, s.pdinv32.ptr, s.pt32.ptr, s.mg, 0,
-                    r, z);
-    launch_scatter_value(d, s.npb, s.pb_idx.ptr, 0.0, z);
+    run_captured(d, s.pcycle_graph, [&] {
+        mg_cycle<float>(d, s.mats[NSFEM_PRESSURE_STIFFNESS], s.P, s.pdinv32.ptr, s.pt32.ptr, s.mg,
+                        0, s.pb32.ptr, s.pz32.ptr);
+        launch_scatter_value(d, s.npb, s.pb_idx.ptr, 0.0, s.pz32.ptr);
+    });
 }
 
 // a . b over n entries, through a one-entry device scratch (one host
@@ -1113,7 +1161,7 @@ void pcg_pressure(Simulation &s, SolveOut &o) {
         ensure_pressure_fp32(s);
         status = pcg_run32(d, A, s.P, s.prhs.ptr, s.p.ptr, w, s.pb32.ptr, s.pz32.ptr, s.opt.rtol,
                            s.opt.max_iterations,
-                           [&](const float *r, float *z) { fine_vcycle32(s, r, z); }, o, 0);
+                           [&](const float *, float *) { fine_vcycle32(s); }, o, 0);
     } else {
         status = pcg_run(d, A, s.P, s.prhs.ptr, s.p.ptr, w, s.opt.rtol, s.opt.max_iterations,
                          [&](const double *r, double *z) { fine_vcycle(s, r, z); }, o, 0);
@@ -1163,11 +1211,14 @@ void ensure_velocity_fp32(Simulation &s) {
     for (DeviceArray<float> *a : {&s.vb32, &s.vz32, &s.vt32}) a->resize((size_t)V);
 }
 
-void velocity_vcycle32(Simulation &s, const float *r, float *z) {
+// z = M r in FP32 on the fixed vectors r = vb32, z = vz32 (captured)
+void velocity_vcycle32(Simulation &s) {
     Device &d = *s.dev;
-    mg_cycle<float>(d, s.mats[NSFEM_DIFFUSION], s.V, s.vdinv32.ptr, s.vt32.ptr, s.vmg, 0, r, z,
-                    true, kMgSweeps, kVmgFineOmega);
-    launch_scatter_value(d, s.nvb, s.vb_idx.ptr, 0.0, z);
+    run_captured(d, s.vcycle_graph, [&] {
+        mg_cycle<float>(d, s.mats[NSFEM_DIFFUSION], s.V, s.vdinv32.ptr, s.vt32.ptr, s.vmg, 0,
+                        s.vb32.ptr, s.vz32.ptr, true, kMgSweeps, kVmgFineOmega);
+        launch_scatter_value(d, s.nvb, s.vb_idx.ptr, 0.0, s.vz32.ptr);
+    });
 }
 
 // Column c (0: u, 1: v) of the diffusion solve by CG preconditioned with
@@ -1184,7 +1235,7 @@ void pcg_velocity(Simulation &s, int c, SolveOut &o) {
         ensure_velocity_fp32(s);
         status = pcg_run32(d, A, s.V, s.rhs.ptr + off, x, w, s.vb32.ptr, s.vz32.ptr, s.opt.rtol,
                            s.opt.max_iterations,
-                           [&](const float *r, float *z) { velocity_vcycle32(s, r, z); }, o, 0);
+                           [&](const float *, float *) { velocity_vcycle32(s); }, o, 0);
     } else {
         status = pcg_run(d, A, s.V, s.rhs.ptr + off, x, w, s.opt.rtol, s.opt.max_iterations,
                          [&](const double *r, double *z) { velocity_vcycle(s, r, z); }, o, 0);
@@ -1364,6 +1415,7 @@ void simulation_add_coarse_level(Simulation &s, int count, int64_t nnz, const in
     NSFEM_REQUIRE(count > 0 && count < above && nnz > 0 && rs && col && val,
                   NSFEM_INVALID_ARGUMENT,
                   "coarse level: need 0 < count < the level above's size and a non-empty P");
+    drop_graph(s.pcycle_graph);
     append_level(d, s.mg, above, s.mg.deep_rs, s.mg.deep_col, s.mg.deep_val, count, nnz, rs, col,
                  val, host_threads(s));
 }
@@ -1403,6 +1455,7 @@ namespace {
 // Drops the velocity hierarchy and the V-cycle preconditioned diffusion
 // solves.
 void release_velocity_multigrid(Simulation &s) {
+    drop_graph(s.vcycle_graph);
     s.vmg.release();
     s.vmg_mode = 0;
     for (DeviceArray<double> *a : {&s.vdinv, &s.vr, &s.vz, &s.vp, &s.vq, &s.vt, &s.vdot})
@@ -1490,6 +1543,7 @@ void simulation_add_velocity_coarse_level(Simulation &s, int count, int64_t nnz,
                   NSFEM_INVALID_ARGUMENT,
                   "velocity coarse level: need 0 < count < the level above's size and a "
                   "non-empty P");
+    drop_graph(s.vcycle_graph);
     append_level(d, s.vmg, above, s.vmg.deep_rs, s.vmg.deep_col, s.vmg.deep_val, count, nnz, rs,
                  col, val, host_threads(s));
 }
