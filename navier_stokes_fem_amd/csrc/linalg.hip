@@ -368,8 +368,12 @@ namespace {
 // IT = int16_t reads the columns as 16-bit offsets from the tile's first row
 // (bcol, square matrices whose offsets fit): 2 bytes less per nonzero.
 // T = double: the FP64 values; T = float: the FP32 copy (val32) with FP32
-// vectors and sums (the multigrid smoothing of mixed-precision solves).
-template <int MODE, typename IT, typename T>
+// x.  S: the type of the products, sums, lhs and y: T (FP32 throughout for
+// the multigrid smoothing of mixed-precision solves), or double with FP32
+// values and x (each FP32 x FP32 product exact in FP64, summed in FP64: the
+// product of a CG's FP32 direction, free of the cancellation an FP32 sum
+// suffers on smooth vectors).
+template <int MODE, typename IT, typename T, typename S = T>
 __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_t nnz,
                                                  const int32_t *__restrict__ tiles,
                                                  const int32_t *__restrict__ rs,
@@ -377,11 +381,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_
                                                  const int16_t *__restrict__ bcol,
                                                  const T *__restrict__ val,
                                                  const T *__restrict__ x,
-                                                 const T *lhs, T *y,
-                                                 const T *__restrict__ dinv, T omega) {
+                                                 const S *lhs, S *y,
+                                                 const S *__restrict__ dinv, S omega) {
     constexpr bool F64 = std::is_same<T, double>::value;
-    __shared__ __align__(16) T prod[kTileNnz];
-    __shared__ T red[kWaves];
+    constexpr bool S64 = std::is_same<S, double>::value;
+    __shared__ __align__(16) S prod[kTileNnz];
+    __shared__ S red[kWaves];
     const int t = xcd_swizzle(blockIdx.x, gridDim.x);
     if (t >= n_tiles) return;
     const int tid = threadIdx.x;
@@ -461,16 +466,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
-                for (int j = 0; j < 4; ++j) prod[chunk + lane + 64 * j] = xv[j];
+                for (int j = 0; j < 4; ++j) prod[chunk + lane + 64 * j] = (S)xv[j];
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
-                if constexpr (F64) {
+                if constexpr (S64) {
                     f64x2 p0 = *reinterpret_cast<const f64x2 *>(prod + chunk + 4 * lane);
                     f64x2 p1 = *reinterpret_cast<const f64x2 *>(prod + chunk + 4 * lane + 2);
-                    p0.x = v[g][0] * p0.x;
-                    p0.y = v[g][1] * p0.y;
-                    p1.x = v[g][2] * p1.x;
-                    p1.y = v[g][3] * p1.y;
+                    p0.x = (double)v[g][0] * p0.x;
+                    p0.y = (double)v[g][1] * p0.y;
+                    p1.x = (double)v[g][2] * p1.x;
+                    p1.y = (double)v[g][3] * p1.y;
                     *reinterpret_cast<f64x2 *>(prod + chunk + 4 * lane) = p0;
                     *reinterpret_cast<f64x2 *>(prod + chunk + 4 * lane + 2) = p1;
                 } else {
@@ -487,31 +492,31 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n_tiles, int n_rows, int64_
         const int row = r0 + tid;
         if (row < r1) {
             const int a = rs[row] - ka, b = rs[row + 1] - ka;
-            T s = T(0);
+            S s = S(0);
             for (int k = a; k < b; ++k) s += prod[k];
             if (MODE == 0) y[row] = s;
             if (MODE == 1) y[row] = lhs[row] - s;
             if (MODE == 2) y[row] = s - lhs[row];
             if (MODE == 3) {
-                const T r = lhs[row] - s;
-                y[row] = __builtin_fma(omega * dinv[row], r, x[row]);
+                const S r = lhs[row] - s;
+                y[row] = __builtin_fma(omega * dinv[row], r, (S)x[row]);
             }
         }
     } else {  // one row longer than a tile
-        T s = T(0);
-        for (int k = tid; k < nz; k += kBlock) s += val[k0 + k] * x[col[k0 + k]];
+        S s = S(0);
+        for (int k = tid; k < nz; k += kBlock) s += (S)val[k0 + k] * (S)x[col[k0 + k]];
         s = wave_sum(s);
         if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
         if (tid == 0) {
-            T tot = T(0);
+            S tot = S(0);
             for (int i = 0; i < kWaves; ++i) tot += red[i];
             if (MODE == 0) y[r0] = tot;
             if (MODE == 1) y[r0] = lhs[r0] - tot;
             if (MODE == 2) y[r0] = tot - lhs[r0];
             if (MODE == 3) {
-                const T r = lhs[r0] - tot;
-                y[r0] = __builtin_fma(omega * dinv[r0], r, x[r0]);
+                const S r = lhs[r0] - tot;
+                y[r0] = __builtin_fma(omega * dinv[r0], r, (S)x[r0]);
             }
         }
     }
@@ -554,12 +559,12 @@ __global__ __launch_bounds__(kBlock) void k_dot_final(int nb, const double *__re
     if (threadIdx.x == 0) result[0] += s[0];
 }
 
-// The update of a CG with an FP32 preconditioner: x += a p, r -= a q, r32 =
-// r rounded to FP32 (the preconditioner's input), and the partial sums of
-// (r, r) per contiguous chunk in k_dot_partial's order (k_dot_final adds
-// them).
+// The update of a CG with FP32 directions p (q = A p in FP64) and an FP32
+// preconditioner: x += a p and r -= a q in FP64, r32 = r rounded to FP32
+// (the preconditioner's input), and the partial sums of (r, r) per
+// contiguous chunk in k_dot_partial's order (k_dot_final adds them).
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int chunk, double a,
-                                                       const double *__restrict__ p,
+                                                       const float *__restrict__ p,
                                                        const double *__restrict__ q,
                                                        double *__restrict__ x,
                                                        double *__restrict__ r,
@@ -570,7 +575,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int chunk, double 
     const int hi = min(n, lo + chunk);
     double s[1] = {0.0};
     for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
-        x[i] = __builtin_fma(a, p[i], x[i]);
+        x[i] = __builtin_fma(a, (double)p[i], x[i]);
         const double ri = __builtin_fma(-a, q[i], r[i]);
         r[i] = ri;
         r32[i] = (float)ri;
@@ -580,27 +585,29 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int chunk, double 
     if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
-// Partial sums of (r, z) with z in FP32, widened, per chunk.
-__global__ __launch_bounds__(kBlock) void k_dot_mixed_partial(int n, int chunk,
-                                                              const double *__restrict__ r,
-                                                              const float *__restrict__ z,
-                                                              double *__restrict__ partial) {
+// Partial sums of (a, b), each operand FP64 or FP32 (widened), per chunk,
+// in k_dot_partial's order.
+template <typename A, typename B>
+__global__ __launch_bounds__(kBlock) void k_dot_any_partial(int n, int chunk,
+                                                            const A *__restrict__ a,
+                                                            const B *__restrict__ b,
+                                                            double *__restrict__ partial) {
     __shared__ double lds[kWaves];
     const int lo = blockIdx.x * chunk;
     const int hi = min(n, lo + chunk);
     double s[1] = {0.0};
     for (int i = lo + threadIdx.x; i < hi; i += kBlock)
-        s[0] = __builtin_fma(r[i], (double)z[i], s[0]);
+        s[0] = __builtin_fma((double)a[i], (double)b[i], s[0]);
     block_sum<1, kBlock>(s, lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
-// p = z + b p with z in FP32 (the CG's new direction)
+// p = z + b p in FP32 storage (the CG's new direction; the sum in FP64)
 __global__ void k_pcg_direction(int n, const float *__restrict__ z, double b,
-                                double *__restrict__ p) {
+                                float *__restrict__ p) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = __builtin_fma(b, p[i], (double)z[i]);
+        p[i] = (float)__builtin_fma(b, (double)p[i], (double)z[i]);
 }
 
 __global__ void k_scatter(int count, const int32_t *__restrict__ idx,
@@ -681,21 +688,21 @@ const float *values<float>(Matrix &m) {
     return m.val32.ptr;
 }
 
-template <int MODE, typename IT, typename T>
-void spmv_launch(Device &d, Matrix &m, const T *x, const T *lhs, T *y, const T *dinv,
-                 T omega) {
-    k_spmv<MODE, IT, T><<<m.n_tiles, kBlock, 0, d.stream>>>(
+template <int MODE, typename IT, typename T, typename S>
+void spmv_launch(Device &d, Matrix &m, const T *x, const S *lhs, S *y, const S *dinv,
+                 S omega) {
+    k_spmv<MODE, IT, T, S><<<m.n_tiles, kBlock, 0, d.stream>>>(
         m.n_tiles, m.rows, m.nnz, m.tiles.ptr, m.row_start.ptr, m.col.ptr, m.bcol.ptr,
         values<T>(m), x, lhs, y, dinv, omega);
 }
 
-template <int MODE, typename T>
-void spmv_launch(Device &d, Matrix &m, const T *x, const T *lhs, T *y,
-                 const T *dinv = nullptr, T omega = T(0)) {
+template <int MODE, typename T, typename S>
+void spmv_launch(Device &d, Matrix &m, const T *x, const S *lhs, S *y,
+                 const S *dinv = nullptr, S omega = S(0)) {
     if (m.bcol_ok && want_idx16())
-        spmv_launch<MODE, int16_t>(d, m, x, lhs, y, dinv, omega);
+        spmv_launch<MODE, int16_t, T, S>(d, m, x, lhs, y, dinv, omega);
     else
-        spmv_launch<MODE, int32_t>(d, m, x, lhs, y, dinv, omega);
+        spmv_launch<MODE, int32_t, T, S>(d, m, x, lhs, y, dinv, omega);
 }
 
 template <typename T>
@@ -722,6 +729,14 @@ void launch_spmv(Device &d, Matrix &m, const float *x, const float *lhs, float *
     NSFEM_REQUIRE(m.rows == 0 || m.val32.ptr, NSFEM_NOT_READY,
                   "FP32 product without the FP32 copy of the values");
     spmv_any(d, m, x, lhs, y, mode);
+}
+
+// y = A x with FP32 values and x, the products and sums in FP64
+void launch_spmv_acc64(Device &d, Matrix &m, const float *x, double *y) {
+    if (m.rows == 0) return;
+    NSFEM_REQUIRE(m.val32.ptr, NSFEM_NOT_READY, "FP32 product without the FP32 copy of the values");
+    spmv_launch<0, float, double>(d, m, x, (const double *)nullptr, y);
+    NSFEM_HIP(hipGetLastError());
 }
 
 void launch_sweep(Device &d, Matrix &m, double w, const double *dinv, const double *x,
@@ -784,7 +799,7 @@ void dot_shape(int n, int &nb, int &chunk) {
 }
 }  // namespace
 
-void launch_pcg_update(Device &d, int n, double a, const double *p, const double *q, double *x,
+void launch_pcg_update(Device &d, int n, double a, const float *p, const double *q, double *x,
                        double *r, float *r32, double *rr) {
     if (n <= 0) return;
     d.dot_partial.ensure(kDotBlocks);
@@ -795,18 +810,34 @@ void launch_pcg_update(Device &d, int n, double a, const double *p, const double
     NSFEM_HIP(hipGetLastError());
 }
 
-void launch_dot_mixed_accumulate(Device &d, int n, const double *r, const float *z,
-                                 double *result) {
+namespace {
+template <typename A, typename B>
+void dot_any(Device &d, int n, const A *a, const B *b, double *result) {
     if (n <= 0) return;
     d.dot_partial.ensure(kDotBlocks);
     int nb, chunk;
     dot_shape(n, nb, chunk);
-    k_dot_mixed_partial<<<nb, kBlock, 0, d.stream>>>(n, chunk, r, z, d.dot_partial.ptr);
+    k_dot_any_partial<<<nb, kBlock, 0, d.stream>>>(n, chunk, a, b, d.dot_partial.ptr);
     k_dot_final<<<1, kBlock, 0, d.stream>>>(nb, d.dot_partial.ptr, result);
     NSFEM_HIP(hipGetLastError());
 }
+}  // namespace
 
-void launch_pcg_direction(Device &d, int n, const float *z, double b, double *p) {
+void launch_dot_accumulate(Device &d, int n, const double *a, const float *b, double *result) {
+    dot_any(d, n, a, b, result);
+}
+
+void launch_dot_accumulate(Device &d, int n, const float *a, const float *b, double *result) {
+    dot_any(d, n, a, b, result);
+}
+
+void launch_copy(Device &d, int n, const float *a, float *y) {
+    if (n <= 0) return;
+    k_convert<<<ew_grid(n), kBlock, 0, d.stream>>>(n, a, y);
+    NSFEM_HIP(hipGetLastError());
+}
+
+void launch_pcg_direction(Device &d, int n, const float *z, double b, float *p) {
     if (n <= 0) return;
     k_pcg_direction<<<ew_grid(n), kBlock, 0, d.stream>>>(n, z, b, p);
     NSFEM_HIP(hipGetLastError());

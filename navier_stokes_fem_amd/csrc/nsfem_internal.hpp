@@ -312,15 +312,20 @@ void launch_scatter(Device &d, int count, const int32_t *idx, const double *val,
 void launch_scatter_value(Device &d, int count, const int32_t *idx, double v,
                           double *dst);  // dst[idx[i]] = v
 void launch_scatter_value(Device &d, int count, const int32_t *idx, double v, float *dst);
-// The CG with an FP32 preconditioner: x += a p, r -= a q, r32 = (float) r,
-// *rr += (r, r) (deterministic order, as launch_dot_accumulate)
-void launch_pcg_update(Device &d, int n, double a, const double *p, const double *q, double *x,
+// The CG with FP32 directions and an FP32 preconditioner: x += a p and
+// r -= a q in FP64, r32 = (float) r, *rr += (r, r) (deterministic order, as
+// launch_dot_accumulate)
+void launch_pcg_update(Device &d, int n, double a, const float *p, const double *q, double *x,
                        double *r, float *r32, double *rr);
-// *result += (r, z) with z in FP32
-void launch_dot_mixed_accumulate(Device &d, int n, const double *r, const float *z,
-                                 double *result);
-// p = z + b p with z in FP32
-void launch_pcg_direction(Device &d, int n, const float *z, double b, double *p);
+// y = A x with the FP32 copy of the values and FP32 x, products and sums in
+// FP64 (ensure_val32 first)
+void launch_spmv_acc64(Device &d, Matrix &m, const float *x, double *y);
+// *result += (a, b) accumulated in FP64 with FP32 operands widened
+void launch_dot_accumulate(Device &d, int n, const double *a, const float *b, double *result);
+void launch_dot_accumulate(Device &d, int n, const float *a, const float *b, double *result);
+void launch_copy(Device &d, int n, const float *src, float *dst);
+// p = z + b p, FP32 storage
+void launch_pcg_direction(Device &d, int n, const float *z, double b, float *p);
 void launch_add(Device &d, int n, const double *a, double *y);  // y += a
 void launch_add(Device &d, int n, const float *a, float *y);
 // x += w * dinv .* r

@@ -157,7 +157,7 @@ struct Simulation {
     // FP32 cycle's fine-level vectors
     bool pmg = false;
     DeviceArray<double> pdinv, pr, pz, pp, pq, pt, pdot;
-    DeviceArray<float> pdinv32, pb32, pz32, pt32;
+    DeviceArray<float> pdinv32, pb32, pz32, pt32, pp32;
     // the FP32 cycles of the pressure and the diffusion solves (fixed
     // kernels on fixed vectors), captured in a HIP graph on their first run
     // and replayed; dropped with their hierarchies
@@ -173,6 +173,7 @@ struct Simulation {
     int vmg_mode = 0;
     DeviceArray<double> vdinv, vr, vz, vp, vq, vt, vdot;
     DeviceArray<float> vdinv32, vb32, vz32, vt32;  // the FP32 cycle's fine level
+    DeviceArray<float> vp32;                       // the FP32 CG direction
     double vmg_sec[2][2] = {{0.0, 0.0}, {0.0, 0.0}};  // [column][plain, V-cycle]
     int vmg_age[2][2] = {{0, 0}, {0, 0}};
     // projection guess of the diffusion solves' corrections u - u~ (u~ = the
@@ -705,7 +706,7 @@ void drop_graph(hipGraphExec_t &g);
 void release_pressure_multigrid(Simulation &s) {
     s.pmg = false;
     drop_graph(s.pcycle_graph);
-    for (DeviceArray<float> *a : {&s.pdinv32, &s.pb32, &s.pz32, &s.pt32}) a->release();
+    for (DeviceArray<float> *a : {&s.pdinv32, &s.pb32, &s.pz32, &s.pt32, &s.pp32}) a->release();
     s.pdinv.release();
     s.pr.release();
     s.pz.release();
@@ -989,7 +990,7 @@ void ensure_pressure_fp32(Simulation &s) {
     ensure_val32(d, s.mats[NSFEM_PRESSURE_STIFFNESS]);
     s.pdinv32.resize((size_t)n);
     launch_convert(d, n, s.pdinv.ptr, s.pdinv32.ptr);
-    for (DeviceArray<float> *a : {&s.pb32, &s.pz32, &s.pt32}) a->resize((size_t)n);
+    for (DeviceArray<float> *a : {&s.pb32, &s.pz32, &s.pt32, &s.pp32}) a->resize((size_t)n);
 }
 
 // z = M r in FP32 on the fixed vectors r = pb32, z = pz32 (captured)
@@ -1085,16 +1086,31 @@ double take_scalar(Device &d, double *dot) {
     return h;
 }
 
-// pcg_run with an FP32 preconditioner z32 = precond(r32): the update writes
-// r rounded to FP32 beside r and returns (r, r) from the same pass, and z
-// is used widened (in (r, z) and p = z + beta p), never stored in FP64.
-// The stopping rule is pcg_run's.
+// pcg_run with an FP32 preconditioner z32 = precond(r32) and FP32
+// directions: p in FP32, q = A p on the FP32 values with every product and
+// sum in FP64 (an FP32 sum would cancel away smooth directions' products on
+// the pressure matrix), x and r updated in FP64 in one pass that also
+// writes r rounded to FP32 and returns (r, r); every dot accumulates in
+// FP64.  As in the mixed persistent CG, the recursion on the FP32 values
+// only steers: convergence is decided on a recomputed FP64 b - A x, which
+// then replaces the recursive residual (a reliable update), as it does
+// whenever the recursive residual has fallen kPcgUpdateDrop-fold since the
+// last one (the FP32 directions keep conjugacy to ~1e-7, so a longer
+// recursion would stall).  The stopping rule is pcg_run's.  Returns
+// NSFEM_STAGNATION after kPcgRecomputeMax recomputed residuals above the
+// tolerance where the recursion had met it, or after kPcgIterationsMax
+// iterations (the V-cycle preconditioned solves take tens); x is then the
+// last iterate, for the FP64 form to finish.
+constexpr int kPcgRecomputeMax = 8;
+constexpr double kPcgUpdateDrop = 1e3;
+constexpr int64_t kPcgIterationsMax = 500;
+
 template <class Pre>
 int pcg_run32(Device &d, Matrix &A, int n, const double *b, double *x, const PcgWork &w,
-              float *r32, float *z32, double rtol, int64_t max_iterations, Pre precond,
-              SolveOut &o, int c) {
+              float *r32, float *z32, float *p, double rtol, int64_t max_iterations,
+              Pre precond, SolveOut &o, int c) {
     const auto t0 = std::chrono::steady_clock::now();
-    double *r = w.r, *p = w.p, *q = w.q;
+    double *r = w.r, *q = w.q;
     const double bb = vec_dot(d, n, b, b, w.dot);
     launch_spmv(d, A, x, b, r, 1);
     double rr = vec_dot(d, n, r, r, w.dot);
@@ -1103,17 +1119,19 @@ int pcg_run32(Device &d, Matrix &A, int n, const double *b, double *x, const Pcg
     const int64_t max_iter = max_iterations > 0 ? max_iterations : n;
     int64_t k = 0;
     int status = NSFEM_OK;
+    int recomputed = 0;
+    double rr_true = rr;  // the last recomputed residual
     if (rr > tol2) {
         status = NSFEM_MAX_ITERATIONS_REACHED;
         launch_fill(d, 1, 0.0, w.dot);
         launch_convert(d, n, r, r32);
         precond(r32, z32);
-        launch_convert(d, n, z32, p);
-        launch_dot_mixed_accumulate(d, n, r, z32, w.dot);
+        launch_copy(d, n, z32, p);
+        launch_dot_accumulate(d, n, r, z32, w.dot);
         double rz = take_scalar(d, w.dot);
         while (k < max_iter) {
-            launch_spmv(d, A, p, nullptr, q, 0);
-            launch_dot_accumulate(d, n, p, q, w.dot);
+            launch_spmv_acc64(d, A, p, q);
+            launch_dot_accumulate(d, n, q, p, w.dot);
             const double pq = take_scalar(d, w.dot);
             if (!(pq > 0.0) || !(rz > 0.0)) {
                 status = NSFEM_BREAKDOWN;
@@ -1123,18 +1141,28 @@ int pcg_run32(Device &d, Matrix &A, int n, const double *b, double *x, const Pcg
             launch_pcg_update(d, n, alpha, p, q, x, r, r32, w.dot);
             rr = take_scalar(d, w.dot);
             ++k;
-            if (rr <= tol2) {
+            const bool met = rr <= tol2;
+            if (met || rr * kPcgUpdateDrop * kPcgUpdateDrop <= rr_true) {
                 launch_spmv(d, A, x, b, r, 1);
                 launch_dot_accumulate(d, n, r, r, w.dot);
                 rr = take_scalar(d, w.dot);
+                rr_true = rr;
                 if (rr <= tol2) {
                     status = NSFEM_OK;
                     break;
                 }
+                if (met && ++recomputed >= kPcgRecomputeMax) {
+                    status = NSFEM_STAGNATION;
+                    break;
+                }
                 launch_convert(d, n, r, r32);
             }
+            if (k >= kPcgIterationsMax) {
+                status = NSFEM_STAGNATION;
+                break;
+            }
             precond(r32, z32);
-            launch_dot_mixed_accumulate(d, n, r, z32, w.dot);
+            launch_dot_accumulate(d, n, r, z32, w.dot);
             const double rz_new = take_scalar(d, w.dot);
             const double beta = rz_new / rz;
             rz = rz_new;
@@ -1156,15 +1184,21 @@ void pcg_pressure(Simulation &s, SolveOut &o) {
     launch_scatter(d, s.npb, s.pb_idx.ptr, s.pb_val.ptr, s.p.ptr);
     const PcgWork w{s.pr.ptr, s.pz.ptr, s.pp.ptr, s.pq.ptr, s.pdot.ptr};
     Matrix &A = s.mats[NSFEM_PRESSURE_STIFFNESS];
-    int status;
+    int status = NSFEM_OK;
+    SolveOut first{};
     if (pressure_cycle_fp32(s)) {
         ensure_pressure_fp32(s);
-        status = pcg_run32(d, A, s.P, s.prhs.ptr, s.p.ptr, w, s.pb32.ptr, s.pz32.ptr, s.opt.rtol,
-                           s.opt.max_iterations,
+        status = pcg_run32(d, A, s.P, s.prhs.ptr, s.p.ptr, w, s.pb32.ptr, s.pz32.ptr, s.pp32.ptr,
+                           s.opt.rtol, s.opt.max_iterations,
                            [&](const float *, float *) { fine_vcycle32(s); }, o, 0);
-    } else {
+    }
+    if (!pressure_cycle_fp32(s) || status == NSFEM_STAGNATION) {
+        // FP64 throughout (from the last iterate when the FP32 form stalled)
+        first = o;
         status = pcg_run(d, A, s.P, s.prhs.ptr, s.p.ptr, w, s.opt.rtol, s.opt.max_iterations,
                          [&](const double *r, double *z) { fine_vcycle(s, r, z); }, o, 0);
+        o.iters[0] += first.iters[0];
+        o.seconds += first.seconds;
     }
     if (status != NSFEM_OK)
         fail(status, "V-cycle preconditioned CG on the pressure stiffness matrix did not "
@@ -1208,7 +1242,7 @@ void ensure_velocity_fp32(Simulation &s) {
     ensure_val32(d, s.mats[NSFEM_DIFFUSION]);
     s.vdinv32.resize((size_t)V);
     launch_convert(d, V, s.vdinv.ptr, s.vdinv32.ptr);
-    for (DeviceArray<float> *a : {&s.vb32, &s.vz32, &s.vt32}) a->resize((size_t)V);
+    for (DeviceArray<float> *a : {&s.vb32, &s.vz32, &s.vt32, &s.vp32}) a->resize((size_t)V);
 }
 
 // z = M r in FP32 on the fixed vectors r = vb32, z = vz32 (captured)
@@ -1230,15 +1264,21 @@ void pcg_velocity(Simulation &s, int c, SolveOut &o) {
     launch_scatter(d, s.nvb, s.vb_idx.ptr, c ? s.vb_v.ptr : s.vb_u.ptr, x);
     const PcgWork w{s.vr.ptr, s.vz.ptr, s.vp.ptr, s.vq.ptr, s.vdot.ptr};
     Matrix &A = s.mats[NSFEM_DIFFUSION];
-    int status;
+    int status = NSFEM_OK;
+    SolveOut first{};
     if (velocity_cycle_fp32(s)) {
         ensure_velocity_fp32(s);
-        status = pcg_run32(d, A, s.V, s.rhs.ptr + off, x, w, s.vb32.ptr, s.vz32.ptr, s.opt.rtol,
-                           s.opt.max_iterations,
+        status = pcg_run32(d, A, s.V, s.rhs.ptr + off, x, w, s.vb32.ptr, s.vz32.ptr, s.vp32.ptr,
+                           s.opt.rtol, s.opt.max_iterations,
                            [&](const float *, float *) { velocity_vcycle32(s); }, o, 0);
-    } else {
+    }
+    if (!velocity_cycle_fp32(s) || status == NSFEM_STAGNATION) {
+        // FP64 throughout (from the last iterate when the FP32 form stalled)
+        first = o;
         status = pcg_run(d, A, s.V, s.rhs.ptr + off, x, w, s.opt.rtol, s.opt.max_iterations,
                          [&](const double *r, double *z) { velocity_vcycle(s, r, z); }, o, 0);
+        o.iters[0] += first.iters[0];
+        o.seconds += first.seconds;
     }
     if (status != NSFEM_OK)
         fail(status, "V-cycle preconditioned CG on the diffusion matrix did not converge "
@@ -1460,7 +1500,7 @@ void release_velocity_multigrid(Simulation &s) {
     s.vmg_mode = 0;
     for (DeviceArray<double> *a : {&s.vdinv, &s.vr, &s.vz, &s.vp, &s.vq, &s.vt, &s.vdot})
         a->release();
-    for (DeviceArray<float> *a : {&s.vdinv32, &s.vb32, &s.vz32, &s.vt32}) a->release();
+    for (DeviceArray<float> *a : {&s.vdinv32, &s.vb32, &s.vz32, &s.vt32, &s.vp32}) a->release();
     for (int c = 0; c < 2; ++c)
         for (int w = 0; w < 2; ++w) {
             s.vmg_sec[c][w] = 0.0;
