@@ -932,12 +932,17 @@ void drop_graph(hipGraphExec_t &g) {
 // synchronise nothing (an FP32 cycle over a fixed hierarchy and fixed
 // vectors: products, sweeps, transfers and the dense deepest solve).  A
 // graph replaces ~100 launches of the cycle, most of them on levels so
-// small that their launch latency is their cost.  NSFEM_VCYCLE_GRAPH=0
-// runs the launches directly (read at each call).
+// small that their launch latency is their cost.  Opt-in with
+// NSFEM_VCYCLE_GRAPH=1 (read at each call): at 2048^2 it gained 0.6%
+// (19.67 against 19.55 steps/s), and one kernel-traced run of the bench
+// (rocprofv3 --kernel-trace) ended in a host-side segmentation fault inside
+// a replay, in the accuracy run that follows the timed ones; the same
+// command without the profiler and the replay-against-direct test pass.
+// Until that is understood the cycles launch directly by default.
 template <class F>
 void run_captured(Device &d, hipGraphExec_t &exec, F launches) {
     const char *e = std::getenv("NSFEM_VCYCLE_GRAPH");
-    if (e && std::atoi(e) == 0) {
+    if (!(e && std::atoi(e) == 1)) {
         launches();
         return;
     }

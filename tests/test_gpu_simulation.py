@@ -557,13 +557,12 @@ def test_mass_projection(gpu):
 
 
 def test_captured_vcycles_replay_like_eager_launches(gpu, monkeypatch):
-    """The FP32 V-cycles of the pressure and diffusion solves run from a HIP
-    graph captured on their first use and replayed on every later one, with
-    the CG's own launches, dots and host synchronisations between the
-    replays: 6 steps (dozens of replays of each graph) give bit-identical
-    fields and iteration counts to the same steps with every cycle launched
-    directly (NSFEM_VCYCLE_GRAPH=0), and adding a level afterwards (a new
-    hierarchy, a new capture) keeps the solve within its tolerance."""
+    """With NSFEM_VCYCLE_GRAPH=1 the FP32 V-cycles of the pressure and
+    diffusion solves run from a HIP graph captured on their first use and
+    replayed on every later one, with the CG's own launches, dots and host
+    synchronisations between the replays: 6 steps (dozens of replays of
+    each graph) give bit-identical fields and iteration counts to the same
+    steps with every cycle launched directly (the default), twice over."""
     from navier_stokes_fem_amd import mesh
     from navier_stokes_fem_amd.simulation import Simulation
 
