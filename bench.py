@@ -905,7 +905,11 @@ def main():
     achieved = bytes_textbook / max(k_secs, 1e-12) / 1e9
     measured = MEASURED_BYTES_PER_ITERATION.get((mixed, name)) if args.n == 2048 else None
     traffic = measured * k_iters / launches if measured else None
-    if cols == 2:
+    if cols == 2 and mixed and os.environ.get("NSFEM_CG_PAIRED") != "twophase":
+        # the packed single-reduction iteration (block size, loads in flight)
+        bs = 768 if os.environ.get("NSFEM_CG_P2_BLOCK") == "768" else 1024
+        kernel = "k_cg_p2<%d, short, %d>" % (bs, 8 if os.environ.get("NSFEM_CG_P2_LOADS") == "8" else 4)
+    elif cols == 2:
         kernel = "k_cg<2, 1024, %s, short, 8>" % ("true" if mixed else "false")
     elif mixed:
         kernel = "k_cg<1, 768, true, short, 1>"

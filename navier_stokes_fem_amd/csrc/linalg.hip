@@ -1052,7 +1052,8 @@ struct CgArgs {
     const double *b;  // [NC][n]
     double *x;        // [NC][n]
     void *r, *p0, *p1, *q;  // interleaved [n][NC] (two-column solves)
-    void *st[2];            // packed {r, p, q} rows of one-column mixed solves
+    void *st[2];            // packed {r, p, q} rows of mixed solves
+    int packed2;            // two-column mixed solves: the packed iteration (else two-phase)
     double rtol, atol, rdelta;
     int max_iter, max_refine;
     int ru_gap;      // mixed: force a reliable update after this many iterations
@@ -1421,6 +1422,21 @@ __device__ __forceinline__ void st_state(void *st, int i, const RowState &v) {
     reinterpret_cast<RowState *>(st)[i] = v;
 }
 
+// Two-column mixed solves keep both columns' FP32 {r, p} of a row in one
+// 16-byte word {r0, r1, p0, p1} and their q in an 8-byte word {q0, q1}: a
+// state copy holds the n words of the first kind, then the n of the second.
+// A neighbour's contribution to both columns takes two gathers.
+__device__ __forceinline__ float4 *st2_rp(void *st) { return reinterpret_cast<float4 *>(st); }
+__device__ __forceinline__ const float4 *st2_rp(const void *st) {
+    return reinterpret_cast<const float4 *>(st);
+}
+__device__ __forceinline__ float2 *st2_q(void *st, int n) {
+    return reinterpret_cast<float2 *>(reinterpret_cast<float4 *>(st) + n);
+}
+__device__ __forceinline__ const float2 *st2_q(const void *st, int n) {
+    return reinterpret_cast<const float2 *>(reinterpret_cast<const float4 *>(st) + n);
+}
+
 // Column word of nonzero k: the column (IT = int32_t) or its offset from the
 // first row of the nonzero's wave tile (IT = int16_t).
 template <typename IT>
@@ -1453,7 +1469,7 @@ template <int NC, int BS, bool MIXED, bool SETUP, typename IT, bool PACKED = fal
           typename MP = const int4 *>
 __device__ void residual_phase(const CgArgs &A, MP meta, int T0, int T1, double *lds_wave,
                                double (&acc)[3 * NC], void *st = nullptr) {
-    static_assert(!PACKED || (NC == 1 && MIXED), "packed state: one FP32 column");
+    static_assert(!PACKED || ((NC == 1 || NC == 2) && MIXED), "packed state: FP32 columns");
     using VT = typename std::conditional<MIXED, float, double>::type;
     constexpr int NW = BS / 64;
     const int lane = threadIdx.x & 63;
@@ -1527,11 +1543,20 @@ __device__ void residual_phase(const CgArgs &A, MP meta, int T0, int T1, double 
                 if (MIXED) acc[2 * NC + c] += (double)rv.v[c] * (double)rv.v[c];
                 zero.v[c] = (VT)0;
             }
-            if constexpr (PACKED) {
+            if constexpr (PACKED && NC == 1) {
                 if constexpr (SETUP)
                     st_state(st, row, RowState{(float)rv.v[0], 0.f, 0.f});
                 else
                     reinterpret_cast<RowState *>(st)[row].r = (float)rv.v[0];
+            } else if constexpr (PACKED) {
+                float4 *rp = st2_rp(st);
+                if constexpr (SETUP) {
+                    rp[row] = make_float4((float)rv.v[0], (float)rv.v[1], 0.f, 0.f);
+                    st2_q(st, A.n)[row] = make_float2(0.f, 0.f);
+                } else {
+                    reinterpret_cast<float2 *>(rp + row)[0] =
+                        make_float2((float)rv.v[0], (float)rv.v[1]);
+                }
             } else {
                 stv<NC, VT>(r, row, rv);
                 if (SETUP) {
@@ -2411,6 +2436,352 @@ __device__ __forceinline__ void cg_body1(const CgArgs &A, const int bid, const i
     }
 }
 
+// ---------------------------------------------------------------------
+// Two-column mixed solves: the single-reduction iteration of the one-column
+// solves (phase_f1) over both columns at once.  Each nonzero's column and
+// value load once for both columns; a neighbour's {r0, r1, p0, p1} and
+// {q0, q1} come in two gathers; each column's p_k is recomputed from its
+// gathered state by the same two roundings as its owner's, and the FP64
+// sums (p_k, q_k), (r_k, r_k), (r_k, q_k), (q_k, q_k) of both columns take
+// one grid reduction of 8 values.  sums[4c + i] holds column c's i-th sum.
+// IB: nonzeros per lane whose loads are in flight together (4 or 8).
+template <int BS, typename IT, int IB, typename MP>
+__device__ void phase_f2(const CgArgs &A, MP meta, int T0, int T1, double *lds_wave,
+                         const void *st_old, void *st_new, const float (&alpha)[2],
+                         const float (&beta)[2], const double (&pend)[2], const bool has_pend,
+                         double (&sums)[8]) {
+    constexpr int NW = BS / 64;
+    constexpr int WI = kWItems;
+    const int n = A.n;
+    const float *val = A.val32;
+    const float4 *rp_old = st2_rp(st_old);
+    const float2 *q_old = st2_q(st_old, n);
+    float4 *rp_new = st2_rp(st_new);
+    float2 *q_new = st2_q(st_new, n);
+    const bool pend0 = has_pend && pend[0] != 0.0, pend1 = has_pend && pend[1] != 0.0;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int t = T0 + wave;
+    if (t >= T1) return;
+    WTile w = wave_tile(meta, t);
+    int cn[WI];
+    {
+        const int kf = w.nz > 0 ? w.k0 : 0;
+#pragma unroll
+        for (int i = 0; i < WI; ++i) {
+            const int k = lane + 64 * i;
+            cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
+        }
+    }
+    for (; t < T1; t += NW) {
+        const WTile cur = w;
+        int cj[WI];
+#pragma unroll
+        for (int i = 0; i < WI; ++i) cj[i] = col_of<IT>(cn[i], cur.r0);
+        const int tn = t + NW;
+        if (tn < T1) {
+            w = wave_tile(meta, tn);
+            const int kf = w.nz > 0 ? w.k0 : 0;
+#pragma unroll
+            for (int i = 0; i < WI; ++i) {
+                const int k = lane + 64 * i;
+                cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
+            }
+        }
+        const int row = cur.r0 + lane;
+        const bool has_row = row < cur.r1;
+        const int rowc = has_row ? row : cur.r0;
+        const int rsa = A.rs[rowc];
+        const int rsb = A.rs[rowc + 1];
+        const float4 so = rp_old[rowc];
+        const float2 sq = q_old[rowc];
+        // the row's solution entries and scale load with its state, ahead of
+        // the gathers (a late load would wait behind the next tile's
+        // prefetch)
+        double xo0 = 0.0, xo1 = 0.0, scl = 1.0;
+        if (pend0) xo0 = A.x[rowc];
+        if (pend1) xo1 = A.x[(size_t)n + rowc];
+        if (has_pend && A.scale) scl = A.scale[rowc];
+        double s[2] = {0.0, 0.0};
+        if (cur.nz == 0) {
+            // rows without nonzeros: q = 0
+        } else if (cur.nz <= 64 * WI) {
+#pragma unroll
+            for (int i0 = 0; i0 < WI; i0 += IB) {
+                float vj[IB];
+                float4 gj[IB];
+                float2 gq[IB];
+#pragma unroll
+                for (int i = 0; i < IB; ++i) {
+                    const int k = lane + 64 * (i0 + i);
+                    vj[i] = ld_stream(val + (k < cur.nz ? cur.k0 + k : 0));
+                    gj[i] = rp_old[cj[i0 + i]];
+                    gq[i] = q_old[cj[i0 + i]];
+                }
+#pragma unroll
+                for (int i = 0; i < IB; ++i) {
+                    const int k = lane + 64 * (i0 + i);
+                    const double v = (double)vj[i];
+                    lds_wave[2 * k] =
+                        v * (double)__builtin_fmaf(beta[0], gj[i].z, next_r(alpha[0], gj[i].x, gq[i].x));
+                    lds_wave[2 * k + 1] =
+                        v * (double)__builtin_fmaf(beta[1], gj[i].w, next_r(alpha[1], gj[i].y, gq[i].y));
+                }
+            }
+            __syncwarp();
+            const int ra = rsa - cur.k0;
+            const int rb = has_row ? rsb - cur.k0 : ra;
+            row_sums<2>(lds_wave, ra, rb, s);
+            __syncwarp();
+        } else {
+            // a single row longer than a tile: wave-wide sums
+            double part0 = 0.0, part1 = 0.0;
+            for (int k = lane; k < cur.nz; k += 64) {
+                const int j = col_at<IT>(A, cur.r0, cur.k0 + k);
+                const float4 g = rp_old[j];
+                const float2 q = q_old[j];
+                const double v = (double)val[cur.k0 + k];
+                part0 += v * (double)__builtin_fmaf(beta[0], g.z, next_r(alpha[0], g.x, q.x));
+                part1 += v * (double)__builtin_fmaf(beta[1], g.w, next_r(alpha[1], g.y, q.y));
+            }
+            s[0] = wave_sum(part0);
+            s[1] = wave_sum(part1);
+        }
+        if (has_row) {
+            const float r0 = next_r(alpha[0], so.x, sq.x), r1 = next_r(alpha[1], so.y, sq.y);
+            const float p0 = __builtin_fmaf(beta[0], so.z, r0);
+            const float p1 = __builtin_fmaf(beta[1], so.w, r1);
+            const float q0 = (float)s[0], q1 = (float)s[1];
+            const double r0d = (double)r0, q0d = (double)q0, r1d = (double)r1, q1d = (double)q1;
+            sums[0] += (double)p0 * s[0];
+            sums[1] += r0d * r0d;
+            sums[2] += r0d * q0d;
+            sums[3] += q0d * q0d;
+            sums[4] += (double)p1 * s[1];
+            sums[5] += r1d * r1d;
+            sums[6] += r1d * q1d;
+            sums[7] += q1d * q1d;
+            rp_new[row] = make_float4(r0, r1, p0, p1);
+            q_new[row] = make_float2(q0, q1);
+            // x_c += x_step(pend_c) p_c, x_step = pend_c * scale[row] (or pend_c)
+            if (pend0) A.x[row] = __builtin_fma(pend[0] * scl, (double)so.z, xo0);
+            if (pend1) A.x[(size_t)n + row] = __builtin_fma(pend[1] * scl, (double)so.w, xo1);
+        }
+    }
+}
+
+// x_c += pend_c p_c over the block's rows (p from the packed state)
+template <int BS>
+__device__ void flush_x2(const CgArgs &A, int R0, int R1, const void *st,
+                         const double (&pend)[2]) {
+    const float4 *rp = st2_rp(st);
+    for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) {
+        const float4 v = rp[row];
+        if (pend[0] != 0.0) A.x[row] = __builtin_fma(x_step(A, pend[0], row), (double)v.z, A.x[row]);
+        if (pend[1] != 0.0) {
+            double *x = A.x + (size_t)A.n + row;
+            *x = __builtin_fma(x_step(A, pend[1], row), (double)v.w, *x);
+        }
+    }
+}
+
+// Control state of a two-column solve (LDS, updated by thread 0): the
+// one-column rules per column, with one iteration count and one reliable
+// update for both (the FP64 residual pass reads the matrix once for both
+// columns; a column that did not ask for the update restarts its recursion
+// from the true residual, like one that did).
+struct CgCtl2 {
+    double tol2[2], gamma[2], rmax2[2], last_rr[2], ru_rr[2], rho[2];
+    double pend[2], alpha[2], beta[2], g[2];
+    int iters[2], status[2], stall[2], active[2], dropped[2];
+    int k, last_ru, refines, ru, has_pend, out_of_iters, any;
+};
+
+__device__ void ctl2_after_phase(CgCtl2 &C, const CgArgs &A, const double (&sums)[8]) {
+    const int k = ++C.k;
+    const double d2 = A.rdelta * A.rdelta;
+    bool want = false;
+    for (int c = 0; c < 2; ++c) {
+        double alpha = 0.0, g = 0.0;
+        bool dropped = false;
+        if (C.active[c]) {
+            const double delta = sums[4 * c], gk = sums[4 * c + 1];
+            if (!(delta > 0.0)) {
+                C.status[c] = NSFEM_BREAKDOWN;
+                C.active[c] = 0;
+                C.last_rr[c] = gk;
+            } else {
+                alpha = gk / delta;
+                g = fmax(gk - 2.0 * alpha * sums[4 * c + 2] + alpha * alpha * sums[4 * c + 3], 0.0);
+                C.gamma[c] = gk;
+                C.iters[c] = k;
+                dropped = g * C.rho[c] <= C.tol2[c] ||
+                          (g < d2 * C.rmax2[c] &&
+                           (k - C.last_ru >= A.ru_min_gap || g < d2 * d2 * C.rmax2[c]));
+            }
+        }
+        want = want || dropped;
+        C.pend[c] = alpha;
+        C.alpha[c] = alpha;
+        C.g[c] = g;
+        C.dropped[c] = dropped;
+    }
+    C.has_pend = 1;
+    C.out_of_iters = k >= A.max_iter;
+    const bool any = C.active[0] || C.active[1];
+    C.ru = any && (want || k - C.last_ru >= A.ru_gap || C.out_of_iters);
+    if (C.ru) C.last_ru = k;
+}
+
+// After a reliable update; ra as residual_phase<2>: ra[c] = FP64 (r, r),
+// ra[4 + c] = (FP32 r, FP32 r).
+__device__ void ctl2_after_update(CgCtl2 &C, const CgArgs &A, const double (&ra)[6]) {
+    C.has_pend = 0;
+    ++C.refines;
+    for (int c = 0; c < 2; ++c) {
+        C.pend[c] = 0.0;
+        C.alpha[c] = 0.0;
+        C.last_rr[c] = ra[c];
+        if (!C.active[c]) continue;
+        if (ra[c] <= C.tol2[c]) {
+            C.active[c] = 0;
+            continue;
+        }
+        C.g[c] = ra[4 + c];
+        C.rmax2[c] = A.scale ? ra[4 + c] : ra[c];
+        C.rho[c] = A.scale && ra[4 + c] > 0.0 ? ra[c] / ra[4 + c] : 1.0;
+        if (C.dropped[c]) {
+            const double sr = 0.5 * (1.0 + A.rdelta);
+            C.stall[c] = ra[c] > sr * sr * C.ru_rr[c] ? C.stall[c] + 1 : 0;
+        }
+        C.ru_rr[c] = fmin(C.ru_rr[c], ra[c]);
+        if (C.stall[c] >= 3) {
+            C.status[c] = NSFEM_STAGNATION;
+            C.active[c] = 0;
+        } else if (C.refines >= A.max_refine) {
+            C.status[c] = NSFEM_MAX_ITERATIONS_REACHED;
+            C.active[c] = 0;
+        }
+    }
+}
+
+__device__ void ctl2_end(CgCtl2 &C) {
+    for (int c = 0; c < 2; ++c) {
+        if (C.active[c] && C.out_of_iters) {
+            C.status[c] = NSFEM_MAX_ITERATIONS_REACHED;
+            C.active[c] = 0;
+            C.last_rr[c] = C.g[c];
+        }
+        C.beta[c] = C.active[c] ? C.g[c] / C.gamma[c] : 0.0;
+        if (!C.active[c]) C.alpha[c] = 0.0;
+    }
+    C.any = C.active[0] || C.active[1];
+}
+
+// The two-column mixed persistent CG run by one block (arguments as
+// cg_body), single-reduction iteration over the packed state.
+template <int BS, typename IT, int IB, typename MP>
+__device__ __forceinline__ void cg_body2_m(const CgArgs &A, MP meta, const int bid, const int nb,
+                                           const int L, double *prod_raw, double *lds) {
+    __shared__ CgCtl2 C;
+    const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
+    const int R0 = A.tiles[T0], R1 = A.tiles[T1];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double *lds64 = prod_raw + wave * kWaveNnz * 2;
+    unsigned epoch = 0;
+    const bool t0 = threadIdx.x == 0;
+    const bool writer = (L == 0 && t0);
+    int cur = 0;  // st[cur]: state of step k - 1
+
+    {
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        residual_phase<2, BS, true, true, IT, true, MP>(A, meta, T0, T1, lds64, acc, A.st[0]);
+        if (!grid_sum<6, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
+        if (t0) {
+            for (int c = 0; c < 2; ++c) {
+                const double bn = sqrt(acc[2 + c]), r0n = sqrt(acc[c]);
+                const double ref = bn > 0.0 ? bn : r0n;
+                const double tol = fmax(A.rtol * ref, A.atol);
+                C.tol2[c] = tol * tol;
+                C.active[c] = acc[c] > C.tol2[c];
+                C.gamma[c] = acc[4 + c];
+                C.rmax2[c] = A.scale ? acc[4 + c] : acc[c];
+                C.last_rr[c] = C.ru_rr[c] = acc[c];
+                C.rho[c] = A.scale && acc[4 + c] > 0.0 ? acc[c] / acc[4 + c] : 1.0;
+                C.g[c] = 0.0;
+                C.iters[c] = 0;
+                C.status[c] = NSFEM_OK;
+                C.stall[c] = C.dropped[c] = 0;
+                C.pend[c] = C.alpha[c] = C.beta[c] = 0.0;
+                if (writer) {
+                    A.res[c].rhs_norm = bn;
+                    A.res[c].initial_residual = r0n;
+                    A.res[c].tolerance = tol;
+                }
+            }
+            C.k = C.last_ru = C.refines = C.ru = C.has_pend = C.out_of_iters = 0;
+            C.any = C.active[0] || C.active[1];
+        }
+        __syncthreads();
+    }
+    while (C.any) {
+        const int nx = cur ^ 1;
+        const float alpha[2] = {(float)C.alpha[0], (float)C.alpha[1]};
+        const float beta[2] = {(float)C.beta[0], (float)C.beta[1]};
+        const double pend[2] = {C.pend[0], C.pend[1]};
+        const bool has_pend = C.has_pend != 0;
+        double sums[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        CG_TRACE_STAMP(C.k, 0);
+        phase_f2<BS, IT, IB, MP>(A, meta, T0, T1, lds64, A.st[cur], A.st[nx], alpha, beta, pend,
+                             has_pend, sums);
+        CG_TRACE_MAX(C.k, 1);
+        if (!grid_sum<8, BS>(A.sync, sums, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
+        CG_TRACE_STAMP(C.k, 2);
+        if (t0) {
+            ctl2_after_phase(C, A, sums);
+            if (!C.ru) ctl2_end(C);
+        }
+        __syncthreads();
+        if (C.ru) {
+            // apply the pending x += alpha p_k (FP64), then replace both
+            // recursive residuals: r_k := b - A x, alpha := 0
+            const double pf[2] = {C.pend[0], C.pend[1]};
+            flush_x2<BS>(A, R0, R1, A.st[nx], pf);
+            double dummy[1] = {0.0};
+            if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
+            double ra[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            residual_phase<2, BS, true, false, IT, true, MP>(A, meta, T0, T1, lds64, ra, A.st[nx]);
+            if (!grid_sum<6, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
+            if (t0) {
+                ctl2_after_update(C, A, ra);
+                ctl2_end(C);
+            }
+            __syncthreads();
+        }
+        if (!C.any) {
+            if (C.has_pend) {
+                const double pf[2] = {C.pend[0], C.pend[1]};
+                flush_x2<BS>(A, R0, R1, A.st[nx], pf);
+            }
+            break;
+        }
+        cur = nx;
+    }
+    if (writer) {
+        for (int c = 0; c < 2; ++c) {
+            A.res[c].iterations = C.iters[c];
+            A.res[c].refinements = C.refines;
+            A.res[c].status = C.status[c];
+            A.res[c].last_residual = sqrt(C.last_rr[c]);
+        }
+    }
+    return;
+timeout:
+    if (writer) {
+        for (int c = 0; c < 2; ++c) A.res[c].status = NSFEM_BARRIER_TIMEOUT;
+    }
+}
+
 // One block per CU: BS / 256 waves per SIMD (4 for 1024 threads, 128 VGPRs
 // per wave; 2 for 512 threads, up to 256 VGPRs).
 // WI (items per lane of a tile) < 8 is used by the one-column mixed solve only.
@@ -2424,6 +2795,19 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg(const CgArgs A) {
         cg_body1<BS, IT, WI>(A, blockIdx.x, gridDim.x, L, prod_raw, lds);
     else
         cg_body<NC, BS, MIXED, IT>(A, blockIdx.x, gridDim.x, L, prod_raw, lds);
+}
+
+// The two-column mixed solve's packed iteration: a kernel of its own, so
+// that its registers are allocated for it alone.  (The tile descriptors
+// stay in global memory: a copy in LDS beside the product slices measured
+// no faster, 2,495 against 2,525 GB/s in the 2048^2 cavity's mass solve.)
+template <int BS, typename IT, int IB>
+__global__ __launch_bounds__(BS, BS / 256) void k_cg_p2(const CgArgs A) {
+    constexpr int NW = BS / 64;
+    __shared__ double prod_raw[NW * kWaveNnz * 2];
+    __shared__ double lds[kMaxRed * NW];
+    const int L = xcd_swizzle(blockIdx.x, gridDim.x);
+    cg_body2_m<BS, IT, IB>(A, A.meta, blockIdx.x, gridDim.x, L, prod_raw, lds);
 }
 
 // Many independent systems in one persistent launch: block i works for
@@ -2530,12 +2914,50 @@ int phase_form(const Matrix &m, bool groups_default) {
     return -1;
 }
 
+template <typename IT, int BS, int IB>
+void cg_launch_p2(Device &d, const CgArgs &args, int grid) {
+    void *kargs[] = {(void *)&args};
+    NSFEM_HIP(hipLaunchCooperativeKernel((const void *)k_cg_p2<BS, IT, IB>, dim3(grid), dim3(BS),
+                                         kargs, 0, d.stream));
+}
+template <typename IT, int BS, int IB>
+int cg_max_grid_p2(Device &d) {
+    int per_cu = 0;
+    NSFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_p2<BS, IT, IB>, BS, 0));
+    return std::max(1, std::min(per_cu * d.cu_count, kMaxGridBlocks));
+}
+// Calls f.template run<IT, BS, IB>() for the packed two-column iteration's
+// block size (768 or 1024) and loads in flight (4 or 8 nonzeros per lane).
+template <typename IT, typename F>
+void p2_dispatch(int bs, int ib, F &f) {
+    if (bs == kCgBlockWide) {
+        if (ib == 8) f.template run2<IT, kCgBlockWide, 8>();
+        else f.template run2<IT, kCgBlockWide, 4>();
+    } else {
+        if (ib == 8) f.template run2<IT, kCgBlock, 8>();
+        else f.template run2<IT, kCgBlock, 4>();
+    }
+}
+
+// p2: the two-column mixed solve's packed iteration (k_cg_p2)
 struct CgMaxGrid {
     Device &d;
     int out;
     int bs = kCgBlock;
+    bool p2 = false;
+    int ib = 4;
+    template <typename IT, int BS, int IB>
+    void run2() {
+        out = cg_max_grid_p2<IT, BS, IB>(d);
+    }
     template <int NC, bool MX, typename IT, int WI>
     void run() {
+        if constexpr (NC == 2 && MX) {
+            if (p2) {
+                p2_dispatch<IT>(bs, ib, *this);
+                return;
+            }
+        }
         if constexpr (NC == 1 && MX) {
             if (bs == kCgBlockWide) {
                 out = cg_max_grid<NC, MX, IT, WI, kCgBlockWide>(d);
@@ -2550,8 +2972,20 @@ struct CgLaunch {
     const CgArgs &a;
     int grid;
     int bs = kCgBlock;
+    bool p2 = false;
+    int ib = 4;
+    template <typename IT, int BS, int IB>
+    void run2() {
+        cg_launch_p2<IT, BS, IB>(d, a, grid);
+    }
     template <int NC, bool MX, typename IT, int WI>
     void run() {
+        if constexpr (NC == 2 && MX) {
+            if (p2) {
+                p2_dispatch<IT>(bs, ib, *this);
+                return;
+            }
+        }
         if constexpr (NC == 1 && MX) {
             if (bs == kCgBlockWide) {
                 cg_launch<NC, MX, IT, WI, kCgBlockWide>(d, a, grid);
@@ -2663,6 +3097,26 @@ void set_state_buffers(CgArgs &a, unsigned char *w, size_t vb) {
     a.st[1] = w + 3 * vb;
 }
 
+// Iteration of two-column mixed solves: the packed single-reduction form
+// unless NSFEM_CG_PAIRED=twophase (read at every solve; comparison runs
+// switch it in one process).  The batched solves run the two-phase form.
+bool paired_packed() {
+    const char *e = std::getenv("NSFEM_CG_PAIRED");
+    return !(e && std::string(e) == "twophase");
+}
+// Block size of the packed two-column iteration: NSFEM_CG_P2_BLOCK=768 or
+// 1024 (read at every solve).
+int paired_block() {
+    const char *e = std::getenv("NSFEM_CG_P2_BLOCK");
+    return (e && std::atoi(e) == kCgBlockWide) ? kCgBlockWide : kCgBlock;
+}
+// Nonzeros per lane whose loads the packed two-column iteration keeps in
+// flight together: NSFEM_CG_P2_LOADS=4 or 8 (read at every solve).
+int paired_loads() {
+    const char *e = std::getenv("NSFEM_CG_P2_LOADS");
+    return (e && std::atoi(e) == 8) ? 8 : 4;
+}
+
 struct CgPass {
     CgColumnResult res[2];
     int iters[2] = {0, 0}, refines[2] = {0, 0};
@@ -2687,10 +3141,12 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
     // VGPRs) was slower on every measured solve.  One-column mixed solves
     // run 768 (cg_block).
     const bool idx16 = m.dcol_ok && want_idx16();
-    const int bs = cg_block(nc, mixed);
+    const bool p2 = nc == 2 && mixed && paired_packed();
+    const int bs = p2 ? paired_block() : cg_block(nc, mixed);
     const int form = idx16 && mixed && nc == 1 ? phase_form(m, bs == kCgBlockWide) : -1;
     const int wi = form >= 0 ? form : items_per_lane(m);
-    CgMaxGrid mg{d, 0, bs};
+    const int ib = p2 ? paired_loads() : 4;
+    CgMaxGrid mg{d, 0, bs, p2, ib};
     cg_dispatch(nc, mixed, idx16, wi, mg);
     const int maxg = mg.out;
     int want = req.grid;
@@ -2728,6 +3184,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
     a.p1 = w + 2 * vec_bytes;
     a.q = w + 3 * vec_bytes;
     set_state_buffers(a, w, vec_bytes);
+    a.packed2 = p2;
     set_cg_params(d, req, n, a);
     a.sync = ws.sync;
     a.res = ws.res;
@@ -2743,7 +3200,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
         NSFEM_HIP(hipEventRecord(d.ev_start, d.stream));
         k_sync_reset<<<(ngroups + 255) / 256, 256, 0, d.stream>>>(a.sync, ngroups);
         NSFEM_HIP(hipGetLastError());
-        CgLaunch ln{d, a, grid, bs};
+        CgLaunch ln{d, a, grid, bs, p2, ib};
         cg_dispatch(nc, mixed, idx16, wi, ln);
         NSFEM_HIP(hipEventRecord(d.ev_stop, d.stream));
         NSFEM_HIP(hipMemcpyAsync(P.res, ws.res, sizeof(CgColumnResult) * nc,

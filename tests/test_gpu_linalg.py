@@ -188,7 +188,14 @@ def test_cg_matches_oracle(gpu, oracle, which, precision):
         assert st["refinements"] >= 1
 
 
-def test_cg_two_columns(gpu, oracle):
+# two-column mixed forms: the packed single-reduction iteration in both
+# block sizes, and the two-phase iteration
+PAIRED_FORMS = [("fp64", {}), ("fp32_refined", {}),
+                ("fp32_refined", {"NSFEM_CG_P2_BLOCK": "768", "NSFEM_CG_P2_LOADS": "8"}),
+                ("fp32_refined", {"NSFEM_CG_PAIRED": "twophase"})]
+
+
+def test_cg_two_columns(gpu, oracle, monkeypatch):
     m, A = _cavity_system(oracle, 20, "diffusion")
     rs, col, val = A
     n = len(rs) - 1
@@ -196,7 +203,10 @@ def test_cg_two_columns(gpu, oracle):
     b = rng.standard_normal(2 * n)
     b[n:] *= 3.0
     gpu.upload_csr(11, rs, col, val)
-    for precision in ("fp64", "fp32_refined"):
+    for precision, env in PAIRED_FORMS:
+        print(f"two columns {precision} {env}", flush=True)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
         bx = _buf(gpu, np.zeros(2 * n))
         st = gpu.cg_solve(11, _buf(gpu, b), bx, columns=2, rtol=1e-10, precision=precision)
         x = bx.download()
@@ -210,6 +220,55 @@ def test_cg_two_columns(gpu, oracle):
                 single = _buf(gpu, np.zeros(n))
                 s1 = gpu.cg_solve(11, _buf(gpu, bc), single, rtol=1e-10)[0]
                 assert abs(s1["iterations"] - st[c]["iterations"]) <= 1
+            else:
+                # within a few iterations of the one-column mixed solve
+                single = _buf(gpu, np.zeros(n))
+                s1 = gpu.cg_solve(11, _buf(gpu, bc), single, rtol=1e-10,
+                                  precision=precision)[0]
+                assert abs(s1["iterations"] - st[c]["iterations"]) <= 0.1 * s1["iterations"] + 3
+        for k in env:
+            monkeypatch.delenv(k)
+
+
+@pytest.mark.parametrize("env", [{}, {"NSFEM_CG_P2_BLOCK": "768", "NSFEM_CG_P2_LOADS": "8"},
+                                 {"NSFEM_CG_PAIRED": "twophase"}],
+                         ids=["packed1024", "packed768", "twophase"])
+def test_cg_two_columns_one_converged(gpu, oracle, monkeypatch, env):
+    """A pair whose second column starts at its solution (zero right-hand
+    side, zero guess) and one whose second column converges in a handful of
+    iterations: the first column still reaches the tolerance and the second
+    is left exactly / within tolerance, on a system large enough for many
+    persistent blocks."""
+    from navier_stokes_fem_amd import device as D
+    from navier_stokes_fem_amd import mesh
+
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    m = mesh.cavity(96)
+    a = D.assemble(m, 0.01, 0.01)
+    rs, col, val, _ = a["velocity_mass"]
+    A = (rs, col, val)
+    n = len(rs) - 1
+    gpu.upload_csr(15, rs, col, val)
+    rng = np.random.default_rng(41)
+    b0 = rng.standard_normal(n)
+    e = np.zeros(n)
+    e[n // 2] = 1.0
+    for second, name in ((np.zeros(n), "zero"), (oracle.spmv_sub(A, np.zeros(n), -e), "one node")):
+        print(f"two columns, second {name}, {env}", flush=True)
+        b = np.concatenate([b0, second])
+        bx = _buf(gpu, np.zeros(2 * n))
+        st = gpu.cg_solve(15, _buf(gpu, b), bx, columns=2, rtol=1e-10, precision="fp32_refined")
+        x = bx.download()
+        assert st[0]["status"] == 0 and st[1]["status"] == 0, st
+        r0 = np.linalg.norm(oracle.spmv_sub(A, b0, x[:n]))
+        assert r0 <= 1e-10 * np.linalg.norm(b0) * (1 + 1e-9)
+        if name == "zero":
+            assert st[1]["iterations"] == 0
+            assert np.all(x[n:] == 0.0)
+        else:
+            r1 = np.linalg.norm(oracle.spmv_sub(A, second, x[n:]))
+            assert r1 <= 1e-10 * np.linalg.norm(second) * (1 + 1e-9)
 
 
 def test_cg_reference_semantics(gpu, oracle):
