@@ -907,9 +907,13 @@ def main():
     measured = MEASURED_BYTES_PER_ITERATION.get((mixed, name)) if args.n == 2048 else None
     traffic = measured * k_iters / launches if measured else None
     if cols == 2 and mixed and os.environ.get("NSFEM_CG_PAIRED") != "twophase":
-        # the packed single-reduction iteration (block size, loads in flight)
+        # the single-reduction two-column iteration: form, block size and
+        # loads in flight as the library picks them
+        form = "packed" if os.environ.get("NSFEM_CG_PAIRED") == "packed" else "narrow-gather"
         bs = 768 if os.environ.get("NSFEM_CG_P2_BLOCK") == "768" else 1024
-        kernel = "k_cg_p2<%d, short, %d>" % (bs, 8 if os.environ.get("NSFEM_CG_P2_LOADS") == "8" else 4)
+        ib = os.environ.get("NSFEM_CG_P2_LOADS")
+        ib = int(ib) if ib in ("4", "8") else (8 if form == "narrow-gather" else 4)
+        kernel = "k_cg_p2<%d, short, %d>, %s form" % (bs, ib, form)
     elif cols == 2:
         kernel = "k_cg<2, 1024, %s, short, 8>" % ("true" if mixed else "false")
     elif mixed:

@@ -1053,7 +1053,7 @@ struct CgArgs {
     double *x;        // [NC][n]
     void *r, *p0, *p1, *q;  // interleaved [n][NC] (two-column solves)
     void *st[2];            // packed {r, p, q} rows of mixed solves
-    int packed2;            // two-column mixed solves: the packed iteration (else two-phase)
+    int packed2;  // two-column mixed solves: 1 packed, 2 narrow-gather iteration (0 two-phase)
     double rtol, atol, rdelta;
     int max_iter, max_refine;
     int ru_gap;      // mixed: force a reliable update after this many iterations
@@ -2782,6 +2782,238 @@ timeout:
     }
 }
 
+// ---------------------------------------------------------------------
+// Two-column mixed solves, narrow-gather form: the packed iteration's
+// arithmetic with p_k stored once instead of recomputed by every reader.
+// After the reduction, a stream pass over the block's own rows forms
+//   r_k = r_{k-1} - alpha q_{k-1},  p_k = r_k + beta p_{k-1},
+//   x += pend p_{k-1} (FP64),
+// a grid barrier follows, and the product q_k = A p_k gathers only the
+// neighbours' {p0, p1} (8 bytes, one load per nonzero instead of two) with
+// the sums (p_k, q_k), (r_k, r_k), (r_k, q_k), (q_k, q_k) of the packed
+// form.  r, p and q are interleaved [n][2] FP32 arrays (one copy each: the
+// stream pass touches only its own rows, the product writes only q).  The
+// FP32 roundings, products and sums are those of the packed form, so the
+// iterates are the same bits.
+template <int BS>
+__device__ void phase_u2(const CgArgs &A, int R0, int R1, float2 *r, float2 *p, const float2 *q,
+                         const float (&alpha)[2], const float (&beta)[2], const double (&pend)[2],
+                         const bool has_pend) {
+    const bool pend0 = has_pend && pend[0] != 0.0, pend1 = has_pend && pend[1] != 0.0;
+    const size_t n = (size_t)A.n;
+    for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) {
+        const float2 ro = r[row], po = p[row], qo = q[row];
+        double x0 = 0.0, x1 = 0.0, scl = 1.0;
+        if (pend0) x0 = A.x[row];
+        if (pend1) x1 = A.x[n + row];
+        if (has_pend && A.scale) scl = A.scale[row];
+        const float r0 = next_r(alpha[0], ro.x, qo.x), r1 = next_r(alpha[1], ro.y, qo.y);
+        r[row] = make_float2(r0, r1);
+        p[row] = make_float2(__builtin_fmaf(beta[0], po.x, r0), __builtin_fmaf(beta[1], po.y, r1));
+        if (pend0) A.x[row] = __builtin_fma(pend[0] * scl, (double)po.x, x0);
+        if (pend1) A.x[n + row] = __builtin_fma(pend[1] * scl, (double)po.y, x1);
+    }
+}
+
+template <int BS, typename IT, int IB, typename MP>
+__device__ void phase_q2(const CgArgs &A, MP meta, int T0, int T1, double *lds_wave,
+                         const float2 *p, const float2 *r, float2 *q, double (&sums)[8]) {
+    constexpr int NW = BS / 64;
+    constexpr int WI = kWItems;
+    const float *val = A.val32;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int t = T0 + wave;
+    if (t >= T1) return;
+    WTile w = wave_tile(meta, t);
+    int cn[WI];
+    {
+        const int kf = w.nz > 0 ? w.k0 : 0;
+#pragma unroll
+        for (int i = 0; i < WI; ++i) {
+            const int k = lane + 64 * i;
+            cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
+        }
+    }
+    for (; t < T1; t += NW) {
+        const WTile cur = w;
+        int cj[WI];
+#pragma unroll
+        for (int i = 0; i < WI; ++i) cj[i] = col_of<IT>(cn[i], cur.r0);
+        const int tn = t + NW;
+        if (tn < T1) {
+            w = wave_tile(meta, tn);
+            const int kf = w.nz > 0 ? w.k0 : 0;
+#pragma unroll
+            for (int i = 0; i < WI; ++i) {
+                const int k = lane + 64 * i;
+                cn[i] = col_word<IT>(A, k < w.nz ? w.k0 + k : kf);
+            }
+        }
+        const int row = cur.r0 + lane;
+        const bool has_row = row < cur.r1;
+        const int rowc = has_row ? row : cur.r0;
+        const int rsa = A.rs[rowc];
+        const int rsb = A.rs[rowc + 1];
+        const float2 po = p[rowc], ro = r[rowc];
+        double s[2] = {0.0, 0.0};
+        if (cur.nz == 0) {
+            // rows without nonzeros: q = 0
+        } else if (cur.nz <= 64 * WI) {
+#pragma unroll
+            for (int i0 = 0; i0 < WI; i0 += IB) {
+                float vj[IB];
+                float2 gj[IB];
+#pragma unroll
+                for (int i = 0; i < IB; ++i) {
+                    const int k = lane + 64 * (i0 + i);
+                    vj[i] = ld_stream(val + (k < cur.nz ? cur.k0 + k : 0));
+                    gj[i] = p[cj[i0 + i]];
+                }
+#pragma unroll
+                for (int i = 0; i < IB; ++i) {
+                    const int k = lane + 64 * (i0 + i);
+                    const double v = (double)vj[i];
+                    lds_wave[2 * k] = v * (double)gj[i].x;
+                    lds_wave[2 * k + 1] = v * (double)gj[i].y;
+                }
+            }
+            __syncwarp();
+            const int ra = rsa - cur.k0;
+            const int rb = has_row ? rsb - cur.k0 : ra;
+            row_sums<2>(lds_wave, ra, rb, s);
+            __syncwarp();
+        } else {
+            // a single row longer than a tile: wave-wide sums
+            double part0 = 0.0, part1 = 0.0;
+            for (int k = lane; k < cur.nz; k += 64) {
+                const float2 g = p[col_at<IT>(A, cur.r0, cur.k0 + k)];
+                const double v = (double)val[cur.k0 + k];
+                part0 += v * (double)g.x;
+                part1 += v * (double)g.y;
+            }
+            s[0] = wave_sum(part0);
+            s[1] = wave_sum(part1);
+        }
+        if (has_row) {
+            const float q0 = (float)s[0], q1 = (float)s[1];
+            const double r0d = (double)ro.x, q0d = (double)q0, r1d = (double)ro.y, q1d = (double)q1;
+            sums[0] += (double)po.x * s[0];
+            sums[1] += r0d * r0d;
+            sums[2] += r0d * q0d;
+            sums[3] += q0d * q0d;
+            sums[4] += (double)po.y * s[1];
+            sums[5] += r1d * r1d;
+            sums[6] += r1d * q1d;
+            sums[7] += q1d * q1d;
+            q[row] = make_float2(q0, q1);
+        }
+    }
+}
+
+// The narrow-gather two-column mixed CG run by one block (control as
+// cg_body2_m; one grid barrier and one reduction an iteration).
+template <int BS, typename IT, int IB, typename MP>
+__device__ __forceinline__ void cg_body2n_m(const CgArgs &A, MP meta, const int bid, const int nb,
+                                            const int L, double *prod_raw, double *lds) {
+    __shared__ CgCtl2 C;
+    const int T0 = A.block_tiles[L], T1 = A.block_tiles[L + 1];
+    const int R0 = A.tiles[T0], R1 = A.tiles[T1];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double *lds64 = prod_raw + wave * kWaveNnz * 2;
+    float2 *r = (float2 *)A.r, *p = (float2 *)A.p0, *q = (float2 *)A.q;
+    unsigned epoch = 0;
+    const bool t0 = threadIdx.x == 0;
+    const bool writer = (L == 0 && t0);
+
+    {
+        // r = b - A x (FP64, stored FP32), p = 0, q = 0
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        residual_phase<2, BS, true, true, IT, false, MP>(A, meta, T0, T1, lds64, acc);
+        for (int row = R0 + (int)threadIdx.x; row < R1; row += BS) q[row] = make_float2(0.f, 0.f);
+        if (!grid_sum<6, BS>(A.sync, acc, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
+        if (t0) {
+            for (int c = 0; c < 2; ++c) {
+                const double bn = sqrt(acc[2 + c]), r0n = sqrt(acc[c]);
+                const double ref = bn > 0.0 ? bn : r0n;
+                const double tol = fmax(A.rtol * ref, A.atol);
+                C.tol2[c] = tol * tol;
+                C.active[c] = acc[c] > C.tol2[c];
+                C.gamma[c] = acc[4 + c];
+                C.rmax2[c] = A.scale ? acc[4 + c] : acc[c];
+                C.last_rr[c] = C.ru_rr[c] = acc[c];
+                C.rho[c] = A.scale && acc[4 + c] > 0.0 ? acc[c] / acc[4 + c] : 1.0;
+                C.g[c] = 0.0;
+                C.iters[c] = 0;
+                C.status[c] = NSFEM_OK;
+                C.stall[c] = C.dropped[c] = 0;
+                C.pend[c] = C.alpha[c] = C.beta[c] = 0.0;
+                if (writer) {
+                    A.res[c].rhs_norm = bn;
+                    A.res[c].initial_residual = r0n;
+                    A.res[c].tolerance = tol;
+                }
+            }
+            C.k = C.last_ru = C.refines = C.ru = C.has_pend = C.out_of_iters = 0;
+            C.any = C.active[0] || C.active[1];
+        }
+        __syncthreads();
+    }
+    while (C.any) {
+        {
+            const float alpha[2] = {(float)C.alpha[0], (float)C.alpha[1]};
+            const float beta[2] = {(float)C.beta[0], (float)C.beta[1]};
+            const double pend[2] = {C.pend[0], C.pend[1]};
+            phase_u2<BS>(A, R0, R1, r, p, q, alpha, beta, pend, C.has_pend != 0);
+        }
+        double dummy[1] = {0.0};
+        if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
+        double sums[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        phase_q2<BS, IT, IB, MP>(A, meta, T0, T1, lds64, p, r, q, sums);
+        if (!grid_sum<8, BS>(A.sync, sums, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
+        if (t0) {
+            ctl2_after_phase(C, A, sums);
+            if (!C.ru) ctl2_end(C);
+        }
+        __syncthreads();
+        if (C.ru) {
+            // apply the pending x += alpha p_k (FP64), then replace both
+            // recursive residuals: r_k := b - A x, alpha := 0
+            const double pf[2] = {C.pend[0], C.pend[1]};
+            flush_x<2, BS, float>(A, R0, R1, (const float *)p, pf);
+            if (!grid_sum<1, BS>(A.sync, dummy, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
+            double ra[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            residual_phase<2, BS, true, false, IT, false, MP>(A, meta, T0, T1, lds64, ra);
+            if (!grid_sum<6, BS>(A.sync, ra, epoch, lds, A.timeout_ticks, bid, nb, A.flat)) goto timeout;
+            if (t0) {
+                ctl2_after_update(C, A, ra);
+                ctl2_end(C);
+            }
+            __syncthreads();
+        }
+        if (!C.any) {
+            if (C.has_pend) {
+                const double pf[2] = {C.pend[0], C.pend[1]};
+                flush_x<2, BS, float>(A, R0, R1, (const float *)p, pf);
+            }
+            break;
+        }
+    }
+    if (writer) {
+        for (int c = 0; c < 2; ++c) {
+            A.res[c].iterations = C.iters[c];
+            A.res[c].refinements = C.refines;
+            A.res[c].status = C.status[c];
+            A.res[c].last_residual = sqrt(C.last_rr[c]);
+        }
+    }
+    return;
+timeout:
+    if (writer) {
+        for (int c = 0; c < 2; ++c) A.res[c].status = NSFEM_BARRIER_TIMEOUT;
+    }
+}
+
 // One block per CU: BS / 256 waves per SIMD (4 for 1024 threads, 128 VGPRs
 // per wave; 2 for 512 threads, up to 256 VGPRs).
 // WI (items per lane of a tile) < 8 is used by the one-column mixed solve only.
@@ -2807,7 +3039,10 @@ __global__ __launch_bounds__(BS, BS / 256) void k_cg_p2(const CgArgs A) {
     __shared__ double prod_raw[NW * kWaveNnz * 2];
     __shared__ double lds[kMaxRed * NW];
     const int L = xcd_swizzle(blockIdx.x, gridDim.x);
-    cg_body2_m<BS, IT, IB>(A, A.meta, blockIdx.x, gridDim.x, L, prod_raw, lds);
+    if (A.packed2 == 2)
+        cg_body2n_m<BS, IT, IB>(A, A.meta, blockIdx.x, gridDim.x, L, prod_raw, lds);
+    else
+        cg_body2_m<BS, IT, IB>(A, A.meta, blockIdx.x, gridDim.x, L, prod_raw, lds);
 }
 
 // Many independent systems in one persistent launch: block i works for
@@ -3097,12 +3332,20 @@ void set_state_buffers(CgArgs &a, unsigned char *w, size_t vb) {
     a.st[1] = w + 3 * vb;
 }
 
-// Iteration of two-column mixed solves: the packed single-reduction form
-// unless NSFEM_CG_PAIRED=twophase (read at every solve; comparison runs
-// switch it in one process).  The batched solves run the two-phase form.
+// Iteration of two-column mixed solves: the single-reduction iteration
+// (k_cg_p2) unless NSFEM_CG_PAIRED=twophase, in its narrow-gather form
+// unless NSFEM_CG_PAIRED=packed (read at every solve; comparison runs switch
+// it in one process).  Measured on the 2048^2 cavity's mass solve (textbook
+// bytes / time): narrow 2,581 (4 loads in flight) and 2,603 GB/s (8),
+// packed 2,521, two-phase 2,463.  The batched solves run the two-phase form.
 bool paired_packed() {
     const char *e = std::getenv("NSFEM_CG_PAIRED");
     return !(e && std::string(e) == "twophase");
+}
+// 2: the narrow-gather form, 1: the packed form.
+int paired_form() {
+    const char *e = std::getenv("NSFEM_CG_PAIRED");
+    return (e && std::string(e) == "packed") ? 1 : 2;
 }
 // Block size of the packed two-column iteration: NSFEM_CG_P2_BLOCK=768 or
 // 1024 (read at every solve).
@@ -3110,11 +3353,14 @@ int paired_block() {
     const char *e = std::getenv("NSFEM_CG_P2_BLOCK");
     return (e && std::atoi(e) == kCgBlockWide) ? kCgBlockWide : kCgBlock;
 }
-// Nonzeros per lane whose loads the packed two-column iteration keeps in
-// flight together: NSFEM_CG_P2_LOADS=4 or 8 (read at every solve).
-int paired_loads() {
+// Nonzeros per lane whose loads the two-column iteration keeps in flight
+// together: NSFEM_CG_P2_LOADS=4 or 8 (read at every solve); by default 8 for
+// the narrow-gather form (one 8-byte gather a nonzero), 4 for the packed
+// one (two gathers a nonzero; 8 measured 2,497 against 2,527 GB/s).
+int paired_loads(int form) {
     const char *e = std::getenv("NSFEM_CG_P2_LOADS");
-    return (e && std::atoi(e) == 8) ? 8 : 4;
+    if (e && (std::atoi(e) == 4 || std::atoi(e) == 8)) return std::atoi(e);
+    return form == 2 ? 8 : 4;
 }
 
 struct CgPass {
@@ -3145,7 +3391,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
     const int bs = p2 ? paired_block() : cg_block(nc, mixed);
     const int form = idx16 && mixed && nc == 1 ? phase_form(m, bs == kCgBlockWide) : -1;
     const int wi = form >= 0 ? form : items_per_lane(m);
-    const int ib = p2 ? paired_loads() : 4;
+    const int ib = p2 ? paired_loads(paired_form()) : 4;
     CgMaxGrid mg{d, 0, bs, p2, ib};
     cg_dispatch(nc, mixed, idx16, wi, mg);
     const int maxg = mg.out;
@@ -3184,7 +3430,7 @@ void cg_pass(Device &d, const CgRequest &req, bool mixed, CgPass &P,
     a.p1 = w + 2 * vec_bytes;
     a.q = w + 3 * vec_bytes;
     set_state_buffers(a, w, vec_bytes);
-    a.packed2 = p2;
+    a.packed2 = p2 ? paired_form() : 0;
     set_cg_params(d, req, n, a);
     a.sync = ws.sync;
     a.res = ws.res;

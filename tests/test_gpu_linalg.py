@@ -188,10 +188,11 @@ def test_cg_matches_oracle(gpu, oracle, which, precision):
         assert st["refinements"] >= 1
 
 
-# two-column mixed forms: the packed single-reduction iteration in both
-# block sizes, and the two-phase iteration
+# two-column mixed forms: the single-reduction iteration (narrow-gather by
+# default, packed) in both block sizes, and the two-phase iteration
 PAIRED_FORMS = [("fp64", {}), ("fp32_refined", {}),
-                ("fp32_refined", {"NSFEM_CG_P2_BLOCK": "768", "NSFEM_CG_P2_LOADS": "8"}),
+                ("fp32_refined", {"NSFEM_CG_P2_BLOCK": "768", "NSFEM_CG_P2_LOADS": "4"}),
+                ("fp32_refined", {"NSFEM_CG_PAIRED": "packed"}),
                 ("fp32_refined", {"NSFEM_CG_PAIRED": "twophase"})]
 
 
@@ -230,9 +231,10 @@ def test_cg_two_columns(gpu, oracle, monkeypatch):
             monkeypatch.delenv(k)
 
 
-@pytest.mark.parametrize("env", [{}, {"NSFEM_CG_P2_BLOCK": "768", "NSFEM_CG_P2_LOADS": "8"},
+@pytest.mark.parametrize("env", [{}, {"NSFEM_CG_P2_BLOCK": "768"}, {"NSFEM_CG_PAIRED": "packed"},
+                                 {"NSFEM_CG_PAIRED": "packed", "NSFEM_CG_P2_BLOCK": "768"},
                                  {"NSFEM_CG_PAIRED": "twophase"}],
-                         ids=["packed1024", "packed768", "twophase"])
+                         ids=["narrow1024", "narrow768", "packed1024", "packed768", "twophase"])
 def test_cg_two_columns_one_converged(gpu, oracle, monkeypatch, env):
     """A pair whose second column starts at its solution (zero right-hand
     side, zero guess) and one whose second column converges in a handful of
@@ -328,3 +330,31 @@ def test_cg_large_grid(gpu, oracle):
         x = bx.download()
         r = np.linalg.norm(oracle.spmv_sub((rs, col, val), b, x))
         assert st["status"] == 0 and r <= 1e-8 * np.linalg.norm(b) * (1 + 1e-9), (precision, st)
+
+
+def test_cg_two_columns_narrow_equals_packed(gpu, monkeypatch):
+    """The narrow-gather two-column iteration stores p_k once where the
+    packed one recomputes it from each neighbour's {r, p, q} by the same
+    FP32 roundings, with the same products and sums: solutions, iteration
+    and refinement counts are bit-identical, on a system of many persistent
+    blocks."""
+    from navier_stokes_fem_amd import device as D
+    from navier_stokes_fem_amd import mesh
+
+    m = mesh.cavity(96)
+    a = D.assemble(m, 0.01, 0.01)
+    rs, col, val, _ = a["velocity_mass"]
+    n = len(rs) - 1
+    gpu.upload_csr(15, rs, col, val)
+    rng = np.random.default_rng(43)
+    b = rng.standard_normal(2 * n)
+    out = {}
+    for form in ("packed", "narrow"):
+        print(f"two columns {form}", flush=True)
+        monkeypatch.setenv("NSFEM_CG_PAIRED", form)
+        bx = _buf(gpu, np.zeros(2 * n))
+        st = gpu.cg_solve(15, _buf(gpu, b), bx, columns=2, rtol=1e-10, precision="fp32_refined")
+        out[form] = (bx.download(), [(c["iterations"], c["refinements"], c["status"]) for c in st])
+    np.testing.assert_array_equal(out["packed"][0], out["narrow"][0])
+    assert out["packed"][1] == out["narrow"][1]
+    assert all(c[2] == 0 for c in out["narrow"][1])
