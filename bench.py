@@ -68,12 +68,12 @@ def cg_bytes(nnz, n, iters, refinements, mixed, columns=1):
 # FETCH_SIZE and WRITE_SIZE (scaled by a k_axpy calibration) on this
 # workload, the launches' set-up and reliable-update residual passes
 # included; source: profiles/k_cg_cavity2048_hbm_traffic.md (the mass row
-# on the default command: the packed two-column kernel k_cg_p2, ~10.6
+# on the default command: k_cg_p2's narrow-gather form, ~10.6
 # iterations and ~2.9 FP64 passes a launch;
 # the others on the forms that still run those solves by the persistent
 # CG).  roofline.traffic = this figure x the iterations of a launch.
 MEASURED_BYTES_PER_ITERATION = {(False, "pressure"): 721.9e6, (True, "pressure"): 435.4e6,
-                                (True, "diffusion v"): 2723.9e6, (True, "mass u+v"): 5895.7e6}
+                                (True, "diffusion v"): 2723.9e6, (True, "mass u+v"): 5079.9e6}
 
 
 def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
