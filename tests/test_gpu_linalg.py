@@ -358,3 +358,49 @@ def test_cg_two_columns_narrow_equals_packed(gpu, monkeypatch):
     np.testing.assert_array_equal(out["packed"][0], out["narrow"][0])
     assert out["packed"][1] == out["narrow"][1]
     assert all(c[2] == 0 for c in out["narrow"][1])
+
+
+@pytest.mark.parametrize("env", [{}, {"NSFEM_CG_PAIRED": "packed"}, {"NSFEM_CG_PAIRED": "twophase"}],
+                         ids=["narrow", "packed", "twophase"])
+def test_cg_two_columns_long_row(gpu, oracle, monkeypatch, env):
+    """Two columns on an SPD arrow matrix whose first row and column are
+    dense (3,000 nonzeros, longer than a wave tile, summed wave-wide) around
+    a tridiagonal block: every two-column form reaches the tolerance, FP64
+    and FP32 + refinement."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    n = 3000
+    a = 0.5 / np.sqrt(n)  # eigenvalues in [0.75, 3.0]
+    rows, cols, vals = [0] * n, list(range(n)), [2.0] + [a] * (n - 1)
+    for j in range(1, n):
+        rows += [j, j]
+        cols += [0, j]
+        vals += [a, 2.0]
+        if j > 1:
+            rows.append(j)
+            cols.append(j - 1)
+            vals.append(-0.5)
+        if j < n - 1:
+            rows.append(j)
+            cols.append(j + 1)
+            vals.append(-0.5)
+    order = np.lexsort((np.array(cols), np.array(rows)))
+    r_, c_, v_ = np.array(rows)[order], np.array(cols, np.int32)[order], np.array(vals)[order]
+    rs = np.zeros(n + 1, np.int32)
+    np.add.at(rs, r_ + 1, 1)
+    rs = np.cumsum(rs).astype(np.int32)
+    A = (rs, c_, v_)
+    assert rs[1] - rs[0] == n  # the dense row
+    gpu.upload_csr(14, rs, c_, v_)
+    rng = np.random.default_rng(47)
+    b = rng.standard_normal(2 * n)
+    for precision in ("fp64", "fp32_refined"):
+        print(f"arrow matrix, two columns, {precision}, {env}", flush=True)
+        bx = _buf(gpu, np.zeros(2 * n))
+        st = gpu.cg_solve(14, _buf(gpu, b), bx, columns=2, rtol=1e-10, precision=precision)
+        x = bx.download()
+        for c in range(2):
+            bc = b[c * n:(c + 1) * n]
+            assert st[c]["status"] == 0, st
+            r = np.linalg.norm(oracle.spmv_sub(A, bc, x[c * n:(c + 1) * n]))
+            assert r <= 1e-10 * np.linalg.norm(bc) * (1 + 1e-9)
