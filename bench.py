@@ -73,7 +73,7 @@ def cg_bytes(nnz, n, iters, refinements, mixed, columns=1):
 # the others on the forms that still run those solves by the persistent
 # CG).  roofline.traffic = this figure x the iterations of a launch.
 MEASURED_BYTES_PER_ITERATION = {(False, "pressure"): 721.9e6, (True, "pressure"): 435.4e6,
-                                (True, "diffusion v"): 2723.9e6, (True, "mass u+v"): 5079.9e6}
+                                (True, "diffusion v"): 2723.9e6, (True, "mass u+v"): 4898.0e6}
 
 
 def spmv_bench(dev, sim, slot, n_rows, n_cols, nnz, reps=100):
