@@ -528,6 +528,27 @@ __global__ void k_axpy(int n, double a, const double *__restrict__ x, double *y)
         y[i] = __builtin_fma(a, x[i], y[i]);
 }
 
+// y = y0 + a_0 X_0 + ... + a_{l-1} X_{l-1} (y0 = 0, or y's own values when
+// `accumulate`), X_k = X + k * stride, one fused multiply-add per term in k
+// order: the bits of a fill (or nothing) followed by l k_axpy launches, in
+// one pass over the vectors.  y2, when given, receives the same values.
+constexpr int kCombineMax = 8;
+struct CombineCoef {
+    double a[kCombineMax];
+};
+
+__global__ void k_combine(int n, int l, CombineCoef c, const double *__restrict__ X,
+                          size_t stride, int accumulate, double *__restrict__ y,
+                          double *__restrict__ y2) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double t = accumulate ? y[i] : 0.0;
+        for (int k = 0; k < l; ++k) t = __builtin_fma(c.a[k], X[(size_t)k * stride + i], t);
+        y[i] = t;
+        if (y2) y2[i] = t;
+    }
+}
+
 __global__ void k_axpby(int n, double a, double b, const double *x, const double *y,
                         double *out) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -762,6 +783,28 @@ void launch_axpy(Device &d, int n, double a, const double *x, double *y) {
     if (n <= 0) return;
     k_axpy<<<ew_grid(n), kBlock, 0, d.stream>>>(n, a, x, y);
     NSFEM_HIP(hipGetLastError());
+}
+
+void launch_combine(Device &d, int n, int l, const double *a, const double *X, size_t stride,
+                    bool accumulate, double *y, double *y2) {
+    if (n <= 0) return;
+    if (l <= 0) {
+        if (!accumulate) launch_fill(d, n, 0.0, y);
+        if (y2) launch_copy(d, n, y, y2);
+        return;
+    }
+    // terms beyond kCombineMax continue from the stored partial sum (an FP64
+    // store and load: the same bits as one longer chain)
+    for (int k0 = 0; k0 < l; k0 += kCombineMax) {
+        const int m = std::min(kCombineMax, l - k0);
+        CombineCoef c{};
+        for (int k = 0; k < m; ++k) c.a[k] = a[k0 + k];
+        const bool last = k0 + m == l;
+        k_combine<<<ew_grid(n), kBlock, 0, d.stream>>>(n, m, c, X + (size_t)k0 * stride, stride,
+                                                        (accumulate || k0 > 0) ? 1 : 0, y,
+                                                        last ? y2 : nullptr);
+        NSFEM_HIP(hipGetLastError());
+    }
 }
 
 void launch_axpby(Device &d, int n, double a, double b, const double *x, const double *y,

@@ -264,6 +264,11 @@ void launch_spmv(Device &d, Matrix &m, const float *x, const float *lhs, float *
 void launch_axpy(Device &d, int n, double a, const double *x, double *y);
 void launch_axpby(Device &d, int n, double a, double b, const double *x,
                   const double *y, double *out);
+// y = (accumulate ? y : 0) + sum_{k<l} a[k] * X[k * stride ...], one FMA per
+// term in k order (the bits of a fill and l launch_axpy calls) in one pass;
+// y2 (optional) gets the same values
+void launch_combine(Device &d, int n, int l, const double *a, const double *X, size_t stride,
+                    bool accumulate, double *y, double *y2 = nullptr);
 void launch_dot_accumulate(Device &d, int n, const double *a, const double *b,
                            double *result);
 

@@ -558,10 +558,10 @@ void projection_guess(Device &d, ProjBasis &B, const double *b, double *x) {
     if (l > 0) {
         std::vector<double> a(l);
         proj_dots(d, B, l, b, a.data());
-        launch_fill(d, B.n, 0.0, x);
-        for (int i = 0; i < l; ++i) launch_axpy(d, B.n, a[i], B.x.ptr + (size_t)i * B.n, x);
+        launch_combine(d, B.n, l, a.data(), B.x.ptr, (size_t)B.n, false, x, B.guess.ptr);
+    } else {
+        launch_copy(d, B.n, x, B.guess.ptr);
     }
-    launch_copy(d, B.n, x, B.guess.ptr);
 }
 
 // After the solve x: the correction dx = x - guess (the whole solution after
@@ -588,10 +588,9 @@ void projection_update(Device &d, ProjBasis &B, Matrix &A, const double *x) {
     double n1 = n0, before = n0;
     for (int pass = 0; pass < 2 && l > 0; ++pass) {
         proj_dots(d, B, l, w, c.data());  // X_i . A dx = (A X_i) . dx
-        for (int i = 0; i < l; ++i) {
-            launch_axpy(d, n, -c[i], B.x.ptr + (size_t)i * n, dx);
-            launch_axpy(d, n, -c[i], B.ax.ptr + (size_t)i * n, w);
-        }
+        for (int i = 0; i < l; ++i) c[i] = -c[i];
+        launch_combine(d, n, l, c.data(), B.x.ptr, (size_t)n, true, dx);
+        launch_combine(d, n, l, c.data(), B.ax.ptr, (size_t)n, true, w);
         n1 = dev_dot(d, B, dx, w);
         if (n1 > 0.5 * before) break;
         before = n1;
@@ -1765,7 +1764,7 @@ void diffusion_guess(Simulation &s) {
         double *x = s.uv.ptr + (size_t)c * V, *del = s.vdel.ptr + (size_t)c * V;
         ProjBasis &B = s.vproj[c];
         launch_spmv(d, A, x, s.rhs.ptr + (size_t)c * V, B.w.ptr, 1);  // r0 = b - A u~
-        launch_fill(d, V, 0.0, del);
+        if (B.dim == 0) launch_fill(d, V, 0.0, del);  // else the projection writes all of del
         projection_guess(d, B, B.w.ptr, del);
         launch_copy(d, V, x, s.du.ptr + (size_t)c * V);
         launch_add(d, V, del, x);
@@ -1921,10 +1920,13 @@ void simulation_step(Simulation &s, int steps, nsfem_step_stats *stats) {
         pressure_after_solve(s);
         // 4. velocity correction
         launch_spmv(d, PD, s.p.ptr, nullptr, s.rhs.ptr, 0);
-        launch_copy(d, 2 * V, s.uv.ptr, s.du.ptr);
-        if (s.mproj[0].max > 0)
-            for (int c = 0; c < 2; ++c)
-                projection_guess(d, s.mproj[c], s.rhs.ptr + (size_t)c * V, s.du.ptr + (size_t)c * V);
+        for (int c = 0; c < 2; ++c) {
+            ProjBasis &B = s.mproj[c];
+            double *x0 = s.du.ptr + (size_t)c * V;
+            // the velocity input, unless a non-empty basis writes all of x0
+            if (B.max == 0 || B.dim == 0) launch_copy(d, V, s.uv.ptr + (size_t)c * V, x0);
+            if (B.max > 0) projection_guess(d, B, s.rhs.ptr + (size_t)c * V, x0);
+        }
         const bool pair_m = pair_columns(s, 1);
         if (pair_m) {
             solve(s, NSFEM_VELOCITY_MASS, s.rhs.ptr, s.du.ptr, 2, o[3]);
